@@ -1,0 +1,267 @@
+"""Triangular surface meshes in the layout of the reference's ``Mesh3d``.
+
+Mirrors ``Mesh3d`` (src/utils/mesh.jl:23-34): ``coordinates`` 3 x Nc, ``topology``
+n_g x E, ``sources`` 3 x N, ``physics_topology`` n_p x E, all column-major.  Indices
+are stored 0-based here; the C-ABI takes an ``index_base`` so 1-based Julia arrays
+pass through unchanged.
+
+Generators:
+  * ``icosphere(f, order)``: geodesic sphere, every icosahedron face split f x f,
+    vertices projected to the sphere; 10 f^2 + 2 corner nodes, 20 f^2 triangles.
+    Quadratic meshes add the 30 f^2 edge mid nodes (projected), corners first.
+  * ``ellipsoid(f, axes, order)``: the same sphere scaled per axis.
+  * ``load_comsol(path, geometry_order, physics_order)``: reader for the ``.mphtxt``
+    files the reference's tests use (src/3d/read_comsol.jl:9-119), including the
+    swap of local nodes 5 and 6 of 6-node triangles (read_comsol.jl:49-52).
+
+Element node order follows the reference (src/3d/SurfaceFunction.jl:40-57): corners
+1,2,3 then mid(1,2), mid(2,3), mid(3,1); (x2-x1) x (x3-x1) points out of the body,
+as in the bundled COMSOL spheres, so the reference's negated normal
+(src/3d/jacobian.jl:63-70) points into the body.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+
+import numpy as np
+
+TRI3 = 3
+TRI6 = 6
+
+
+@dataclasses.dataclass
+class Mesh3d:
+    """Data model of the reference's ``Mesh3d`` restricted to what assembly reads."""
+
+    coordinates: np.ndarray        # (3, Nc) float64
+    topology: np.ndarray           # (n_g, E) int64, 0-based
+    sources: np.ndarray            # (3, N) float64, collocation nodes
+    physics_topology: np.ndarray   # (n_p, E) int64, 0-based, indexes ``sources``
+    geometry_kind: int             # TRI3 | TRI6
+    physics_kind: int              # TRI3 | TRI6
+
+    def __post_init__(self):
+        self.coordinates = np.ascontiguousarray(self.coordinates, dtype=np.float64)
+        self.sources = np.ascontiguousarray(self.sources, dtype=np.float64)
+        self.topology = np.ascontiguousarray(self.topology, dtype=np.int64)
+        self.physics_topology = np.ascontiguousarray(self.physics_topology, dtype=np.int64)
+        if self.coordinates.shape[0] != 3 or self.sources.shape[0] != 3:
+            raise ValueError("coordinates and sources must be 3 x n")
+        if self.topology.shape[0] != self.geometry_kind:
+            raise ValueError("topology rows must equal the geometry node count")
+        if self.physics_topology.shape[0] != self.physics_kind:
+            raise ValueError("physics_topology rows must equal the physics node count")
+        if self.topology.shape[1] != self.physics_topology.shape[1]:
+            raise ValueError("topology and physics_topology disagree on element count")
+
+    @property
+    def n_elements(self) -> int:
+        return int(self.topology.shape[1])
+
+    @property
+    def n_sources(self) -> int:
+        return int(self.sources.shape[1])
+
+    @property
+    def n_coordinates(self) -> int:
+        return int(self.coordinates.shape[1])
+
+    def fortran_views(self):
+        """Arrays laid out column-major (Julia order) as flat contiguous buffers."""
+        return (np.asfortranarray(self.coordinates), np.asfortranarray(self.topology),
+                np.asfortranarray(self.sources), np.asfortranarray(self.physics_topology))
+
+
+# ---------------------------------------------------------------------------------------
+# Geodesic sphere
+# ---------------------------------------------------------------------------------------
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    # orient every face so (b-a) x (c-a) points outward
+    for i, (a, b, c) in enumerate(f):
+        n = np.cross(v[b] - v[a], v[c] - v[a])
+        if np.dot(n, v[a] + v[b] + v[c]) < 0:
+            f[i] = [a, c, b]
+    return v, f
+
+
+def _subdivide(f_level: int):
+    """Split every icosahedron face into f^2 triangles; returns unprojected vertices."""
+    v0, faces0 = _icosahedron()
+    key_to_idx: dict = {}
+    verts = []
+
+    # Exact integer keys: a point on face (a,b,c) at barycentric (f-i-j, i, j)/f is keyed
+    # by the sorted (vertex, weight) pairs, which is identical on shared edges.
+    def vid(a, b, c, i, j):
+        w = {}
+        for node, wt in ((a, f_level - i - j), (b, i), (c, j)):
+            if wt:
+                w[node] = w.get(node, 0) + wt
+        key = tuple(sorted(w.items()))
+        idx = key_to_idx.get(key)
+        if idx is None:
+            idx = len(verts)
+            key_to_idx[key] = idx
+            p = sum(v0[node] * wt for node, wt in key) / f_level
+            verts.append(p)
+        return idx
+
+    tris = []
+    for a, b, c in faces0:
+        grid = {}
+        for i in range(f_level + 1):
+            for j in range(f_level + 1 - i):
+                grid[i, j] = vid(a, b, c, i, j)
+        for i in range(f_level):
+            for j in range(f_level - i):
+                tris.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
+                if i + j + 2 <= f_level:
+                    tris.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
+    return np.array(verts), np.array(tris, dtype=np.int64)
+
+
+def _project(p, radius=1.0):
+    return radius * p / np.linalg.norm(p, axis=-1, keepdims=True)
+
+
+def icosphere(f: int, order: str = "linear", radius: float = 1.0,
+              physics: str = "geometry") -> Mesh3d:
+    """Geodesic sphere mesh (nodes 10 f^2 + 2 linear, 40 f^2 + 2 quadratic)."""
+    if f < 1:
+        raise ValueError("f must be >= 1")
+    verts, tris = _subdivide(f)
+    corners = _project(verts, 1.0)
+    # orientation: (x2-x1) x (x3-x1) outward
+    a, b, c = corners[tris[:, 0]], corners[tris[:, 1]], corners[tris[:, 2]]
+    flip = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    if order == "linear":
+        coords = corners * radius
+        topo = tris.T.copy()
+        gk = TRI3
+    elif order == "quadratic":
+        edge_id: dict = {}
+        mids = []
+        n_c = corners.shape[0]
+        quad = np.empty((tris.shape[0], 6), dtype=np.int64)
+        for e, (p, q, r) in enumerate(tris):
+            quad[e, :3] = (p, q, r)
+            for slot, (u, w) in zip((3, 4, 5), ((p, q), (q, r), (r, p))):
+                key = (min(u, w), max(u, w))
+                idx = edge_id.get(key)
+                if idx is None:
+                    idx = n_c + len(mids)
+                    edge_id[key] = idx
+                    mids.append(_project(0.5 * (corners[u] + corners[w]), 1.0))
+                quad[e, slot] = idx
+        coords = np.vstack([corners, np.array(mids)]) * radius
+        topo = quad.T.copy()
+        gk = TRI6
+    else:
+        raise ValueError("order must be 'linear' or 'quadratic'")
+    return _with_physics(coords.T.copy(), topo, gk, physics)
+
+
+def ellipsoid(f: int, axes=(1.0, 1.0, 1.0), order: str = "linear",
+              physics: str = "geometry") -> Mesh3d:
+    """Icosphere scaled by semi-axes (a, b, c); orientation is preserved."""
+    m = icosphere(f, order, 1.0, physics="geometry")
+    s = np.asarray(axes, dtype=np.float64).reshape(3, 1)
+    return _with_physics(m.coordinates * s, m.topology, m.geometry_kind, physics)
+
+
+def _with_physics(coords, topo, gk, physics):
+    if physics == "geometry":
+        return Mesh3d(coords, topo, coords.copy(), topo.copy(), gk, gk)
+    if physics == "linear":
+        # read_comsol.jl:87-93: corner connectivity; sources are the used corner nodes.
+        ptop = topo[:3, :]
+        used = np.unique(ptop)
+        remap = -np.ones(coords.shape[1], dtype=np.int64)
+        remap[used] = np.arange(used.size)
+        return Mesh3d(coords, topo, coords[:, used].copy(), remap[ptop], gk, TRI3)
+    raise ValueError("physics must be 'geometry' or 'linear'")
+
+
+# ---------------------------------------------------------------------------------------
+# COMSOL .mphtxt reader
+# ---------------------------------------------------------------------------------------
+def read_comsol(path: str):
+    """Return (coordinates 3xNc, topology n x E 0-based) as in read_comsol.jl:9-54."""
+    if not path.endswith(".mphtxt"):
+        path = path + ".mphtxt"
+    coords, topo = [], []
+    with open(path, "r") as fh:
+        lines = fh.read().splitlines()
+    i = 0
+    while i < len(lines):
+        line = lines[i]
+        if line == "# Mesh vertex coordinates":
+            i += 1
+            while i < len(lines) and lines[i].strip():
+                coords.append([float(x) for x in lines[i].split()])
+                i += 1
+        elif line == "# Elements":
+            i += 1
+            while i < len(lines) and lines[i].strip():
+                topo.append([int(x) for x in lines[i].split()])
+                i += 1
+        i += 1
+    coords = np.array(coords, dtype=np.float64).T.copy()
+    topo = np.array(topo, dtype=np.int64).T.copy()
+    if topo.shape[0] == 6:
+        topo[[4, 5], :] = topo[[5, 4], :]   # read_comsol.jl:49-52
+    return coords, topo
+
+
+def _remove_unused(topo):
+    """Compact renumbering preserving order (src/3d/mesh.jl:6-33)."""
+    used = np.unique(topo)
+    remap = -np.ones(int(topo.max()) + 1, dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    return remap[topo], used
+
+
+def load_comsol(path: str, geometry_order: str = "quadratic",
+                physics_order: str = "geometry") -> Mesh3d:
+    """Restatement of ``load3dTriangularComsolMesh`` (read_comsol.jl:61-119) for
+    geometry in {linear, quadratic} and physics in {geometry, linear}."""
+    coords0, topo0 = read_comsol(path)
+    if geometry_order == "quadratic":
+        if topo0.shape[0] != 6:
+            raise ValueError("quadratic geometry needs a 6-node triangle mesh")
+        gk = TRI6
+    elif geometry_order == "linear":
+        topo0 = topo0[:3, :]
+        gk = TRI3
+    else:
+        raise ValueError("Only quadratic and linear geometries are supported")
+    topo, used = _remove_unused(topo0)
+    coords = coords0[:, used]
+    if physics_order == "geometry":
+        return Mesh3d(coords, topo, coords[:, np.unique(topo)].copy(), topo.copy(), gk, gk)
+    if physics_order == "linear":
+        ptop = topo[:3, :]
+        used_p = np.unique(ptop)
+        # The reference indexes ``sources`` with ``topology[1:3,:]`` directly; that is only
+        # consistent when corner nodes come first, which holds for the bundled files.
+        if used_p.size and int(used_p.max()) != used_p.size - 1:
+            raise ValueError("corner nodes are not numbered first; the reference's "
+                             "linear-physics indexing would be inconsistent")
+        return Mesh3d(coords, topo, coords[:, used_p].copy(), ptop.copy(), gk, TRI3)
+    raise ValueError("physics_order must be 'geometry' or 'linear'")
+
+
+def golden_dir() -> str:
+    here = os.path.dirname(os.path.abspath(__file__))
+    return os.path.normpath(os.path.join(here, "..", "..", "tests", "golden"))

@@ -1,0 +1,147 @@
+"""TEST INFRASTRUCTURE — independent NumPy restatement of the dense collocation assembly.
+
+Only tests/ may import this module.  It is written separately from bie_oracle.cpp
+(vectorised over targets, elements and quadrature points; Gauss-Legendre from
+numpy.polynomial.legendre.leggauss) so that the two restatements check each other on
+small meshes.  Reference functions restated (file:line):
+  curveLinearQuadpoints        src/2d/quadrature.jl:5-11
+  triangularQuadpoints         src/3d/quadrature.jl:70-82
+  rotated_triangular_quadpoints src/3d/quadrature.jl:99-113
+  rotated_midpoint_triangular_quadpoints src/3d/quadrature.jl:117-144
+  create_rotated_element       src/3d/assembly_collocation.jl:122-136
+  triangularLinear/Quadratic   src/3d/surface_functions.jl:191-233
+  interpolate_elements/jacobian! src/3d/assembly_collocation.jl:27-66, src/3d/jacobian.jl:7-87
+  find_closest_corner          src/3d/assembly_collocation.jl:102-115
+  greens3d!/freens3d!/freens3dk0! src/3d/kernels.jl:13-55
+  computing_integrals!         src/3d/assembly_collocation.jl:213-252
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def gl01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def tensor(n, m):
+    nx, wx = gl01(n)
+    ny, wy = gl01(m)
+    u = np.kron(np.ones(m), nx)
+    v = np.kron(ny, np.ones(n))
+    w = np.kron(wy, wx)
+    return u, v, w
+
+
+def rule(c, n, m):
+    if c == 1:
+        u, v, w = tensor(n, m)
+        return u * v, (1.0 - u) * v, w * v
+    if c in (2, 3):
+        u, v, w = tensor(n, m)
+        xi, eta, W = u, v * (1.0 - u), w * np.abs(1.0 - u)
+        return (xi, eta, W) if c == 2 else (eta, xi, W)
+    mm = int(np.ceil(m / 2.0)) * 2
+    u, v, w = tensor(n, mm)
+    X = np.concatenate([0.5 * u, (u - 0.5) * v + 0.5])
+    Y = np.concatenate([v * (1.0 - u), v * (1.0 - u)])
+    W = np.concatenate([w * (0.5 * (1.0 - u)), w * (0.5 * v)])
+    if c == 4:
+        return X, Y, W
+    if c == 5:
+        return (1.0 - X) - Y, X, W
+    return Y, (1.0 - X) - Y, W
+
+
+def basis(kind, u, v):
+    """Returns N, dN/du, dN/dv with shape (Q, kind)."""
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    o, z = np.ones_like(u), np.zeros_like(u)
+    if kind == 3:
+        N = np.stack([(1.0 - u) - v, u, v], axis=1)
+        du = np.stack([-o, o, z], axis=1)
+        dv = np.stack([-o, z, o], axis=1)
+        return N, du, dv
+    l = (1.0 - v) - u
+    N = np.stack([l * ((1.0 - 2.0 * v) - 2.0 * u), u * (2.0 * u - 1.0), v * (2.0 * v - 1.0),
+                  4.0 * u * l, 4.0 * u * v, 4.0 * v * l], axis=1)
+    du = np.stack([-(2.0 * l - 1.0) - 2.0 * l, 4.0 * u - 1.0, z, 4.0 * l - 4.0 * u,
+                   4.0 * v, -4.0 * v], axis=1)
+    dv = np.stack([-(2.0 * l - 1.0) - 2.0 * l, z, 4.0 * v - 1.0, -4.0 * u, 4.0 * u,
+                   4.0 * l - 4.0 * v], axis=1)
+    return N, du, dv
+
+
+def julia_hypot3(a, b, c):
+    """Julia Base n-ary hypot, elementwise (NaN/Inf handling as in select_exact.cpp)."""
+    a, b, c = np.abs(a), np.abs(b), np.abs(c)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        m = np.maximum(np.maximum(a, b), c)
+        safe = np.where(m == 0, 1.0, m)
+        s = (a / safe) ** 2
+        s = s + (b / safe) ** 2
+        s = s + (c / safe) ** 2
+        out = m * np.sqrt(s)
+    out = np.where(m == 0, 0.0, out)
+    out = np.where(np.isinf(a) | np.isinf(b) | np.isinf(c), np.inf, out)
+    return out
+
+
+def element_geometry(mesh, c, n, m):
+    u, v, w = rule(c, n, m)
+    N, du, dv = basis(mesh.geometry_kind, u, v)
+    X = mesh.coordinates[:, mesh.topology]            # (3, ng, E)
+    y = np.einsum("dke,qk->eqd", X, N)
+    tu = np.einsum("dke,qk->eqd", X, du)
+    tv = np.einsum("dke,qk->eqd", X, dv)
+    cr = np.cross(tu, tv)
+    J = julia_hypot3(cr[..., 0], cr[..., 1], cr[..., 2])
+    nrm = -cr / J[..., None]
+    return y, nrm, J * w[None, :]
+
+
+def select(mesh, targets):
+    """Rule index (1..R) per (target, element), first strict minimum."""
+    P = mesh.sources[:, mesh.physics_topology]         # (3, np, E)
+    d = julia_hypot3(P[0][None] - targets[0][:, None, None],
+                     P[1][None] - targets[1][:, None, None],
+                     P[2][None] - targets[2][:, None, None])   # (M, np, E)
+    # first strict minimum: argmin returns the first occurrence of the minimum
+    idx = np.argmin(np.where(np.isnan(d), np.inf, d), axis=1) + 1
+    R = 6 if mesh.geometry_kind == 6 else 3
+    return np.minimum(idx, R).astype(np.int8)
+
+
+def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True):
+    targets = mesh.sources if targets is None else np.asarray(targets, dtype=np.float64)
+    M, N, E = targets.shape[1], mesh.n_sources, mesh.n_elements
+    R = 6 if mesh.geometry_kind == 6 else 3
+    sel = select(mesh, targets)
+    F = np.zeros((M, N), dtype=np.complex128)
+    G = np.zeros((M, N), dtype=np.complex128)
+    C = np.zeros(M, dtype=np.complex128)
+    k = complex(k)
+    for c in range(1, R + 1):
+        ii, ee = np.nonzero(sel == c)
+        if ii.size == 0:
+            continue
+        y, nrm, jw = element_geometry(mesh, c, n, m)
+        u, v, _ = rule(c, n, m)
+        phi, _, _ = basis(mesh.physics_kind, u, v)            # (Q, np)
+        d = y[ee] - targets.T[ii][:, None, :]                 # (P, Q, 3)
+        r = julia_hypot3(d[..., 0], d[..., 1], d[..., 2])
+        nd = np.einsum("pqd,pqd->pq", nrm[ee], d)
+        ekr = np.exp(1j * k * r)
+        g = ekr / (4 * np.pi * r) * jw[ee]
+        f = ekr * (1j * k * r - 1.0) * nd / (4 * np.pi * r ** 3) * jw[ee]
+        k0 = -nd / (4 * np.pi * r ** 3) * jw[ee]
+        cols = mesh.physics_topology[:, ee].T                 # (P, np)
+        if gOn:
+            np.add.at(G, (ii[:, None], cols), g @ phi)
+        if fOn:
+            np.add.at(F, (ii[:, None], cols), f @ phi)
+        if cOn:
+            np.add.at(C, ii, k0.sum(axis=1))
+    return F, G, C
