@@ -1,0 +1,111 @@
+"""ctypes binding of libbie_amd.so (the C ABI in include/bie_amd.h).
+
+Loading fails loudly when the HIP library has not been built: there is no CPU
+fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import mesh as _mesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbie_amd.so")
+
+BIE_OK = 0
+BIE_EINVAL = 1
+BIE_EUNSUPPORTED = 2
+BIE_ENOMEM = 3
+BIE_EHIP = 4
+BIE_F, BIE_G, BIE_C = 1, 2, 4
+
+_dp = ctypes.c_void_p
+
+
+class BieMesh(ctypes.Structure):
+    _fields_ = [("coordinates", _dp), ("topology", _dp), ("sources", _dp),
+                ("physics_topology", _dp), ("n_coordinates", ctypes.c_int64),
+                ("n_elements", ctypes.c_int64), ("n_sources", ctypes.c_int64),
+                ("geometry_kind", ctypes.c_int32), ("physics_kind", ctypes.c_int32),
+                ("index_base", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class BieError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bie_amd error {code}: {msg}")
+        self.code = code
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "bie_last_error": (ctypes.c_char_p, []),
+    "bie_version": (ctypes.c_int, []),
+    "bie_assemble": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_double, ctypes.c_double,
+                                    _dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_uint32, _dp, _dp, _dp]),
+    "bie_dmesh_create": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_int,
+                                        ctypes.POINTER(_dp)]),
+    "bie_dmesh_destroy": (ctypes.c_int, [_dp]),
+    "bie_dmesh_n_sources": (ctypes.c_int64, [_dp]),
+    "bie_dmesh_n_elements": (ctypes.c_int64, [_dp]),
+    "bie_dmesh_sources": (_dp, [_dp]),
+    "bie_assemble_device": (ctypes.c_int, [_dp, ctypes.c_double, ctypes.c_double, _dp,
+                                           ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_uint32, _dp, _dp, _dp, _dp]),
+    "bie_select_rules_device": (ctypes.c_int, [_dp, _dp, ctypes.c_int64, _dp, _dp]),
+    "bie_element_geometry": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, _dp, _dp, _dp]),
+    "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
+    "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_int64)]),
+    "bie_device_malloc": (ctypes.c_int, [ctypes.POINTER(_dp), ctypes.c_int64]),
+    "bie_device_free": (ctypes.c_int, [_dp]),
+    "bie_memcpy_h2d": (ctypes.c_int, [_dp, _dp, ctypes.c_int64]),
+    "bie_memcpy_d2h": (ctypes.c_int, [_dp, _dp, ctypes.c_int64]),
+    "bie_device_synchronize": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != BIE_OK:
+        msg = lib().bie_last_error()
+        raise BieError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class HostMesh:
+    """Keeps Fortran-ordered copies alive while a bie_mesh struct points at them."""
+
+    def __init__(self, m: "_mesh.Mesh3d"):
+        self.coords = np.asfortranarray(m.coordinates, dtype=np.float64)
+        self.topo = np.asfortranarray(m.topology, dtype=np.int64)
+        self.sources = np.asfortranarray(m.sources, dtype=np.float64)
+        self.ptopo = np.asfortranarray(m.physics_topology, dtype=np.int64)
+        self.struct = BieMesh(_ptr(self.coords), _ptr(self.topo), _ptr(self.sources),
+                              _ptr(self.ptopo), m.n_coordinates, m.n_elements, m.n_sources,
+                              m.geometry_kind, m.physics_kind, 0, 0)

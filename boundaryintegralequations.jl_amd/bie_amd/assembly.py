@@ -1,0 +1,116 @@
+"""Host-side mirror of the reference's assembly entry point, backed by the HIP library.
+
+``assemble_parallel(mesh, k, in_sources; fOn, gOn, cOn, sparse, m=4, n=4, progress)``
+mirrors ``assemble_parallel!`` (src/3d/assembly_collocation.jl:259-268): same keyword
+names, defaults and argument meaning, returning ``(F, G, C)`` with F, G complex128
+M x N in column-major order and C complex128 of length M.  Errors raise
+``ValueError``/``NotImplementedError`` where the reference throws ``ArgumentError`` or
+has no method.
+
+``DeviceMesh`` is the device-resident form (bie_dmesh) for callers that keep F, G, C
+in HBM: it takes raw device pointers (e.g. ``torch.Tensor.data_ptr()``) and a HIP
+stream handle.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh3d
+
+
+def _flags(fOn, gOn, cOn):
+    return (_lib.BIE_F if fOn else 0) | (_lib.BIE_G if gOn else 0) | (_lib.BIE_C if cOn else 0)
+
+
+def assemble_parallel(mesh: Mesh3d, k, in_sources=None, *, fOn=True, gOn=True, cOn=True,
+                      sparse=False, m=4, n=4, progress=True, depth=2, offset=None):
+    """Dense F, G, C on linear/quadratic triangles (drop-in for assemble_parallel!)."""
+    if sparse:
+        raise NotImplementedError(
+            "sparse=true (the near-field assembly behind the fast operators) is not part of "
+            "this library; call the reference's sparse_assemble_parallel!")
+    del progress, depth, offset  # accepted for signature parity; unused by dense assembly
+    hm = _lib.HostMesh(mesh)
+    if in_sources is None or in_sources is mesh.sources:
+        targets = None
+        M = mesh.n_sources
+    else:
+        targets = np.asfortranarray(in_sources, dtype=np.float64)
+        if targets.ndim != 2 or targets.shape[0] != 3:
+            raise ValueError("in_sources must be a 3 x M array")
+        M = targets.shape[1]
+    N = mesh.n_sources
+    F = np.empty((M, N), dtype=np.complex128, order="F")
+    G = np.empty((M, N), dtype=np.complex128, order="F")
+    C = np.empty(M, dtype=np.complex128)
+    k = complex(k)
+    rc = _lib.lib().bie_assemble(ctypes.byref(hm.struct), k.real, k.imag,
+                                 _lib._ptr(targets), M, int(n), int(m), _flags(fOn, gOn, cOn),
+                                 _lib._ptr(F), _lib._ptr(G), _lib._ptr(C))
+    if rc == _lib.BIE_EINVAL:
+        raise ValueError(_lib.lib().bie_last_error().decode())
+    if rc == _lib.BIE_EUNSUPPORTED:
+        raise NotImplementedError(_lib.lib().bie_last_error().decode())
+    _lib.check(rc)
+    return F, G, C
+
+
+class DeviceMesh:
+    """A mesh resident on one GPU (bie_dmesh), with its cached rule tables."""
+
+    def __init__(self, mesh: Mesh3d, device: int = 0):
+        self.mesh = mesh
+        hm = _lib.HostMesh(mesh)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().bie_dmesh_create(ctypes.byref(hm.struct), int(device),
+                                               ctypes.byref(h)))
+        self.handle = h
+        self.n_sources = mesh.n_sources
+        self.n_elements = mesh.n_elements
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().bie_dmesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def sources_ptr(self) -> int:
+        return _lib.lib().bie_dmesh_sources(self.handle)
+
+    def assemble(self, k, F_ptr: int, G_ptr: int, C_ptr: int, *, targets_ptr: int = 0,
+                 n_targets: int | None = None, n=4, m=4, fOn=True, gOn=True, cOn=True,
+                 stream: int = 0):
+        """Asynchronous device assembly into caller-owned device buffers."""
+        k = complex(k)
+        M = self.n_sources if n_targets is None else int(n_targets)
+        _lib.check(_lib.lib().bie_assemble_device(
+            self.handle, k.real, k.imag, ctypes.c_void_p(targets_ptr or None), M, int(n),
+            int(m), _flags(fOn, gOn, cOn), ctypes.c_void_p(F_ptr), ctypes.c_void_p(G_ptr),
+            ctypes.c_void_p(C_ptr), ctypes.c_void_p(stream or None)))
+
+    def select_rules(self, rules_ptr: int, *, targets_ptr: int = 0, n_targets=None,
+                     stream: int = 0):
+        M = self.n_sources if n_targets is None else int(n_targets)
+        _lib.check(_lib.lib().bie_select_rules_device(
+            self.handle, ctypes.c_void_p(targets_ptr or None), M, ctypes.c_void_p(rules_ptr),
+            ctypes.c_void_p(stream or None)))
+
+    def element_geometry(self, rule: int, n: int, m: int):
+        from . import rules as _r
+        Q = _r.rule_size(rule, n, m)
+        E = self.n_elements
+        y = np.zeros((E, Q, 3))
+        nr = np.zeros((E, Q, 3))
+        jw = np.zeros((E, Q))
+        _lib.check(_lib.lib().bie_element_geometry(self.handle, rule, n, m, _lib._ptr(y),
+                                                   _lib._ptr(nr), _lib._ptr(jw)))
+        return y, nr, jw

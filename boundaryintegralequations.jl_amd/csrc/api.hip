@@ -1,0 +1,440 @@
+// C-ABI entry points of libbie_amd (declared in include/bie_amd.h).
+//
+// Ownership follows the drop-in contract: callers own F, G, C (host or device); the
+// library owns only device mesh copies and rule-dependent tables inside a bie_dmesh,
+// released by bie_dmesh_destroy.  Errors become status codes plus a thread-local
+// message, never exceptions across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "bie_common.hpp"
+
+namespace bie {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return BIE_OK;
+  } catch (const Error& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return BIE_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BIE_EINVAL;
+  }
+}
+
+template <class T>
+T* dalloc(size_t count) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error{BIE_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e)};
+  }
+  return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void validate_mesh(const bie_mesh* m) {
+  BIE_REQUIRE(m != nullptr, BIE_EINVAL, "mesh is NULL");
+  BIE_REQUIRE(m->geometry_kind == BIE_TRI3 || m->geometry_kind == BIE_TRI6, BIE_EUNSUPPORTED,
+              "geometry_kind must be BIE_TRI3 or BIE_TRI6");
+  BIE_REQUIRE(m->physics_kind == BIE_TRI3 || m->physics_kind == BIE_TRI6, BIE_EUNSUPPORTED,
+              "physics_kind must be BIE_TRI3 or BIE_TRI6");
+  BIE_REQUIRE(!(m->geometry_kind == BIE_TRI3 && m->physics_kind == BIE_TRI6), BIE_EUNSUPPORTED,
+              "quadratic physics on linear geometry is not a reference configuration");
+  BIE_REQUIRE(m->index_base == 0 || m->index_base == 1, BIE_EINVAL, "index_base must be 0 or 1");
+  BIE_REQUIRE(m->n_elements >= 0 && m->n_coordinates >= 0 && m->n_sources >= 0, BIE_EINVAL,
+              "negative mesh size");
+  BIE_REQUIRE(m->n_elements <= (1LL << 28) && m->n_coordinates < (1LL << 31) &&
+                  m->n_sources < (1LL << 31),
+              BIE_EINVAL, "mesh too large for 32-bit element/node indices");
+  if (m->n_elements > 0) {
+    BIE_REQUIRE(m->coordinates && m->topology && m->sources && m->physics_topology, BIE_EINVAL,
+                "mesh array pointer is NULL");
+  }
+}
+
+std::vector<int> convert_topology(const int64_t* t, int rows, int64_t E, int64_t base,
+                                  int64_t limit, const char* what) {
+  std::vector<int> out((size_t)rows * E);
+  for (int64_t e = 0; e < E; ++e)
+    for (int r = 0; r < rows; ++r) {
+      const int64_t v = t[r + e * rows] - base;
+      if (v < 0 || v >= limit)
+        throw Error{BIE_EINVAL, std::string(what) + " index out of range at element " +
+                                    std::to_string(e + base)};
+      out[(size_t)e * rows + r] = (int)v;
+    }
+  return out;
+}
+
+void ensure_rules(bie_dmesh* dm, int n, int m, hipStream_t s) {
+  if (dm->cn == n && dm->cm == m && dm->d_elem) return;
+  RuleSet rs = build_ruleset(dm->gk, dm->pk, n, m);
+  dfree(dm->d_pts);
+  dfree(dm->d_phi);
+  dfree(dm->d_elem);
+  dm->d_pts = dm->d_phi = dm->d_elem = nullptr;
+  dm->cn = dm->cm = -1;
+  dm->d_pts = dalloc<double>(rs.pts.size());
+  dm->d_phi = dalloc<double>(rs.phi.size());
+  dm->d_elem = dalloc<double>((size_t)std::max<int64_t>(dm->E, 1) * rs.Pe * 8);
+  BIE_HIP_CHECK(hipMemcpyAsync(dm->d_pts, rs.pts.data(), rs.pts.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+  BIE_HIP_CHECK(hipMemcpyAsync(dm->d_phi, rs.phi.data(), rs.phi.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+  BIE_HIP_CHECK(hipStreamSynchronize(s));
+  dm->rs = std::move(rs);
+  dm->cn = n;
+  dm->cm = m;
+}
+
+// Event record for bie_timing_*: (call start, kernel start, kernel end) per call.
+struct Timing {
+  std::mutex mu;
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::array<hipEvent_t, 3>> rec;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    BIE_HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+};
+Timing g_timing;
+
+AssemblyArgs make_args(bie_dmesh* dm, const double* d_targets, int64_t M, double kr,
+                       double ki, uint32_t flags, double* dF, double* dG, double* dC) {
+  AssemblyArgs a{};
+  a.targets = d_targets;
+  a.M = M;
+  a.N = dm->N;
+  a.E = (int)dm->E;
+  a.NP = dm->pk;
+  a.elem = dm->d_elem;
+  a.Pe = dm->rs.Pe;
+  a.selnodes = dm->d_selnodes;
+  a.ptopo = dm->d_ptopo;
+  a.phi = dm->d_phi;
+  a.Qmax = dm->rs.Qmax;
+  for (int c = 0; c < 6; ++c) {
+    a.off[c] = dm->rs.off[c];
+    a.len[c] = dm->rs.len[c];
+  }
+  a.R = dm->rs.R;
+  a.kr = kr;
+  a.ki = ki;
+  a.flags = flags;
+  a.F = dF;
+  a.G = dG;
+  a.C = dC;
+  return a;
+}
+
+void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_targets,
+                          int64_t M, int n, int m, uint32_t flags, double* dF, double* dG,
+                          double* dC, hipStream_t s) {
+  BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+  BIE_REQUIRE(M >= 0, BIE_EINVAL, "negative number of targets");
+  BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
+  BIE_REQUIRE(dF && dG && dC, BIE_EINVAL, "output pointer is NULL");
+  BIE_REQUIRE(std::isfinite(kr) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+  if (d_targets == nullptr) {
+    d_targets = dm->d_sources;
+    BIE_REQUIRE(M == dm->N, BIE_EINVAL, "collocation (NULL targets) needs n_targets == n_sources");
+  }
+  BIE_HIP_CHECK(hipSetDevice(dm->device));
+  std::lock_guard<std::mutex> lk(dm->mu);
+  ensure_rules(dm, n, m, s);
+  std::array<hipEvent_t, 3> ev{};
+  bool timed = false;
+  {
+    std::lock_guard<std::mutex> tl(g_timing.mu);
+    if (g_timing.on) {
+      timed = true;
+      for (auto& e : ev) e = g_timing.get();
+    }
+  }
+  if (timed) BIE_HIP_CHECK(hipEventRecord(ev[0], s));
+  launch_fill_zero(dF, 2 * M * dm->N, s);
+  launch_fill_zero(dG, 2 * M * dm->N, s);
+  launch_fill_zero(dC, 2 * M, s);
+  if (M > 0 && dm->E > 0) {
+    launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
+                    dm->d_elem, s);
+    AssemblyArgs a = make_args(dm, d_targets, M, kr, ki, flags, dF, dG, dC);
+    if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
+    launch_assemble(a, s);
+  } else if (timed) {
+    BIE_HIP_CHECK(hipEventRecord(ev[1], s));
+  }
+  if (timed) {
+    BIE_HIP_CHECK(hipEventRecord(ev[2], s));
+    std::lock_guard<std::mutex> tl(g_timing.mu);
+    g_timing.rec.push_back(ev);
+  }
+}
+
+}  // namespace
+}  // namespace bie
+
+using namespace bie;
+
+extern "C" {
+
+const char* bie_last_error(void) { return g_last_error.c_str(); }
+
+int bie_version(void) { return 1; }
+
+int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
+  return guarded([&] {
+    BIE_REQUIRE(out != nullptr, BIE_EINVAL, "out is NULL");
+    *out = nullptr;
+    validate_mesh(mesh);
+    // Host-side validation first: a malformed mesh is rejected before any HIP call.
+    const int64_t base = mesh->index_base;
+    std::vector<int> topo = convert_topology(mesh->topology, mesh->geometry_kind,
+                                             mesh->n_elements, base, mesh->n_coordinates,
+                                             "topology");
+    std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
+                                              mesh->n_elements, base, mesh->n_sources,
+                                              "physics_topology");
+    BIE_REQUIRE(device >= 0, BIE_EINVAL, "device ordinal out of range");
+    int ndev = 0;
+    BIE_HIP_CHECK(hipGetDeviceCount(&ndev));
+    BIE_REQUIRE(device < ndev, BIE_EINVAL, "device ordinal out of range");
+    BIE_HIP_CHECK(hipSetDevice(device));
+    std::unique_ptr<bie_dmesh> dm(new bie_dmesh);
+    dm->device = device;
+    dm->gk = mesh->geometry_kind;
+    dm->pk = mesh->physics_kind;
+    dm->Nc = mesh->n_coordinates;
+    dm->E = mesh->n_elements;
+    dm->N = mesh->n_sources;
+    try {
+      dm->d_coords = dalloc<double>((size_t)3 * dm->Nc);
+      dm->d_sources = dalloc<double>((size_t)3 * dm->N);
+      dm->d_topo = dalloc<int>(topo.size());
+      dm->d_ptopo = dalloc<int>(ptopo.size());
+      dm->d_selnodes = dalloc<double>((size_t)dm->E * dm->pk * 4);
+      if (dm->Nc)
+        BIE_HIP_CHECK(hipMemcpy(dm->d_coords, mesh->coordinates, 24 * dm->Nc, hipMemcpyHostToDevice));
+      if (dm->N)
+        BIE_HIP_CHECK(hipMemcpy(dm->d_sources, mesh->sources, 24 * dm->N, hipMemcpyHostToDevice));
+      if (!topo.empty())
+        BIE_HIP_CHECK(hipMemcpy(dm->d_topo, topo.data(), topo.size() * sizeof(int), hipMemcpyHostToDevice));
+      if (!ptopo.empty())
+        BIE_HIP_CHECK(hipMemcpy(dm->d_ptopo, ptopo.data(), ptopo.size() * sizeof(int), hipMemcpyHostToDevice));
+      launch_gather_selnodes(dm->d_sources, dm->d_ptopo, dm->pk, (int)dm->E, dm->d_selnodes, nullptr);
+      BIE_HIP_CHECK(hipDeviceSynchronize());
+    } catch (...) {
+      bie_dmesh_destroy(dm.release());
+      throw;
+    }
+    *out = dm.release();
+  });
+}
+
+int bie_dmesh_destroy(bie_dmesh* dm) {
+  if (!dm) return BIE_OK;
+  (void)hipSetDevice(dm->device);
+  dfree(dm->d_coords);
+  dfree(dm->d_topo);
+  dfree(dm->d_sources);
+  dfree(dm->d_ptopo);
+  dfree(dm->d_selnodes);
+  dfree(dm->d_pts);
+  dfree(dm->d_phi);
+  dfree(dm->d_elem);
+  delete dm;
+  return BIE_OK;
+}
+
+int64_t bie_dmesh_n_sources(const bie_dmesh* dm) { return dm ? dm->N : -1; }
+int64_t bie_dmesh_n_elements(const bie_dmesh* dm) { return dm ? dm->E : -1; }
+const double* bie_dmesh_sources(const bie_dmesh* dm) { return dm ? dm->d_sources : nullptr; }
+
+int bie_assemble_device(bie_dmesh* dm, double k_re, double k_im, const double* d_targets,
+                        int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* dF,
+                        double* dG, double* dC, void* stream) {
+  return guarded([&] {
+    assemble_device_impl(dm, k_re, k_im, d_targets, n_targets, n, m, flags, dF, dG, dC,
+                         static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_select_rules_device(bie_dmesh* dm, const double* d_targets, int64_t n_targets,
+                            int8_t* d_rules, void* stream) {
+  return guarded([&] {
+    BIE_REQUIRE(dm && d_rules, BIE_EINVAL, "NULL argument");
+    BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
+    if (!d_targets) {
+      d_targets = dm->d_sources;
+      BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
+    }
+    BIE_HIP_CHECK(hipSetDevice(dm->device));
+    launch_select(d_targets, n_targets, dm->d_selnodes, dm->pk, (int)dm->E,
+                  dm->gk == 6 ? 6 : 3, d_rules, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, double* y,
+                         double* normals, double* jw) {
+  return guarded([&] {
+    BIE_REQUIRE(dm && y && normals && jw, BIE_EINVAL, "NULL argument");
+    BIE_HIP_CHECK(hipSetDevice(dm->device));
+    std::lock_guard<std::mutex> lk(dm->mu);
+    ensure_rules(dm, n, m, nullptr);
+    BIE_REQUIRE(rule >= 1 && rule <= dm->rs.R, BIE_EINVAL, "rule index out of range");
+    const RuleSet& rs = dm->rs;
+    launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, rs.Pe, dm->d_elem,
+                    nullptr);
+    std::vector<double> h((size_t)dm->E * rs.Pe * 8);
+    BIE_HIP_CHECK(hipMemcpy(h.data(), dm->d_elem, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int Q = rs.len[rule - 1], off = rs.off[rule - 1];
+    for (int64_t e = 0; e < dm->E; ++e)
+      for (int q = 0; q < Q; ++q) {
+        const double* p = &h[((size_t)e * rs.Pe + off + q) * 8];
+        const size_t o = (size_t)e * Q + q;
+        for (int d = 0; d < 3; ++d) {
+          y[3 * o + d] = p[d];
+          normals[3 * o + d] = p[3 + d];
+        }
+        jw[o] = p[6] * 4.0 * kPi;
+      }
+  });
+}
+
+int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* targets,
+                 int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
+                 double* G, double* C) {
+  bie_dmesh* dm = nullptr;
+  int rc = bie_dmesh_create(mesh, 0, &dm);
+  if (rc != BIE_OK) return rc;
+  double* dT = nullptr;
+  double* dF = nullptr;
+  double* dG = nullptr;
+  double* dC = nullptr;
+  rc = guarded([&] {
+    BIE_REQUIRE(F && G && C, BIE_EINVAL, "output pointer is NULL");
+    BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
+    const int64_t M = targets ? n_targets : dm->N;
+    BIE_REQUIRE(targets != nullptr || n_targets == dm->N || n_targets == 0, BIE_EINVAL,
+                "collocation (NULL targets) needs n_targets == n_sources");
+    const size_t fg = (size_t)2 * M * dm->N;
+    dF = dalloc<double>(fg);
+    dG = dalloc<double>(fg);
+    dC = dalloc<double>((size_t)2 * M);
+    if (targets) {
+      dT = dalloc<double>((size_t)3 * M);
+      if (M) BIE_HIP_CHECK(hipMemcpy(dT, targets, 24 * M, hipMemcpyHostToDevice));
+    }
+    assemble_device_impl(dm, k_re, k_im, targets ? dT : nullptr, M, n, m, flags, dF, dG, dC,
+                         nullptr);
+    BIE_HIP_CHECK(hipDeviceSynchronize());
+    if (fg) {
+      BIE_HIP_CHECK(hipMemcpy(F, dF, fg * sizeof(double), hipMemcpyDeviceToHost));
+      BIE_HIP_CHECK(hipMemcpy(G, dG, fg * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (M) BIE_HIP_CHECK(hipMemcpy(C, dC, 16 * M, hipMemcpyDeviceToHost));
+  });
+  dfree(dT);
+  dfree(dF);
+  dfree(dG);
+  dfree(dC);
+  std::string keep = g_last_error;
+  bie_dmesh_destroy(dm);
+  g_last_error = keep;
+  return rc;
+}
+
+int bie_timing_enable(int on) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> tl(g_timing.mu);
+    g_timing.on = on != 0;
+  });
+}
+
+int bie_timing_collect(double* kernel_ms, double* call_ms, int64_t* n_calls) {
+  return guarded([&] {
+    std::vector<std::array<hipEvent_t, 3>> rec;
+    {
+      std::lock_guard<std::mutex> tl(g_timing.mu);
+      rec.swap(g_timing.rec);
+    }
+    double ks = 0.0, cs = 0.0;
+    for (auto& ev : rec) {
+      BIE_HIP_CHECK(hipEventSynchronize(ev[2]));
+      float a = 0.f, b = 0.f;
+      BIE_HIP_CHECK(hipEventElapsedTime(&a, ev[1], ev[2]));
+      BIE_HIP_CHECK(hipEventElapsedTime(&b, ev[0], ev[2]));
+      ks += a;
+      cs += b;
+    }
+    {
+      std::lock_guard<std::mutex> tl(g_timing.mu);
+      for (auto& ev : rec)
+        for (auto e : ev) g_timing.pool.push_back(e);
+    }
+    if (kernel_ms) *kernel_ms = ks;
+    if (call_ms) *call_ms = cs;
+    if (n_calls) *n_calls = (int64_t)rec.size();
+  });
+}
+
+int bie_device_malloc(void** ptr, int64_t bytes) {
+  return guarded([&] {
+    BIE_REQUIRE(ptr && bytes >= 0, BIE_EINVAL, "bad argument");
+    *ptr = dalloc<char>((size_t)bytes);
+  });
+}
+
+int bie_device_free(void* ptr) {
+  return guarded([&] { dfree(ptr); });
+}
+
+int bie_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    if (bytes > 0) BIE_HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int bie_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    if (bytes > 0) BIE_HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int bie_device_synchronize(void) {
+  return guarded([&] { BIE_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// Shared host-side declarations of the bie_amd library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bie_amd.h"
+
+namespace bie {
+
+void set_error(const std::string& msg);
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define BIE_HIP_CHECK(expr)                                                       \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess)                                                         \
+      throw ::bie::Error{BIE_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+  } while (0)
+
+#define BIE_REQUIRE(cond, code, msg)                         \
+  do {                                                       \
+    if (!(cond)) throw ::bie::Error{(code), std::string(msg)}; \
+  } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Point record of the concatenated rule set: u, v, w, then N, dN/du, dN/dv of the
+// geometry basis (6 slots each, unused slots zero).
+constexpr int kPtStride = 21;
+
+// All quadrature rules one assembly uses, for geometry kind gk and physics kind pk.
+// Rules 1..3 cluster at the corners (Q = n*m each); rules 4..6 at the edge mid points of
+// quadratic elements (Q = 2*n*m', m' = m rounded up to even).
+struct RuleSet {
+  int gk = 3, pk = 3, n = 0, m = 0;
+  int R = 3;          // number of rules
+  int Qc = 0, Qm = 0; // corner / mid-edge rule sizes
+  int Pe = 0;         // points per element over all rules
+  int Qmax = 0;
+  int off[6] = {0, 0, 0, 0, 0, 0};
+  int len[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> pts;  // Pe x kPtStride
+  std::vector<double> phi;  // R x Qmax x pk physics basis at the rule points
+};
+
+RuleSet build_ruleset(int gk, int pk, int n, int m);
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
+
+// Device-side launch parameters for one assembly.
+struct AssemblyArgs {
+  const double* targets;   // 3 x M
+  int64_t M;
+  int64_t N;               // columns of F, G
+  int E;
+  int NP;                  // physics nodes per element
+  const double* elem;      // E x Pe x 8: y(3), n(3), J*w/(4 pi), 0
+  int Pe;
+  const double* selnodes;  // E x NP x 4 selection node coordinates
+  const int* ptopo;        // E x NP physics node indices (0-based, element-major)
+  const double* phi;       // R x Qmax x NP
+  int Qmax;
+  int off[6];
+  int len[6];
+  int R;
+  double kr, ki;
+  unsigned flags;
+  double* F;
+  double* G;
+  double* C;
+};
+
+// Launchers (assemble.hip)
+void launch_fill_zero(double* p, int64_t n_doubles, hipStream_t s);
+void launch_geometry(const double* coords, const int* topo, int gk, int E,
+                     const double* d_pts, int Pe, double* elem, hipStream_t s);
+void launch_gather_selnodes(const double* sources, const int* ptopo, int NP, int E,
+                            double* selnodes, hipStream_t s);
+void launch_assemble(const AssemblyArgs& a, hipStream_t s);
+void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,
+                   int R, int8_t* out, hipStream_t s);
+
+}  // namespace bie
+
+// Device-resident mesh and its cached rule-dependent tables.
+struct bie_dmesh {
+  int device = 0;
+  int gk = 3, pk = 3;
+  int64_t Nc = 0, E = 0, N = 0;
+  double* d_coords = nullptr;
+  int* d_topo = nullptr;        // E x gk, element-major
+  double* d_sources = nullptr;  // 3 x N
+  int* d_ptopo = nullptr;       // E x pk, element-major
+  double* d_selnodes = nullptr; // E x pk x 4
+  // rule-dependent cache
+  int cn = -1, cm = -1;
+  bie::RuleSet rs;
+  double* d_pts = nullptr;
+  double* d_phi = nullptr;
+  double* d_elem = nullptr;
+  std::mutex mu;
+};

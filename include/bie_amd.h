@@ -1,0 +1,113 @@
+/*
+ * bie_amd.h — C ABI of the MI355X (gfx950) dense 3D collocation assembly library.
+ *
+ * Drop-in boundary for BoundaryIntegralEquations.jl's
+ *     assemble_parallel!(mesh::Mesh3d, k, in_sources; fOn, gOn, cOn, m=4, n=4) -> (F, G, C)
+ * (src/3d/assembly_collocation.jl:259-268 dispatcher, :270-353 linear triangles,
+ *  :355-479 quadratic triangles).  All arrays are column-major (Julia order); complex
+ * values are interleaved (re, im) float64 pairs, identical to Julia's ComplexF64 and
+ * HIP's hipDoubleComplex.  Every entry point returns BIE_OK (0) or an error code and
+ * leaves a message for bie_last_error() on the calling thread.
+ */
+#ifndef BIE_AMD_H
+#define BIE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BIE_OK 0
+#define BIE_EINVAL 1        /* bad argument: null pointer, size, index, kind  (ArgumentError) */
+#define BIE_EUNSUPPORTED 2  /* element family outside linear/quadratic triangles            */
+#define BIE_ENOMEM 3        /* device or host allocation failed                              */
+#define BIE_EHIP 4          /* HIP runtime error                                             */
+
+#define BIE_TRI3 3          /* TriangularLinear    (src/3d/SurfaceFunction.jl:31)  */
+#define BIE_TRI6 6          /* TriangularQuadratic (src/3d/SurfaceFunction.jl:58)  */
+
+#define BIE_F 1u            /* fOn */
+#define BIE_G 2u            /* gOn */
+#define BIE_C 4u            /* cOn */
+#define BIE_FGC 7u
+
+/* Mesh3d (src/utils/mesh.jl:23-34), the fields assembly reads. */
+typedef struct bie_mesh {
+  const double* coordinates;        /* 3 x n_coordinates                              */
+  const int64_t* topology;          /* geometry_kind x n_elements                     */
+  const double* sources;            /* 3 x n_sources (collocation nodes)               */
+  const int64_t* physics_topology;  /* physics_kind x n_elements, indexes sources      */
+  int64_t n_coordinates;
+  int64_t n_elements;
+  int64_t n_sources;
+  int32_t geometry_kind;            /* BIE_TRI3 | BIE_TRI6 */
+  int32_t physics_kind;             /* BIE_TRI3 | BIE_TRI6 */
+  int32_t index_base;               /* 1 for Julia arrays, 0 for C/Python              */
+  int32_t reserved;
+} bie_mesh;
+
+typedef struct bie_dmesh bie_dmesh;  /* device-resident mesh + cached rule tables */
+
+const char* bie_last_error(void);
+int bie_version(void);
+
+/* ---- Host-array drop-in ------------------------------------------------------------
+ * Replaces assemble_parallel!(mesh, k, in_sources; fOn, gOn, cOn, m, n)
+ * (assembly_collocation.jl:259-268).  k = k_re + i k_im (complex k as in the lossy caller,
+ * LossyGlobalMatrix.jl:119).  targets: 3 x M (NULL = mesh sources).  F, G: M x N complex,
+ * C: M complex, caller-allocated; disabled outputs are zero-filled (as :308-310).
+ * Rule selection for any target reads mesh sources (defined behaviour; the reference
+ * indexes the targets there, :325/:436, which is only valid for collocation). */
+int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* targets,
+                 int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
+                 double* G, double* C);
+
+/* ---- Device-resident path ------------------------------------------------------------
+ * Uploads the mesh once (the element loop's inputs, Mesh3d src/utils/mesh.jl:23-34). */
+int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out);
+int bie_dmesh_destroy(bie_dmesh* dm);
+int64_t bie_dmesh_n_sources(const bie_dmesh* dm);
+int64_t bie_dmesh_n_elements(const bie_dmesh* dm);
+/* Device copy of the collocation nodes (3 x N), for use as targets. */
+const double* bie_dmesh_sources(const bie_dmesh* dm);
+
+/* Same semantics as bie_assemble, with device pointers and an optional hipStream_t
+ * (NULL = the legacy default stream).  d_targets NULL = collocation at mesh sources.
+ * Asynchronous with respect to the host. */
+int bie_assemble_device(bie_dmesh* dm, double k_re, double k_im, const double* d_targets,
+                        int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* dF,
+                        double* dG, double* dC, void* stream);
+
+/* Rule index 1..R chosen for every (target i, element e), written as int8 to d_rules
+ * [i * n_elements + e] (find_closest_corner, assembly_collocation.jl:102-115). */
+int bie_select_rules_device(bie_dmesh* dm, const double* d_targets, int64_t n_targets,
+                            int8_t* d_rules, void* stream);
+
+/* Element geometry at rule c (1..6) points: y (E x Q x 3), normal (E x Q x 3, the
+ * reference's negated normal), J*w (E x Q): interpolate_elements
+ * (assembly_collocation.jl:27-66, jacobian.jl:78-87).  Host output arrays. */
+int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, double* y,
+                         double* normals, double* jw);
+
+/* ---- Kernel timing ------------------------------------------------------------------
+ * While enabled, every device assembly records three HIP events on its launch stream:
+ * call start, before and after the dense assembly kernel.  bie_timing_collect waits for
+ * the recorded events, returns the summed kernel and whole-call durations (ms) and the
+ * number of calls, and clears the record.  No counterpart in the reference (it only
+ * wraps calls in @time, examples/sphere.jl:51). */
+int bie_timing_enable(int on);
+int bie_timing_collect(double* kernel_ms, double* call_ms, int64_t* n_calls);
+
+/* ---- Device memory helpers (for callers without their own allocator) -------------- */
+int bie_device_malloc(void** ptr, int64_t bytes);
+int bie_device_free(void* ptr);
+int bie_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+int bie_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+int bie_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIE_AMD_H */

@@ -1,0 +1,191 @@
+"""Parity of the HIP assembly (through the C ABI) with the CPU oracle.
+
+Bar: rule indices bit-exact; F, G, C within 1e-12 relative Frobenius error of the
+oracle (the north star's tolerance; the reference itself runs @fastmath, so bitwise
+agreement of floating-point sums is not expected).  Full-size checks use rows sampled
+from the 10242-node quadratic sphere plus size-independent identities.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import analytic
+import oracle
+from bie_amd import _lib
+from bie_amd import mesh as M
+from bie_amd.assembly import DeviceMesh, assemble_parallel
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+class DevBuf:
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().bie_device_malloc(ctypes.byref(p), int(nbytes)))
+        self.ptr = p.value or 0
+        self.nbytes = nbytes
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        _lib.check(_lib.lib().bie_memcpy_h2d(ctypes.c_void_p(self.ptr), _lib._ptr(arr), arr.nbytes))
+
+    def download(self, arr):
+        _lib.check(_lib.lib().bie_memcpy_d2h(_lib._ptr(arr), ctypes.c_void_p(self.ptr), arr.nbytes))
+        return arr
+
+    def __del__(self):
+        if self.ptr:
+            _lib.lib().bie_device_free(ctypes.c_void_p(self.ptr))
+
+
+def gpu_rules(mesh, targets=None):
+    dm = DeviceMesh(mesh)
+    if targets is None:
+        M_ = mesh.n_sources
+        tbuf = None
+    else:
+        M_ = targets.shape[1]
+        tbuf = DevBuf(24 * M_)
+        tbuf.upload(np.asfortranarray(targets).ravel(order="F"))
+    out = DevBuf(M_ * mesh.n_elements)
+    dm.select_rules(out.ptr, targets_ptr=tbuf.ptr if tbuf else 0, n_targets=M_)
+    _lib.check(_lib.lib().bie_device_synchronize())
+    return out.download(np.zeros((M_, mesh.n_elements), dtype=np.int8))
+
+
+MESHES = {
+    "lin8": lambda: M.icosphere(8, "linear"),
+    "quad4": lambda: M.icosphere(4, "quadratic"),
+    "quad3_linphys": lambda: M.icosphere(3, "quadratic", physics="linear"),
+    "sphere_1m": lambda: M.load_comsol(os.path.join(GOLD, "sphere_1m")),
+    "sphere_1m_lin": lambda: M.load_comsol(os.path.join(GOLD, "sphere_1m"), "linear", "linear"),
+    "ellipsoid": lambda: M.ellipsoid(5, (0.7, 1.3, 2.0), "linear"),
+}
+
+
+@pytest.mark.parametrize("name", ["lin8", "quad4", "sphere_1m", "ellipsoid"])
+def test_rule_selection_bit_exact(name):
+    mesh = MESHES[name]()
+    print(f"[case] selection {name} N={mesh.n_sources}", flush=True)
+    assert np.array_equal(gpu_rules(mesh), oracle.select(mesh))
+
+
+def test_rule_selection_ties_and_field_points():
+    mesh = M.icosphere(6, "quadratic")
+    rng = np.random.default_rng(7)
+    c = mesh.coordinates
+    # exact midpoints between node pairs, element centroids, and scaled copies: tie-heavy
+    i, j = rng.integers(0, c.shape[1], (2, 400))
+    T = [0.5 * (c[:, i] + c[:, j]), c[:, mesh.topology[:3, :50]].mean(axis=1),
+         1.7 * c[:, :200], 0.3 * c[:, :200], rng.standard_normal((3, 300))]
+    T = np.hstack(T)
+    assert np.array_equal(gpu_rules(mesh, T), oracle.select(mesh, T))
+
+
+@pytest.mark.parametrize("rule", [1, 2, 3, 4, 5, 6])
+def test_element_geometry_matches_oracle(rule):
+    mesh = M.icosphere(3, "quadratic")
+    dm = DeviceMesh(mesh)
+    y, nr, jw = dm.element_geometry(rule, 4, 3)
+    yo, no, jo = oracle.element_geometry(mesh, rule, 4, 3)
+    assert np.max(np.abs(y - yo)) < 1e-14
+    assert np.max(np.abs(nr - no)) < 1e-13
+    assert np.max(np.abs(jw - jo)) < 1e-15
+
+
+CASES = [
+    ("lin8", 1.0, 2, 2),          # config 1: 642-node linear sphere, k=1, 4-point rule
+    ("lin8", 7.5, 4, 4),
+    ("quad4", 10.0, 3, 2),        # quadratic sphere at the full-size k and 6-point rule
+    ("quad4", 10.0, 4, 4),
+    ("quad4", 2.0 + 0.4j, 3, 3),  # complex k (lossy caller)
+    ("quad3_linphys", 3.0, 4, 4), # quadratic geometry, linear physics
+    ("sphere_1m", 2 * np.pi * 100 / 340, 4, 4),
+    ("sphere_1m_lin", 4.0, 3, 2),
+    ("ellipsoid", 5.0, 4, 4),
+]
+
+
+@pytest.mark.parametrize("name,k,n,m", CASES)
+def test_assembly_parity(name, k, n, m):
+    mesh = MESHES[name]()
+    print(f"[case] assembly {name} N={mesh.n_sources} k={k} (n,m)=({n},{m})", flush=True)
+    F, G, C = assemble_parallel(mesh, k, mesh.sources, n=n, m=m)
+    Fo, Go, Co = oracle.assemble(mesh, k, n=n, m=m)
+    assert rel(F, Fo) < TOL and rel(G, Go) < TOL and rel(C, Co) < TOL
+    assert np.isfortran(F) and F.shape == (mesh.n_sources, mesh.n_sources)
+
+
+def test_field_points_and_flags():
+    mesh = MESHES["quad4"]()
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((3, 77))
+    X *= (0.2 + 2.5 * rng.random(77)) / np.linalg.norm(X, axis=0)
+    print("[case] field points quad4 M=77", flush=True)
+    F, G, C = assemble_parallel(mesh, 4.0, X, n=3, m=2)
+    Fo, Go, Co = oracle.assemble(mesh, 4.0, targets=X, n=3, m=2)
+    assert rel(F, Fo) < TOL and rel(G, Go) < TOL and rel(C, Co) < TOL
+    F, G, C = assemble_parallel(mesh, 4.0, X, n=3, m=2, fOn=False, cOn=False)
+    assert not F.any() and not C.any() and rel(G, Go) < TOL
+    F, G, C = assemble_parallel(mesh, 4.0, X, n=3, m=2, gOn=False)
+    assert not G.any() and rel(F, Fo) < TOL and rel(C, Co) < TOL
+
+
+def test_empty_targets():
+    mesh = M.icosphere(2, "linear")
+    F, G, C = assemble_parallel(mesh, 1.0, np.zeros((3, 0)), n=2, m=2)
+    assert F.shape == (0, mesh.n_sources) and C.shape == (0,)
+
+
+def test_reference_tolerance_table_on_gpu_output():
+    # test/3d/assembly_test.jl:12-44 with the GPU's F, C
+    k = 2 * np.pi * 100.0 / 340.0
+    for go, po, tol in [("quadratic", "geometry", 2.8e-5), ("linear", "linear", 0.037),
+                        ("quadratic", "linear", 0.048)]:
+        mesh = M.load_comsol(os.path.join(GOLD, "sphere_1m"), go, po)
+        F, _, C = assemble_parallel(mesh, k, mesh.sources, gOn=False)
+        pI = analytic.incoming_wave([np.pi / 2, 0.0], 1.0, mesh.sources, k)
+        p = np.linalg.solve(F + np.diag(1.0 - C), pI)
+        th = np.arccos(np.clip(mesh.sources[0], -1, 1))
+        pa = analytic.plane_wave_scattering_sphere(k, 1.0, 1.0, th, 1e-6)
+        assert rel(p, pa) < tol, (go, po)
+
+
+def test_config1_pulsating_sphere_lu():
+    mesh = M.icosphere(8, "linear")
+    Z0 = 1.21 * 343.0
+    F, G, C = assemble_parallel(mesh, 1.0, mesh.sources, n=2, m=2)
+    p = np.linalg.solve(F + np.diag(1 - C), -1j * Z0 * (G @ np.ones(mesh.n_sources)))
+    pa = analytic.pulsating_sphere_exterior(1.0, 1.0, 1.0, Z0, 1.0)
+    assert np.max(np.abs(p - pa)) / abs(pa) < 5e-3
+
+
+@pytest.mark.parametrize("nm", [(3, 2), (4, 4)])
+def test_config2_full_size_rows_and_identities(nm):
+    n, m = nm
+    mesh = M.icosphere(16, "quadratic")   # 10242 nodes, 5120 quadratic triangles
+    k = 10.0
+    print(f"[case] full-size quadratic sphere N={mesh.n_sources} k={k} (n,m)={nm}", flush=True)
+    F, G, C = assemble_parallel(mesh, k, mesh.sources, n=n, m=m)
+    assert np.isfinite(F).all() and np.isfinite(G).all() and np.isfinite(C).all()
+    rows = np.sort(np.random.default_rng(11).choice(mesh.n_sources, 40, replace=False))
+    Fo, Go, Co = oracle.assemble(mesh, k, targets=mesh.sources[:, rows], n=n, m=m)
+    assert rel(F[rows], Fo) < TOL and rel(G[rows], Go) < TOL and rel(C[rows], Co) < TOL
+    # size-independent identities over all rows
+    assert np.max(np.abs(C - 0.5)) < 1e-3
+    rg, rf = analytic.sphere_row_sum_G(k, 1.0), analytic.sphere_row_sum_F(k, 1.0)
+    assert np.max(np.abs(G.sum(axis=1) - rg)) / abs(rg) < 5e-3
+    assert np.max(np.abs(F.sum(axis=1) - rf)) / abs(rf) < 2e-2
+    del F, G
+    F0, _, C0 = assemble_parallel(mesh, 0.0, mesh.sources, n=n, m=m, gOn=False)
+    assert np.max(np.abs(F0.sum(axis=1) - C0)) < 1e-12

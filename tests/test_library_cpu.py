@@ -1,0 +1,98 @@
+"""C-ABI library checks that need no GPU: it loads, exports every declared symbol, and
+rejects malformed input on the host before any HIP call (the reference's
+ArgumentError / missing-method paths)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from bie_amd import _lib
+from bie_amd import mesh as M
+from bie_amd.assembly import assemble_parallel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(bie_[a-z0-9_]+)\s*\(",
+                                text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    # and the Python binding declares a signature for each of them
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_version():
+    assert _lib.lib().bie_version() >= 1
+
+
+def _struct(mesh, **over):
+    hm = _lib.HostMesh(mesh)
+    for k, v in over.items():
+        setattr(hm.struct, k, v)
+    return hm
+
+
+def _call(hm, M_=None):
+    N = hm.struct.n_sources
+    M_ = N if M_ is None else M_
+    F = np.zeros((M_, N), dtype=np.complex128)
+    G = np.zeros_like(F)
+    C = np.zeros(M_, dtype=np.complex128)
+    return _lib.lib().bie_assemble(ctypes.byref(hm.struct), 1.0, 0.0, None, M_, 2, 2, 7,
+                                   _lib._ptr(F), _lib._ptr(G), _lib._ptr(C))
+
+
+def test_rejects_unsupported_element_kinds():
+    mesh = M.icosphere(1, "linear")
+    assert _call(_struct(mesh, geometry_kind=4)) == _lib.BIE_EUNSUPPORTED
+    assert _call(_struct(mesh, physics_kind=6)) == _lib.BIE_EUNSUPPORTED  # quad physics, linear geometry
+    assert b"TRI" in _lib.lib().bie_last_error() or b"reference" in _lib.lib().bie_last_error()
+
+
+def test_rejects_out_of_range_topology_and_null():
+    mesh = M.icosphere(1, "linear")
+    bad = M.icosphere(1, "linear")
+    bad.topology[0, 3] = bad.n_coordinates + 5
+    assert _call(_lib.HostMesh(bad)) == _lib.BIE_EINVAL
+    assert b"out of range" in _lib.lib().bie_last_error()
+    assert _call(_struct(mesh, index_base=2)) == _lib.BIE_EINVAL
+    assert _call(_struct(mesh, coordinates=None)) == _lib.BIE_EINVAL
+    rc = _lib.lib().bie_assemble(None, 1.0, 0.0, None, 0, 2, 2, 7, None, None, None)
+    assert rc == _lib.BIE_EINVAL
+
+
+def test_one_based_indices_are_validated_too():
+    mesh = M.icosphere(1, "linear")
+    hm = _lib.HostMesh(mesh)
+    hm.struct.index_base = 1   # 0-based data read as 1-based -> index 0 is out of range
+    assert _call(hm) == _lib.BIE_EINVAL
+
+
+def test_python_mirror_argument_errors():
+    mesh = M.icosphere(1, "linear")
+    with pytest.raises(NotImplementedError):
+        assemble_parallel(mesh, 1.0, sparse=True)
+    with pytest.raises(ValueError):
+        assemble_parallel(mesh, 1.0, np.zeros((2, 5)))
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ImportError):
+        _lib.lib()
