@@ -58,6 +58,8 @@ SIGNATURES = {
     "bie_select_rules_device": (ctypes.c_int, [_dp, _dp, ctypes.c_int64, _dp, _dp]),
     "bie_element_geometry": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, _dp, _dp, _dp]),
+    "bie_schedule_check": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int64)]),
     "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),

@@ -9,11 +9,13 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
 
 #include "bie_common.hpp"
+#include "schedule.hpp"
 
 namespace bie {
 
@@ -131,6 +133,66 @@ struct Timing {
 };
 Timing g_timing;
 
+// Which assembly kernel to use: the LDS-slot kernel (default) or the all-atomics kernel
+// (BIE_AMD_KERNEL=atomic), kept for small problems and as a cross-check.
+bool use_atomic_kernel() {
+  const char* v = std::getenv("BIE_AMD_KERNEL");
+  return v && std::strcmp(v, "atomic") == 0;
+}
+
+template <class T>
+T* upload_vec(const std::vector<T>& v, hipStream_t s) {
+  T* d = dalloc<T>(v.size());
+  if (!v.empty())
+    BIE_HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return d;
+}
+
+void free_schedule(bie_dmesh* dm) {
+  for (int** p : {&dm->d_seg_step_begin, &dm->d_step_elems, &dm->d_slot_of, &dm->d_flush_begin,
+                  &dm->d_flush_node, &dm->d_flush_slot, &dm->d_zero_cols}) {
+    dfree(*p);
+    *p = nullptr;
+  }
+  dm->sched_key = -1;
+}
+
+void ensure_schedule(bie_dmesh* dm, int min_segments, hipStream_t s) {
+  if (dm->sched_key == min_segments && dm->d_step_elems) return;
+  free_schedule(dm);
+  std::vector<double> cent((size_t)dm->E * 3);
+  for (int64_t e = 0; e < dm->E; ++e)
+    for (int d = 0; d < 3; ++d) {
+      double acc = 0;
+      for (int k = 0; k < 3; ++k) acc += dm->h_coords[3 * (size_t)dm->h_topo[e * dm->gk + k] + d];
+      cent[3 * e + d] = acc / 3.0;
+    }
+  SlotSchedule sc = build_slot_schedule(dm->h_ptopo.data(), dm->pk, (int)dm->E, (int)dm->N,
+                                        cent.data(), kSlotWaves, kSlotCount, min_segments);
+  dm->d_seg_step_begin = upload_vec(sc.seg_step_begin, s);
+  dm->d_step_elems = upload_vec(sc.step_elems, s);
+  dm->d_slot_of = upload_vec(sc.slot_of, s);
+  dm->d_flush_begin = upload_vec(sc.flush_begin, s);
+  dm->d_flush_node = upload_vec(sc.flush_node, s);
+  dm->d_flush_slot = upload_vec(sc.flush_slot, s);
+  dm->d_zero_cols = upload_vec(sc.zero_cols, s);
+  BIE_HIP_CHECK(hipStreamSynchronize(s));
+  dm->sched_segments = sc.n_segments;
+  dm->sched_zero_cols = (int)sc.zero_cols.size();
+  dm->sched_max_live = sc.max_live;
+  dm->sched_key = min_segments;
+}
+
+void ensure_mid_queue(bie_dmesh* dm, int64_t cap) {
+  if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
+  if (dm->mid_cap >= cap && dm->d_mid_queue) return;
+  dfree(dm->d_mid_queue);
+  dm->d_mid_queue = nullptr;
+  dm->mid_cap = 0;
+  dm->d_mid_queue = dalloc<int2>((size_t)cap);
+  dm->mid_cap = cap;
+}
+
 AssemblyArgs make_args(bie_dmesh* dm, const double* d_targets, int64_t M, double kr,
                        double ki, uint32_t flags, double* dF, double* dG, double* dC) {
   AssemblyArgs a{};
@@ -184,15 +246,50 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     }
   }
   if (timed) BIE_HIP_CHECK(hipEventRecord(ev[0], s));
-  launch_fill_zero(dF, 2 * M * dm->N, s);
-  launch_fill_zero(dG, 2 * M * dm->N, s);
+  const bool atomic_path = use_atomic_kernel() || M == 0 || dm->E == 0;
+  if (atomic_path || !(flags & BIE_F)) launch_fill_zero(dF, 2 * M * dm->N, s);
+  if (atomic_path || !(flags & BIE_G)) launch_fill_zero(dG, 2 * M * dm->N, s);
   launch_fill_zero(dC, 2 * M, s);
   if (M > 0 && dm->E > 0) {
     launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
                     dm->d_elem, s);
     AssemblyArgs a = make_args(dm, d_targets, M, kr, ki, flags, dF, dG, dC);
-    if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_assemble(a, s);
+    if (use_atomic_kernel()) {
+      if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
+      launch_assemble(a, s);
+    } else {
+      const int64_t rowblocks = (M + 63) / 64;
+      const int min_segments = (int)std::max<int64_t>(1, std::min<int64_t>(dm->E, (2048 + rowblocks - 1) / rowblocks));
+      ensure_schedule(dm, min_segments, s);
+      SlotArgs sa{};
+      sa.W = kSlotWaves;
+      sa.S = kSlotCount;
+      sa.n_segments = dm->sched_segments;
+      sa.n_tiles = rowblocks * dm->sched_segments;
+      sa.phi_in_lds = (sizeof(double) * (size_t)a.R * a.Qmax * a.NP <= 24 * 1024) ? 1 : 0;
+      sa.seg_step_begin = dm->d_seg_step_begin;
+      sa.step_elems = dm->d_step_elems;
+      sa.slot_of = dm->d_slot_of;
+      sa.flush_begin = dm->d_flush_begin;
+      sa.flush_node = dm->d_flush_node;
+      sa.flush_slot = dm->d_flush_slot;
+      if (dm->pk == 6) {
+        ensure_mid_queue(dm, M * dm->E);
+        launch_fill_zero_u32(dm->d_mid_count, s);
+      } else if (!dm->d_mid_count) {
+        dm->d_mid_count = dalloc<unsigned>(1);
+      }
+      sa.mid_count = dm->d_mid_count;
+      sa.mid_queue = dm->d_mid_queue;
+      sa.mid_cap = dm->mid_cap;
+      // The slot kernel stores owned columns once; only shared/untouched columns need
+      // zeros (a disabled output is zeroed whole above and never written).
+      if (flags & BIE_F) launch_zero_columns(dF, M, dm->d_zero_cols, dm->sched_zero_cols, s);
+      if (flags & BIE_G) launch_zero_columns(dG, M, dm->d_zero_cols, dm->sched_zero_cols, s);
+      if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
+      launch_assemble_slots(a, sa, s);
+      launch_assemble_mid(a, sa, s);
+    }
   } else if (timed) {
     BIE_HIP_CHECK(hipEventRecord(ev[1], s));
   }
@@ -233,6 +330,9 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     BIE_REQUIRE(device < ndev, BIE_EINVAL, "device ordinal out of range");
     BIE_HIP_CHECK(hipSetDevice(device));
     std::unique_ptr<bie_dmesh> dm(new bie_dmesh);
+    dm->h_topo = topo;
+    dm->h_ptopo = ptopo;
+    dm->h_coords.assign(mesh->coordinates, mesh->coordinates + 3 * mesh->n_coordinates);
     dm->device = device;
     dm->gk = mesh->geometry_kind;
     dm->pk = mesh->physics_kind;
@@ -274,6 +374,9 @@ int bie_dmesh_destroy(bie_dmesh* dm) {
   dfree(dm->d_pts);
   dfree(dm->d_phi);
   dfree(dm->d_elem);
+  free_schedule(dm);
+  dfree(dm->d_mid_count);
+  dfree(dm->d_mid_queue);
   delete dm;
   return BIE_OK;
 }
@@ -374,6 +477,41 @@ int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* t
   bie_dmesh_destroy(dm);
   g_last_error = keep;
   return rc;
+}
+
+int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stats) {
+  return guarded([&] {
+    validate_mesh(mesh);
+    BIE_REQUIRE(min_segments >= 1, BIE_EINVAL, "min_segments must be >= 1");
+    const int64_t base = mesh->index_base;
+    std::vector<int> topo = convert_topology(mesh->topology, mesh->geometry_kind,
+                                             mesh->n_elements, base, mesh->n_coordinates,
+                                             "topology");
+    std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
+                                              mesh->n_elements, base, mesh->n_sources,
+                                              "physics_topology");
+    const int64_t E = mesh->n_elements;
+    std::vector<double> cent((size_t)E * 3);
+    for (int64_t e = 0; e < E; ++e)
+      for (int d = 0; d < 3; ++d) {
+        double acc = 0;
+        for (int k = 0; k < 3; ++k)
+          acc += mesh->coordinates[3 * (size_t)topo[e * mesh->geometry_kind + k] + d];
+        cent[3 * e + d] = acc / 3.0;
+      }
+    SlotSchedule sc = build_slot_schedule(ptopo.data(), mesh->physics_kind, (int)E,
+                                          (int)mesh->n_sources, cent.data(), kSlotWaves,
+                                          kSlotCount, min_segments);
+    validate_slot_schedule(sc, ptopo.data(), mesh->physics_kind, (int)E, (int)mesh->n_sources);
+    if (stats) {
+      stats[0] = sc.n_segments;
+      stats[1] = sc.n_steps;
+      stats[2] = sc.max_live;
+      stats[3] = sc.n_shared;
+      stats[4] = (int64_t)sc.zero_cols.size();
+      stats[5] = (int64_t)sc.flush_node.size();
+    }
+  });
 }
 
 int bie_timing_enable(int on) {

@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "bie_common.hpp"
+#include "schedule.hpp"
 
 namespace bie {
 namespace {
@@ -161,9 +162,59 @@ k_select(const double* __restrict__ targets, int64_t M, const double* __restrict
   }
 }
 
+// Quadrature of one (target, element) pair with rule points tab[0..Q) and physics basis
+// ph[q * NP + s]: accumulates the F and G contributions of the NP element nodes and the
+// C integrand.  Single-layer g = e^{ikr}/(4 pi r), double-layer
+// f = e^{ikr}(ikr - 1)(n.d)/(4 pi r^3) and k0 = -(n.d)/(4 pi r^3) (kernels.jl:13-55) share
+// one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].
+template <int NP, bool CPLX>
+__device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
+                                                const double* __restrict__ tab,
+                                                const double* __restrict__ ph, int Q,
+                                                double kr, double ki, double* fr, double* fi,
+                                                double* gr, double* gi, double& csum) {
+  for (int q = 0; q < Q; ++q) {
+    const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
+    const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+    const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double rinv = rsqrt_nr(r2);
+    const double r = r2 * rinv;
+    const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
+    const double g = B.z * rinv;             // J w / (4 pi r)
+    const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
+    csum -= ak;
+    double sn, cs;
+    sincos(kr * r, &sn, &cs);
+    double Fr, Fi;
+    if (CPLX) {
+      const double ex = exp(-ki * r);
+      cs *= ex;
+      sn *= ex;
+      const double ar = fma(-ki, r, -1.0), ai = kr * r;
+      Fr = ak * fma(cs, ar, -sn * ai);
+      Fi = ak * fma(cs, ai, sn * ar);
+    } else {
+      const double krr = kr * r;
+      Fr = -ak * fma(krr, sn, cs);
+      Fi = ak * fma(krr, cs, -sn);
+    }
+    const double Gr = g * cs, Gi = g * sn;
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      const double p = ph[q * NP + s];
+      fr[s] = fma(Fr, p, fr[s]);
+      fi[s] = fma(Fi, p, fi[s]);
+      gr[s] = fma(Gr, p, gr[s]);
+      gi[s] = fma(Gi, p, gi[s]);
+    }
+  }
+}
+
 // One thread per target row; blockIdx.y selects a contiguous chunk of elements.
 // Every (target, element) contribution is added into zeroed F, G with hardware FP64
-// atomics; C is accumulated per thread and added once per chunk.
+// atomics; C is accumulated per thread and added once per chunk.  Used for small
+// problems, where the slot schedule does not pay.
 template <int NP, bool CPLX>
 __global__ void __launch_bounds__(256)
 k_assemble(const AssemblyArgs a, int ech, int phi_in_lds) {
@@ -179,53 +230,15 @@ k_assemble(const AssemblyArgs a, int ech, int phi_in_lds) {
   const int e0 = blockIdx.y * ech;
   const int e1 = min(a.E, e0 + ech);
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
-  const double kr = a.kr, ki = a.ki;
   double csum = 0.0;
   for (int e = e0; e < e1; ++e) {
     const int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
-    const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
-    const double* ph = phi + c * a.Qmax * NP;
-    const int Q = a.len[c];
     double fr[NP], fi[NP], gr[NP], gi[NP];
 #pragma unroll
     for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
-    for (int q = 0; q < Q; ++q) {
-      const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
-      const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-      const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
-      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-      const double rinv = rsqrt_nr(r2);
-      const double r = r2 * rinv;
-      const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
-      const double g = B.z * rinv;             // J w / (4 pi r)
-      const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
-      csum -= ak;
-      double sn, cs;
-      sincos(kr * r, &sn, &cs);
-      double Gr, Gi, Fr, Fi;
-      if (CPLX) {
-        const double ex = exp(-ki * r);
-        cs *= ex;
-        sn *= ex;
-        const double ar = fma(-ki, r, -1.0), ai = kr * r;
-        Fr = ak * fma(cs, ar, -sn * ai);
-        Fi = ak * fma(cs, ai, sn * ar);
-      } else {
-        const double krr = kr * r;
-        Fr = -ak * fma(krr, sn, cs);
-        Fi = ak * fma(krr, cs, -sn);
-      }
-      Gr = g * cs;
-      Gi = g * sn;
-#pragma unroll
-      for (int s = 0; s < NP; ++s) {
-        const double p = ph[q * NP + s];
-        fr[s] = fma(Fr, p, fr[s]);
-        fi[s] = fma(Fi, p, fi[s]);
-        gr[s] = fma(Gr, p, gr[s]);
-        gi[s] = fma(Gi, p, gi[s]);
-      }
-    }
+    pair_quadrature<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
+                              phi + c * a.Qmax * NP, a.len[c], a.kr, a.ki, fr, fi, gr, gi,
+                              csum);
     const int* cols = a.ptopo + (size_t)e * NP;
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
@@ -243,7 +256,176 @@ k_assemble(const AssemblyArgs a, int ech, int phi_in_lds) {
   if (a.flags & BIE_C) unsafeAtomicAdd(a.C + 2 * i, csum);
 }
 
+// LDS-slot kernel.  Workgroup = W waves sharing 64 target rows (lane = row) and one
+// segment of elements; each step the waves take one element each, add the element's
+// node contributions into LDS slots (ds_add_f64: waves may hit the same slot), and after
+// a barrier the nodes whose last element was in this step are flushed: owned columns
+// with one coalesced store per row block, shared columns with FP64 atomics.  Pairs whose
+// rule is a mid-edge split rule (quadratic physics only, ~1% of pairs) are appended to a
+// queue and integrated by k_assemble_mid.
+template <int NP, bool CPLX, int W>
+__global__ void __launch_bounds__(64 * W)
+k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
+  extern __shared__ double lds[];
+  double* acc = lds;                              // [slot][4][64]
+  double* s_phi = lds + (size_t)sa.S * 256;
+  const int nphi = a.R * a.Qmax * NP;
+  // XCD-aware tile order: consecutive block ids go to different XCDs, so give each XCD
+  // a contiguous run of (row block, segment) tiles, row-block major.
+  const int b = blockIdx.x;
+  const int per_xcd = (int)((sa.n_tiles + 7) / 8);
+  const int64_t tile = (int64_t)(b & 7) * per_xcd + (b >> 3);
+  if ((b >> 3) >= per_xcd || tile >= sa.n_tiles) return;
+  const int rb = (int)(tile / sa.n_segments);
+  const int seg = (int)(tile - (int64_t)rb * sa.n_segments);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t i = (int64_t)rb * 64 + lane;
+  const bool active = i < a.M;
+  if (sa.phi_in_lds)
+    for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
+  for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
+  __syncthreads();
+  const double* phi = sa.phi_in_lds ? s_phi : a.phi;
+  const int64_t ic = active ? i : (a.M - 1);
+  const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  double csum = 0.0;
+  const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
+  for (int st = st0; st < st1; ++st) {
+    const int e = sa.step_elems[(size_t)st * W + wave];
+    if (e >= 0) {
+      int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
+      bool mid = NP == 6 && c >= 3;
+      if (NP == 6) {
+        const bool push = mid && active;
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+          unsigned base = 0;
+          const int leader = __ffsll((long long)mask) - 1;
+          if (lane == leader) base = atomicAdd(sa.mid_count, (unsigned)__popcll(mask));
+          base = __shfl(base, leader);
+          if (push) {
+            const unsigned pos =
+                base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            if ((int64_t)pos < sa.mid_cap)
+              sa.mid_queue[pos] = make_int2((int)i, e | ((c - 3) << 28));
+          }
+        }
+      }
+      const bool doit = active && !mid;
+      double fr[NP], fi[NP], gr[NP], gi[NP];
+#pragma unroll
+      for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
+      if (doit) {
+        pair_quadrature<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
+                                  phi + c * a.Qmax * NP, a.len[c], a.kr, a.ki, fr, fi, gr,
+                                  gi, csum);
+        const int* so = sa.slot_of + (size_t)e * NP;
+#pragma unroll
+        for (int s = 0; s < NP; ++s) {
+          double* p = acc + (size_t)so[s] * 256 + lane;
+          atomicAdd(p, fr[s]);
+          atomicAdd(p + 64, fi[s]);
+          atomicAdd(p + 128, gr[s]);
+          atomicAdd(p + 192, gi[s]);
+        }
+      }
+    }
+    __syncthreads();
+    const int f1 = sa.flush_begin[st + 1];
+    for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
+      const int node = sa.flush_node[f];
+      const int ss = sa.flush_slot[f];
+      double* p = acc + (size_t)(ss & 0xffff) * 256 + lane;
+      const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
+      p[0] = 0.0;
+      p[64] = 0.0;
+      p[128] = 0.0;
+      p[192] = 0.0;
+      if (active) {
+        const size_t idx = 2 * ((size_t)i + (size_t)node * (size_t)a.M);
+        if (ss & kSharedBit) {
+          if (doF) {
+            unsafeAtomicAdd(a.F + idx, vr);
+            unsafeAtomicAdd(a.F + idx + 1, vi);
+          }
+          if (doG) {
+            unsafeAtomicAdd(a.G + idx, wr);
+            unsafeAtomicAdd(a.G + idx + 1, wi);
+          }
+        } else {
+          if (doF) *reinterpret_cast<double2*>(a.F + idx) = make_double2(vr, vi);
+          if (doG) *reinterpret_cast<double2*>(a.G + idx) = make_double2(wr, wi);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
+}
+
+// Mid-edge split-rule pairs from the queue (rotated_midpoint_triangular_quadpoints,
+// quadrature.jl:117-144): one thread per pair, results added atomically after the slot
+// kernel has written its columns.
+__global__ void __launch_bounds__(256)
+k_assemble_mid(const AssemblyArgs a, const unsigned* __restrict__ count,
+               const int2* __restrict__ queue, int64_t cap, int cplx) {
+  const int64_t n = min((int64_t)*count, cap);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int2 en = queue[t];
+    const int64_t i = en.x;
+    const int e = en.y & 0x0fffffff;
+    const int c = 3 + (en.y >> 28);
+    const double x0 = a.targets[3 * i], x1 = a.targets[3 * i + 1], x2 = a.targets[3 * i + 2];
+    double fr[6], fi[6], gr[6], gi[6], csum = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
+    const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
+    const double* ph = a.phi + c * a.Qmax * 6;
+    if (cplx)
+      pair_quadrature<6, true>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
+    else
+      pair_quadrature<6, false>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
+    const int* cols = a.ptopo + (size_t)e * 6;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const size_t idx = 2 * ((size_t)i + (size_t)cols[s] * (size_t)a.M);
+      if (doF) {
+        unsafeAtomicAdd(a.F + idx, fr[s]);
+        unsafeAtomicAdd(a.F + idx + 1, fi[s]);
+      }
+      if (doG) {
+        unsafeAtomicAdd(a.G + idx, gr[s]);
+        unsafeAtomicAdd(a.G + idx + 1, gi[s]);
+      }
+    }
+    if (a.flags & BIE_C) unsafeAtomicAdd(a.C + 2 * i, csum);
+  }
+}
+
+// Zero whole columns j of a column-major M x N complex matrix, j from a list.
+__global__ void k_zero_columns(double2* __restrict__ X, int64_t M, const int* __restrict__ cols,
+                               int ncols) {
+  const int64_t total = (int64_t)ncols * M;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += stride) {
+    const int64_t c = t / M;
+    const int64_t r = t - c * M;
+    X[(size_t)cols[c] * M + r] = make_double2(0.0, 0.0);
+  }
+}
+
 }  // namespace
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
+void launch_fill_zero_u32(unsigned* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, p);
+  BIE_HIP_CHECK(hipGetLastError());
+}
 
 void launch_fill_zero(double* p, int64_t n_doubles, hipStream_t s) {
   if (n_doubles <= 0) return;
@@ -286,6 +468,52 @@ void launch_select(const double* targets, int64_t M, const double* sel, int NP, 
     hipLaunchKernelGGL(k_select<3>, grid, dim3(256), 0, s, targets, M, sel, E, R, out);
   else
     hipLaunchKernelGGL(k_select<6>, grid, dim3(256), 0, s, targets, M, sel, E, R, out);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s) {
+  if (ncols <= 0 || M <= 0) return;
+  int64_t blocks = ((int64_t)ncols * M + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_zero_columns, dim3((unsigned)blocks), dim3(256), 0, s,
+                     reinterpret_cast<double2*>(X), M, cols, ncols);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s) {
+  if (a.M == 0 || a.E == 0) return;
+  BIE_REQUIRE(sa.W == kSlotWaves, BIE_EINVAL, "slot schedule built for another wave count");
+  const int64_t per_xcd = (sa.n_tiles + 7) / 8;
+  BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
+  const dim3 grid((unsigned)(per_xcd * 8));
+  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 +
+                                         (sa.phi_in_lds ? (size_t)a.R * a.Qmax * a.NP : 0));
+  BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
+  const bool cplx = a.ki != 0.0;
+  constexpr int W = kSlotWaves;
+  auto go = [&](auto kern) {
+    BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    hipLaunchKernelGGL(kern, grid, dim3(64 * W), shmem, s, a, sa);
+  };
+  if (a.NP == 3) {
+    if (cplx)
+      go(k_assemble_slots<3, true, W>);
+    else
+      go(k_assemble_slots<3, false, W>);
+  } else {
+    if (cplx)
+      go(k_assemble_slots<6, true, W>);
+    else
+      go(k_assemble_slots<6, false, W>);
+  }
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s) {
+  if (a.NP != 6 || a.M == 0 || a.E == 0) return;
+  hipLaunchKernelGGL(k_assemble_mid, dim3(2048), dim3(256), 0, s, a, sa.mid_count,
+                     sa.mid_queue, sa.mid_cap, a.ki != 0.0 ? 1 : 0);
   BIE_HIP_CHECK(hipGetLastError());
 }
 

@@ -78,13 +78,37 @@ struct AssemblyArgs {
   double* C;
 };
 
+// Device view of a slot schedule plus the mid-edge pair queue.
+constexpr int kSlotWaves = 8;   // waves per workgroup of the slot kernel
+constexpr int kSlotCount = 64;  // LDS slots (64 rows x 4 doubles each)
+
+struct SlotArgs {
+  int W, S;
+  int n_segments;
+  int64_t n_tiles;
+  int phi_in_lds;
+  const int* seg_step_begin;
+  const int* step_elems;
+  const int* slot_of;
+  const int* flush_begin;
+  const int* flush_node;
+  const int* flush_slot;
+  unsigned* mid_count;
+  int2* mid_queue;
+  int64_t mid_cap;
+};
+
 // Launchers (assemble.hip)
 void launch_fill_zero(double* p, int64_t n_doubles, hipStream_t s);
+void launch_fill_zero_u32(unsigned* p, hipStream_t s);
 void launch_geometry(const double* coords, const int* topo, int gk, int E,
                      const double* d_pts, int Pe, double* elem, hipStream_t s);
 void launch_gather_selnodes(const double* sources, const int* ptopo, int NP, int E,
                             double* selnodes, hipStream_t s);
 void launch_assemble(const AssemblyArgs& a, hipStream_t s);
+void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
+void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
+void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,
                    int R, int8_t* out, hipStream_t s);
 
@@ -106,5 +130,24 @@ struct bie_dmesh {
   double* d_pts = nullptr;
   double* d_phi = nullptr;
   double* d_elem = nullptr;
+  // host copies for scheduling
+  std::vector<double> h_coords;
+  std::vector<int> h_topo, h_ptopo;
+  // slot schedule cache (mesh-only; keyed by the requested minimum segment count)
+  int sched_key = -1;
+  int sched_segments = 0;
+  int sched_zero_cols = 0;
+  int sched_max_live = 0;
+  int* d_seg_step_begin = nullptr;
+  int* d_step_elems = nullptr;
+  int* d_slot_of = nullptr;
+  int* d_flush_begin = nullptr;
+  int* d_flush_node = nullptr;
+  int* d_flush_slot = nullptr;
+  int* d_zero_cols = nullptr;
+  // mid-edge pair queue
+  unsigned* d_mid_count = nullptr;
+  int2* d_mid_queue = nullptr;
+  int64_t mid_cap = 0;
   std::mutex mu;
 };

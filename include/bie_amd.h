@@ -90,6 +90,12 @@ int bie_select_rules_device(bie_dmesh* dm, const double* d_targets, int64_t n_ta
 int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, double* y,
                          double* normals, double* jw);
 
+/* Builds the LDS-slot schedule of the assembly kernel for a host mesh and replays it,
+ * checking every invariant the kernel relies on (no counterpart in the reference; the
+ * reference's threads own whole rows, assembly_collocation.jl:314).  stats (6 values):
+ * segments, steps, max live slots, shared nodes, zeroed columns, flush entries. */
+int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stats);
+
 /* ---- Kernel timing ------------------------------------------------------------------
  * While enabled, every device assembly records three HIP events on its launch stream:
  * call start, before and after the dense assembly kernel.  bie_timing_collect waits for

@@ -96,3 +96,27 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         _lib.lib()
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "c4", "sphere_1m", "quadlin", "ellipsoid"])
+@pytest.mark.parametrize("min_segments", [1, 13, 64, 300])
+def test_slot_schedule_invariants(name, min_segments):
+    """The LDS-slot kernel's schedule: every element swept once, slots never double-held,
+    every touched node flushed exactly once per segment, shared/owned flags consistent."""
+    mesh = {"c1": lambda: M.icosphere(8, "linear"),
+            "c2": lambda: M.icosphere(16, "quadratic"),
+            "c3": lambda: M.icosphere(20, "linear"),
+            "c4": lambda: M.icosphere(16, "linear"),
+            "sphere_1m": lambda: M.load_comsol(os.path.join(ROOT, "tests", "golden", "sphere_1m")),
+            "quadlin": lambda: M.icosphere(6, "quadratic", physics="linear"),
+            "ellipsoid": lambda: M.ellipsoid(9, (0.5, 1.0, 2.0), "linear")}[name]()
+    hm = _lib.HostMesh(mesh)
+    st = (ctypes.c_int64 * 6)()
+    rc = _lib.lib().bie_schedule_check(ctypes.byref(hm.struct), min_segments, st)
+    assert rc == 0, _lib.lib().bie_last_error()
+    n_seg, n_steps, max_live, n_shared, n_zero, n_flush = list(st)
+    max_size = -(-mesh.n_elements // min_segments)   # segments hold at most this many elements
+    assert n_seg >= -(-mesh.n_elements // max_size)
+    assert max_live <= 64
+    assert n_zero == n_shared            # closed surfaces: every node is touched
+    assert n_flush >= mesh.n_sources     # each node flushed at least once
