@@ -246,7 +246,8 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     }
   }
   if (timed) BIE_HIP_CHECK(hipEventRecord(ev[0], s));
-  const bool atomic_path = use_atomic_kernel() || M == 0 || dm->E == 0;
+  const bool atomic_path = use_atomic_kernel() || M == 0 || dm->E == 0 ||
+                           sizeof(double) * (size_t)dm->rs.R * dm->rs.Qmax * dm->pk > 24 * 1024;
   if (atomic_path || !(flags & BIE_F)) launch_fill_zero(dF, 2 * M * dm->N, s);
   if (atomic_path || !(flags & BIE_G)) launch_fill_zero(dG, 2 * M * dm->N, s);
   launch_fill_zero(dC, 2 * M, s);
@@ -254,7 +255,8 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
                     dm->d_elem, s);
     AssemblyArgs a = make_args(dm, d_targets, M, kr, ki, flags, dF, dG, dC);
-    if (use_atomic_kernel()) {
+    const bool phi_fits = sizeof(double) * (size_t)a.R * a.Qmax * a.NP <= 24 * 1024;
+    if (use_atomic_kernel() || !phi_fits) {
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
       launch_assemble(a, s);
     } else {

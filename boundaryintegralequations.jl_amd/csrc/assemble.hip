@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "bie_common.hpp"
+#include "kernels_math.hpp"
 #include "schedule.hpp"
 
 namespace bie {
@@ -139,15 +140,6 @@ __device__ __forceinline__ int select_rule(double x0, double x1, double x2,
   return c;
 }
 
-// 1/sqrt(x) from the hardware estimate refined by two Newton steps.
-__device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x;
-  y = y * fma(-h, y * y, 1.5);
-  y = y * fma(-h, y * y, 1.5);
-  return y;
-}
-
 template <int NP>
 __global__ void __launch_bounds__(256)
 k_select(const double* __restrict__ targets, int64_t M, const double* __restrict__ sel,
@@ -173,9 +165,14 @@ __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
                                                 const double* __restrict__ ph, int Q,
                                                 double kr, double ki, double* fr, double* fi,
                                                 double* gr, double* gi, double& csum) {
+  // one point ahead: the next row of the element table is in flight while this point
+  // is evaluated
+  double4 A = *reinterpret_cast<const double4*>(tab);
+  double4 B = *reinterpret_cast<const double4*>(tab + 4);
   for (int q = 0; q < Q; ++q) {
-    const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
-    const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+    const int qn = (q + 1 < Q) ? q + 1 : q;
+    const double4 An = *reinterpret_cast<const double4*>(tab + 8 * qn);
+    const double4 Bn = *reinterpret_cast<const double4*>(tab + 8 * qn + 4);
     const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double rinv = rsqrt_nr(r2);
@@ -185,7 +182,7 @@ __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
     const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
     csum -= ak;
     double sn, cs;
-    sincos(kr * r, &sn, &cs);
+    sincos_f64(kr * r, &sn, &cs);
     double Fr, Fi;
     if (CPLX) {
       const double ex = exp(-ki * r);
@@ -208,6 +205,8 @@ __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
       gr[s] = fma(Gr, p, gr[s]);
       gi[s] = fma(Gi, p, gi[s]);
     }
+    A = An;
+    B = Bn;
   }
 }
 
@@ -217,13 +216,8 @@ __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
 // problems, where the slot schedule does not pay.
 template <int NP, bool CPLX>
 __global__ void __launch_bounds__(256)
-k_assemble(const AssemblyArgs a, int ech, int phi_in_lds) {
-  extern __shared__ double s_phi[];
-  const int nphi = a.R * a.Qmax * NP;
-  if (phi_in_lds)
-    for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
-  __syncthreads();
-  const double* phi = phi_in_lds ? s_phi : a.phi;
+k_assemble(const AssemblyArgs a, int ech) {
+  const double* __restrict__ phi = a.phi;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= a.M) return;
   const double x0 = a.targets[3 * i], x1 = a.targets[3 * i + 1], x2 = a.targets[3 * i + 2];
@@ -279,21 +273,20 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   const int rb = (int)(tile / sa.n_segments);
   const int seg = (int)(tile - (int64_t)rb * sa.n_segments);
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = (int64_t)rb * 64 + lane;
   const bool active = i < a.M;
-  if (sa.phi_in_lds)
-    for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
+  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
   for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
   __syncthreads();
-  const double* phi = sa.phi_in_lds ? s_phi : a.phi;
+  const double* phi = s_phi;
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   double csum = 0.0;
   const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
   for (int st = st0; st < st1; ++st) {
-    const int e = sa.step_elems[(size_t)st * W + wave];
+    const int e = __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st * W + wave]);
     if (e >= 0) {
       int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
       bool mid = NP == 6 && c >= 3;
@@ -335,8 +328,8 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
     __syncthreads();
     const int f1 = sa.flush_begin[st + 1];
     for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
-      const int node = sa.flush_node[f];
-      const int ss = sa.flush_slot[f];
+      const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
+      const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
       double* p = acc + (size_t)(ss & 0xffff) * 256 + lane;
       const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
       p[0] = 0.0;
@@ -486,8 +479,8 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
   const int64_t per_xcd = (sa.n_tiles + 7) / 8;
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
-  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 +
-                                         (sa.phi_in_lds ? (size_t)a.R * a.Qmax * a.NP : 0));
+  BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
+  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + (size_t)a.R * a.Qmax * a.NP);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
   const bool cplx = a.ki != 0.0;
   constexpr int W = kSlotWaves;
@@ -528,23 +521,17 @@ void launch_assemble(const AssemblyArgs& a, hipStream_t s) {
   const int ech = (int)((a.E + chunks - 1) / chunks);
   chunks = (a.E + ech - 1) / ech;
   dim3 grid((unsigned)rowblocks, (unsigned)chunks);
-  size_t shmem = sizeof(double) * (size_t)a.R * a.Qmax * a.NP;
-  int phi_in_lds = 1;
-  if (shmem > 48 * 1024) {  // very high orders: read the basis table from global memory
-    shmem = 0;
-    phi_in_lds = 0;
-  }
   const bool cplx = a.ki != 0.0;
   if (a.NP == 3) {
     if (cplx)
-      hipLaunchKernelGGL((k_assemble<3, true>), grid, dim3(256), shmem, s, a, ech, phi_in_lds);
+      hipLaunchKernelGGL((k_assemble<3, true>), grid, dim3(256), 0, s, a, ech);
     else
-      hipLaunchKernelGGL((k_assemble<3, false>), grid, dim3(256), shmem, s, a, ech, phi_in_lds);
+      hipLaunchKernelGGL((k_assemble<3, false>), grid, dim3(256), 0, s, a, ech);
   } else {
     if (cplx)
-      hipLaunchKernelGGL((k_assemble<6, true>), grid, dim3(256), shmem, s, a, ech, phi_in_lds);
+      hipLaunchKernelGGL((k_assemble<6, true>), grid, dim3(256), 0, s, a, ech);
     else
-      hipLaunchKernelGGL((k_assemble<6, false>), grid, dim3(256), shmem, s, a, ech, phi_in_lds);
+      hipLaunchKernelGGL((k_assemble<6, false>), grid, dim3(256), 0, s, a, ech);
   }
   BIE_HIP_CHECK(hipGetLastError());
 }

@@ -1,0 +1,57 @@
+// FP64 device math for the assembly kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bie {
+
+// sin and cos of x for |x| < 2^26 by Cody-Waite reduction modulo pi/2 with two fused
+// multiply-add terms (the product n * pio2_hi is exact inside the FMA, the dropped tail of
+// pi/2 is ~1e-33, so the reduced argument carries only two roundings of ~5.5e-17), then
+// degree-11 (sin) and degree-12 (cos) polynomials on |r| <= pi/4 whose coefficients were
+// fitted by a Remez exchange in 60-digit arithmetic (csrc/fit_sincos.py; max error of the
+// polynomials 1.4e-17 and 8.7e-19).  Absolute error of the results ~2e-16, which is what
+// e^{ikr} needs.  Larger arguments take the library sincos.
+__device__ __forceinline__ void sincos_f64(double x, double* s, double* c) {
+  if (fabs(x) >= 67108864.0) {  // 2^26
+    sincos(x, s, c);
+    return;
+  }
+  constexpr double kTwoOverPi = 0.63661977236758138243;
+  constexpr double kPio2Hi = 1.5707963267948965580;   // pi/2 rounded to double
+  constexpr double kPio2Lo = 6.1232339957367658e-17;  // pi/2 - kPio2Hi rounded to double
+  const double n = rint(x * kTwoOverPi);
+  double r = fma(-n, kPio2Hi, x);
+  r = fma(-n, kPio2Lo, r);
+  const int q = (int)n;
+  const double t = r * r;
+  double S = fma(t, 1.5918142569704652e-10, -2.505113204972727e-08);
+  S = fma(t, S, 2.7557316103657487e-06);
+  S = fma(t, S, -0.00019841269836761008);
+  S = fma(t, S, 0.00833333333333095);
+  S = fma(t, S, -0.16666666666666666);
+  const double sr = fma(r * t, S, r);
+  double C = fma(t, -1.1382639805756885e-11, 2.0876146382220145e-09);
+  C = fma(t, C, -2.7557317272344146e-07);
+  C = fma(t, C, 2.480158729876704e-05);
+  C = fma(t, C, -0.0013888888888887398);
+  C = fma(t, C, 0.041666666666666664);
+  const double cr = fma(t * t, C, fma(-0.5, t, 1.0));
+  double ss = (q & 1) ? cr : sr;
+  double cc = (q & 1) ? sr : cr;
+  if (q & 2) ss = -ss;
+  if ((q + 1) & 2) cc = -cc;
+  *s = ss;
+  *c = cc;
+}
+
+// 1/sqrt(x) from the hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h, y * y, 1.5);
+  y = y * fma(-h, y * y, 1.5);
+  return y;
+}
+
+}  // namespace bie
