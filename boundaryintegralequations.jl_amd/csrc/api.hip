@@ -149,11 +149,13 @@ T* upload_vec(const std::vector<T>& v, hipStream_t s) {
 }
 
 void free_schedule(bie_dmesh* dm) {
-  for (int** p : {&dm->d_seg_step_begin, &dm->d_step_elems, &dm->d_slot_of, &dm->d_flush_begin,
+  for (int** p : {&dm->d_seg_step_begin, &dm->d_step_elems, &dm->d_step_slot, &dm->d_flush_begin,
                   &dm->d_flush_node, &dm->d_flush_slot, &dm->d_zero_cols}) {
     dfree(*p);
     *p = nullptr;
   }
+  dfree(dm->d_step_sel);
+  dm->d_step_sel = nullptr;
   dm->sched_key = -1;
 }
 
@@ -171,7 +173,23 @@ void ensure_schedule(bie_dmesh* dm, int min_segments, hipStream_t s) {
                                         cent.data(), kSlotWaves, kSlotCount, min_segments);
   dm->d_seg_step_begin = upload_vec(sc.seg_step_begin, s);
   dm->d_step_elems = upload_vec(sc.step_elems, s);
-  dm->d_slot_of = upload_vec(sc.slot_of, s);
+  // per (step, wave) copies of the selection nodes and slots, in the order the kernel
+  // reads them
+  const size_t nsw = (size_t)sc.n_steps * kSlotWaves;
+  std::vector<int> step_slot(nsw * dm->pk, 0);
+  std::vector<double> step_sel(nsw * dm->pk * 4, 0.0);
+  for (size_t t = 0; t < nsw; ++t) {
+    const int e = sc.step_elems[t];
+    if (e < 0) continue;
+    for (int k = 0; k < dm->pk; ++k) {
+      step_slot[t * dm->pk + k] = sc.slot_of[(size_t)e * dm->pk + k];
+      const int node = dm->h_ptopo[(size_t)e * dm->pk + k];
+      for (int d = 0; d < 3; ++d)
+        step_sel[(t * dm->pk + k) * 4 + d] = dm->h_sources[3 * (size_t)node + d];
+    }
+  }
+  dm->d_step_slot = upload_vec(step_slot, s);
+  dm->d_step_sel = upload_vec(step_sel, s);
   dm->d_flush_begin = upload_vec(sc.flush_begin, s);
   dm->d_flush_node = upload_vec(sc.flush_node, s);
   dm->d_flush_slot = upload_vec(sc.flush_slot, s);
@@ -267,11 +285,17 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       sa.W = kSlotWaves;
       sa.S = kSlotCount;
       sa.n_segments = dm->sched_segments;
+      sa.n_rowblocks = (int)rowblocks;
       sa.n_tiles = rowblocks * dm->sched_segments;
+      {
+        const char* ord = std::getenv("BIE_AMD_TILE_ORDER");
+        sa.seg_major = (ord && std::strcmp(ord, "rowblock") == 0) ? 0 : 1;
+      }
       sa.phi_in_lds = (sizeof(double) * (size_t)a.R * a.Qmax * a.NP <= 24 * 1024) ? 1 : 0;
       sa.seg_step_begin = dm->d_seg_step_begin;
       sa.step_elems = dm->d_step_elems;
-      sa.slot_of = dm->d_slot_of;
+      sa.step_slot = dm->d_step_slot;
+      sa.step_sel = dm->d_step_sel;
       sa.flush_begin = dm->d_flush_begin;
       sa.flush_node = dm->d_flush_node;
       sa.flush_slot = dm->d_flush_slot;
@@ -335,6 +359,7 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     dm->h_topo = topo;
     dm->h_ptopo = ptopo;
     dm->h_coords.assign(mesh->coordinates, mesh->coordinates + 3 * mesh->n_coordinates);
+    dm->h_sources.assign(mesh->sources, mesh->sources + 3 * mesh->n_sources);
     dm->device = device;
     dm->gk = mesh->geometry_kind;
     dm->pk = mesh->physics_kind;

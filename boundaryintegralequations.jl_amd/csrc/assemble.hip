@@ -160,54 +160,102 @@ k_select(const double* __restrict__ targets, int64_t M, const double* __restrict
 // f = e^{ikr}(ikr - 1)(n.d)/(4 pi r^3) and k0 = -(n.d)/(4 pi r^3) (kernels.jl:13-55) share
 // one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].
 template <int NP, bool CPLX>
+__device__ __forceinline__ void point_kernel(double x0, double x1, double x2, const double4 A,
+                                             const double4 B, const double* __restrict__ p,
+                                             double kr, double ki, double* fr, double* fi,
+                                             double* gr, double* gi, double& csum) {
+  const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rinv = rsqrt_nr(r2);
+  const double r = r2 * rinv;
+  const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
+  const double g = B.z * rinv;             // J w / (4 pi r)
+  const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
+  csum -= ak;
+  double sn, cs;
+  sincos_f64(kr * r, &sn, &cs);
+  double Fr, Fi;
+  if (CPLX) {
+    const double ex = exp(-ki * r);
+    cs *= ex;
+    sn *= ex;
+    const double ar = fma(-ki, r, -1.0), ai = kr * r;
+    Fr = ak * fma(cs, ar, -sn * ai);
+    Fi = ak * fma(cs, ai, sn * ar);
+  } else {
+    const double krr = kr * r;
+    Fr = -ak * fma(krr, sn, cs);
+    Fi = ak * fma(krr, cs, -sn);
+  }
+  const double Gr = g * cs, Gi = g * sn;
+#pragma unroll
+  for (int s = 0; s < NP; ++s) {
+    const double ps = p[s];
+    fr[s] = fma(Fr, ps, fr[s]);
+    fi[s] = fma(Fi, ps, fi[s]);
+    gr[s] = fma(Gr, ps, gr[s]);
+    gi[s] = fma(Gi, ps, gi[s]);
+  }
+}
+
+// Quadrature of one (target, element) pair over rule points tab[0..Q) with physics basis
+// ph[q * NP + s]: accumulates the F and G contributions of the NP element nodes and the
+// C integrand.  Single-layer g = e^{ikr}/(4 pi r), double-layer
+// f = e^{ikr}(ikr - 1)(n.d)/(4 pi r^3) and k0 = -(n.d)/(4 pi r^3) (kernels.jl:13-55) share
+// one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].  Points are taken two at
+// a time from two register sets so the next table row is always in flight.
+template <int NP, bool CPLX>
 __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
                                                 const double* __restrict__ tab,
                                                 const double* __restrict__ ph, int Q,
                                                 double kr, double ki, double* fr, double* fi,
                                                 double* gr, double* gi, double& csum) {
-  // one point ahead: the next row of the element table is in flight while this point
-  // is evaluated
   double4 A = *reinterpret_cast<const double4*>(tab);
   double4 B = *reinterpret_cast<const double4*>(tab + 4);
-  for (int q = 0; q < Q; ++q) {
-    const int qn = (q + 1 < Q) ? q + 1 : q;
-    const double4 An = *reinterpret_cast<const double4*>(tab + 8 * qn);
-    const double4 Bn = *reinterpret_cast<const double4*>(tab + 8 * qn + 4);
-    const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double rinv = rsqrt_nr(r2);
-    const double r = r2 * rinv;
-    const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
-    const double g = B.z * rinv;             // J w / (4 pi r)
-    const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
-    csum -= ak;
-    double sn, cs;
-    sincos_f64(kr * r, &sn, &cs);
-    double Fr, Fi;
-    if (CPLX) {
-      const double ex = exp(-ki * r);
-      cs *= ex;
-      sn *= ex;
-      const double ar = fma(-ki, r, -1.0), ai = kr * r;
-      Fr = ak * fma(cs, ar, -sn * ai);
-      Fi = ak * fma(cs, ai, sn * ar);
-    } else {
-      const double krr = kr * r;
-      Fr = -ak * fma(krr, sn, cs);
-      Fi = ak * fma(krr, cs, -sn);
-    }
-    const double Gr = g * cs, Gi = g * sn;
-#pragma unroll
-    for (int s = 0; s < NP; ++s) {
-      const double p = ph[q * NP + s];
-      fr[s] = fma(Fr, p, fr[s]);
-      fi[s] = fma(Fi, p, fi[s]);
-      gr[s] = fma(Gr, p, gr[s]);
-      gi[s] = fma(Gi, p, gi[s]);
-    }
-    A = An;
-    B = Bn;
+  int q = 0;
+  for (; q + 1 < Q; q += 2) {
+    const double4 A2 = *reinterpret_cast<const double4*>(tab + 8 * (q + 1));
+    const double4 B2 = *reinterpret_cast<const double4*>(tab + 8 * (q + 1) + 4);
+    point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+    const int qn = (q + 2 < Q) ? q + 2 : q + 1;
+    A = *reinterpret_cast<const double4*>(tab + 8 * qn);
+    B = *reinterpret_cast<const double4*>(tab + 8 * qn + 4);
+    point_kernel<NP, CPLX>(x0, x1, x2, A2, B2, ph + (q + 1) * NP, kr, ki, fr, fi, gr, gi, csum);
   }
+  if (q < Q) point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+}
+
+// As pair_quadrature, with the first two table rows (points 0 and 1, or 0 twice when
+// Q == 1) already loaded by the caller, so their latency overlaps earlier work.
+template <int NP, bool CPLX>
+__device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double x2,
+                                                    const double* __restrict__ tab,
+                                                    const double* __restrict__ ph, int Q,
+                                                    double kr, double ki, double4 A, double4 B,
+                                                    double4 A2, double4 B2, double* fr,
+                                                    double* fi, double* gr, double* gi,
+                                                    double& csum) {
+  int q = 0;
+  for (; q + 1 < Q; q += 2) {
+    point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+    const int q2 = (q + 2 < Q) ? q + 2 : q + 1;
+    A = *reinterpret_cast<const double4*>(tab + 8 * q2);
+    B = *reinterpret_cast<const double4*>(tab + 8 * q2 + 4);
+    point_kernel<NP, CPLX>(x0, x1, x2, A2, B2, ph + (q + 1) * NP, kr, ki, fr, fi, gr, gi, csum);
+    const int q3 = (q + 3 < Q) ? q + 3 : q + 1;
+    A2 = *reinterpret_cast<const double4*>(tab + 8 * q3);
+    B2 = *reinterpret_cast<const double4*>(tab + 8 * q3 + 4);
+  }
+  if (q < Q) point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+}
+
+__device__ __forceinline__ void load_rows01(const double* __restrict__ tab, int Q, double4& A,
+                                            double4& B, double4& A2, double4& B2) {
+  A = *reinterpret_cast<const double4*>(tab);
+  B = *reinterpret_cast<const double4*>(tab + 4);
+  const int q1 = Q > 1 ? 1 : 0;
+  A2 = *reinterpret_cast<const double4*>(tab + 8 * q1);
+  B2 = *reinterpret_cast<const double4*>(tab + 8 * q1 + 4);
 }
 
 // One thread per target row; blockIdx.y selects a contiguous chunk of elements.
@@ -254,9 +302,16 @@ k_assemble(const AssemblyArgs a, int ech) {
 // segment of elements; each step the waves take one element each, add the element's
 // node contributions into LDS slots (ds_add_f64: waves may hit the same slot), and after
 // a barrier the nodes whose last element was in this step are flushed: owned columns
-// with one coalesced store per row block, shared columns with FP64 atomics.  Pairs whose
-// rule is a mid-edge split rule (quadratic physics only, ~1% of pairs) are appended to a
-// queue and integrated by k_assemble_mid.
+// with one coalesced store per row block, shared columns with FP64 atomics.
+//
+// Latency: all waves leave a barrier together, so nothing would hide a load issued
+// there.  Each wave therefore selects the rule of its *next* element and starts loading
+// that element's first table rows before the barrier; selection data is laid out in
+// step order and read one step ahead.
+//
+// Pairs whose rule is a mid-edge split rule (quadratic physics only, ~1% of pairs) are
+// queued in LDS, moved to the global queue once per segment, and integrated by
+// k_assemble_mid.
 template <int NP, bool CPLX, int W>
 __global__ void __launch_bounds__(64 * W)
 k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
@@ -264,57 +319,59 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   double* acc = lds;                              // [slot][4][64]
   double* s_phi = lds + (size_t)sa.S * 256;
   const int nphi = a.R * a.Qmax * NP;
+  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
   // XCD-aware tile order: consecutive block ids go to different XCDs, so give each XCD
-  // a contiguous run of (row block, segment) tiles, row-block major.
+  // a contiguous run of (row block, segment) tiles.
   const int b = blockIdx.x;
   const int per_xcd = (int)((sa.n_tiles + 7) / 8);
   const int64_t tile = (int64_t)(b & 7) * per_xcd + (b >> 3);
   if ((b >> 3) >= per_xcd || tile >= sa.n_tiles) return;
-  const int rb = (int)(tile / sa.n_segments);
-  const int seg = (int)(tile - (int64_t)rb * sa.n_segments);
+  int rb, seg;
+  if (sa.seg_major) {  // an XCD's concurrent workgroups share one segment's tables in L2
+    seg = (int)(tile / sa.n_rowblocks);
+    rb = (int)(tile - (int64_t)seg * sa.n_rowblocks);
+  } else {             // an XCD's concurrent workgroups share row blocks (shared-column atomics)
+    rb = (int)(tile / sa.n_segments);
+    seg = (int)(tile - (int64_t)rb * sa.n_segments);
+  }
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = (int64_t)rb * 64 + lane;
   const bool active = i < a.M;
   for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
   for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
+  if (threadIdx.x == 0) *s_qn = 0u;
   __syncthreads();
   const double* phi = s_phi;
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  const double kr = a.kr, ki = a.ki;
   double csum = 0.0;
   const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
+  int e = (st0 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st0 * W + wave]) : -1;
+  int c = 0;
+  double4 A0 = make_double4(0, 0, 0, 0), B0 = A0, A1 = A0, B1 = A0;
+  if (e >= 0) {
+    c = select_rule<NP>(x0, x1, x2, sa.step_sel + ((size_t)st0 * W + wave) * NP * 4, a.R) - 1;
+    if (!(NP == 6 && c >= 3))
+      load_rows01(a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, a.len[c], A0, B0, A1, B1);
+  }
   for (int st = st0; st < st1; ++st) {
-    const int e = __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st * W + wave]);
+    const size_t nidx = (size_t)(st + 1) * W + wave;
+    const int e_nxt =
+        (st + 1 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[nidx]) : -1;
     if (e >= 0) {
-      int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
-      bool mid = NP == 6 && c >= 3;
-      if (NP == 6) {
-        const bool push = mid && active;
-        const unsigned long long mask = __ballot(push);
-        if (mask) {
-          unsigned base = 0;
-          const int leader = __ffsll((long long)mask) - 1;
-          if (lane == leader) base = atomicAdd(sa.mid_count, (unsigned)__popcll(mask));
-          base = __shfl(base, leader);
-          if (push) {
-            const unsigned pos =
-                base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-            if ((int64_t)pos < sa.mid_cap)
-              sa.mid_queue[pos] = make_int2((int)i, e | ((c - 3) << 28));
-          }
-        }
-      }
-      const bool doit = active && !mid;
-      double fr[NP], fi[NP], gr[NP], gi[NP];
+      const bool mid = NP == 6 && c >= 3;
+      if (active && !mid) {
+        double fr[NP], fi[NP], gr[NP], gi[NP];
 #pragma unroll
-      for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
-      if (doit) {
-        pair_quadrature<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
-                                  phi + c * a.Qmax * NP, a.len[c], a.kr, a.ki, fr, fi, gr,
-                                  gi, csum);
-        const int* so = sa.slot_of + (size_t)e * NP;
+        for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
+        pair_quadrature_pre<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
+                                      phi + c * a.Qmax * NP, a.len[c], kr, ki, A0, B0, A1, B1,
+                                      fr, fi, gr, gi, csum);
+        const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
 #pragma unroll
         for (int s = 0; s < NP; ++s) {
           double* p = acc + (size_t)so[s] * 256 + lane;
@@ -324,6 +381,34 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
           atomicAdd(p + 192, gi[s]);
         }
       }
+      if (NP == 6) {
+        const bool push = mid && active;
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+          const int leader = __ffsll((long long)mask) - 1;
+          unsigned base = 0;
+          if (lane == leader) base = atomicAdd(s_qn, (unsigned)__popcll(mask));
+          base = __shfl(base, leader);
+          if (push) {
+            const unsigned pos = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            const int2 entry = make_int2((int)i, e | ((c - 3) << 28));
+            if (pos < (unsigned)kMidLds) {
+              s_q[pos] = entry;
+            } else {  // LDS queue full: go straight to the global queue
+              const unsigned g = atomicAdd(sa.mid_count, 1u);
+              if ((int64_t)g < sa.mid_cap) sa.mid_queue[g] = entry;
+            }
+          }
+        }
+      }
+    }
+    // next element: rule selection and its first rows, in flight across the barrier
+    int c_nxt = 0;
+    if (e_nxt >= 0) {
+      c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
+      if (!(NP == 6 && c_nxt >= 3))
+        load_rows01(a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8, a.len[c_nxt], A0, B0,
+                    A1, B1);
     }
     __syncthreads();
     const int f1 = sa.flush_begin[st + 1];
@@ -354,6 +439,18 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
       }
     }
     __syncthreads();
+    e = e_nxt;
+    c = c_nxt;
+  }
+  if (NP == 6 && wave == 0) {  // the loop ended on a barrier: the LDS queue is complete
+    const unsigned n = min(*s_qn, (unsigned)kMidLds);
+    if (n) {
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(sa.mid_count, n);
+      base = __shfl(base, 0);
+      for (unsigned t = lane; t < n; t += 64)
+        if ((int64_t)(base + t) < sa.mid_cap) sa.mid_queue[base + t] = s_q[t];
+    }
   }
   if (active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
 }
@@ -480,7 +577,8 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
   BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
-  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + (size_t)a.R * a.Qmax * a.NP);
+  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + (size_t)a.R * a.Qmax * a.NP) +
+                       sizeof(int2) * kMidLds + 16;
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
   const bool cplx = a.ki != 0.0;
   constexpr int W = kSlotWaves;

@@ -81,15 +81,19 @@ struct AssemblyArgs {
 // Device view of a slot schedule plus the mid-edge pair queue.
 constexpr int kSlotWaves = 8;   // waves per workgroup of the slot kernel
 constexpr int kSlotCount = 64;  // LDS slots (64 rows x 4 doubles each)
+constexpr int kMidLds = 512;    // per-workgroup LDS queue of mid-edge pairs
 
 struct SlotArgs {
   int W, S;
   int n_segments;
+  int n_rowblocks;
+  int seg_major;
   int64_t n_tiles;
   int phi_in_lds;
   const int* seg_step_begin;
   const int* step_elems;
-  const int* slot_of;
+  const int* step_slot;    // (step, wave, node) slot, schedule order
+  const double* step_sel;  // (step, wave, node) selection node xyz_, schedule order
   const int* flush_begin;
   const int* flush_node;
   const int* flush_slot;
@@ -131,7 +135,7 @@ struct bie_dmesh {
   double* d_phi = nullptr;
   double* d_elem = nullptr;
   // host copies for scheduling
-  std::vector<double> h_coords;
+  std::vector<double> h_coords, h_sources;
   std::vector<int> h_topo, h_ptopo;
   // slot schedule cache (mesh-only; keyed by the requested minimum segment count)
   int sched_key = -1;
@@ -140,7 +144,8 @@ struct bie_dmesh {
   int sched_max_live = 0;
   int* d_seg_step_begin = nullptr;
   int* d_step_elems = nullptr;
-  int* d_slot_of = nullptr;
+  int* d_step_slot = nullptr;
+  double* d_step_sel = nullptr;
   int* d_flush_begin = nullptr;
   int* d_flush_node = nullptr;
   int* d_flush_slot = nullptr;
