@@ -60,6 +60,9 @@ SIGNATURES = {
                                             ctypes.c_int32, _dp, _dp, _dp]),
     "bie_schedule_check": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int64)]),
+    "bie_zgemm_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp,
+                                         ctypes.c_int64, _dp, ctypes.c_int64, _dp,
+                                         ctypes.c_int64, _dp]),
     "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),

@@ -541,6 +541,13 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
   });
 }
 
+int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                     const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream) {
+  return guarded([&] {
+    zgemm_device(M, nrhs, K, dA, lda, dX, ldx, dY, ldy, static_cast<hipStream_t>(stream));
+  });
+}
+
 int bie_timing_enable(int on) {
   return guarded([&] {
     std::lock_guard<std::mutex> tl(g_timing.mu);

@@ -113,6 +113,8 @@ void launch_assemble(const AssemblyArgs& a, hipStream_t s);
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
+void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                  const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,
                    int R, int8_t* out, hipStream_t s);
 

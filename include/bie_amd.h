@@ -96,6 +96,16 @@ int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, doub
  * segments, steps, max live slots, shared nodes, zeroed columns, flush entries. */
 int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stats);
 
+/* ---- Dense operator apply --------------------------------------------------------------
+ * Y = A X for complex128 column-major A (M x K, leading dimension lda), X (K x nrhs, ldx),
+ * Y (M x nrhs, ldy), all device pointers, on FP64 MFMA; asynchronous on stream.
+ * Replaces the BLAS ZGEMM behind `F*X` / `G*vs` in the reference's callers
+ * (examples/3d_pulsating_sphere.jl:42,62), batched like the FMM operators' 32-RHS path
+ * (src/3d/fast_multipole_method.jl:249-287).  Uses a cached per-device workspace of
+ * 16 * K * ceil(nrhs/32)*32 bytes. */
+int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                     const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream);
+
 /* ---- Kernel timing ------------------------------------------------------------------
  * While enabled, every device assembly records three HIP events on its launch stream:
  * call start, before and after the dense assembly kernel.  bie_timing_collect waits for
