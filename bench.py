@@ -39,6 +39,48 @@ def algorithmic_flops(rule_hist, rs_len, n_s):
     return pts * (35 + 8 * n_s) + pairs * 10 * n_s, pts, pairs
 
 
+def measure_apply(torch, F, N, steps, stream, nrhs=32):
+    """Y = F X for 32 complex N(0,1) right-hand sides (re, im ~ N(0, 1/2), seed 0) on the
+    assembled F: the library's MFMA ZGEMM, and torch.matmul (rocBLAS ZGEMM) beside it."""
+    from bie_amd.operators import zgemm_device
+    g = torch.Generator(device="cpu").manual_seed(0)
+    Xh = torch.complex(torch.randn(nrhs, N, generator=g, dtype=torch.float64),
+                       torch.randn(nrhs, N, generator=g, dtype=torch.float64)) / (2 ** 0.5)
+    X = Xh.to(F.device)                  # row c of X is column c of the N x nrhs matrix
+    Y = torch.empty_like(X)
+    Fc = torch.view_as_complex(F.view(N, N, 2))   # Fc[j, i] = F[i, j] (column-major F)
+
+    def ours():
+        zgemm_device(N, nrhs, N, F.data_ptr(), N, X.data_ptr(), N, Y.data_ptr(), N, stream)
+
+    def ref():
+        return X @ Fc                     # (nrhs x N) = (F X)^T
+
+    res = {}
+    for name, fn in (("mfma_zgemm", ours), ("torch_rocblas_zgemm", ref)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            out = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        bytes_ = 16 * N * N + 2 * 16 * N * nrhs
+        res[name] = {"ms": ms, "GB_per_s": bytes_ / (ms * 1e-3) / 1e9,
+                     "TFLOP_per_s": 8 * N * N * nrhs / (ms * 1e-3) / 1e12,
+                     "frac_fp64_mfma_peak": 8 * N * N * nrhs / (ms * 1e-3) / PEAK_FP64,
+                     "frac_hbm_peak": bytes_ / (ms * 1e-3) / PEAK_HBM}
+    ours()
+    torch.cuda.synchronize()
+    Yr = ref()
+    res["rel_err_vs_torch"] = float(torch.linalg.norm(Y - Yr) / torch.linalg.norm(Yr))
+    res["nrhs"] = nrhs
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +164,9 @@ def main():
         run(n, m, 1, 0)
     else:
         run(n, m, 1, 0)
+
+    if not args.no_extra:
+        extra["apply"] = measure_apply(torch, F, N, args.steps, stream)
 
     cpu = None
     if not args.no_cpu:

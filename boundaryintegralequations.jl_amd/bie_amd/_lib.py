@@ -63,6 +63,16 @@ SIGNATURES = {
     "bie_zgemm_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp]),
+    "bie_assemble_sweep_device": (ctypes.c_int, [_dp, _dp, ctypes.c_int32, _dp, ctypes.c_int64,
+                                                  ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.c_uint32, _dp, _dp, ctypes.c_int64,
+                                                  _dp, _dp]),
+    "bie_sweep_graph_create": (ctypes.c_int, [_dp, _dp, ctypes.c_int32, _dp, ctypes.c_int64,
+                                              ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                              _dp, _dp, ctypes.c_int64, _dp,
+                                              ctypes.POINTER(_dp)]),
+    "bie_graph_launch": (ctypes.c_int, [_dp, _dp]),
+    "bie_graph_destroy": (ctypes.c_int, [_dp]),
     "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),

@@ -58,6 +58,27 @@ def assemble_parallel(mesh: Mesh3d, k, in_sources=None, *, fOn=True, gOn=True, c
     return F, G, C
 
 
+class Graph:
+    """An instantiated HIP graph (bie_graph); launch() replays it on a stream."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream: int = 0):
+        _lib.check(_lib.lib().bie_graph_launch(self.handle, ctypes.c_void_p(stream or None)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().bie_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceMesh:
     """A mesh resident on one GPU (bie_dmesh), with its cached rule tables."""
 
@@ -103,6 +124,30 @@ class DeviceMesh:
         _lib.check(_lib.lib().bie_select_rules_device(
             self.handle, ctypes.c_void_p(targets_ptr or None), M, ctypes.c_void_p(rules_ptr),
             ctypes.c_void_p(stream or None)))
+
+    def assemble_sweep(self, k_ptr: int, nk: int, F_ptr: int, G_ptr: int, stride: int,
+                       C_ptr: int, *, targets_ptr: int = 0, n_targets: int | None = None,
+                       n=4, m=4, fOn=True, gOn=True, cOn=True, stream: int = 0):
+        """F_q, G_q for nk real wavenumbers (device array at k_ptr); F_q at
+        F_ptr + 16 * q * stride bytes.  Asynchronous."""
+        M = self.n_sources if n_targets is None else int(n_targets)
+        _lib.check(_lib.lib().bie_assemble_sweep_device(
+            self.handle, ctypes.c_void_p(k_ptr), int(nk), ctypes.c_void_p(targets_ptr or None),
+            M, int(n), int(m), _flags(fOn, gOn, cOn), ctypes.c_void_p(F_ptr),
+            ctypes.c_void_p(G_ptr), int(stride), ctypes.c_void_p(C_ptr),
+            ctypes.c_void_p(stream or None)))
+
+    def sweep_graph(self, k_ptr: int, nk: int, F_ptr: int, G_ptr: int, stride: int, C_ptr: int,
+                    *, targets_ptr: int = 0, n_targets: int | None = None, n=4, m=4,
+                    fOn=True, gOn=True, cOn=True) -> "Graph":
+        """The same sweep captured into a HIP graph."""
+        M = self.n_sources if n_targets is None else int(n_targets)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().bie_sweep_graph_create(
+            self.handle, ctypes.c_void_p(k_ptr), int(nk), ctypes.c_void_p(targets_ptr or None),
+            M, int(n), int(m), _flags(fOn, gOn, cOn), ctypes.c_void_p(F_ptr),
+            ctypes.c_void_p(G_ptr), int(stride), ctypes.c_void_p(C_ptr), ctypes.byref(h)))
+        return Graph(h)
 
     def element_geometry(self, rule: int, n: int, m: int):
         from . import rules as _r

@@ -148,20 +148,28 @@ T* upload_vec(const std::vector<T>& v, hipStream_t s) {
   return d;
 }
 
-void free_schedule(bie_dmesh* dm) {
-  for (int** p : {&dm->d_seg_step_begin, &dm->d_step_elems, &dm->d_step_slot, &dm->d_flush_begin,
-                  &dm->d_flush_node, &dm->d_flush_slot, &dm->d_zero_cols}) {
+void free_schedule(bie_dmesh::Sched& sc) {
+  for (int** p : {&sc.d_seg_step_begin, &sc.d_step_elems, &sc.d_step_slot, &sc.d_flush_begin,
+                  &sc.d_flush_node, &sc.d_flush_slot, &sc.d_zero_cols}) {
     dfree(*p);
     *p = nullptr;
   }
-  dfree(dm->d_step_sel);
-  dm->d_step_sel = nullptr;
-  dm->sched_key = -1;
+  dfree(sc.d_step_sel);
+  sc.d_step_sel = nullptr;
+  sc.key = -1;
+  sc.S = -1;
 }
 
-void ensure_schedule(bie_dmesh* dm, int min_segments, hipStream_t s) {
-  if (dm->sched_key == min_segments && dm->d_step_elems) return;
-  free_schedule(dm);
+void free_schedules(bie_dmesh* dm) {
+  for (auto& sc : dm->sched) free_schedule(sc);
+}
+
+// Schedule `which` (0: single-k kernel, 1: sweep kernel) for S live slots.
+bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segments,
+                                  hipStream_t s) {
+  bie_dmesh::Sched& out = dm->sched[which];
+  if (out.key == min_segments && out.S == S && out.d_step_elems) return out;
+  free_schedule(out);
   std::vector<double> cent((size_t)dm->E * 3);
   for (int64_t e = 0; e < dm->E; ++e)
     for (int d = 0; d < 3; ++d) {
@@ -170,9 +178,9 @@ void ensure_schedule(bie_dmesh* dm, int min_segments, hipStream_t s) {
       cent[3 * e + d] = acc / 3.0;
     }
   SlotSchedule sc = build_slot_schedule(dm->h_ptopo.data(), dm->pk, (int)dm->E, (int)dm->N,
-                                        cent.data(), kSlotWaves, kSlotCount, min_segments);
-  dm->d_seg_step_begin = upload_vec(sc.seg_step_begin, s);
-  dm->d_step_elems = upload_vec(sc.step_elems, s);
+                                        cent.data(), kSlotWaves, S, min_segments);
+  out.d_seg_step_begin = upload_vec(sc.seg_step_begin, s);
+  out.d_step_elems = upload_vec(sc.step_elems, s);
   // per (step, wave) copies of the selection nodes and slots, in the order the kernel
   // reads them
   const size_t nsw = (size_t)sc.n_steps * kSlotWaves;
@@ -188,17 +196,42 @@ void ensure_schedule(bie_dmesh* dm, int min_segments, hipStream_t s) {
         step_sel[(t * dm->pk + k) * 4 + d] = dm->h_sources[3 * (size_t)node + d];
     }
   }
-  dm->d_step_slot = upload_vec(step_slot, s);
-  dm->d_step_sel = upload_vec(step_sel, s);
-  dm->d_flush_begin = upload_vec(sc.flush_begin, s);
-  dm->d_flush_node = upload_vec(sc.flush_node, s);
-  dm->d_flush_slot = upload_vec(sc.flush_slot, s);
-  dm->d_zero_cols = upload_vec(sc.zero_cols, s);
+  out.d_step_slot = upload_vec(step_slot, s);
+  out.d_step_sel = upload_vec(step_sel, s);
+  out.d_flush_begin = upload_vec(sc.flush_begin, s);
+  out.d_flush_node = upload_vec(sc.flush_node, s);
+  out.d_flush_slot = upload_vec(sc.flush_slot, s);
+  out.d_zero_cols = upload_vec(sc.zero_cols, s);
   BIE_HIP_CHECK(hipStreamSynchronize(s));
-  dm->sched_segments = sc.n_segments;
-  dm->sched_zero_cols = (int)sc.zero_cols.size();
-  dm->sched_max_live = sc.max_live;
-  dm->sched_key = min_segments;
+  out.segments = sc.n_segments;
+  out.zero_cols = (int)sc.zero_cols.size();
+  out.max_live = sc.max_live;
+  out.key = min_segments;
+  out.S = S;
+  return out;
+}
+
+SlotArgs slot_args(bie_dmesh* dm, const bie_dmesh::Sched& sc, int64_t rowblocks) {
+  SlotArgs sa{};
+  sa.W = kSlotWaves;
+  sa.S = sc.S;
+  sa.n_segments = sc.segments;
+  sa.n_rowblocks = (int)rowblocks;
+  sa.n_tiles = rowblocks * sc.segments;
+  const char* ord = std::getenv("BIE_AMD_TILE_ORDER");
+  sa.seg_major = (ord && std::strcmp(ord, "rowblock") == 0) ? 0 : 1;
+  sa.phi_in_lds = 1;
+  sa.seg_step_begin = sc.d_seg_step_begin;
+  sa.step_elems = sc.d_step_elems;
+  sa.step_slot = sc.d_step_slot;
+  sa.step_sel = sc.d_step_sel;
+  sa.flush_begin = sc.d_flush_begin;
+  sa.flush_node = sc.d_flush_node;
+  sa.flush_slot = sc.d_flush_slot;
+  sa.mid_count = dm->d_mid_count;
+  sa.mid_queue = dm->d_mid_queue;
+  sa.mid_cap = dm->mid_cap;
+  return sa;
 }
 
 void ensure_mid_queue(bie_dmesh* dm, int64_t cap) {
@@ -280,38 +313,18 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     } else {
       const int64_t rowblocks = (M + 63) / 64;
       const int min_segments = (int)std::max<int64_t>(1, std::min<int64_t>(dm->E, (2048 + rowblocks - 1) / rowblocks));
-      ensure_schedule(dm, min_segments, s);
-      SlotArgs sa{};
-      sa.W = kSlotWaves;
-      sa.S = kSlotCount;
-      sa.n_segments = dm->sched_segments;
-      sa.n_rowblocks = (int)rowblocks;
-      sa.n_tiles = rowblocks * dm->sched_segments;
-      {
-        const char* ord = std::getenv("BIE_AMD_TILE_ORDER");
-        sa.seg_major = (ord && std::strcmp(ord, "rowblock") == 0) ? 0 : 1;
-      }
-      sa.phi_in_lds = (sizeof(double) * (size_t)a.R * a.Qmax * a.NP <= 24 * 1024) ? 1 : 0;
-      sa.seg_step_begin = dm->d_seg_step_begin;
-      sa.step_elems = dm->d_step_elems;
-      sa.step_slot = dm->d_step_slot;
-      sa.step_sel = dm->d_step_sel;
-      sa.flush_begin = dm->d_flush_begin;
-      sa.flush_node = dm->d_flush_node;
-      sa.flush_slot = dm->d_flush_slot;
+      const bie_dmesh::Sched& sc = ensure_schedule(dm, 0, kSlotCount, min_segments, s);
       if (dm->pk == 6) {
         ensure_mid_queue(dm, M * dm->E);
         launch_fill_zero_u32(dm->d_mid_count, s);
       } else if (!dm->d_mid_count) {
         dm->d_mid_count = dalloc<unsigned>(1);
       }
-      sa.mid_count = dm->d_mid_count;
-      sa.mid_queue = dm->d_mid_queue;
-      sa.mid_cap = dm->mid_cap;
+      SlotArgs sa = slot_args(dm, sc, rowblocks);
       // The slot kernel stores owned columns once; only shared/untouched columns need
       // zeros (a disabled output is zeroed whole above and never written).
-      if (flags & BIE_F) launch_zero_columns(dF, M, dm->d_zero_cols, dm->sched_zero_cols, s);
-      if (flags & BIE_G) launch_zero_columns(dG, M, dm->d_zero_cols, dm->sched_zero_cols, s);
+      if (flags & BIE_F) launch_zero_columns(dF, M, sc.d_zero_cols, sc.zero_cols, s);
+      if (flags & BIE_G) launch_zero_columns(dG, M, sc.d_zero_cols, sc.zero_cols, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
       launch_assemble_slots(a, sa, s);
       launch_assemble_mid(a, sa, s);
@@ -324,6 +337,65 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     std::lock_guard<std::mutex> tl(g_timing.mu);
     g_timing.rec.push_back(ev);
   }
+}
+
+// Batched sweep over nk real wavenumbers (d_k on the device).  prepare_only builds every
+// cache (rules, schedule, queues) without launching, so that a later call can be
+// captured into a graph.
+void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d_targets,
+                       int64_t M, int n, int m, uint32_t flags, double* dF, double* dG,
+                       int64_t stride, double* dC, hipStream_t s, bool prepare_only) {
+  BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+  BIE_REQUIRE(nk >= 0 && M >= 0, BIE_EINVAL, "negative size");
+  BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
+  BIE_REQUIRE(nk == 0 || d_k, BIE_EINVAL, "k list is NULL");
+  BIE_REQUIRE(dF && dG && dC, BIE_EINVAL, "output pointer is NULL");
+  BIE_REQUIRE(stride >= M * dm->N, BIE_EINVAL, "stride smaller than M * N");
+  if (d_targets == nullptr) {
+    d_targets = dm->d_sources;
+    BIE_REQUIRE(M == dm->N, BIE_EINVAL, "collocation (NULL targets) needs n_targets == n_sources");
+  }
+  BIE_HIP_CHECK(hipSetDevice(dm->device));
+  ensure_rules(dm, n, m, s);
+  const int KB = sweep_kb(dm->pk);
+  BIE_REQUIRE(sizeof(double) * (size_t)dm->rs.R * dm->rs.Qmax * dm->pk <= 24 * 1024,
+              BIE_EUNSUPPORTED, "sweep supports quadrature orders up to n*m' = 36");
+  const bie_dmesh::Sched& sc = ensure_schedule(dm, 1, kSlotCount / KB, 1, s);
+  if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
+  if (dm->pk == 6) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
+  const int nkb = (nk + KB - 1) / KB;
+  if (dm->block_uniform_cap < std::max(nkb, 1)) {
+    dfree(dm->d_block_uniform);
+    dm->d_block_uniform = nullptr;
+    dm->block_uniform_cap = 0;
+    dm->d_block_uniform = dalloc<int>((size_t)std::max(nkb, 1));
+    dm->block_uniform_cap = std::max(nkb, 1);
+  }
+  if (prepare_only) return;
+  launch_fill_zero(dC, 2 * M, s);
+  launch_fill_zero_u32(dm->d_mid_count, s);
+  for (int which = 0; which < 2; ++which) {
+    double* X = which == 0 ? dF : dG;
+    const bool on = (flags & (which == 0 ? BIE_F : BIE_G)) != 0;
+    if (on)
+      launch_zero_columns_multi(X, stride, nk, M, sc.d_zero_cols, sc.zero_cols, s);
+    else
+      for (int q = 0; q < nk; ++q) launch_fill_zero(X + 2 * (size_t)q * stride, 2 * M * dm->N, s);
+  }
+  if (M == 0 || dm->E == 0 || nk == 0) return;
+  launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
+                  dm->d_elem, s);
+  launch_block_uniform(d_k, nk, KB, dm->d_block_uniform, s);
+  const int64_t rowblocks = (M + 63) / 64;
+  AssemblyArgs a = make_args(dm, d_targets, M, 0.0, 0.0, flags, dF, dG, dC);
+  SlotArgs sa = slot_args(dm, sc, rowblocks);
+  SweepArgs sw{};
+  sw.k = d_k;
+  sw.nk = nk;
+  sw.block_uniform = dm->d_block_uniform;
+  sw.stride = stride;
+  sw.n_tiles = (int64_t)nkb * sc.segments * rowblocks;
+  launch_sweep(a, sa, sw, s);
 }
 
 }  // namespace
@@ -401,7 +473,8 @@ int bie_dmesh_destroy(bie_dmesh* dm) {
   dfree(dm->d_pts);
   dfree(dm->d_phi);
   dfree(dm->d_elem);
-  free_schedule(dm);
+  free_schedules(dm);
+  dfree(dm->d_block_uniform);
   dfree(dm->d_mid_count);
   dfree(dm->d_mid_queue);
   delete dm;
@@ -546,6 +619,83 @@ int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64
   return guarded([&] {
     zgemm_device(M, nrhs, K, dA, lda, dX, ldx, dY, ldy, static_cast<hipStream_t>(stream));
   });
+}
+
+int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
+                              const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
+                              uint32_t flags, double* dF, double* dG, int64_t stride,
+                              double* dC, void* stream) {
+  return guarded([&] {
+    BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+    std::lock_guard<std::mutex> lk(dm->mu);
+    sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC,
+                      static_cast<hipStream_t>(stream), false);
+  });
+}
+
+struct bie_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
+                           const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
+                           uint32_t flags, double* dF, double* dG, int64_t stride, double* dC,
+                           bie_graph** out) {
+  return guarded([&] {
+    BIE_REQUIRE(dm != nullptr && out != nullptr, BIE_EINVAL, "NULL argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(dm->mu);
+    BIE_HIP_CHECK(hipSetDevice(dm->device));
+    hipStream_t cs = nullptr;
+    BIE_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    std::unique_ptr<bie_graph> g(new bie_graph);
+    g->device = dm->device;
+    try {
+      // build every cache and set kernel attributes outside the capture
+      sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC, cs,
+                        true);
+      sweep_prepare_attributes(dm->pk, dm->rs.R * dm->rs.Qmax * dm->pk);
+      BIE_HIP_CHECK(hipStreamSynchronize(cs));
+      BIE_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      try {
+        sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC,
+                          cs, false);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(cs, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      BIE_HIP_CHECK(hipStreamEndCapture(cs, &g->graph));
+      BIE_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      (void)hipStreamDestroy(cs);
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      throw;
+    }
+    (void)hipStreamDestroy(cs);
+    *out = g.release();
+  });
+}
+
+int bie_graph_launch(bie_graph* g, void* stream) {
+  return guarded([&] {
+    BIE_REQUIRE(g != nullptr, BIE_EINVAL, "graph is NULL");
+    BIE_HIP_CHECK(hipSetDevice(g->device));
+    BIE_HIP_CHECK(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  });
+}
+
+int bie_graph_destroy(bie_graph* g) {
+  if (!g) return BIE_OK;
+  (void)hipSetDevice(g->device);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return BIE_OK;
 }
 
 int bie_timing_enable(int on) {

@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "bie_common.hpp"
+#include "device_common.hpp"
 #include "kernels_math.hpp"
 #include "schedule.hpp"
 
@@ -74,70 +75,6 @@ __global__ void k_gather_selnodes(const double* __restrict__ sources,
   o[1] = sources[3 * (size_t)node + 1];
   o[2] = sources[3 * (size_t)node + 2];
   o[3] = 0.0;
-}
-
-// Julia Base n-ary hypot for Float64 (maxabs * sqrt(sum((x_i/maxabs)^2)), left to
-// right), with no fused multiply-add so that the result is bit-identical to the CPU.
-__device__ __noinline__ double julia_hypot3(double x, double y, double z) {
-#pragma clang fp contract(off)
-  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
-  if (isinf(ax) || isinf(ay) || isinf(az)) return INFINITY;
-  if (isnan(ax) || isnan(ay) || isnan(az)) return NAN;
-  double mx = ax > ay ? ax : ay;
-  mx = mx > az ? mx : az;
-  if (mx == 0.0) return mx;
-  const double a = ax / mx, b = ay / mx, c = az / mx;
-  double s = a * a;
-  s = s + b * b;
-  s = s + c * c;
-  return mx * sqrt(s);
-}
-
-// 1-based rule index of the node nearest to x (first strict minimum of the Julia hypot
-// distance).  A squared-distance filter decides every pair whose two nearest nodes are
-// separated by more than 1e-12 relative (far beyond the few-ulp error of either
-// distance, so the exact argmin is the same); near ties, zero, inf and NaN distances go
-// through the exact sequence.  Index 0 (no distance below Inf) maps to the last rule,
-// as the reference's if/elseif chain does.
-template <int NP>
-__device__ __forceinline__ int select_rule(double x0, double x1, double x2,
-                                           const double* __restrict__ P, int R) {
-  double d2[NP];
-#pragma unroll
-  for (int s = 0; s < NP; ++s) {
-    const double dx = P[4 * s] - x0, dy = P[4 * s + 1] - x1, dz = P[4 * s + 2] - x2;
-    d2[s] = fma(dx, dx, fma(dy, dy, dz * dz));
-  }
-  int jb = 0;
-  double b = d2[0];
-#pragma unroll
-  for (int s = 1; s < NP; ++s)
-    if (d2[s] < b) {
-      b = d2[s];
-      jb = s;
-    }
-  const double lim = b * (1.0 + 1e-12);
-  bool ok = (b <= 1.0e300);
-#pragma unroll
-  for (int s = 0; s < NP; ++s)
-    if (s != jb && !(d2[s] > lim)) ok = false;
-  int c;
-  if (ok) {
-    c = jb + 1;
-  } else {
-    double best = INFINITY;
-    int idx = 0;
-    for (int s = 0; s < NP; ++s) {
-      const double d = julia_hypot3(P[4 * s] - x0, P[4 * s + 1] - x1, P[4 * s + 2] - x2);
-      if (d < best) {
-        best = d;
-        idx = s + 1;
-      }
-    }
-    c = idx;
-  }
-  if (c < 1 || c > R) c = R;
-  return c;
 }
 
 template <int NP>
@@ -247,15 +184,6 @@ __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double
     B2 = *reinterpret_cast<const double4*>(tab + 8 * q3 + 4);
   }
   if (q < Q) point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
-}
-
-__device__ __forceinline__ void load_rows01(const double* __restrict__ tab, int Q, double4& A,
-                                            double4& B, double4& A2, double4& B2) {
-  A = *reinterpret_cast<const double4*>(tab);
-  B = *reinterpret_cast<const double4*>(tab + 4);
-  const int q1 = Q > 1 ? 1 : 0;
-  A2 = *reinterpret_cast<const double4*>(tab + 8 * q1);
-  B2 = *reinterpret_cast<const double4*>(tab + 8 * q1 + 4);
 }
 
 // One thread per target row; blockIdx.y selects a contiguous chunk of elements.

@@ -102,6 +102,15 @@ struct SlotArgs {
   int64_t mid_cap;
 };
 
+// Batched frequency sweep (sweep.hip).
+struct SweepArgs {
+  const double* k;           // nk real wavenumbers (device)
+  int nk;
+  const int* block_uniform;  // per k block: 1 when its k are uniformly spaced
+  int64_t stride;            // complex elements between consecutive F_k (and G_k)
+  int64_t n_tiles;           // k blocks x segments x row blocks
+};
+
 // Launchers (assemble.hip)
 void launch_fill_zero(double* p, int64_t n_doubles, hipStream_t s);
 void launch_fill_zero_u32(unsigned* p, hipStream_t s);
@@ -115,6 +124,13 @@ void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t 
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
+void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
+                               int ncols, hipStream_t s);
+int sweep_kb(int NP);
+void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw,
+                  hipStream_t s);
+void sweep_prepare_attributes(int NP, int nphi);
+void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,
                    int R, int8_t* out, hipStream_t s);
 
@@ -139,19 +155,21 @@ struct bie_dmesh {
   // host copies for scheduling
   std::vector<double> h_coords, h_sources;
   std::vector<int> h_topo, h_ptopo;
-  // slot schedule cache (mesh-only; keyed by the requested minimum segment count)
-  int sched_key = -1;
-  int sched_segments = 0;
-  int sched_zero_cols = 0;
-  int sched_max_live = 0;
-  int* d_seg_step_begin = nullptr;
-  int* d_step_elems = nullptr;
-  int* d_step_slot = nullptr;
-  double* d_step_sel = nullptr;
-  int* d_flush_begin = nullptr;
-  int* d_flush_node = nullptr;
-  int* d_flush_slot = nullptr;
-  int* d_zero_cols = nullptr;
+  // slot schedules (mesh-only): [0] single-k kernel, [1] sweep kernel
+  struct Sched {
+    int S = -1, key = -1;
+    int segments = 0, zero_cols = 0, max_live = 0;
+    int* d_seg_step_begin = nullptr;
+    int* d_step_elems = nullptr;
+    int* d_step_slot = nullptr;
+    double* d_step_sel = nullptr;
+    int* d_flush_begin = nullptr;
+    int* d_flush_node = nullptr;
+    int* d_flush_slot = nullptr;
+    int* d_zero_cols = nullptr;
+  } sched[2];
+  int* d_block_uniform = nullptr;
+  int block_uniform_cap = 0;
   // mid-edge pair queue
   unsigned* d_mid_count = nullptr;
   int2* d_mid_queue = nullptr;

@@ -1,0 +1,385 @@
+// Batched frequency sweep: dense F_k, G_k for a list of real wavenumbers k_0..k_{nk-1}
+// on one mesh, in one launch per kernel (graph-capturable: no host synchronisation, no
+// memset nodes).
+//
+// Each wave carries KB wavenumbers at once.  Everything that does not depend on k is
+// shared between them: rule selection (find_closest_corner,
+// src/3d/assembly_collocation.jl:102-115), the table loads, d = y - x, 1/r, n.d, the
+// C integrand (C is k-independent, kernels.jl:48-55, and is formed once by the first
+// k block).  For a block of uniformly spaced k, e^{i k_b r} = e^{i k_0 r} (e^{i dk r})^b
+// replaces KB sincos evaluations by two plus KB-1 complex products; other blocks evaluate
+// each e^{i k_b r} directly, so any list of k is exact.  Accumulation uses the LDS-slot
+// schedule of the single-k kernel with KB accumulators per slot (the schedule is built
+// for kSlotCount / KB live slots).  The reference has no batched path: it calls
+// assemble_parallel! once per k (e.g. the Taylor sweep of src/3d/rosebem.jl:56-157
+// shares geometry across derivative orders in the same way).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "bie_common.hpp"
+#include "device_common.hpp"
+#include "kernels_math.hpp"
+#include "schedule.hpp"
+
+namespace bie {
+namespace {
+
+template <int NP, int KB>
+__device__ __forceinline__ void sweep_point(double x0, double x1, double x2, const double4 A,
+                                            const double4 B, const double* __restrict__ p,
+                                            const double* kv, double dk, bool rec,
+                                            double (&fr)[KB][NP], double (&fi)[KB][NP],
+                                            double (&gr)[KB][NP], double (&gi)[KB][NP],
+                                            double& csum) {
+  const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rinv = rsqrt_nr(r2);
+  const double r = r2 * rinv;
+  const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
+  const double g = B.z * rinv;
+  const double ak = nd * rinv * rinv * g;
+  csum -= ak;
+  double sn, cs, zs = 0.0, zc = 1.0;
+  sincos_f64(kv[0] * r, &sn, &cs);
+  if (rec) sincos_f64(dk * r, &zs, &zc);
+  double ps[NP];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) ps[s] = p[s];
+#pragma unroll
+  for (int b = 0; b < KB; ++b) {
+    if (b > 0) {
+      if (rec) {
+        const double c2 = fma(cs, zc, -sn * zs);
+        sn = fma(cs, zs, sn * zc);
+        cs = c2;
+      } else {
+        sincos_f64(kv[b] * r, &sn, &cs);
+      }
+    }
+    const double krr = kv[b] * r;
+    const double Fr = -ak * fma(krr, sn, cs);
+    const double Fi = ak * fma(krr, cs, -sn);
+    const double Gr = g * cs, Gi = g * sn;
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      fr[b][s] = fma(Fr, ps[s], fr[b][s]);
+      fi[b][s] = fma(Fi, ps[s], fi[b][s]);
+      gr[b][s] = fma(Gr, ps[s], gr[b][s]);
+      gi[b][s] = fma(Gi, ps[s], gi[b][s]);
+    }
+  }
+}
+
+template <int NP, int KB, int W>
+__global__ void __launch_bounds__(64 * W)
+k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
+  extern __shared__ double lds[];
+  double* acc = lds;  // [slot][b][4][64]
+  const int nphi = a.R * a.Qmax * NP;
+  double* s_phi = lds + (size_t)sa.S * KB * 256;
+  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
+  const int b = blockIdx.x;
+  const int64_t per_xcd = (sw.n_tiles + 7) / 8;
+  const int64_t tile = (int64_t)(b & 7) * per_xcd + (b >> 3);
+  if ((b >> 3) >= per_xcd || tile >= sw.n_tiles) return;
+  // k block outermost, then segment, then row block
+  const int64_t per_kb = (int64_t)sa.n_segments * sa.n_rowblocks;
+  const int kb = (int)(tile / per_kb);
+  const int64_t rest = tile - (int64_t)kb * per_kb;
+  const int seg = (int)(rest / sa.n_rowblocks);
+  const int rb = (int)(rest - (int64_t)seg * sa.n_rowblocks);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t i = (int64_t)rb * 64 + lane;
+  const bool active = i < a.M;
+  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
+  for (int t = threadIdx.x; t < sa.S * KB * 256; t += blockDim.x) acc[t] = 0.0;
+  if (threadIdx.x == 0) *s_qn = 0u;
+  __syncthreads();
+  const int64_t ic = active ? i : (a.M - 1);
+  const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  const bool first_kb = kb == 0;
+  double kv[KB];
+  const int k0 = kb * KB;
+#pragma unroll
+  for (int q = 0; q < KB; ++q) kv[q] = sw.k[min(k0 + q, sw.nk - 1)];
+  const bool rec = sw.block_uniform[kb] != 0;
+  const double dk = KB > 1 ? kv[1] - kv[0] : 0.0;
+  double csum = 0.0;
+  const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
+  int e = (st0 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st0 * W + wave]) : -1;
+  int c = 0;
+  double4 A0 = make_double4(0, 0, 0, 0), B0 = A0;
+  if (e >= 0) {
+    c = select_rule<NP>(x0, x1, x2, sa.step_sel + ((size_t)st0 * W + wave) * NP * 4, a.R) - 1;
+    if (!(NP == 6 && c >= 3)) {
+      const double* t0 = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
+      A0 = *reinterpret_cast<const double4*>(t0);
+      B0 = *reinterpret_cast<const double4*>(t0 + 4);
+    }
+  }
+  for (int st = st0; st < st1; ++st) {
+    const size_t nidx = (size_t)(st + 1) * W + wave;
+    const int e_nxt =
+        (st + 1 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[nidx]) : -1;
+    if (e >= 0) {
+      const bool mid = NP == 6 && c >= 3;
+      if (active && !mid) {
+        double fr[KB][NP], fi[KB][NP], gr[KB][NP], gi[KB][NP];
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+#pragma unroll
+          for (int s = 0; s < NP; ++s) fr[q][s] = fi[q][s] = gr[q][s] = gi[q][s] = 0.0;
+        const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
+        const double* ph = s_phi + c * a.Qmax * NP;
+        const int Q = a.len[c];
+        double cs_local = 0.0;
+        // point 0 was fetched before the barrier; later rows are loaded in turn
+        sweep_point<NP, KB>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi, cs_local);
+        for (int q = 1; q < Q; ++q) {
+          const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
+          const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+          sweep_point<NP, KB>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, fr, fi, gr, gi,
+                              cs_local);
+        }
+        csum += cs_local;
+        const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
+#pragma unroll
+        for (int s = 0; s < NP; ++s)
+#pragma unroll
+          for (int q2 = 0; q2 < KB; ++q2) {
+            double* p = acc + ((size_t)so[s] * KB + q2) * 256 + lane;
+            atomicAdd(p, fr[q2][s]);
+            atomicAdd(p + 64, fi[q2][s]);
+            atomicAdd(p + 128, gr[q2][s]);
+            atomicAdd(p + 192, gi[q2][s]);
+          }
+      }
+      if (NP == 6 && first_kb) {  // mid-edge pairs are queued once, for every k
+        const bool push = mid && active;
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+          const int leader = __ffsll((long long)mask) - 1;
+          unsigned base = 0;
+          if (lane == leader) base = atomicAdd(s_qn, (unsigned)__popcll(mask));
+          base = __shfl(base, leader);
+          if (push) {
+            const unsigned pos = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            const int2 entry = make_int2((int)i, e | ((c - 3) << 28));
+            if (pos < (unsigned)kMidLds) {
+              s_q[pos] = entry;
+            } else {
+              const unsigned g = atomicAdd(sa.mid_count, 1u);
+              if ((int64_t)g < sa.mid_cap) sa.mid_queue[g] = entry;
+            }
+          }
+        }
+      }
+    }
+    int c_nxt = 0;
+    if (e_nxt >= 0) {
+      c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
+      if (!(NP == 6 && c_nxt >= 3)) {
+        const double* t0 = a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8;
+        A0 = *reinterpret_cast<const double4*>(t0);
+        B0 = *reinterpret_cast<const double4*>(t0 + 4);
+      }
+    }
+    __syncthreads();
+    const int f1 = sa.flush_begin[st + 1];
+    for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
+      const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
+      const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
+#pragma unroll
+      for (int q2 = 0; q2 < KB; ++q2) {
+        double* p = acc + ((size_t)(ss & 0xffff) * KB + q2) * 256 + lane;
+        const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
+        p[0] = 0.0;
+        p[64] = 0.0;
+        p[128] = 0.0;
+        p[192] = 0.0;
+        const int kidx = k0 + q2;
+        if (active && kidx < sw.nk) {
+          const size_t idx = 2 * ((size_t)kidx * sw.stride + (size_t)i + (size_t)node * a.M);
+          if (ss & kSharedBit) {
+            if (doF) {
+              unsafeAtomicAdd(a.F + idx, vr);
+              unsafeAtomicAdd(a.F + idx + 1, vi);
+            }
+            if (doG) {
+              unsafeAtomicAdd(a.G + idx, wr);
+              unsafeAtomicAdd(a.G + idx + 1, wi);
+            }
+          } else {
+            if (doF) *reinterpret_cast<double2*>(a.F + idx) = make_double2(vr, vi);
+            if (doG) *reinterpret_cast<double2*>(a.G + idx) = make_double2(wr, wi);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    e = e_nxt;
+    c = c_nxt;
+  }
+  if (NP == 6 && first_kb && wave == 0) {
+    const unsigned n = min(*s_qn, (unsigned)kMidLds);
+    if (n) {
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(sa.mid_count, n);
+      base = __shfl(base, 0);
+      for (unsigned t = lane; t < n; t += 64)
+        if ((int64_t)(base + t) < sa.mid_cap) sa.mid_queue[base + t] = s_q[t];
+    }
+  }
+  if (first_kb && active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
+}
+
+// Mid-edge pairs for every k: one thread per (pair, k).
+__global__ void __launch_bounds__(256)
+k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
+  const int64_t n = min((int64_t)*sa.mid_count, sa.mid_cap);
+  const int64_t total = n * sw.nk;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += stride) {
+    const int kidx = (int)(t / n);
+    const int2 en = sa.mid_queue[t - (int64_t)kidx * n];
+    const int64_t i = en.x;
+    const int e = en.y & 0x0fffffff;
+    const int c = 3 + (en.y >> 28);
+    const double x0 = a.targets[3 * i], x1 = a.targets[3 * i + 1], x2 = a.targets[3 * i + 2];
+    double fr[1][6], fi[1][6], gr[1][6], gi[1][6], csum = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) fr[0][s] = fi[0][s] = gr[0][s] = gi[0][s] = 0.0;
+    const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
+    const double* ph = a.phi + c * a.Qmax * 6;
+    const double kk = sw.k[kidx];
+    for (int q = 0; q < a.len[c]; ++q) {
+      const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
+      const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+      sweep_point<6, 1>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi, csum);
+    }
+    const int* cols = a.ptopo + (size_t)e * 6;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const size_t idx = 2 * ((size_t)kidx * sw.stride + (size_t)i + (size_t)cols[s] * a.M);
+      if (doF) {
+        unsafeAtomicAdd(a.F + idx, fr[0][s]);
+        unsafeAtomicAdd(a.F + idx + 1, fi[0][s]);
+      }
+      if (doG) {
+        unsafeAtomicAdd(a.G + idx, gr[0][s]);
+        unsafeAtomicAdd(a.G + idx + 1, gi[0][s]);
+      }
+    }
+    if (kidx == 0 && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
+  }
+}
+
+// Zero listed columns of nk column-major M x N complex matrices spaced stride apart.
+__global__ void k_zero_columns_multi(double2* __restrict__ X, int64_t stride, int nk, int64_t M,
+                                     const int* __restrict__ cols, int ncols) {
+  const int64_t per = (int64_t)ncols * M;
+  const int64_t total = per * nk;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += step) {
+    const int64_t kq = t / per;
+    const int64_t u = t - kq * per;
+    const int64_t c = u / M;
+    const int64_t r = u - c * M;
+    X[kq * stride + (size_t)cols[c] * M + r] = make_double2(0.0, 0.0);
+  }
+}
+
+}  // namespace
+
+void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
+                               int ncols, hipStream_t s) {
+  if (ncols <= 0 || M <= 0 || nk <= 0) return;
+  int64_t blocks = ((int64_t)ncols * M * nk + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_zero_columns_multi, dim3((unsigned)blocks), dim3(256), 0, s,
+                     reinterpret_cast<double2*>(X), stride, nk, M, cols, ncols);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+int sweep_kb(int NP) { return NP == 3 ? 4 : 2; }
+
+namespace {
+
+__global__ void k_block_uniform(const double* __restrict__ k, int nk, int KB, int* out) {
+  const int kb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kb * KB >= nk) return;
+  const double k0 = k[kb * KB];
+  const double dk = (kb * KB + 1 < nk) ? k[kb * KB + 1] - k0 : 0.0;
+  int ok = (kb * KB + KB <= nk) ? 1 : 0;  // a partial block evaluates directly
+  for (int b = 2; b < KB && ok; ++b) {
+    const double kb_ = k[kb * KB + b];
+    // the recurrence produces e^{i (k0 + b dk) r}; accept it when it differs from
+    // e^{i k_b r} by at most ~2 ulp of k_b r (a rounded linspace passes)
+    if (fabs(kb_ - (k0 + b * dk)) > 4.5e-16 * fabs(kb_)) ok = 0;
+  }
+  out[kb] = ok;
+}
+
+size_t g_sweep_attr[2] = {0, 0};
+
+}  // namespace
+
+void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s) {
+  const int nkb = (nk + KB - 1) / KB;
+  if (nkb <= 0) return;
+  hipLaunchKernelGGL(k_block_uniform, dim3((nkb + 63) / 64), dim3(64), 0, s, k, nk, KB, out);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+size_t sweep_shmem(int NP, int nphi) {
+  const int KB = sweep_kb(NP);
+  return sizeof(double) * ((size_t)(kSlotCount / KB) * KB * 256 + (size_t)nphi) +
+         sizeof(int2) * kMidLds + 16;
+}
+
+void sweep_prepare_attributes(int NP, int nphi) {
+  const size_t shmem = sweep_shmem(NP, nphi);
+  constexpr int W = kSlotWaves;
+  const void* kern = NP == 3 ? reinterpret_cast<const void*>(k_sweep_slots<3, 4, W>)
+                             : reinterpret_cast<const void*>(k_sweep_slots<6, 2, W>);
+  size_t& done = g_sweep_attr[NP == 3 ? 0 : 1];
+  if (shmem > done) {
+    BIE_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)std::max(shmem, (size_t)(144 * 1024))));
+    done = std::max(shmem, (size_t)(144 * 1024));
+  }
+}
+
+void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw,
+                  hipStream_t s) {
+  if (a.M == 0 || a.E == 0 || sw.nk == 0) return;
+  const int KB = sweep_kb(a.NP);
+  BIE_REQUIRE(sa.W == kSlotWaves && sa.S * KB <= kSlotCount, BIE_EINVAL,
+              "sweep schedule does not match the kernel");
+  const int64_t per_xcd = (sw.n_tiles + 7) / 8;
+  BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many sweep tiles");
+  const size_t shmem = sizeof(double) * ((size_t)sa.S * KB * 256 + (size_t)a.R * a.Qmax * a.NP) +
+                       sizeof(int2) * kMidLds + 16;
+  BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "sweep kernel LDS request exceeds 160 KiB");
+  sweep_prepare_attributes(a.NP, a.R * a.Qmax * a.NP);
+  constexpr int W = kSlotWaves;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * 8)), dim3(64 * W), shmem, s, a, sa, sw);
+  };
+  if (a.NP == 3)
+    go(k_sweep_slots<3, 4, W>);
+  else
+    go(k_sweep_slots<6, 2, W>);
+  BIE_HIP_CHECK(hipGetLastError());
+  if (a.NP == 6) {
+    hipLaunchKernelGGL(k_sweep_mid, dim3(4096), dim3(256), 0, s, a, sa, sw);
+    BIE_HIP_CHECK(hipGetLastError());
+  }
+}
+
+}  // namespace bie

@@ -48,6 +48,7 @@ typedef struct bie_mesh {
 } bie_mesh;
 
 typedef struct bie_dmesh bie_dmesh;  /* device-resident mesh + cached rule tables */
+typedef struct bie_graph bie_graph;  /* an instantiated HIP graph of one batched assembly */
 
 const char* bie_last_error(void);
 int bie_version(void);
@@ -95,6 +96,27 @@ int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, doub
  * reference's threads own whole rows, assembly_collocation.jl:314).  stats (6 values):
  * segments, steps, max live slots, shared nodes, zeroed columns, flush entries. */
 int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stats);
+
+/* ---- Batched frequency sweep ------------------------------------------------------------
+ * F_q, G_q for nk real wavenumbers d_k[q] (device array) on one device mesh, equal to nk
+ * calls of bie_assemble_device(k = d_k[q]) (each the reference's assemble_parallel!,
+ * assembly_collocation.jl:259-479, which a frequency sweep calls once per k).  F_q starts
+ * at dF + 2*q*stride doubles (stride >= n_targets * n_sources complex elements); likewise
+ * G_q.  C is k-independent and written once to dC.  Everything k-independent (rule
+ * selection, geometry, table loads, 1/r, n.d) is shared by the wavenumbers of a block, and
+ * uniformly spaced blocks replace sincos by a recurrence.  Asynchronous on stream. */
+int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
+                              const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
+                              uint32_t flags, double* dF, double* dG, int64_t stride,
+                              double* dC, void* stream);
+/* The same sweep captured once into a HIP graph (kernels only, no memset nodes); launching
+ * the graph replays it on any stream.  The graph keeps the pointers it was built with. */
+int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
+                           const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
+                           uint32_t flags, double* dF, double* dG, int64_t stride, double* dC,
+                           bie_graph** out);
+int bie_graph_launch(bie_graph* g, void* stream);
+int bie_graph_destroy(bie_graph* g);
 
 /* ---- Dense operator apply --------------------------------------------------------------
  * Y = A X for complex128 column-major A (M x K, leading dimension lda), X (K x nrhs, ldx),
