@@ -81,6 +81,63 @@ def measure_apply(torch, F, N, steps, stream, nrhs=32):
     return res
 
 
+def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
+    """Config 3: 4002-node linear sphere, 64 wavenumbers uniform in [0.5, 30], (n,m)=(4,4).
+    Time of 64 single-k device assemblies vs one replay of the captured batched sweep,
+    each writing all 64 (F_k, G_k) pairs; parity between the two on every k."""
+    mesh = bie_amd.icosphere(f, "linear")
+    N = mesh.n_sources
+    dm = bie_amd.DeviceMesh(mesh, 0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    ks = torch.linspace(0.5, 30.0, nk, dtype=torch.float64)
+    kd = ks.to(dev)
+    stride = N * N
+    F = torch.empty(nk * 2 * stride, dtype=torch.float64, device=dev)
+    G = torch.empty_like(F)
+    C = torch.empty(2 * N, dtype=torch.float64, device=dev)
+    Fp = torch.empty_like(F)
+    Gp = torch.empty_like(F)
+
+    def plain():
+        for q in range(nk):
+            dm.assemble(float(ks[q]), Fp.data_ptr() + 16 * q * stride,
+                        Gp.data_ptr() + 16 * q * stride, C.data_ptr(), n=n, m=m, stream=stream)
+
+    g = dm.sweep_graph(kd.data_ptr(), nk, F.data_ptr(), G.data_ptr(), stride, C.data_ptr(),
+                       n=n, m=m)
+
+    def batched():
+        g.launch(stream)
+
+    res = {}
+    for name, fn in (("per_k_launches", plain), ("batched_graph", batched)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name + "_ms"] = e0.elapsed_time(e1) / steps
+    err = 0.0
+    for q in range(nk):
+        sl = slice(2 * q * stride, 2 * (q + 1) * stride)
+        err = max(err, float(torch.linalg.norm(F[sl] - Fp[sl]) / torch.linalg.norm(Fp[sl])),
+                  float(torch.linalg.norm(G[sl] - Gp[sl]) / torch.linalg.norm(Gp[sl])))
+    g.close()
+    res["speedup"] = res["per_k_launches_ms"] / res["batched_graph_ms"]
+    res["max_rel_err_batched_vs_per_k"] = err
+    res["workload"] = (f"linear geodesic sphere N={N}, E={mesh.n_elements}, {nk} k uniform in "
+                       f"[0.5, 30], (n,m)=({n},{m}), F_k and G_k for every k")
+    res["entries_per_s_batched"] = nk * 2 * N * N / (res["batched_graph_ms"] * 1e-3)
+    del F, G, Fp, Gp
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +224,13 @@ def main():
 
     if not args.no_extra:
         extra["apply"] = measure_apply(torch, F, N, args.steps, stream)
+
+    if not args.no_extra:
+        del G
+        torch.cuda.empty_cache()
+        extra["sweep"] = measure_sweep(torch, bie_amd)
+        G = torch.empty(2 * N * N, dtype=torch.float64, device=dev)
+        run(n, m, 1, 0)
 
     cpu = None
     if not args.no_cpu:

@@ -262,6 +262,32 @@ def load_comsol(path: str, geometry_order: str = "quadratic",
     raise ValueError("physics_order must be 'geometry' or 'linear'")
 
 
+def with_coordinates(mesh: Mesh3d, coords) -> Mesh3d:
+    """The same mesh with new coordinates; collocation nodes that are coordinates move
+    with them (physics = geometry, or the corner nodes for linear physics)."""
+    coords = np.asarray(coords, dtype=np.float64)
+    lut = {tuple(c): j for j, c in enumerate(mesh.coordinates.T)}
+    idx = np.array([lut[tuple(s)] for s in mesh.sources.T], dtype=np.int64)
+    return Mesh3d(coords, mesh.topology, coords[:, idx].copy(), mesh.physics_topology,
+                  mesh.geometry_kind, mesh.physics_kind)
+
+
+def perturb_radial(mesh: Mesh3d, sigma: float, seed: int) -> Mesh3d:
+    """Scale every coordinate by (1 + eps), eps ~ N(0, sigma^2) per node (seeded)."""
+    rng = np.random.default_rng(seed)
+    eps = rng.normal(0.0, sigma, mesh.n_coordinates)
+    return with_coordinates(mesh, mesh.coordinates * (1.0 + eps))
+
+
+def fibonacci_sphere(n: int, radius: float = 1.0) -> np.ndarray:
+    """n nearly uniform points on a sphere (Fibonacci lattice), 3 x n."""
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    phi = np.pi * (1.0 + 5.0 ** 0.5) * i
+    rho = np.sqrt(1.0 - z * z)
+    return radius * np.vstack([rho * np.cos(phi), rho * np.sin(phi), z])
+
+
 def golden_dir() -> str:
     here = os.path.dirname(os.path.abspath(__file__))
     return os.path.normpath(os.path.join(here, "..", "..", "tests", "golden"))

@@ -1,0 +1,182 @@
+// Vector-Jacobian product of the dense collocation assembly (gfx950, FP64):
+//
+//   grad_X, grad_k of  Phi = Re[ sum_i alpha_i sum_j F_ij(X, k) p_j + sum_i beta_i C_i(X) ]
+//
+// for complex weights alpha (per target), beta (per target) and p (per collocation node),
+// with the rule selection held fixed (it is piecewise constant in X).  X are the mesh
+// coordinates (3 x Nc); targets that are mesh nodes (tvert[i] >= 0) move with them, field
+// targets stay fixed.  This is the building block of the adjoint gradient of objectives
+// that go through F, C and a solve; the reference has no counterpart (its shape
+// optimisation example differentiates by finite differences,
+// examples/2d_optimization.jl:184-216).
+//
+// Quadrature is recomputed per pair instead of stored: at each rule point
+// y = X N, t_u = X dN/du, t_v = X dN/dv, c = t_u x t_v.  Since the reference's normal is
+// n = -c/|c| and J = |c|, the integrands times J are
+//   f J w = -(c.d) w h(r),   h = e^{ikr}(ikr - 1)/(4 pi r^3)      (kernels.jl:29-37)
+//   k0 J w = (c.d) w / (4 pi r^3)                                 (kernels.jl:48-55)
+// with d = y - x, so Phi's integrand is (c.d) H with H = -A1 h + A2/(4 pi r^3),
+// A1 = alpha p_h w (p_h = physics-basis interpolant of p), A2 = beta w, and
+//   dPsi/dd = H c + (c.d) H_r d / r,  dPsi/dc = H d,  dPsi/dk = (c.d) A1 k e^{ikr}/(4 pi r),
+//   H_r = -A1 h' + A2 (-3/(4 pi r^4)),  h' = -k^2 e^{ikr}/(4 pi r^2) - 3 h / r.
+#include <hip/hip_runtime.h>
+
+#include "bie_common.hpp"
+#include "device_common.hpp"
+#include "kernels_math.hpp"
+
+namespace bie {
+namespace {
+
+constexpr double kInv4Pi = 0.079577471545947667884;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int NG, int NP>
+__global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool active = i < a.M;
+  const int64_t ic = active ? i : (a.M - 1);
+  const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
+  double2 al = active ? a.alpha[ic] : make_double2(0.0, 0.0);
+  double2 be = (active && a.beta) ? a.beta[ic] : make_double2(0.0, 0.0);
+  const double k = a.k;
+  double gx0 = 0, gx1 = 0, gx2 = 0, gk = 0;
+  const int e0 = blockIdx.y * ech;
+  const int e1 = min(a.E, e0 + ech);
+  for (int e = e0; e < e1; ++e) {
+    const int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
+    double X[NG][3];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      const int node = a.topo[(size_t)e * NG + q];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) X[q][d] = a.coords[3 * (size_t)node + d];
+    }
+    double pr[NP], pim[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      const double2 v = a.p[a.ptopo[(size_t)e * NP + s]];
+      pr[s] = v.x;
+      pim[s] = v.y;
+    }
+    double gX[NG][3];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) gX[q][0] = gX[q][1] = gX[q][2] = 0.0;
+    const int Q = a.len[c];
+    const double* P0 = a.pts + (size_t)a.off[c] * kPtStride;
+    const double* ph0 = a.phi + (size_t)c * a.Qmax * NP;
+    for (int q = 0; q < Q; ++q) {
+      const double* P = P0 + (size_t)q * kPtStride;
+      const double w = P[2];
+      double y[3] = {0, 0, 0}, tu[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const double N = P[3 + g], du = P[9 + g], dv = P[15 + g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          y[d] = fma(X[g][d], N, y[d]);
+          tu[d] = fma(X[g][d], du, tu[d]);
+          tv[d] = fma(X[g][d], dv, tv[d]);
+        }
+      }
+      const double cv[3] = {tu[1] * tv[2] - tu[2] * tv[1], tu[2] * tv[0] - tu[0] * tv[2],
+                            tu[0] * tv[1] - tu[1] * tv[0]};
+      const double d[3] = {y[0] - x0, y[1] - x1, y[2] - x2};
+      const double r2 = fma(d[0], d[0], fma(d[1], d[1], d[2] * d[2]));
+      const double rinv = rsqrt_nr(r2);
+      const double r = r2 * rinv;
+      const double cd = fma(cv[0], d[0], fma(cv[1], d[1], cv[2] * d[2]));
+      // p interpolated at the point, times alpha and w
+      double phr = 0, phi_ = 0;
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        const double b = ph0[q * NP + s];
+        phr = fma(b, pr[s], phr);
+        phi_ = fma(b, pim[s], phi_);
+      }
+      const double A1r = w * (al.x * phr - al.y * phi_), A1i = w * (al.x * phi_ + al.y * phr);
+      const double A2r = w * be.x;
+      double sn, cs;
+      sincos_f64(k * r, &sn, &cs);
+      const double i4r = kInv4Pi * rinv;            // 1/(4 pi r)
+      const double i4r3 = i4r * rinv * rinv;        // 1/(4 pi r^3)
+      const double kr = k * r;
+      const double hr = (-cs - kr * sn) * i4r3, hi = (kr * cs - sn) * i4r3;
+      const double k2i4r2 = k * k * i4r * rinv;     // k^2/(4 pi r^2)
+      const double hpr = -k2i4r2 * cs - 3.0 * hr * rinv, hpi = -k2i4r2 * sn - 3.0 * hi * rinv;
+      // Re(H) = -Re(A1 h) + A2r/(4 pi r^3);  Re(H_r) = -Re(A1 h') - 3 A2r/(4 pi r^4)
+      const double ReH = -(A1r * hr - A1i * hi) + A2r * i4r3;
+      const double ReHr = -(A1r * hpr - A1i * hpi) - 3.0 * A2r * i4r3 * rinv;
+      // dPsi/dk = cd * Re(A1 k e^{ikr}) / (4 pi r)
+      gk += cd * k * i4r * (A1r * cs - A1i * sn);
+      const double t = ReHr * cd * rinv;
+      const double gd[3] = {fma(t, d[0], ReH * cv[0]), fma(t, d[1], ReH * cv[1]),
+                            fma(t, d[2], ReH * cv[2])};
+      const double gc[3] = {ReH * d[0], ReH * d[1], ReH * d[2]};
+      gx0 -= gd[0];
+      gx1 -= gd[1];
+      gx2 -= gd[2];
+      // d(g.(t_u x t_v))/dt_u = t_v x g,  /dt_v = g x t_u
+      const double gtu[3] = {tv[1] * gc[2] - tv[2] * gc[1], tv[2] * gc[0] - tv[0] * gc[2],
+                             tv[0] * gc[1] - tv[1] * gc[0]};
+      const double gtv[3] = {gc[1] * tu[2] - gc[2] * tu[1], gc[2] * tu[0] - gc[0] * tu[2],
+                             gc[0] * tu[1] - gc[1] * tu[0]};
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const double N = P[3 + g], du = P[9 + g], dv = P[15 + g];
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd)
+          gX[g][dd] = fma(N, gd[dd], fma(du, gtu[dd], fma(dv, gtv[dd], gX[g][dd])));
+      }
+    }
+    // the wave shares element e: sum its node gradients over the lanes, one atomic each
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const int node = a.topo[(size_t)e * NG + g];
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) {
+        const double v = wave_sum(gX[g][dd]);
+        if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(a.gradX + 3 * (size_t)node + dd, v);
+      }
+    }
+  }
+  if (active && a.tvert) {
+    const int v = a.tvert[i];
+    if (v >= 0) {
+      unsafeAtomicAdd(a.gradX + 3 * (size_t)v, gx0);
+      unsafeAtomicAdd(a.gradX + 3 * (size_t)v + 1, gx1);
+      unsafeAtomicAdd(a.gradX + 3 * (size_t)v + 2, gx2);
+    }
+  }
+  const double gks = wave_sum(gk);
+  if ((threadIdx.x & 63) == 0 && a.gradk) unsafeAtomicAdd(a.gradk, gks);
+}
+
+}  // namespace
+
+void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s) {
+  if (a.M == 0 || a.E == 0) return;
+  const int64_t rowblocks = (a.M + 255) / 256;
+  BIE_REQUIRE(rowblocks < 2147483647LL, BIE_EINVAL, "too many targets");
+  int64_t chunks = (4096 + rowblocks - 1) / rowblocks;
+  if (chunks > a.E) chunks = a.E;
+  if (chunks > 65535) chunks = 65535;
+  if (chunks < 1) chunks = 1;
+  const int ech = (int)((a.E + chunks - 1) / chunks);
+  chunks = (a.E + ech - 1) / ech;
+  const dim3 grid((unsigned)rowblocks, (unsigned)chunks);
+  if (gk == 3 && pk == 3)
+    hipLaunchKernelGGL((k_adjoint<3, 3>), grid, dim3(256), 0, s, a, ech);
+  else if (gk == 6 && pk == 6)
+    hipLaunchKernelGGL((k_adjoint<6, 6>), grid, dim3(256), 0, s, a, ech);
+  else
+    hipLaunchKernelGGL((k_adjoint<6, 3>), grid, dim3(256), 0, s, a, ech);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace bie

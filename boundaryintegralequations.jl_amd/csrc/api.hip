@@ -614,6 +614,50 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
   });
 }
 
+int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, int64_t n_targets,
+                            const int32_t* d_target_node, const double* d_alpha,
+                            const double* d_beta, const double* d_p, int32_t n, int32_t m,
+                            double* d_grad_coords, double* d_grad_k, void* stream) {
+  return guarded([&] {
+    BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+    BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
+    BIE_REQUIRE(d_alpha && d_p && d_grad_coords, BIE_EINVAL, "NULL argument");
+    BIE_REQUIRE(std::isfinite(k), BIE_EINVAL, "k must be finite");
+    if (!d_targets) {
+      d_targets = dm->d_sources;
+      BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
+    }
+    std::lock_guard<std::mutex> lk(dm->mu);
+    BIE_HIP_CHECK(hipSetDevice(dm->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ensure_rules(dm, n, m, s);
+    AdjointArgs a{};
+    a.targets = d_targets;
+    a.M = n_targets;
+    a.tvert = reinterpret_cast<const int*>(d_target_node);
+    a.alpha = reinterpret_cast<const double2*>(d_alpha);
+    a.beta = reinterpret_cast<const double2*>(d_beta);
+    a.p = reinterpret_cast<const double2*>(d_p);
+    a.coords = dm->d_coords;
+    a.topo = dm->d_topo;
+    a.ptopo = dm->d_ptopo;
+    a.selnodes = dm->d_selnodes;
+    a.pts = dm->d_pts;
+    a.phi = dm->d_phi;
+    a.Qmax = dm->rs.Qmax;
+    for (int c = 0; c < 6; ++c) {
+      a.off[c] = dm->rs.off[c];
+      a.len[c] = dm->rs.len[c];
+    }
+    a.R = dm->rs.R;
+    a.E = (int)dm->E;
+    a.k = k;
+    a.gradX = d_grad_coords;
+    a.gradk = d_grad_k;
+    launch_adjoint(a, dm->gk, dm->pk, s);
+  });
+}
+
 int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream) {
   return guarded([&] {

@@ -111,6 +111,30 @@ struct SweepArgs {
   int64_t n_tiles;           // k blocks x segments x row blocks
 };
 
+// Vector-Jacobian product of the assembly (adjoint.hip).
+struct AdjointArgs {
+  const double* targets;   // 3 x M
+  int64_t M;
+  const int* tvert;        // M: mesh node a target sits on (moves with X), -1 fixed; or NULL
+  const double2* alpha;    // M complex weights of F rows
+  const double2* beta;     // M complex weights of C (or NULL)
+  const double2* p;        // N complex values at the collocation nodes
+  const double* coords;    // 3 x Nc
+  const int* topo;         // E x NG
+  const int* ptopo;        // E x NP
+  const double* selnodes;  // E x NP x 4
+  const double* pts;       // rule points (kPtStride)
+  const double* phi;       // R x Qmax x NP
+  int Qmax;
+  int off[6];
+  int len[6];
+  int R;
+  int E;
+  double k;
+  double* gradX;           // 3 x Nc, accumulated
+  double* gradk;           // 1, accumulated (or NULL)
+};
+
 // Launchers (assemble.hip)
 void launch_fill_zero(double* p, int64_t n_doubles, hipStream_t s);
 void launch_fill_zero_u32(unsigned* p, hipStream_t s);
@@ -131,6 +155,7 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
                   hipStream_t s);
 void sweep_prepare_attributes(int NP, int nphi);
 void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s);
+void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,
                    int R, int8_t* out, hipStream_t s);
 

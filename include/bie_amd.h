@@ -118,6 +118,23 @@ int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
 int bie_graph_launch(bie_graph* g, void* stream);
 int bie_graph_destroy(bie_graph* g);
 
+/* ---- Adjoint building block ------------------------------------------------------------
+ * Vector-Jacobian product of the assembly at real k with the rule selection held fixed:
+ *   d_grad_coords[3*v + d] += d/dX[v,d]  Re[ sum_i alpha_i sum_j F_ij p_j + sum_i beta_i C_i ]
+ *   *d_grad_k             += d/dk      (same functional)
+ * X = mesh coordinates (3 x n_coordinates, the array given to bie_dmesh_create).  alpha
+ * (n_targets complex), beta (n_targets complex, NULL = 0) and p (n_sources complex) are
+ * device arrays.  d_target_node[i] >= 0 marks target i as sitting on mesh coordinate
+ * d_target_node[i] (it moves with X); -1 (or a NULL array) keeps it fixed (field points).
+ * The pair quadrature is recomputed, nothing is stored.  No counterpart in the reference,
+ * which differentiates by finite differences (examples/2d_optimization.jl:184-216); the
+ * gradient of a scattering objective through A = F + diag(1 - C) and its solve is
+ * composed from two calls (surface and field operators) on the host. */
+int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, int64_t n_targets,
+                            const int32_t* d_target_node, const double* d_alpha,
+                            const double* d_beta, const double* d_p, int32_t n, int32_t m,
+                            double* d_grad_coords, double* d_grad_k, void* stream);
+
 /* ---- Dense operator apply --------------------------------------------------------------
  * Y = A X for complex128 column-major A (M x K, leading dimension lda), X (K x nrhs, ldx),
  * Y (M x nrhs, ldy), all device pointers, on FP64 MFMA; asynchronous on stream.

@@ -285,7 +285,7 @@ struct Flags {
 // (kernels.jl:94-118).  C is only formed for mo == 0.
 void assemble_impl(const MeshView& mv, double kr, double ki, const double* targets,
                    int64_t M, int n, int m, Flags fl, int mo, double* F, double* G,
-                   double* Cc, int nthreads) {
+                   double* Cc, int nthreads, const int8_t* frozen = nullptr) {
   Tables T = build_tables(mv, n, m);
   const int64_t N = mv.N;
   std::memset(F, 0, sizeof(double) * 2 * M * N);
@@ -303,7 +303,8 @@ void assemble_impl(const MeshView& mv, double kr, double ki, const double* targe
       const double* x = targets + 3 * i;
       double csum = 0.0;
       for (int64_t e = 0; e < mv.E; ++e) {
-        int c = rule_for(mv, x, e, T.R);
+        int c = frozen ? (int)frozen[i * mv.E + e] : rule_for(mv, x, e, T.R);
+        if (c < 1 || c > T.R) c = T.R;
         const ElemTable& t = T.tab[c - 1];
         const int Q = t.Q, np = mv.pk;
         const double* y = &t.y[(size_t)e * Q * 3];
@@ -465,6 +466,21 @@ int orc_assemble_taylor(const double* coords, const int64_t* topo, int gk,
   std::vector<double> C(2 * M);
   assemble_impl(mv, k_re, k_im, targets, M, n, m, Flags{true, true, false}, mo, F, G,
                 C.data(), nthreads);
+  return 0;
+}
+
+// As orc_assemble with the rule index of every (target, element) pair given
+// (rules[i * E + e] in 1..R): the assembly as a smooth function of the geometry and k at
+// a frozen rule selection, for finite-difference checks of derivatives.
+int orc_assemble_frozen(const double* coords, const int64_t* topo, int gk,
+                        const double* sources, int64_t N, const int64_t* ptopo, int pk,
+                        int64_t E, int base, double k_re, double k_im, const double* targets,
+                        int64_t M, int n, int m, int flags, const int8_t* rules, double* F,
+                        double* G, double* C, int nthreads) {
+  if ((gk != 3 && gk != 6) || (pk != 3 && pk != 6) || n < 1 || m < 1 || !rules) return 1;
+  MeshView mv = view(coords, topo, gk, sources, N, ptopo, pk, E, base);
+  Flags fl{(flags & 1) != 0, (flags & 2) != 0, (flags & 4) != 0};
+  assemble_impl(mv, k_re, k_im, targets, M, n, m, fl, 0, F, G, C, nthreads, rules);
   return 0;
 }
 

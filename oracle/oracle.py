@@ -55,6 +55,11 @@ def lib():
                                           ctypes.c_double, ctypes.c_double, _dp,
                                           ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.orc_assemble_frozen.argtypes = [_dp, _i64p, ctypes.c_int, _dp, ctypes.c_int64, _i64p,
+                                          ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, _dp,
+                                          ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _i8p, _dp, _dp, _dp, ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -156,6 +161,29 @@ def assemble_taylor(mesh, k, order, targets=None, n=4, m=4, nthreads=0):
     if rc != 0:
         raise ValueError(f"oracle taylor assemble failed ({rc})")
     return F, G
+
+
+def assemble_frozen(mesh, k, rules, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True,
+                    nthreads=0):
+    """Assembly with the rule index of every (target, element) pair given (int8 [M, E],
+    values 1..R): a smooth function of coordinates and k for finite differences."""
+    coords, topo, src, ptop = _mesh_args(mesh)
+    tg = np.asfortranarray(mesh.sources if targets is None else targets, dtype=np.float64)
+    M, N = tg.shape[1], mesh.n_sources
+    rules = np.ascontiguousarray(rules, dtype=np.int8)
+    assert rules.shape == (M, mesh.n_elements)
+    F = np.zeros((M, N), dtype=np.complex128, order="F")
+    G = np.zeros((M, N), dtype=np.complex128, order="F")
+    C = np.zeros(M, dtype=np.complex128)
+    k = complex(k)
+    flags = (1 if fOn else 0) | (2 if gOn else 0) | (4 if cOn else 0)
+    rc = lib().orc_assemble_frozen(_p(coords), _p(topo, _i64p), mesh.geometry_kind, _p(src), N,
+                                   _p(ptop, _i64p), mesh.physics_kind, mesh.n_elements, 0,
+                                   k.real, k.imag, _p(tg), M, n, m, flags, _p(rules, _i8p),
+                                   _p(F), _p(G), _p(C), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle frozen assemble failed ({rc})")
+    return F, G, C
 
 
 def max_threads() -> int:
