@@ -33,10 +33,32 @@ PEAK_HBM = 8.0e12
 
 
 def algorithmic_flops(rule_hist, rs_len, n_s):
-    """rule_hist[c] = number of (target, element) pairs using rule c (1-based)."""
-    pts = sum(int(rule_hist[c]) * rs_len[c - 1] for c in range(1, len(rs_len) + 1))
+    """Split of the algorithmic FP64 work between the two assembly kernels.
+
+    rule_hist[c] = number of (target, element) pairs using rule c (1-based).  The dense
+    kernel selects the rule of every pair and integrates the pairs of the vertex rules
+    1-3; pairs of the mid-edge rules 4-6 (quadratic physics only) are queued and
+    integrated by the mid-edge kernel."""
+    per_pt = 35 + 8 * n_s
+    pts_dense = sum(int(rule_hist[c]) * rs_len[c - 1] for c in range(1, 4))
+    pts_mid = sum(int(rule_hist[c]) * rs_len[c - 1] for c in range(4, len(rs_len) + 1))
     pairs = int(sum(rule_hist[1:]))
-    return pts * (35 + 8 * n_s) + pairs * 10 * n_s, pts, pairs
+    return dict(dense=pts_dense * per_pt + pairs * 10 * n_s, mid=pts_mid * per_pt,
+                points_dense=pts_dense, points_mid=pts_mid, pairs=pairs)
+
+
+def measured_traffic(N, n, m, k):
+    """HBM bytes per launch of the dense assembly kernel for the default workload, read from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+    kernel of known size); None for other workloads or when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_assemble_quadsphere10242_k10_nm32.txt")
+    if (N, n, m, k) != (10242, 3, 2, 10.0) or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def measure_apply(torch, F, N, steps, stream, nrhs=32):
@@ -145,9 +167,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--f", type=int, default=16, help="icosphere subdivision (16: 10242 nodes)")
     ap.add_argument("--k", type=float, default=10.0)
-    ap.add_argument("--cpu-rows", type=int, default=768, help="rows in the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="target rows in the CPU baseline sample (0: all rows)")
+    ap.add_argument("--cpu-repeats", type=int, default=3,
+                    help="timed CPU runs after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy 3 times (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
@@ -156,6 +183,15 @@ def main():
 
     import numpy as np
     import torch
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "launches": 3}))
+        return
     import bie_amd
     from bie_amd import _lib
     from bie_amd.rules import rule_size
@@ -186,9 +222,11 @@ def main():
         torch.cuda.synchronize()
         _lib.check(L.bie_timing_enable(0))
         total_ms = ev0.elapsed_time(ev1)
-        kms, cms, nc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
-        _lib.check(L.bie_timing_collect(ctypes.byref(kms), ctypes.byref(cms), ctypes.byref(nc)))
-        return total_ms / steps, kms.value / max(nc.value, 1), cms.value / max(nc.value, 1)
+        kms, mms, cms, nc = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _lib.check(L.bie_timing_collect(ctypes.byref(kms), ctypes.byref(mms), ctypes.byref(cms),
+                                        ctypes.byref(nc)))
+        d = max(nc.value, 1)
+        return total_ms / steps, kms.value / d, mms.value / d, cms.value / d
 
     def rule_hist():
         rules = torch.empty(N * E, dtype=torch.int8, device=dev)
@@ -200,12 +238,14 @@ def main():
     entries = 2 * N * N + N
 
     def measure(n, m, steps, warmup):
-        ms, kms, cms = run(n, m, steps, warmup)
+        ms, kms, mms, cms = run(n, m, steps, warmup)
         lens = [rule_size(c, n, m) for c in range(1, 7)]
-        flops, pts, pairs = algorithmic_flops(hist, lens, 6)
-        return dict(ms_per_step=ms, kernel_ms=kms, call_ms=cms, flops=flops, points=pts,
-                    pairs=pairs, entries_per_s=entries / (ms * 1e-3),
-                    kernel_tflops=flops / (kms * 1e-3) / 1e12)
+        fl = algorithmic_flops(hist, lens, 6)
+        return dict(ms_per_step=ms, kernel_ms=kms, mid_ms=mms, call_ms=cms, flops=fl,
+                    entries_per_s=entries / (ms * 1e-3),
+                    kernel_tflops=fl["dense"] / (kms * 1e-3) / 1e12,
+                    mid_tflops=fl["mid"] / (mms * 1e-3) / 1e12 if mms > 0 else None,
+                    pass_tflops=(fl["dense"] + fl["mid"]) / ((kms + mms) * 1e-3) / 1e12)
 
     n, m = 3, 2
     main_res = measure(n, m, args.steps, args.warmup)
@@ -236,12 +276,18 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # the CPU baseline and checker
-        rows = np.linspace(0, N - 1, args.cpu_rows).astype(np.int64)
+        nrows = N if args.cpu_rows <= 0 else min(N, args.cpu_rows)
+        rows = np.linspace(0, N - 1, nrows).astype(np.int64)
         threads = len(os.sched_getaffinity(0))
-        t0 = time.perf_counter()
-        Fo, Go, Co = oracle.assemble(mesh, args.k, targets=mesh.sources[:, rows], n=n, m=m,
-                                     nthreads=threads)
-        t_cpu = time.perf_counter() - t0
+        tg = np.asfortranarray(mesh.sources[:, rows])
+        times = []
+        for rep in range(1 + max(1, args.cpu_repeats)):   # first run is the warm-up
+            t0 = time.perf_counter()
+            Fo, Go, Co = oracle.assemble(mesh, args.k, targets=tg, n=n, m=m, nthreads=threads)
+            times.append(time.perf_counter() - t0)
+            if rep == 0 and times[0] > 20.0:             # bounded sample: one run is enough
+                break
+        t_cpu = float(np.median(times[1:])) if len(times) > 1 else times[0]
         idx = torch.as_tensor(rows, device=dev)
         Fg = F.view(N, N, 2).index_select(1, idx).cpu().numpy()
         Gg = G.view(N, N, 2).index_select(1, idx).cpu().numpy()
@@ -255,11 +301,13 @@ def main():
         cpu = {"value": (2 * len(rows) * N + len(rows)) / t_cpu, "unit": "entries/s",
                "cores": threads, "kind": "port",
                "sample": f"{len(rows)} of {N} target rows of the same workload (all {E} "
-                         f"elements), {t_cpu:.2f} s wall; GPU vs CPU rel Frobenius error on "
-                         f"these rows {err:.2e}",
+                         f"elements), median of {max(1, len(times) - 1)} runs after a warm-up, "
+                         f"{t_cpu:.2f} s wall each on {threads} threads; GPU vs CPU rel "
+                         f"Frobenius error on these rows {err:.2e}",
                "rel_err_gpu_vs_cpu": err}
 
     achieved = main_res["kernel_tflops"]
+    kernel_name = "k_assemble" if os.environ.get("BIE_AMD_KERNEL") == "atomic" else "k_assemble_slots"
     out = {
         "metric": METRIC,
         "value": main_res["entries_per_s"],
@@ -276,12 +324,19 @@ def main():
         "config": {"workload": f"quadratic geodesic sphere r=1, N={N} nodes, E={E} tri6, "
                                f"k={args.k}, collocation, (n,m)=({n},{m}), complex128 F/G/C",
                    "n": n, "m": m, "k": args.k, "n_nodes": N, "n_elements": E},
-        "roofline": {"bound": "mfma" if False else "valu-fp64", "achieved": achieved,
-                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved * 1e12 / PEAK_FP64, "traffic": None,
-                     "kernel": "k_assemble", "kernel_ms": main_res["kernel_ms"],
-                     "flops_per_launch": main_res["flops"],
-                     "quadrature_points": main_res["points"], "pairs": main_res["pairs"]},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP64,
+                     "traffic": measured_traffic(N, n, m, args.k),
+                     "kernel": kernel_name, "kernel_ms": main_res["kernel_ms"],
+                     "flops_per_launch": main_res["flops"]["dense"],
+                     "quadrature_points": main_res["flops"]["points_dense"],
+                     "pairs": main_res["flops"]["pairs"],
+                     "fp64_unit": "VALU (MI355X FP64 vector peak = published FP64 peak)",
+                     "mid_edge_kernel": {"kernel": "k_assemble_mid",
+                                         "kernel_ms": main_res["mid_ms"],
+                                         "flops_per_launch": main_res["flops"]["mid"],
+                                         "tflops": main_res["mid_tflops"]},
+                     "assembly_pass_tflops": main_res["pass_tflops"]},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -76,8 +76,26 @@ SIGNATURES = {
                                               ctypes.POINTER(_dp)]),
     "bie_graph_launch": (ctypes.c_int, [_dp, _dp]),
     "bie_graph_destroy": (ctypes.c_int, [_dp]),
+    "bie_batch_create": (ctypes.c_int, [ctypes.POINTER(_dp), ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.POINTER(_dp)]),
+    "bie_batch_destroy": (ctypes.c_int, [_dp]),
+    "bie_batch_configure": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "bie_batch_info": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int64)]),
+    "bie_batch_assemble": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+                                          ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                          ctypes.POINTER(_dp), _dp]),
+    "bie_batch_tune": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+                                      ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                      ctypes.POINTER(_dp), _dp,
+                                      ctypes.POINTER(ctypes.c_double)]),
     "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int64)]),
     "bie_device_malloc": (ctypes.c_int, [ctypes.POINTER(_dp), ctypes.c_int64]),

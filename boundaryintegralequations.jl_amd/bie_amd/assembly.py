@@ -159,3 +159,72 @@ class DeviceMesh:
         _lib.check(_lib.lib().bie_element_geometry(self.handle, rule, n, m, _lib._ptr(y),
                                                    _lib._ptr(nr), _lib._ptr(jw)))
         return y, nr, jw
+
+
+class Batch:
+    """An uneven batch of device meshes assembled in one persistent launch (bie_batch_*).
+
+    Every mesh shares element kinds and the rule set (n, m); instance i is assembled at its
+    own wavenumber with collocation at its own nodes, into caller-owned device buffers
+    (F_i, G_i: N_i x N_i column-major complex128; C_i: N_i)."""
+
+    def __init__(self, dmeshes, *, n=4, m=4):
+        self.dmeshes = list(dmeshes)           # keeps the meshes alive while the batch is
+        arr = (ctypes.c_void_p * len(self.dmeshes))(*[d.handle.value for d in self.dmeshes])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().bie_batch_create(arr, len(self.dmeshes), int(n), int(m),
+                                               ctypes.byref(h)))
+        self.handle = h
+        self.n, self.m = n, m
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().bie_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.dmeshes)
+
+    def configure(self, elems_per_tile=64, slots=64, order=0):
+        _lib.check(_lib.lib().bie_batch_configure(self.handle, int(elems_per_tile), int(slots),
+                                                  int(order)))
+
+    def info(self):
+        t, s, o = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        nt = ctypes.c_int64()
+        _lib.check(_lib.lib().bie_batch_info(self.handle, ctypes.byref(t), ctypes.byref(s),
+                                             ctypes.byref(o), ctypes.byref(nt)))
+        return dict(elems_per_tile=t.value, slots=s.value, order=o.value, n_tiles=nt.value)
+
+    def _args(self, ks, F_ptrs, G_ptrs, C_ptrs):
+        n = len(self.dmeshes)
+        ks = [complex(k) for k in ks]
+        if not (len(ks) == len(F_ptrs) == len(G_ptrs) == len(C_ptrs) == n):
+            raise ValueError("one k and one F, G, C pointer per instance")
+        kr = (ctypes.c_double * n)(*[k.real for k in ks])
+        ki = (ctypes.c_double * n)(*[k.imag for k in ks])
+        P = ctypes.c_void_p * n
+        return kr, ki, P(*[p or None for p in F_ptrs]), P(*[p or None for p in G_ptrs]), \
+            P(*[p or None for p in C_ptrs])
+
+    def assemble(self, ks, F_ptrs, G_ptrs, C_ptrs, *, fOn=True, gOn=True, cOn=True,
+                 stream: int = 0):
+        """Asynchronous batched assembly."""
+        kr, ki, F, G, C = self._args(ks, F_ptrs, G_ptrs, C_ptrs)
+        _lib.check(_lib.lib().bie_batch_assemble(self.handle, kr, ki, _flags(fOn, gOn, cOn), F, G,
+                                                 C, ctypes.c_void_p(stream or None)))
+
+    def tune(self, ks, F_ptrs, G_ptrs, C_ptrs, *, fOn=True, gOn=True, cOn=True,
+             stream: int = 0) -> float:
+        """Times every schedule candidate, keeps the fastest; returns its time in ms."""
+        kr, ki, F, G, C = self._args(ks, F_ptrs, G_ptrs, C_ptrs)
+        best = ctypes.c_double()
+        _lib.check(_lib.lib().bie_batch_tune(self.handle, kr, ki, _flags(fOn, gOn, cOn), F, G, C,
+                                             ctypes.c_void_p(stream or None), ctypes.byref(best)))
+        return best.value

@@ -96,11 +96,13 @@ std::vector<int> convert_topology(const int64_t* t, int rows, int64_t E, int64_t
 void ensure_rules(bie_dmesh* dm, int n, int m, hipStream_t s) {
   if (dm->cn == n && dm->cm == m && dm->d_elem) return;
   RuleSet rs = build_ruleset(dm->gk, dm->pk, n, m);
+  if (dm->d_elem) (void)hipDeviceSynchronize();  // in-flight kernels may read the old tables
   dfree(dm->d_pts);
   dfree(dm->d_phi);
   dfree(dm->d_elem);
   dm->d_pts = dm->d_phi = dm->d_elem = nullptr;
   dm->cn = dm->cm = -1;
+  dm->geom_valid = false;
   dm->d_pts = dalloc<double>(rs.pts.size());
   dm->d_phi = dalloc<double>(rs.phi.size());
   dm->d_elem = dalloc<double>((size_t)std::max<int64_t>(dm->E, 1) * rs.Pe * 8);
@@ -114,12 +116,20 @@ void ensure_rules(bie_dmesh* dm, int n, int m, hipStream_t s) {
   dm->cm = m;
 }
 
-// Event record for bie_timing_*: (call start, kernel start, kernel end) per call.
+// Element geometry tables (y, n, J w / 4 pi at every rule point) of the cached rule set.
+void refresh_geometry(bie_dmesh* dm, hipStream_t s) {
+  launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
+                  dm->d_elem, s);
+  dm->geom_valid = true;
+}
+
+// Event record for bie_timing_*: (call start, dense kernel start, dense kernel end,
+// mid-edge kernel end) per call.
 struct Timing {
   std::mutex mu;
   bool on = false;
   std::vector<hipEvent_t> pool;
-  std::vector<std::array<hipEvent_t, 3>> rec;
+  std::vector<std::array<hipEvent_t, 4>> rec;
   hipEvent_t get() {
     if (!pool.empty()) {
       hipEvent_t e = pool.back();
@@ -140,6 +150,14 @@ bool use_atomic_kernel() {
   return v && std::strcmp(v, "atomic") == 0;
 }
 
+// LDS slots per workgroup of the single-k slot kernel: 64 (128 KiB of accumulators, one
+// workgroup per CU) unless BIE_AMD_SLOTS selects 32 (64 KiB, two workgroups per CU).
+int plain_slot_count() {
+  const char* v = std::getenv("BIE_AMD_SLOTS");
+  if (v && std::strcmp(v, "32") == 0) return 32;
+  return kSlotCount;
+}
+
 template <class T>
 T* upload_vec(const std::vector<T>& v, hipStream_t s) {
   T* d = dalloc<T>(v.size());
@@ -149,6 +167,8 @@ T* upload_vec(const std::vector<T>& v, hipStream_t s) {
 }
 
 void free_schedule(bie_dmesh::Sched& sc) {
+  // kernels still in flight may read the schedule being replaced
+  if (sc.d_step_elems) (void)hipDeviceSynchronize();
   for (int** p : {&sc.d_seg_step_begin, &sc.d_step_elems, &sc.d_step_slot, &sc.d_flush_begin,
                   &sc.d_flush_node, &sc.d_flush_slot, &sc.d_zero_cols}) {
     dfree(*p);
@@ -164,7 +184,8 @@ void free_schedules(bie_dmesh* dm) {
   for (auto& sc : dm->sched) free_schedule(sc);
 }
 
-// Schedule `which` (0: single-k kernel, 1: sweep kernel) for S live slots.
+// Schedule `which` (0: single-k kernel, 1: sweep kernel, 2: batch kernel) for S live
+// slots and at least min_segments segments.
 bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segments,
                                   hipStream_t s) {
   bie_dmesh::Sched& out = dm->sched[which];
@@ -180,6 +201,7 @@ bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segme
   SlotSchedule sc = build_slot_schedule(dm->h_ptopo.data(), dm->pk, (int)dm->E, (int)dm->N,
                                         cent.data(), kSlotWaves, S, min_segments);
   out.d_seg_step_begin = upload_vec(sc.seg_step_begin, s);
+  out.h_seg_step_begin = sc.seg_step_begin;
   out.d_step_elems = upload_vec(sc.step_elems, s);
   // per (step, wave) copies of the selection nodes and slots, in the order the kernel
   // reads them
@@ -287,7 +309,7 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
   BIE_HIP_CHECK(hipSetDevice(dm->device));
   std::lock_guard<std::mutex> lk(dm->mu);
   ensure_rules(dm, n, m, s);
-  std::array<hipEvent_t, 3> ev{};
+  std::array<hipEvent_t, 4> ev{};
   bool timed = false;
   {
     std::lock_guard<std::mutex> tl(g_timing.mu);
@@ -303,17 +325,17 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
   if (atomic_path || !(flags & BIE_G)) launch_fill_zero(dG, 2 * M * dm->N, s);
   launch_fill_zero(dC, 2 * M, s);
   if (M > 0 && dm->E > 0) {
-    launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
-                    dm->d_elem, s);
+    refresh_geometry(dm, s);
     AssemblyArgs a = make_args(dm, d_targets, M, kr, ki, flags, dF, dG, dC);
     const bool phi_fits = sizeof(double) * (size_t)a.R * a.Qmax * a.NP <= 24 * 1024;
     if (use_atomic_kernel() || !phi_fits) {
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
       launch_assemble(a, s);
+      if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
     } else {
       const int64_t rowblocks = (M + 63) / 64;
       const int min_segments = (int)std::max<int64_t>(1, std::min<int64_t>(dm->E, (2048 + rowblocks - 1) / rowblocks));
-      const bie_dmesh::Sched& sc = ensure_schedule(dm, 0, kSlotCount, min_segments, s);
+      const bie_dmesh::Sched& sc = ensure_schedule(dm, 0, plain_slot_count(), min_segments, s);
       if (dm->pk == 6) {
         ensure_mid_queue(dm, M * dm->E);
         launch_fill_zero_u32(dm->d_mid_count, s);
@@ -327,13 +349,15 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       if (flags & BIE_G) launch_zero_columns(dG, M, sc.d_zero_cols, sc.zero_cols, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
       launch_assemble_slots(a, sa, s);
+      if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
       launch_assemble_mid(a, sa, s);
     }
   } else if (timed) {
     BIE_HIP_CHECK(hipEventRecord(ev[1], s));
+    BIE_HIP_CHECK(hipEventRecord(ev[2], s));
   }
   if (timed) {
-    BIE_HIP_CHECK(hipEventRecord(ev[2], s));
+    BIE_HIP_CHECK(hipEventRecord(ev[3], s));
     std::lock_guard<std::mutex> tl(g_timing.mu);
     g_timing.rec.push_back(ev);
   }
@@ -383,8 +407,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
       for (int q = 0; q < nk; ++q) launch_fill_zero(X + 2 * (size_t)q * stride, 2 * M * dm->N, s);
   }
   if (M == 0 || dm->E == 0 || nk == 0) return;
-  launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, dm->rs.Pe,
-                  dm->d_elem, s);
+  refresh_geometry(dm, s);
   launch_block_uniform(d_k, nk, KB, dm->d_block_uniform, s);
   const int64_t rowblocks = (M + 63) / 64;
   AssemblyArgs a = make_args(dm, d_targets, M, 0.0, 0.0, flags, dF, dG, dC);
@@ -396,6 +419,148 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   sw.stride = stride;
   sw.n_tiles = (int64_t)nkb * sc.segments * rowblocks;
   launch_sweep(a, sa, sw, s);
+}
+
+
+// ---- uneven batch -------------------------------------------------------------------
+
+// Tile list of the batch for its current configuration: every (instance, row block,
+// segment), costed by the segment's step count, ordered longest-first (order 0; ties in
+// instance, segment, row block order) or interleaved across instances (order 1).
+void batch_configure(bie_batch* b, hipStream_t s) {
+  const int key = b->elems_per_tile * 1000 + b->slots * 10 + b->order;
+  bool current = b->configured_key == key && b->d_tiles != nullptr;
+  // a mesh's batch schedule may have been rebuilt by another batch that shares it
+  for (size_t ii = 0; current && ii < b->meshes.size(); ++ii) {
+    const bie_dmesh::Sched& sc = b->meshes[ii]->sched[2];
+    current = sc.d_step_elems != nullptr && sc.S == b->slots && sc.key == b->min_segments[ii];
+  }
+  if (current) return;
+  struct Tile {
+    int steps, inst, rb, seg;
+    int64_t rank;
+  };
+  std::vector<Tile> tiles;
+  const int n_inst = (int)b->meshes.size();
+  b->configured_key = -1;
+  b->rowblocks.assign(n_inst, 0);
+  b->min_segments.assign(n_inst, 0);
+  for (int ii = 0; ii < n_inst; ++ii) {
+    bie_dmesh* dm = b->meshes[ii];
+    const int64_t rbs = (dm->N + 63) / 64;
+    b->rowblocks[ii] = rbs;
+    const int min_seg = (int)std::max<int64_t>(1, (dm->E + b->elems_per_tile - 1) / b->elems_per_tile);
+    b->min_segments[ii] = min_seg;
+    const bie_dmesh::Sched& sc = ensure_schedule(dm, 2, b->slots, min_seg, s);
+    if (dm->N == 0) continue;
+    int64_t rank = 0;
+    for (int g = 0; g < sc.segments; ++g) {
+      const int steps = sc.h_seg_step_begin[g + 1] - sc.h_seg_step_begin[g];
+      for (int64_t rb = 0; rb < rbs; ++rb) tiles.push_back(Tile{steps, ii, (int)rb, g, rank++});
+    }
+  }
+  BIE_REQUIRE(tiles.size() < (size_t)INT32_MAX, BIE_EINVAL, "batch has too many tiles");
+  if (b->order == 0)
+    std::stable_sort(tiles.begin(), tiles.end(),
+                     [](const Tile& x, const Tile& y) { return x.steps > y.steps; });
+  else
+    std::stable_sort(tiles.begin(), tiles.end(),
+                     [](const Tile& x, const Tile& y) { return x.rank < y.rank; });
+  std::vector<int4> h(tiles.size());
+  for (size_t t = 0; t < tiles.size(); ++t)
+    h[t] = make_int4(tiles[t].inst, tiles[t].rb, tiles[t].seg, 0);
+  if (b->d_tiles) (void)hipDeviceSynchronize();  // a previous launch may still read it
+  dfree(b->d_tiles);
+  b->d_tiles = nullptr;
+  b->n_tiles = 0;
+  b->d_tiles = upload_vec(h, s);
+  BIE_HIP_CHECK(hipStreamSynchronize(s));
+  b->n_tiles = (int)h.size();
+  b->configured_key = key;
+}
+
+struct MeshLocks {
+  std::vector<std::unique_lock<std::mutex>> held;
+  explicit MeshLocks(const std::vector<bie_dmesh*>& ms) {
+    std::vector<bie_dmesh*> order(ms);
+    std::sort(order.begin(), order.end());
+    order.erase(std::unique(order.begin(), order.end()), order.end());
+    for (bie_dmesh* dm : order) held.emplace_back(dm->mu);
+  }
+};
+
+void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                         double* const* dF, double* const* dG, double* const* dC,
+                         hipStream_t s) {
+  BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
+  BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
+  BIE_REQUIRE(k_re && dF && dG && dC, BIE_EINVAL, "NULL argument array");
+  const int n_inst = (int)b->meshes.size();
+  for (int ii = 0; ii < n_inst; ++ii) {
+    const double ki = k_im ? k_im[ii] : 0.0;
+    BIE_REQUIRE(std::isfinite(k_re[ii]) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+    BIE_REQUIRE(dC[ii], BIE_EINVAL, "C output pointer is NULL");
+    BIE_REQUIRE(!(flags & BIE_F) || dF[ii], BIE_EINVAL, "F output pointer is NULL");
+    BIE_REQUIRE(!(flags & BIE_G) || dG[ii], BIE_EINVAL, "G output pointer is NULL");
+  }
+  BIE_HIP_CHECK(hipSetDevice(b->device));
+  std::lock_guard<std::mutex> bl(b->mu);
+  MeshLocks locks(b->meshes);
+  for (bie_dmesh* dm : b->meshes) {
+    ensure_rules(dm, b->n, b->m, s);
+    if (!dm->geom_valid) refresh_geometry(dm, s);
+    if (dm->pk == 6)
+      ensure_mid_queue(dm, std::max<int64_t>(1, dm->N * dm->E));
+    else if (!dm->d_mid_count)
+      dm->d_mid_count = dalloc<unsigned>(1);
+  }
+  batch_configure(b, s);
+  BIE_HIP_CHECK(hipEventSynchronize(b->staged));  // the previous upload has left h_inst
+  bool cplx = false;
+  int64_t max_work = 1;
+  for (int ii = 0; ii < n_inst; ++ii) {
+    bie_dmesh* dm = b->meshes[ii];
+    const double ki = k_im ? k_im[ii] : 0.0;
+    cplx = cplx || ki != 0.0;
+    const bie_dmesh::Sched& sc = dm->sched[2];
+    BatchInst& bi = b->h_inst[ii];
+    bi.a = make_args(dm, dm->d_sources, dm->N, k_re[ii], ki, flags, dF[ii], dG[ii], dC[ii]);
+    bi.sa = slot_args(dm, sc, b->rowblocks[ii]);
+    bi.zero_cols = sc.d_zero_cols;
+    bi.n_zero = sc.zero_cols;
+    bi.pad = 0;
+    max_work = std::max<int64_t>(max_work, dm->N * std::max<int64_t>(1, sc.zero_cols));
+    if ((!(flags & BIE_F) && dF[ii]) || (!(flags & BIE_G) && dG[ii]))
+      max_work = std::max<int64_t>(max_work, dm->N * dm->N);
+  }
+  BIE_HIP_CHECK(hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(BatchInst) * n_inst,
+                               hipMemcpyHostToDevice, s));
+  BIE_HIP_CHECK(hipEventRecord(b->staged, s));
+  launch_batch_prep(b->d_inst, n_inst, max_work, s);
+  if (b->n_tiles > 0) {
+    bie_dmesh* d0 = b->meshes[0];
+    const int nphi = d0->rs.R * d0->rs.Qmax * d0->pk;
+    int cus = 0;
+    BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const int per_cu = batch_blocks_per_cu(b->pk, cplx, b->slots, nphi);
+    const int grid = (int)std::min<int64_t>((int64_t)std::max(cus, 1) * per_cu, b->n_tiles);
+    launch_assemble_batch(b->d_inst, b->d_tiles, b->n_tiles, b->d_counter, d0->d_phi, nphi,
+                          b->slots, b->pk, cplx, grid, s);
+  }
+  if (b->pk == 6) launch_assemble_mid_batch(b->d_inst, n_inst, cplx, s);
+}
+
+void batch_destroy_impl(bie_batch* b) {
+  (void)hipSetDevice(b->device);
+  if (b->staged) {
+    (void)hipEventSynchronize(b->staged);
+    (void)hipEventDestroy(b->staged);
+  }
+  if (b->h_inst) (void)hipHostFree(b->h_inst);
+  dfree(b->d_inst);
+  dfree(b->d_tiles);
+  dfree(b->d_counter);
+  delete b;
 }
 
 }  // namespace
@@ -518,8 +683,7 @@ int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, doub
     ensure_rules(dm, n, m, nullptr);
     BIE_REQUIRE(rule >= 1 && rule <= dm->rs.R, BIE_EINVAL, "rule index out of range");
     const RuleSet& rs = dm->rs;
-    launch_geometry(dm->d_coords, dm->d_topo, dm->gk, (int)dm->E, dm->d_pts, rs.Pe, dm->d_elem,
-                    nullptr);
+    refresh_geometry(dm, nullptr);
     std::vector<double> h((size_t)dm->E * rs.Pe * 8);
     BIE_HIP_CHECK(hipMemcpy(h.data(), dm->d_elem, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int Q = rs.len[rule - 1], off = rs.off[rule - 1];
@@ -742,6 +906,152 @@ int bie_graph_destroy(bie_graph* g) {
   return BIE_OK;
 }
 
+int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_t m,
+                     bie_batch** out) {
+  return guarded([&] {
+    BIE_REQUIRE(out != nullptr, BIE_EINVAL, "out is NULL");
+    *out = nullptr;
+    BIE_REQUIRE(meshes != nullptr && n_inst >= 1, BIE_EINVAL, "empty batch");
+    BIE_REQUIRE(n_inst <= 65535, BIE_EINVAL, "too many batch instances");
+    BIE_REQUIRE(n >= 1 && m >= 1, BIE_EINVAL, "quadrature orders must be positive");
+    for (int ii = 0; ii < n_inst; ++ii) {
+      BIE_REQUIRE(meshes[ii] != nullptr, BIE_EINVAL, "batch mesh is NULL");
+      BIE_REQUIRE(meshes[ii]->device == meshes[0]->device, BIE_EINVAL,
+                  "batch meshes live on different devices");
+      BIE_REQUIRE(meshes[ii]->gk == meshes[0]->gk && meshes[ii]->pk == meshes[0]->pk,
+                  BIE_EUNSUPPORTED, "batch meshes must share geometry and physics kinds");
+    }
+    if (meshes[0]->pk == 6) {  // one mid-edge queue per mesh: each may appear once
+      std::vector<bie_dmesh*> ms(meshes, meshes + n_inst);
+      std::sort(ms.begin(), ms.end());
+      BIE_REQUIRE(std::adjacent_find(ms.begin(), ms.end()) == ms.end(), BIE_EINVAL,
+                  "a quadratic-physics mesh may appear only once in a batch");
+    }
+    {
+      RuleSet rs = build_ruleset(meshes[0]->gk, meshes[0]->pk, n, m);
+      BIE_REQUIRE(sizeof(double) * (size_t)rs.R * rs.Qmax * rs.pk <= 24 * 1024, BIE_EUNSUPPORTED,
+                  "batch supports quadrature orders up to n*m' = 36");
+    }
+    BIE_HIP_CHECK(hipSetDevice(meshes[0]->device));
+    std::unique_ptr<bie_batch> b(new bie_batch);
+    b->device = meshes[0]->device;
+    b->gk = meshes[0]->gk;
+    b->pk = meshes[0]->pk;
+    b->n = n;
+    b->m = m;
+    b->meshes.assign(meshes, meshes + n_inst);
+    try {
+      void* h = nullptr;
+      BIE_HIP_CHECK(hipHostMalloc(&h, sizeof(BatchInst) * n_inst, hipHostMallocDefault));
+      b->h_inst = static_cast<BatchInst*>(h);
+      b->d_inst = dalloc<BatchInst>(n_inst);
+      b->d_counter = dalloc<unsigned>(1);
+      BIE_HIP_CHECK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+      BIE_HIP_CHECK(hipEventRecord(b->staged, nullptr));
+    } catch (...) {
+      batch_destroy_impl(b.release());
+      throw;
+    }
+    *out = b.release();
+  });
+}
+
+int bie_batch_destroy(bie_batch* b) {
+  if (!b) return BIE_OK;
+  batch_destroy_impl(b);
+  return BIE_OK;
+}
+
+int bie_batch_configure(bie_batch* b, int32_t elems_per_tile, int32_t slots, int32_t order) {
+  return guarded([&] {
+    BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
+    BIE_REQUIRE(elems_per_tile >= kSlotWaves && elems_per_tile <= (1 << 20), BIE_EINVAL,
+                "elements per tile out of range");
+    BIE_REQUIRE(slots == 32 || slots == 64, BIE_EINVAL, "slots must be 32 or 64");
+    BIE_REQUIRE(order == 0 || order == 1, BIE_EINVAL, "order must be 0 or 1");
+    std::lock_guard<std::mutex> bl(b->mu);
+    b->elems_per_tile = elems_per_tile;
+    b->slots = slots;
+    b->order = order;
+  });
+}
+
+int bie_batch_info(bie_batch* b, int32_t* elems_per_tile, int32_t* slots, int32_t* order,
+                   int64_t* n_tiles) {
+  return guarded([&] {
+    BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
+    std::lock_guard<std::mutex> bl(b->mu);
+    if (elems_per_tile) *elems_per_tile = b->elems_per_tile;
+    if (slots) *slots = b->slots;
+    if (order) *order = b->order;
+    if (n_tiles) *n_tiles = b->configured_key >= 0 ? b->n_tiles : -1;
+  });
+}
+
+int bie_batch_assemble(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                       double* const* dF, double* const* dG, double* const* dC, void* stream) {
+  return guarded([&] {
+    batch_assemble_impl(b, k_re, k_im, flags, dF, dG, dC, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_batch_tune(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                   double* const* dF, double* const* dG, double* const* dC, void* stream,
+                   double* best_ms) {
+  return guarded([&] {
+    BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BIE_HIP_CHECK(hipSetDevice(b->device));
+    hipEvent_t e0, e1;
+    BIE_HIP_CHECK(hipEventCreate(&e0));
+    BIE_HIP_CHECK(hipEventCreate(&e1));
+    struct Cand {
+      int T, S, order;
+    };
+    std::vector<Cand> cands;
+    for (int T : {32, 64, 128})
+      for (int S : {32, 64})
+        for (int o : {0, 1}) cands.push_back(Cand{T, S, o});
+    double best = INFINITY;
+    Cand pick = cands[0];
+    try {
+      for (const Cand& c : cands) {
+        {
+          std::lock_guard<std::mutex> bl(b->mu);
+          b->elems_per_tile = c.T;
+          b->slots = c.S;
+          b->order = c.order;
+        }
+        batch_assemble_impl(b, k_re, k_im, flags, dF, dG, dC, s);  // builds, warms up
+        BIE_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 2; ++r) batch_assemble_impl(b, k_re, k_im, flags, dF, dG, dC, s);
+        BIE_HIP_CHECK(hipEventRecord(e1, s));
+        BIE_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        BIE_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / 2.0 < best) {
+          best = ms / 2.0;
+          pick = c;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+      std::lock_guard<std::mutex> bl(b->mu);
+      b->elems_per_tile = pick.T;
+      b->slots = pick.S;
+      b->order = pick.order;
+    }
+    batch_assemble_impl(b, k_re, k_im, flags, dF, dG, dC, s);  // outputs of the pick
+    if (best_ms) *best_ms = best;
+  });
+}
+
 int bie_timing_enable(int on) {
   return guarded([&] {
     std::lock_guard<std::mutex> tl(g_timing.mu);
@@ -749,21 +1059,23 @@ int bie_timing_enable(int on) {
   });
 }
 
-int bie_timing_collect(double* kernel_ms, double* call_ms, int64_t* n_calls) {
+int bie_timing_collect(double* kernel_ms, double* mid_ms, double* call_ms, int64_t* n_calls) {
   return guarded([&] {
-    std::vector<std::array<hipEvent_t, 3>> rec;
+    std::vector<std::array<hipEvent_t, 4>> rec;
     {
       std::lock_guard<std::mutex> tl(g_timing.mu);
       rec.swap(g_timing.rec);
     }
-    double ks = 0.0, cs = 0.0;
+    double ks = 0.0, ms = 0.0, cs = 0.0;
     for (auto& ev : rec) {
-      BIE_HIP_CHECK(hipEventSynchronize(ev[2]));
-      float a = 0.f, b = 0.f;
+      BIE_HIP_CHECK(hipEventSynchronize(ev[3]));
+      float a = 0.f, b = 0.f, c = 0.f;
       BIE_HIP_CHECK(hipEventElapsedTime(&a, ev[1], ev[2]));
-      BIE_HIP_CHECK(hipEventElapsedTime(&b, ev[0], ev[2]));
+      BIE_HIP_CHECK(hipEventElapsedTime(&b, ev[2], ev[3]));
+      BIE_HIP_CHECK(hipEventElapsedTime(&c, ev[0], ev[3]));
       ks += a;
-      cs += b;
+      ms += b;
+      cs += c;
     }
     {
       std::lock_guard<std::mutex> tl(g_timing.mu);
@@ -771,6 +1083,7 @@ int bie_timing_collect(double* kernel_ms, double* call_ms, int64_t* n_calls) {
         for (auto e : ev) g_timing.pool.push_back(e);
     }
     if (kernel_ms) *kernel_ms = ks;
+    if (mid_ms) *mid_ms = ms;
     if (call_ms) *call_ms = cs;
     if (n_calls) *n_calls = (int64_t)rec.size();
   });

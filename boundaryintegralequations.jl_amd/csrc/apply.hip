@@ -7,9 +7,9 @@
 // (src/3d/fast_multipole_method.jl:249-287).
 //
 // v_mfma_f64_16x16x4_f64 operand layout (one double per lane for A and B, four per lane
-// for the 16 x 16 accumulator):
+// for the 16 x 16 accumulator), as measured on gfx950 with lane-tagged operands:
 //   A[i][k] in lane i + 16 k,  B[k][j] in lane j + 16 k,
-//   D[i][j] in lane j + 16 (i / 4), element i % 4.
+//   D[i][j] in lane j + 16 (i % 4), element i / 4.
 // A complex product uses four real MFMAs (Re += Ar Xr + (-Ai) Xi, Im += Ar Xi + Ai Xr).
 // Each wave owns 32 rows x 32 columns (2 x 2 tiles); a workgroup of 4 waves owns 128
 // rows; K is split over blockIdx.y and the partial sums are added into a zeroed Y with
@@ -114,7 +114,7 @@ k_zgemm_mfma(const double2* __restrict__ A, int64_t lda, const double2* __restri
       if (col >= nrhs) continue;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int64_t row = r0 + a * 16 + 4 * lk + t;
+        const int64_t row = r0 + a * 16 + 4 * t + lk;
         if (row < M) {
           double* y = reinterpret_cast<double*>(Y + row + (int64_t)col * ldy);
           unsafeAtomicAdd(y, re[a][b][t]);

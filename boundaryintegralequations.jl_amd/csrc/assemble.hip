@@ -240,38 +240,20 @@ k_assemble(const AssemblyArgs a, int ech) {
 // Pairs whose rule is a mid-edge split rule (quadratic physics only, ~1% of pairs) are
 // queued in LDS, moved to the global queue once per segment, and integrated by
 // k_assemble_mid.
+// One (row block, segment) tile of the slot kernel: 64 targets (one per lane) against the
+// segment's elements, W elements per step (one per wave).  acc holds S zeroed slots of
+// 4 x 64 doubles and is zero again on return (the schedule flushes every slot it uses by
+// the end of the segment); phi is the basis table in LDS; s_q / s_qn the LDS mid-edge
+// queue, whose count must be zero on entry (it is zero again on return when NP == 6).
+// Ends on a barrier.
 template <int NP, bool CPLX, int W>
-__global__ void __launch_bounds__(64 * W)
-k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
-  extern __shared__ double lds[];
-  double* acc = lds;                              // [slot][4][64]
-  double* s_phi = lds + (size_t)sa.S * 256;
-  const int nphi = a.R * a.Qmax * NP;
-  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
-  unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
-  // XCD-aware tile order: consecutive block ids go to different XCDs, so give each XCD
-  // a contiguous run of (row block, segment) tiles.
-  const int b = blockIdx.x;
-  const int per_xcd = (int)((sa.n_tiles + 7) / 8);
-  const int64_t tile = (int64_t)(b & 7) * per_xcd + (b >> 3);
-  if ((b >> 3) >= per_xcd || tile >= sa.n_tiles) return;
-  int rb, seg;
-  if (sa.seg_major) {  // an XCD's concurrent workgroups share one segment's tables in L2
-    seg = (int)(tile / sa.n_rowblocks);
-    rb = (int)(tile - (int64_t)seg * sa.n_rowblocks);
-  } else {             // an XCD's concurrent workgroups share row blocks (shared-column atomics)
-    rb = (int)(tile / sa.n_segments);
-    seg = (int)(tile - (int64_t)rb * sa.n_segments);
-  }
+__device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
+                                          int seg, double* acc, const double* phi, int2* s_q,
+                                          unsigned* s_qn) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = (int64_t)rb * 64 + lane;
   const bool active = i < a.M;
-  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
-  for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
-  if (threadIdx.x == 0) *s_qn = 0u;
-  __syncthreads();
-  const double* phi = s_phi;
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
@@ -379,20 +361,83 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
       for (unsigned t = lane; t < n; t += 64)
         if ((int64_t)(base + t) < sa.mid_cap) sa.mid_queue[base + t] = s_q[t];
     }
+    if (lane == 0) *s_qn = 0u;
   }
   if (active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
+  __syncthreads();
+}
+
+template <int NP, bool CPLX, int W>
+__global__ void __launch_bounds__(64 * W)
+k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
+  extern __shared__ double lds[];
+  double* acc = lds;                              // [slot][4][64]
+  double* s_phi = lds + (size_t)sa.S * 256;
+  const int nphi = a.R * a.Qmax * NP;
+  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
+  // XCD-aware tile order: consecutive block ids go to different XCDs, so give each XCD
+  // a contiguous run of (row block, segment) tiles.
+  const int b = blockIdx.x;
+  const int per_xcd = (int)((sa.n_tiles + 7) / 8);
+  const int64_t tile = (int64_t)(b & 7) * per_xcd + (b >> 3);
+  if ((b >> 3) >= per_xcd || tile >= sa.n_tiles) return;
+  int rb, seg;
+  if (sa.seg_major) {  // an XCD's concurrent workgroups share one segment's tables in L2
+    seg = (int)(tile / sa.n_rowblocks);
+    rb = (int)(tile - (int64_t)seg * sa.n_rowblocks);
+  } else {             // an XCD's concurrent workgroups share row blocks (shared-column atomics)
+    rb = (int)(tile / sa.n_segments);
+    seg = (int)(tile - (int64_t)rb * sa.n_segments);
+  }
+  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
+  for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
+  if (threadIdx.x == 0) *s_qn = 0u;
+  __syncthreads();
+  slot_tile<NP, CPLX, W>(a, sa, rb, seg, acc, s_phi, s_q, s_qn);
+}
+
+// Uneven batch: many meshes (same element kinds and rule set) in one persistent launch.
+// Each workgroup stages the shared basis table once, then takes (instance, row block,
+// segment) tiles from a global counter until the tile list is exhausted; tiles are
+// ordered by decreasing cost, so the dynamic hand-out is a longest-first list schedule.
+template <int NP, bool CPLX, int W>
+__global__ void __launch_bounds__(64 * W)
+k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ tiles,
+                 int n_tiles, unsigned* __restrict__ counter, const double* __restrict__ phi,
+                 int nphi, int S) {
+  extern __shared__ double lds[];
+  double* acc = lds;
+  double* s_phi = lds + (size_t)S * 256;
+  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
+  int* s_tile = reinterpret_cast<int*>(s_qn + 1);
+  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = phi[t];
+  for (int t = threadIdx.x; t < S * 256; t += blockDim.x) acc[t] = 0.0;
+  if (threadIdx.x == 0) *s_qn = 0u;
+  for (;;) {
+    if (threadIdx.x == 0) *s_tile = (int)min(atomicAdd(counter, 1u), (unsigned)n_tiles);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(*s_tile);
+    __syncthreads();  // every wave has read s_tile before thread 0 may write it again
+    if (t >= n_tiles) break;
+    const int4 td = tiles[t];
+    const int ii = __builtin_amdgcn_readfirstlane(td.x);
+    const int rb = __builtin_amdgcn_readfirstlane(td.y);
+    const int seg = __builtin_amdgcn_readfirstlane(td.z);
+    slot_tile<NP, CPLX, W>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_q, s_qn);
+  }
 }
 
 // Mid-edge split-rule pairs from the queue (rotated_midpoint_triangular_quadpoints,
 // quadrature.jl:117-144): one thread per pair, results added atomically after the slot
 // kernel has written its columns.
-__global__ void __launch_bounds__(256)
-k_assemble_mid(const AssemblyArgs a, const unsigned* __restrict__ count,
-               const int2* __restrict__ queue, int64_t cap, int cplx) {
+__device__ __forceinline__ void mid_pairs(const AssemblyArgs& a, const unsigned* count,
+                                          const int2* queue, int64_t cap, int cplx,
+                                          int64_t first, int64_t stride) {
   const int64_t n = min((int64_t)*count, cap);
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
+  for (int64_t t = first; t < n; t += stride) {
     const int2 en = queue[t];
     const int64_t i = en.x;
     const int e = en.y & 0x0fffffff;
@@ -421,6 +466,53 @@ k_assemble_mid(const AssemblyArgs a, const unsigned* __restrict__ count,
       }
     }
     if (a.flags & BIE_C) unsafeAtomicAdd(a.C + 2 * i, csum);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_assemble_mid(const AssemblyArgs a, const unsigned* __restrict__ count,
+               const int2* __restrict__ queue, int64_t cap, int cplx) {
+  mid_pairs(a, count, queue, cap, cplx, blockIdx.x * (int64_t)blockDim.x + threadIdx.x,
+            (int64_t)gridDim.x * blockDim.x);
+}
+
+// Mid-edge pairs of every quadratic instance of a batch (blockIdx.y = instance).
+__global__ void __launch_bounds__(256)
+k_assemble_mid_batch(const BatchInst* __restrict__ inst, int cplx) {
+  const BatchInst& b = inst[blockIdx.y];
+  if (b.a.NP != 6 || b.a.M == 0) return;
+  mid_pairs(b.a, b.sa.mid_count, b.sa.mid_queue, b.sa.mid_cap, cplx,
+            blockIdx.x * (int64_t)blockDim.x + threadIdx.x, (int64_t)gridDim.x * blockDim.x);
+}
+
+// Per-call preparation of every instance of a batch (blockIdx.y = instance): C zeroed,
+// the shared columns of enabled F / G zeroed (the batch kernel adds into them), disabled
+// non-NULL outputs zeroed whole, and the mid-edge queue count reset.
+__global__ void __launch_bounds__(256) k_batch_prep(const BatchInst* __restrict__ inst) {
+  const BatchInst& b = inst[blockIdx.y];
+  const AssemblyArgs& a = b.a;
+  const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const double2 z = make_double2(0.0, 0.0);
+  if (first == 0 && b.sa.mid_count) *b.sa.mid_count = 0u;
+  if (a.C)
+    for (int64_t t = first; t < a.M; t += stride) reinterpret_cast<double2*>(a.C)[t] = z;
+  double* outs[2] = {a.F, a.G};
+  const unsigned bits[2] = {BIE_F, BIE_G};
+  for (int o = 0; o < 2; ++o) {
+    double2* X = reinterpret_cast<double2*>(outs[o]);
+    if (!X) continue;
+    if (a.flags & bits[o]) {
+      const int64_t total = (int64_t)b.n_zero * a.M;
+      for (int64_t t = first; t < total; t += stride) {
+        const int64_t c = t / a.M;
+        const int64_t r = t - c * a.M;
+        X[(size_t)b.zero_cols[c] * a.M + r] = z;
+      }
+    } else {
+      const int64_t total = a.N * a.M;
+      for (int64_t t = first; t < total; t += stride) X[t] = z;
+    }
   }
 }
 
@@ -533,6 +625,83 @@ void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t 
   if (a.NP != 6 || a.M == 0 || a.E == 0) return;
   hipLaunchKernelGGL(k_assemble_mid, dim3(2048), dim3(256), 0, s, a, sa.mid_count,
                      sa.mid_queue, sa.mid_cap, a.ki != 0.0 ? 1 : 0);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_work, hipStream_t s) {
+  if (n_inst <= 0) return;
+  BIE_REQUIRE(n_inst <= 65535, BIE_EINVAL, "too many batch instances");
+  int64_t blocks = (max_work + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_batch_prep, dim3((unsigned)blocks, (unsigned)n_inst), dim3(256), 0, s,
+                     d_inst);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+size_t batch_shmem(int S, int nphi) {
+  return sizeof(double) * ((size_t)S * 256 + (size_t)nphi) + sizeof(int2) * kMidLds + 16;
+}
+
+void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
+                           unsigned* d_counter, const double* d_phi, int nphi, int S, int NP,
+                           bool cplx, int grid, hipStream_t s) {
+  if (n_tiles <= 0) return;
+  const size_t shmem = batch_shmem(S, nphi);
+  BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
+  BIE_REQUIRE(grid >= 1, BIE_EINVAL, "empty persistent grid");
+  launch_fill_zero_u32(d_counter, s);
+  constexpr int W = kSlotWaves;
+  auto go = [&](auto kern) {
+    BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), shmem, s, d_inst, d_tiles,
+                       n_tiles, d_counter, d_phi, nphi, S);
+  };
+  if (NP == 3) {
+    if (cplx)
+      go(k_assemble_batch<3, true, W>);
+    else
+      go(k_assemble_batch<3, false, W>);
+  } else {
+    if (cplx)
+      go(k_assemble_batch<6, true, W>);
+    else
+      go(k_assemble_batch<6, false, W>);
+  }
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+// Resident workgroups of the batch kernel per CU for slot count S (LDS-bound).
+int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi) {
+  const size_t shmem = batch_shmem(S, nphi);
+  BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
+  constexpr int W = kSlotWaves;
+  int nb = 0;
+  auto q = [&](auto kern) {
+    BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    BIE_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void*>(kern), 64 * W, shmem));
+  };
+  if (NP == 3) {
+    if (cplx)
+      q(k_assemble_batch<3, true, W>);
+    else
+      q(k_assemble_batch<3, false, W>);
+  } else {
+    if (cplx)
+      q(k_assemble_batch<6, true, W>);
+    else
+      q(k_assemble_batch<6, false, W>);
+  }
+  return nb < 1 ? 1 : nb;
+}
+
+void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, hipStream_t s) {
+  if (n_inst <= 0) return;
+  hipLaunchKernelGGL(k_assemble_mid_batch, dim3(256, (unsigned)n_inst), dim3(256), 0, s, d_inst,
+                     cplx ? 1 : 0);
   BIE_HIP_CHECK(hipGetLastError());
 }
 

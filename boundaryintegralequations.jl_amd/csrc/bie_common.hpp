@@ -102,6 +102,16 @@ struct SlotArgs {
   int64_t mid_cap;
 };
 
+// One instance of an uneven batch (assemble.hip, k_assemble_batch): its assembly and
+// slot-schedule arguments, and the columns its shared-column atomics need zeroed.
+struct BatchInst {
+  AssemblyArgs a;
+  SlotArgs sa;
+  const int* zero_cols;
+  int n_zero;
+  int pad;
+};
+
 // Batched frequency sweep (sweep.hip).
 struct SweepArgs {
   const double* k;           // nk real wavenumbers (device)
@@ -146,6 +156,13 @@ void launch_assemble(const AssemblyArgs& a, hipStream_t s);
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
+void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_work, hipStream_t s);
+size_t batch_shmem(int S, int nphi);
+void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
+                           unsigned* d_counter, const double* d_phi, int nphi, int S, int NP,
+                           bool cplx, int grid, hipStream_t s);
+void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, hipStream_t s);
+int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
@@ -177,10 +194,11 @@ struct bie_dmesh {
   double* d_pts = nullptr;
   double* d_phi = nullptr;
   double* d_elem = nullptr;
+  bool geom_valid = false;      // d_elem holds the geometry of the cached rule set
   // host copies for scheduling
   std::vector<double> h_coords, h_sources;
   std::vector<int> h_topo, h_ptopo;
-  // slot schedules (mesh-only): [0] single-k kernel, [1] sweep kernel
+  // slot schedules (mesh-only): [0] single-k kernel, [1] sweep kernel, [2] batch kernel
   struct Sched {
     int S = -1, key = -1;
     int segments = 0, zero_cols = 0, max_live = 0;
@@ -192,12 +210,36 @@ struct bie_dmesh {
     int* d_flush_node = nullptr;
     int* d_flush_slot = nullptr;
     int* d_zero_cols = nullptr;
-  } sched[2];
+    std::vector<int> h_seg_step_begin;  // host copy (tile costs of the batch kernel)
+  } sched[3];
   int* d_block_uniform = nullptr;
   int block_uniform_cap = 0;
   // mid-edge pair queue
   unsigned* d_mid_count = nullptr;
   int2* d_mid_queue = nullptr;
   int64_t mid_cap = 0;
+  std::mutex mu;
+};
+
+// An uneven batch: several device meshes (same element kinds) assembled with one rule set
+// in one persistent launch (bie_batch_* in include/bie_amd.h).
+struct bie_batch {
+  int device = 0;
+  int gk = 3, pk = 3, n = 0, m = 0;
+  std::vector<bie_dmesh*> meshes;
+  // schedule configuration (chosen by bie_batch_tune)
+  int elems_per_tile = 64;   // target elements per segment
+  int slots = 64;            // LDS slots per workgroup
+  int order = 0;             // 0: longest tiles first, 1: instances interleaved
+  int configured_key = -1;   // configuration the tile list was built for
+  // device state
+  bie::BatchInst* d_inst = nullptr;
+  bie::BatchInst* h_inst = nullptr;  // pinned staging for the per-call table
+  hipEvent_t staged = nullptr;       // the last upload of h_inst has completed
+  int4* d_tiles = nullptr;
+  int n_tiles = 0;
+  unsigned* d_counter = nullptr;
+  std::vector<int64_t> rowblocks;
+  std::vector<int> min_segments;     // per instance: the schedule the tile list indexes
   std::mutex mu;
 };

@@ -49,6 +49,7 @@ typedef struct bie_mesh {
 
 typedef struct bie_dmesh bie_dmesh;  /* device-resident mesh + cached rule tables */
 typedef struct bie_graph bie_graph;  /* an instantiated HIP graph of one batched assembly */
+typedef struct bie_batch bie_batch;  /* an uneven batch of device meshes, one rule set */
 
 const char* bie_last_error(void);
 int bie_version(void);
@@ -118,6 +119,37 @@ int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
 int bie_graph_launch(bie_graph* g, void* stream);
 int bie_graph_destroy(bie_graph* g);
 
+/* ---- Uneven batch ---------------------------------------------------------------------
+ * Collocation assembly (targets = each mesh's sources) of many meshes of different sizes,
+ * each with its own wavenumber, in one persistent kernel launch: every workgroup stages
+ * the shared basis table once and takes (instance, 64-row block, element segment) tiles
+ * from a device counter.  All meshes share geometry and physics kinds and the rule set
+ * (n, m); each instance i writes its own F_i, G_i (N_i x N_i, column-major complex128) and
+ * C_i (N_i), same values as bie_assemble_device on that mesh.  Arrays of per-instance
+ * values (k_re, k_im, dF, dG, dC) live on the host; the pointers in dF / dG / dC are device
+ * pointers.  k_im may be NULL (real wavenumbers).  A disabled F / G (flag off) may be NULL;
+ * if given, it is zeroed.  No counterpart in the reference, whose assembly handles one
+ * mesh and one k per call (assembly_collocation.jl:259-268); a batch of uneven meshes was
+ * handled there by looping that call.  The batch keeps the dmesh pointers: destroy it
+ * before them. */
+int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_t m,
+                     bie_batch** out);
+int bie_batch_destroy(bie_batch* b);
+/* Schedule of the persistent launch: target elements per segment (>= 8), LDS slots per
+ * workgroup (32 or 64: two or one resident workgroups per CU) and tile order (0: longest
+ * tiles first, 1: interleaved across instances).  Takes effect at the next call. */
+int bie_batch_configure(bie_batch* b, int32_t elems_per_tile, int32_t slots, int32_t order);
+int bie_batch_info(bie_batch* b, int32_t* elems_per_tile, int32_t* slots, int32_t* order,
+                   int64_t* n_tiles);
+int bie_batch_assemble(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                       double* const* dF, double* const* dG, double* const* dC, void* stream);
+/* Autotuner: times every schedule in {32, 64, 128} elements per tile x {32, 64} slots x
+ * {longest-first, interleaved} on these inputs (synchronising), keeps the fastest and
+ * leaves its outputs in dF / dG / dC; *best_ms (may be NULL) receives its time. */
+int bie_batch_tune(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                   double* const* dF, double* const* dG, double* const* dC, void* stream,
+                   double* best_ms);
+
 /* ---- Adjoint building block ------------------------------------------------------------
  * Vector-Jacobian product of the assembly at real k with the rule selection held fixed:
  *   d_grad_coords[3*v + d] += d/dX[v,d]  Re[ sum_i alpha_i sum_j F_ij p_j + sum_i beta_i C_i ]
@@ -146,13 +178,16 @@ int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream);
 
 /* ---- Kernel timing ------------------------------------------------------------------
- * While enabled, every device assembly records three HIP events on its launch stream:
- * call start, before and after the dense assembly kernel.  bie_timing_collect waits for
- * the recorded events, returns the summed kernel and whole-call durations (ms) and the
- * number of calls, and clears the record.  No counterpart in the reference (it only
- * wraps calls in @time, examples/sphere.jl:51). */
+ * While enabled, every device assembly records four HIP events on its launch stream:
+ * call start, before the dense assembly kernel, between it and the mid-edge kernel (the
+ * pairs that need the split rules 4-6; zero time on linear physics), and after it.
+ * bie_timing_collect waits for the recorded events, returns the summed dense-kernel,
+ * mid-edge-kernel and whole-call durations (ms) and the number of calls, and clears the
+ * record.  No counterpart in the reference (it only wraps calls in @time,
+ * examples/sphere.jl:51). */
 int bie_timing_enable(int on);
-int bie_timing_collect(double* kernel_ms, double* call_ms, int64_t* n_calls);
+int bie_timing_collect(double* kernel_ms, double* mid_ms, double* call_ms,
+                       int64_t* n_calls);
 
 /* ---- Device memory helpers (for callers without their own allocator) -------------- */
 int bie_device_malloc(void** ptr, int64_t bytes);
