@@ -150,12 +150,103 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
         err = max(err, float(torch.linalg.norm(F[sl] - Fp[sl]) / torch.linalg.norm(Fp[sl])),
                   float(torch.linalg.norm(G[sl] - Gp[sl]) / torch.linalg.norm(Gp[sl])))
     g.close()
+    # per-k GMRES (tol 1e-10) for the rigid sphere under p_inc = e^{ikz}: A_k = F_k + diag(1 - C)
+    from bie_amd.solvers import collocation_operator, gmres
+    Fc = torch.view_as_complex(F.view(nk, N, N, 2))
+    Cc = torch.view_as_complex(C.view(N, 2))
+    z = torch.as_tensor(mesh.sources[2], device=dev)
+    its, rrs = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in range(nk):
+        b = torch.exp(1j * float(ks[q]) * z)
+        _, it, rr = gmres(torch, collocation_operator(torch, Fc[q], Cc), b, tol=1e-10)
+        its.append(it)
+        rrs.append(rr)
+    torch.cuda.synchronize()
+    res["gmres"] = {"tol": 1e-10, "total_s": time.perf_counter() - t0,
+                    "iterations_min_mean_max": [min(its), sum(its) / nk, max(its)],
+                    "max_rel_residual": max(rrs)}
     res["speedup"] = res["per_k_launches_ms"] / res["batched_graph_ms"]
     res["max_rel_err_batched_vs_per_k"] = err
     res["workload"] = (f"linear geodesic sphere N={N}, E={mesh.n_elements}, {nk} k uniform in "
                        f"[0.5, 30], (n,m)=({n},{m}), F_k and G_k for every k")
     res["entries_per_s_batched"] = nk * 2 * N * N / (res["batched_graph_ms"] * 1e-3)
     del F, G, Fp, Gp
+    torch.cuda.empty_cache()
+    return res
+
+
+def measure_batch(torch, bie_amd, steps=3, n=4, m=4):
+    """Config 5: 48 ellipsoids (semi-axes U[0.5, 2], N log-uniform in [500, 8000] rounded to
+    10 f^2 + 2, k U[1, 20], seed 3), complex128 F, G, C each, (n,m)=(4,4).  One tuned
+    persistent batch launch vs a loop of 48 single-mesh device assemblies; parity between
+    the two on every instance."""
+    from bie_amd.assembly import Batch, DeviceMesh
+    meshes, ks = bie_amd.mesh.uneven_ellipsoid_batch()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    Ns = [x.n_sources for x in meshes]
+    offs = [0]
+    for N in Ns:
+        offs.append(offs[-1] + N * N)
+    total = offs[-1]
+    FB = torch.empty(2 * total, dtype=torch.float64, device=dev)
+    GB = torch.empty_like(FB)
+    FL = torch.empty_like(FB)
+    GL = torch.empty_like(FB)
+    CB = torch.empty(2 * sum(Ns), dtype=torch.float64, device=dev)
+    CL = torch.empty_like(CB)
+    coff = [0]
+    for N in Ns:
+        coff.append(coff[-1] + N)
+    dms = [DeviceMesh(x, 0) for x in meshes]
+    t0 = time.perf_counter()
+    b = Batch(dms, n=n, m=m)
+    ptr = lambda T, o: T.data_ptr() + 16 * o
+    Fp = [ptr(FB, o) for o in offs[:-1]]
+    Gp = [ptr(GB, o) for o in offs[:-1]]
+    Cp = [ptr(CB, o) for o in coff[:-1]]
+    best_ms = b.tune(ks, Fp, Gp, Cp, stream=stream)
+    tune_s = time.perf_counter() - t0
+
+    def batched():
+        b.assemble(ks, Fp, Gp, Cp, stream=stream)
+
+    def loop():
+        for i, dm in enumerate(dms):
+            dm.assemble(ks[i], ptr(FL, offs[i]), ptr(GL, offs[i]), ptr(CL, coff[i]), n=n, m=m,
+                        stream=stream)
+
+    res = {}
+    for name, fn in (("loop_plain", loop), ("batched_persistent", batched)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name + "_ms"] = e0.elapsed_time(e1) / steps
+    err = 0.0
+    for i in range(len(Ns)):
+        sl = slice(2 * offs[i], 2 * offs[i + 1])
+        err = max(err, float(torch.linalg.norm(FB[sl] - FL[sl]) / torch.linalg.norm(FL[sl])),
+                  float(torch.linalg.norm(GB[sl] - GL[sl]) / torch.linalg.norm(GL[sl])))
+    entries = sum(2 * N * N + N for N in Ns)
+    res.update({
+        "speedup": res["loop_plain_ms"] / res["batched_persistent_ms"],
+        "entries_per_s_batched": entries / (res["batched_persistent_ms"] * 1e-3),
+        "entries_per_s_loop": entries / (res["loop_plain_ms"] * 1e-3),
+        "max_rel_err_batched_vs_loop": err,
+        "schedule": b.info(), "tune_best_ms": best_ms, "tune_wall_s": tune_s,
+        "workload": (f"{len(Ns)} ellipsoids, N {min(Ns)}..{max(Ns)} (sum N^2 = "
+                     f"{total:.3e}), k U[1,20], (n,m)=({n},{m}), F, G, C per instance"),
+    })
+    b.close()
+    del FB, GB, FL, GL, CB, CL
     torch.cuda.empty_cache()
     return res
 
@@ -269,6 +360,7 @@ def main():
         del G
         torch.cuda.empty_cache()
         extra["sweep"] = measure_sweep(torch, bie_amd)
+        extra["uneven_batch"] = measure_batch(torch, bie_amd)
         G = torch.empty(2 * N * N, dtype=torch.float64, device=dev)
         run(n, m, 1, 0)
 

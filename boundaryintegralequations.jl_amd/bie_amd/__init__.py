@@ -6,7 +6,7 @@ its host-side mirror: mesh data model and generators, ctypes binding, and the
 ``assemble_parallel`` entry point with the reference's keyword interface.
 """
 from .mesh import Mesh3d, TRI3, TRI6, icosphere, ellipsoid, load_comsol, read_comsol  # noqa: F401
-from .assembly import assemble_parallel, DeviceMesh  # noqa: F401
+from .assembly import assemble_parallel, Batch, DeviceMesh  # noqa: F401
 
 __all__ = ["Mesh3d", "TRI3", "TRI6", "icosphere", "ellipsoid", "load_comsol", "read_comsol",
            "assemble_parallel", "DeviceMesh"]

@@ -291,3 +291,20 @@ def fibonacci_sphere(n: int, radius: float = 1.0) -> np.ndarray:
 def golden_dir() -> str:
     here = os.path.dirname(os.path.abspath(__file__))
     return os.path.normpath(os.path.join(here, "..", "..", "tests", "golden"))
+
+
+def uneven_ellipsoid_batch(n_inst: int = 48, n_range=(500, 8000), axes_range=(0.5, 2.0),
+                           k_range=(1.0, 20.0), seed: int = 3, order: str = "linear"):
+    """Uneven batch of ellipsoids: semi-axes uniform in axes_range, node counts log-uniform
+    in n_range rounded to the nearest geodesic count 10 f^2 + 2, wavenumbers uniform in
+    k_range.  Returns (meshes, ks)."""
+    rng = np.random.default_rng(seed)
+    meshes, ks = [], []
+    for _ in range(n_inst):
+        axes = rng.uniform(axes_range[0], axes_range[1], 3)
+        n_target = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))
+        f = max(1, int(round(np.sqrt((n_target - 2) / 10.0))))
+        k = rng.uniform(k_range[0], k_range[1])
+        meshes.append(ellipsoid(f, tuple(axes), order))
+        ks.append(float(k))
+    return meshes, ks

@@ -945,7 +945,7 @@ int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_
       BIE_HIP_CHECK(hipHostMalloc(&h, sizeof(BatchInst) * n_inst, hipHostMallocDefault));
       b->h_inst = static_cast<BatchInst*>(h);
       b->d_inst = dalloc<BatchInst>(n_inst);
-      b->d_counter = dalloc<unsigned>(1);
+      b->d_counter = dalloc<unsigned>(kXcds);
       BIE_HIP_CHECK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
       BIE_HIP_CHECK(hipEventRecord(b->staged, nullptr));
     } catch (...) {

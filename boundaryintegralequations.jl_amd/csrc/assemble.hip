@@ -399,12 +399,15 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
 
 // Uneven batch: many meshes (same element kinds and rule set) in one persistent launch.
 // Each workgroup stages the shared basis table once, then takes (instance, row block,
-// segment) tiles from a global counter until the tile list is exhausted; tiles are
-// ordered by decreasing cost, so the dynamic hand-out is a longest-first list schedule.
+// segment) tiles until the tile list is exhausted.  The list is cut into kXcds contiguous
+// queues, one per XCD (workgroup b runs on XCD b mod 8), so an XCD's concurrent tiles
+// share a segment's element tables in its L2; a workgroup whose queue is empty takes tiles
+// from the following queues.  Every queue's counter only grows, so each workgroup leaves
+// after finding all queues exhausted once.
 template <int NP, bool CPLX, int W>
 __global__ void __launch_bounds__(64 * W)
 k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ tiles,
-                 int n_tiles, unsigned* __restrict__ counter, const double* __restrict__ phi,
+                 int n_tiles, unsigned* __restrict__ counters, const double* __restrict__ phi,
                  int nphi, int S) {
   extern __shared__ double lds[];
   double* acc = lds;
@@ -415,17 +418,26 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
   for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = phi[t];
   for (int t = threadIdx.x; t < S * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
-  for (;;) {
-    if (threadIdx.x == 0) *s_tile = (int)min(atomicAdd(counter, 1u), (unsigned)n_tiles);
-    __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(*s_tile);
-    __syncthreads();  // every wave has read s_tile before thread 0 may write it again
-    if (t >= n_tiles) break;
-    const int4 td = tiles[t];
-    const int ii = __builtin_amdgcn_readfirstlane(td.x);
-    const int rb = __builtin_amdgcn_readfirstlane(td.y);
-    const int seg = __builtin_amdgcn_readfirstlane(td.z);
-    slot_tile<NP, CPLX, W>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_q, s_qn);
+  const int x0 = blockIdx.x & (kXcds - 1);
+  for (int q = 0; q < kXcds; ++q) {
+    const int x = (x0 + q) & (kXcds - 1);
+    const int begin = (int)(((int64_t)n_tiles * x) / kXcds);
+    const int end = (int)(((int64_t)n_tiles * (x + 1)) / kXcds);
+    for (;;) {
+      if (threadIdx.x == 0) {
+        const unsigned got = atomicAdd(counters + x, 1u);
+        *s_tile = (got < (unsigned)(end - begin)) ? begin + (int)got : -1;
+      }
+      __syncthreads();
+      const int t = __builtin_amdgcn_readfirstlane(*s_tile);
+      __syncthreads();  // every wave has read s_tile before thread 0 may write it again
+      if (t < 0) break;
+      const int4 td = tiles[t];
+      const int ii = __builtin_amdgcn_readfirstlane(td.x);
+      const int rb = __builtin_amdgcn_readfirstlane(td.y);
+      const int seg = __builtin_amdgcn_readfirstlane(td.z);
+      slot_tile<NP, CPLX, W>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_q, s_qn);
+    }
   }
 }
 
@@ -531,6 +543,10 @@ __global__ void k_zero_columns(double2* __restrict__ X, int64_t M, const int* __
 }  // namespace
 
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
+__global__ void k_zero_u32_n(unsigned* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
 
 void launch_fill_zero_u32(unsigned* p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, p);
@@ -650,7 +666,8 @@ void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_t
   const size_t shmem = batch_shmem(S, nphi);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
   BIE_REQUIRE(grid >= 1, BIE_EINVAL, "empty persistent grid");
-  launch_fill_zero_u32(d_counter, s);
+  hipLaunchKernelGGL(k_zero_u32_n, dim3(1), dim3(64), 0, s, d_counter, kXcds);
+  BIE_HIP_CHECK(hipGetLastError());
   constexpr int W = kSlotWaves;
   auto go = [&](auto kern) {
     BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),

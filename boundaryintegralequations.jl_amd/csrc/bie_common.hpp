@@ -82,6 +82,7 @@ struct AssemblyArgs {
 constexpr int kSlotWaves = 8;   // waves per workgroup of the slot kernel
 constexpr int kSlotCount = 64;  // LDS slots (64 rows x 4 doubles each)
 constexpr int kMidLds = 512;    // per-workgroup LDS queue of mid-edge pairs
+constexpr int kXcds = 8;        // XCDs of an MI355X; workgroup b is dispatched to XCD b mod 8
 
 struct SlotArgs {
   int W, S;

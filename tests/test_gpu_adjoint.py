@@ -91,3 +91,84 @@ def test_scattering_gradient_matches_finite_differences():
         assert abs(grad[d, j] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(grad).max()), (d, j)
     fdk = (objective(X, k + h) - objective(X, k - h)) / (2 * h)
     assert abs(dk - fdk) <= 1e-6 * abs(fdk)
+
+
+def test_config4_adjoint_full_size_against_finite_differences():
+    """Full config 4: 2562-node linear sphere with radial N(0, 0.01^2) perturbation (seed 1),
+    k = 5, 100 Fibonacci field points on r = 3, (n, m) = (4, 4); central differences with
+    h = 1e-6 on 50 coordinates drawn with seed 2 (draws whose +-h evaluations change a rule
+    choice are replaced), and on k.  The differences use the GPU assembly and a GPU LU
+    solve."""
+    import torch
+
+    mesh = M.perturb_radial(M.icosphere(16, "linear"), 0.01, 1)
+    assert mesh.n_sources == 2562 and mesh.n_elements == 5120
+    pts = M.fibonacci_sphere(100, 3.0)
+    k, n, m = 5.0, 4, 4
+    dev = torch.device("cuda:0")
+    print("[case] config4 adjoint full size", flush=True)
+    J, grad, dk = scattering_gradient(mesh, k, pts, n=n, m=m)
+    N, P = mesh.n_sources, pts.shape[1]
+    tp = torch.as_tensor(np.asfortranarray(pts).ravel(order="F"), device=dev)
+
+    def rules(dm):
+        rs = torch.empty(N * dm.n_elements, dtype=torch.int8, device=dev)
+        rf = torch.empty(P * dm.n_elements, dtype=torch.int8, device=dev)
+        dm.select_rules(rs.data_ptr())
+        dm.select_rules(rf.data_ptr(), targets_ptr=tp.data_ptr(), n_targets=P)
+        torch.cuda.synchronize()
+        return rs, rf
+
+    def objective(X, kk):
+        mh = M.with_coordinates(mesh, X)
+        dm = DeviceMesh(mh)
+        F = torch.empty((N, N), dtype=torch.complex128, device=dev)   # F.T in memory
+        G = torch.empty_like(F)
+        C = torch.empty(N, dtype=torch.complex128, device=dev)
+        Ff = torch.empty((N, P), dtype=torch.complex128, device=dev)  # Ff.T in memory
+        Gf = torch.empty_like(Ff)
+        Cf = torch.empty(P, dtype=torch.complex128, device=dev)
+        dm.assemble(kk, F.data_ptr(), G.data_ptr(), C.data_ptr(), n=n, m=m, gOn=False)
+        dm.assemble(kk, Ff.data_ptr(), Gf.data_ptr(), Cf.data_ptr(), targets_ptr=tp.data_ptr(),
+                    n_targets=P, n=n, m=m, gOn=False, cOn=False)
+        torch.cuda.synchronize()
+        A = F.T + torch.diag(1.0 - C)
+        b = torch.as_tensor(plane_wave(mh.sources, kk), device=dev)
+        p = torch.linalg.solve(A, b)
+        u = torch.as_tensor(plane_wave(pts, kk), device=dev) - Ff.T @ p
+        return float(torch.sum(torch.abs(u) ** 2)), rules(dm)
+
+    X = mesh.coordinates
+    J0, (rs0, rf0) = objective(X, k)
+    assert abs(J0 - J) <= 1e-10 * J
+    rng = np.random.default_rng(2)
+    h = 1e-6
+    ad, fd, skipped = [], [], 0
+    # 50 coordinates at which J is differentiable: a draw whose +-h evaluations change a
+    # rule choice (the selection is piecewise constant in X) is replaced by the next one
+    for flat in rng.permutation(X.size):
+        if len(fd) == 50:
+            break
+        d, j = np.unravel_index(flat, X.shape)
+        Xp = X.copy(); Xp[d, j] += h
+        Xm = X.copy(); Xm[d, j] -= h
+        Jp, (rsp, rfp) = objective(Xp, k)
+        Jm, (rsm, rfm) = objective(Xm, k)
+        if not (torch.equal(rsp, rs0) and torch.equal(rsm, rs0) and torch.equal(rfp, rf0)
+                and torch.equal(rfm, rf0)):
+            skipped += 1            # a rule choice flips inside [-h, h]: not differentiable
+            continue
+        fd.append((Jp - Jm) / (2 * h))
+        ad.append(grad[d, j])
+    ad, fd = np.array(ad), np.array(fd)
+    err = np.abs(ad - fd)
+    print(f"[config4] J={J:.6e} skipped={skipped} max|fd|={np.abs(fd).max():.3e} "
+          f"max abs err={err.max():.3e} vec rel err={np.linalg.norm(ad - fd) / np.linalg.norm(fd):.3e}",
+          flush=True)
+    assert len(fd) == 50 and skipped <= 50
+    assert np.linalg.norm(ad - fd) <= 1e-6 * np.linalg.norm(fd)
+    Jkp, _ = objective(X, k + h)
+    Jkm, _ = objective(X, k - h)
+    fdk = (Jkp - Jkm) / (2 * h)
+    print(f"[config4] dJ/dk adjoint={dk:.12e} fd={fdk:.12e}", flush=True)
+    assert abs(dk - fdk) <= 1e-6 * abs(fdk)

@@ -75,3 +75,38 @@ def test_sweep_graph_replays():
     Fo, Go, _ = oracle.assemble(mesh, ks[5], n=3, m=3)
     for Fs, Gs, c in outs:
         assert rel(Fs[5], Fo) < TOL and rel(Gs[5], Go) < TOL
+
+
+def test_sweep_then_gmres_rigid_sphere():
+    """Config-3 shape at reduced size: batched sweep, then per-k GMRES (tol 1e-10) on
+    A_k = F_k + diag(1 - C) for a rigid sphere under p_inc = e^{ikz}; solutions against an
+    oracle dense solve and the analytic series (theta measured from -z for +z incidence)."""
+    import torch
+    import analytic
+    from bie_amd.solvers import collocation_operator, gmres
+
+    mesh = M.icosphere(8, "linear")
+    N = mesh.n_sources
+    ks = np.linspace(0.5, 6.0, 6)
+    dev = torch.device("cuda:0")
+    dm = DeviceMesh(mesh)
+    kd = torch.as_tensor(ks, device=dev)
+    F = torch.empty((len(ks), N, N), dtype=torch.complex128, device=dev)
+    G = torch.empty_like(F)
+    C = torch.empty(N, dtype=torch.complex128, device=dev)
+    print("[case] sweep + gmres", flush=True)
+    dm.assemble_sweep(kd.data_ptr(), len(ks), F.data_ptr(), G.data_ptr(), N * N, C.data_ptr(),
+                      n=4, m=4, gOn=False)
+    torch.cuda.synchronize()
+    z = mesh.sources[2]
+    th = np.arccos(np.clip(-z, -1, 1))
+    for q, k in enumerate(ks):
+        b = torch.as_tensor(np.exp(1j * k * z), device=dev)
+        x, it, rr = gmres(torch, collocation_operator(torch, F[q], C), b, tol=1e-10)
+        assert rr <= 1e-10 and it < 100
+        p = x.cpu().numpy()
+        Fo, _, Co = oracle.assemble(mesh, k, n=4, m=4, gOn=False)
+        pd = np.linalg.solve(Fo + np.diag(1.0 - Co), np.exp(1j * k * z))
+        assert rel(p, pd) < 1e-8
+        pa = analytic.plane_wave_scattering_sphere(k, 1.0, 1.0, th, 1e-8)
+        assert rel(p, pa) < 0.0045 * k + 0.001
