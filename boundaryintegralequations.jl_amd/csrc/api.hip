@@ -150,12 +150,16 @@ bool use_atomic_kernel() {
   return v && std::strcmp(v, "atomic") == 0;
 }
 
-// LDS slots per workgroup of the single-k slot kernel: 64 (128 KiB of accumulators, one
-// workgroup per CU) unless BIE_AMD_SLOTS selects 32 (64 KiB, two workgroups per CU).
-int plain_slot_count() {
-  const char* v = std::getenv("BIE_AMD_SLOTS");
-  if (v && std::strcmp(v, "32") == 0) return 32;
-  return kSlotCount;
+// LDS slots per workgroup of the single-k slot kernel: as many as fit beside the basis
+// table, the staged table rows and the mid-edge queue (one workgroup per CU);
+// BIE_AMD_SLOTS=<n> asks for fewer.
+int plain_slot_count(int nphi, int Qc) {
+  int S = fitting_slots(phi_pad(nphi) + stage_doubles(Qc), 1);
+  if (const char* v = std::getenv("BIE_AMD_SLOTS")) {
+    const int x = std::atoi(v);
+    if (x >= 8 && x < S) S = x;
+  }
+  return S;
 }
 
 template <class T>
@@ -335,7 +339,8 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     } else {
       const int64_t rowblocks = (M + 63) / 64;
       const int min_segments = (int)std::max<int64_t>(1, std::min<int64_t>(dm->E, (2048 + rowblocks - 1) / rowblocks));
-      const bie_dmesh::Sched& sc = ensure_schedule(dm, 0, plain_slot_count(), min_segments, s);
+      const bie_dmesh::Sched& sc =
+          ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0]), min_segments, s);
       if (dm->pk == 6) {
         ensure_mid_queue(dm, M * dm->E);
         launch_fill_zero_u32(dm->d_mid_count, s);
@@ -384,7 +389,8 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   const int KB = sweep_kb(dm->pk);
   BIE_REQUIRE(sizeof(double) * (size_t)dm->rs.R * dm->rs.Qmax * dm->pk <= 24 * 1024,
               BIE_EUNSUPPORTED, "sweep supports quadrature orders up to n*m' = 36");
-  const bie_dmesh::Sched& sc = ensure_schedule(dm, 1, kSlotCount / KB, 1, s);
+  const bie_dmesh::Sched& sc =
+      ensure_schedule(dm, 1, fitting_slots(dm->rs.R * dm->rs.Qmax * dm->pk, KB), 1, s);
   if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
   if (dm->pk == 6) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
   const int nkb = (nk + KB - 1) / KB;
@@ -429,11 +435,16 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
 // instance, segment, row block order) or interleaved across instances (order 1).
 void batch_configure(bie_batch* b, hipStream_t s) {
   const int key = b->elems_per_tile * 1000 + b->slots * 10 + b->order;
-  bool current = b->configured_key == key && b->d_tiles != nullptr;
+  bie_dmesh* d0 = b->meshes[0];
+  const int S = b->slots ? b->slots
+                         : fitting_slots(phi_pad(d0->rs.R * d0->rs.Qmax * d0->pk) +
+                                             stage_doubles(d0->rs.len[0]),
+                                         1);
+  bool current = b->configured_key == key && b->d_tiles != nullptr && b->slots_used == S;
   // a mesh's batch schedule may have been rebuilt by another batch that shares it
   for (size_t ii = 0; current && ii < b->meshes.size(); ++ii) {
     const bie_dmesh::Sched& sc = b->meshes[ii]->sched[2];
-    current = sc.d_step_elems != nullptr && sc.S == b->slots && sc.key == b->min_segments[ii];
+    current = sc.d_step_elems != nullptr && sc.S == S && sc.key == b->min_segments[ii];
   }
   if (current) return;
   struct Tile {
@@ -451,7 +462,7 @@ void batch_configure(bie_batch* b, hipStream_t s) {
     b->rowblocks[ii] = rbs;
     const int min_seg = (int)std::max<int64_t>(1, (dm->E + b->elems_per_tile - 1) / b->elems_per_tile);
     b->min_segments[ii] = min_seg;
-    const bie_dmesh::Sched& sc = ensure_schedule(dm, 2, b->slots, min_seg, s);
+    const bie_dmesh::Sched& sc = ensure_schedule(dm, 2, S, min_seg, s);
     if (dm->N == 0) continue;
     int64_t rank = 0;
     for (int g = 0; g < sc.segments; ++g) {
@@ -476,6 +487,7 @@ void batch_configure(bie_batch* b, hipStream_t s) {
   b->d_tiles = upload_vec(h, s);
   BIE_HIP_CHECK(hipStreamSynchronize(s));
   b->n_tiles = (int)h.size();
+  b->slots_used = S;
   b->configured_key = key;
 }
 
@@ -542,10 +554,11 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
     const int nphi = d0->rs.R * d0->rs.Qmax * d0->pk;
     int cus = 0;
     BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    const int per_cu = batch_blocks_per_cu(b->pk, cplx, b->slots, nphi);
+    const int nstage = stage_doubles(d0->rs.len[0]);
+    const int per_cu = batch_blocks_per_cu(b->pk, cplx, b->slots_used, nphi, nstage);
     const int grid = (int)std::min<int64_t>((int64_t)std::max(cus, 1) * per_cu, b->n_tiles);
     launch_assemble_batch(b->d_inst, b->d_tiles, b->n_tiles, b->d_counter, d0->d_phi, nphi,
-                          b->slots, b->pk, cplx, grid, s);
+                          nstage, b->slots_used, b->pk, cplx, grid, s);
   }
   if (b->pk == 6) launch_assemble_mid_batch(b->d_inst, n_inst, cplx, s);
 }
@@ -967,7 +980,8 @@ int bie_batch_configure(bie_batch* b, int32_t elems_per_tile, int32_t slots, int
     BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
     BIE_REQUIRE(elems_per_tile >= kSlotWaves && elems_per_tile <= (1 << 20), BIE_EINVAL,
                 "elements per tile out of range");
-    BIE_REQUIRE(slots == 32 || slots == 64, BIE_EINVAL, "slots must be 32 or 64");
+    BIE_REQUIRE(slots == 0 || slots == 32 || slots == 64, BIE_EINVAL,
+                "slots must be 32, 64 or 0 (as many as fit)");
     BIE_REQUIRE(order == 0 || order == 1, BIE_EINVAL, "order must be 0 or 1");
     std::lock_guard<std::mutex> bl(b->mu);
     b->elems_per_tile = elems_per_tile;
@@ -1010,7 +1024,7 @@ int bie_batch_tune(bie_batch* b, const double* k_re, const double* k_im, uint32_
     };
     std::vector<Cand> cands;
     for (int T : {32, 64, 128})
-      for (int S : {32, 64})
+      for (int S : {32, 0})
         for (int o : {0, 1}) cands.push_back(Cand{T, S, o});
     double best = INFINITY;
     Cand pick = cands[0];

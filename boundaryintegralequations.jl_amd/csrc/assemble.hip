@@ -186,6 +186,41 @@ __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double
   if (q < Q) point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
 }
 
+// As pair_quadrature over table rows that this wave staged in LDS (rows[8 q .. 8 q + 8)).
+template <int NP, bool CPLX>
+__device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double x2,
+                                                    const double* rows, const double* ph, int Q,
+                                                    double kr, double ki, double* fr, double* fi,
+                                                    double* gr, double* gi, double& csum) {
+#pragma unroll 2
+  for (int q = 0; q < Q; ++q) {
+    const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
+    const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
+    point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+  }
+}
+
+// Per-wave staging of an element's three vertex-rule tables (3 Qc rows of 64 bytes,
+// nchunk = 12 Qc chunks of 16 bytes, at most 3 per lane): global -> registers early,
+// registers -> the wave's LDS rows later, so no global load is pending in the point loop.
+__device__ __forceinline__ void stage_load(const double* __restrict__ elem_e, int nchunk,
+                                           int lane, double2 (&v)[3]) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int j = lane + 64 * t;
+    v[t] = j < nchunk ? reinterpret_cast<const double2*>(elem_e)[j] : make_double2(0.0, 0.0);
+  }
+}
+
+__device__ __forceinline__ void stage_store(double* rows, int nchunk, int lane,
+                                            const double2 (&v)[3]) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int j = lane + 64 * t;
+    if (j < nchunk) reinterpret_cast<double2*>(rows)[j] = v[t];
+  }
+}
+
 // One thread per target row; blockIdx.y selects a contiguous chunk of elements.
 // Every (target, element) contribution is added into zeroed F, G with hardware FP64
 // atomics; C is accumulated per thread and added once per chunk.  Used for small
@@ -240,47 +275,109 @@ k_assemble(const AssemblyArgs a, int ech) {
 // Pairs whose rule is a mid-edge split rule (quadratic physics only, ~1% of pairs) are
 // queued in LDS, moved to the global queue once per segment, and integrated by
 // k_assemble_mid.
+// Flush of one schedule step's finished nodes: each wave takes every W-th entry, reads the
+// node's slot (F re, F im, G re, G im for 64 rows), zeroes it and writes the column
+// segment of F and G: a plain (streaming) store for owned columns, atomics for shared
+// ones.  Runs concurrently with the next step's quadrature, which never touches these
+// slots.
+template <int W>
+__device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs& sa, int st,
+                                           int wave, int lane, int64_t i, bool active,
+                                           double* acc) {
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  const int f1 = sa.flush_begin[st + 1];
+  for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
+    const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
+    const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
+    double* p = acc + (size_t)(ss & 0xffff) * 256 + lane;
+    const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
+    p[0] = 0.0;
+    p[64] = 0.0;
+    p[128] = 0.0;
+    p[192] = 0.0;
+    if (active) {
+      const size_t idx = 2 * ((size_t)i + (size_t)node * (size_t)a.M);
+      if (ss & kSharedBit) {
+        if (doF) {
+          unsafeAtomicAdd(a.F + idx, vr);
+          unsafeAtomicAdd(a.F + idx + 1, vi);
+        }
+        if (doG) {
+          unsafeAtomicAdd(a.G + idx, wr);
+          unsafeAtomicAdd(a.G + idx + 1, wi);
+        }
+      } else {
+        // the output is written once and not read back here: keep it out of the caches
+        if (doF) {
+          __builtin_nontemporal_store(vr, a.F + idx);
+          __builtin_nontemporal_store(vi, a.F + idx + 1);
+        }
+        if (doG) {
+          __builtin_nontemporal_store(wr, a.G + idx);
+          __builtin_nontemporal_store(wi, a.G + idx + 1);
+        }
+      }
+    }
+  }
+}
+
 // One (row block, segment) tile of the slot kernel: 64 targets (one per lane) against the
 // segment's elements, W elements per step (one per wave).  acc holds S zeroed slots of
 // 4 x 64 doubles and is zero again on return (the schedule flushes every slot it uses by
 // the end of the segment); phi is the basis table in LDS; s_q / s_qn the LDS mid-edge
 // queue, whose count must be zero on entry (it is zero again on return when NP == 6).
-// Ends on a barrier.
-template <int NP, bool CPLX, int W>
+// Step st: quadrature of the step's pairs into their slots, then the flush of the nodes
+// whose last element was at step st - 1, then one barrier.  With STAGED, each wave keeps
+// its element's vertex-rule rows in its LDS rows (s_rows + wave * 24 Qc) and loads the
+// next element's rows into registers a whole pair ahead: the global loads never wait
+// behind the flush's stores (vector-memory counters are in order).  Ends on a barrier.
+template <int NP, bool CPLX, int W, bool STAGED>
 __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
-                                          int seg, double* acc, const double* phi, int2* s_q,
-                                          unsigned* s_qn) {
+                                          int seg, double* acc, const double* phi,
+                                          double* s_rows, int2* s_q, unsigned* s_qn) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = (int64_t)rb * 64 + lane;
   const bool active = i < a.M;
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
-  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   const double kr = a.kr, ki = a.ki;
   double csum = 0.0;
+  const int Qc = a.len[0];
+  const int nchunk = 12 * Qc;
+  double* my_rows = STAGED ? s_rows + (size_t)wave * 24 * Qc : nullptr;
+  double2 stg[3];
   const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
   int e = (st0 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st0 * W + wave]) : -1;
   int c = 0;
   double4 A0 = make_double4(0, 0, 0, 0), B0 = A0, A1 = A0, B1 = A0;
   if (e >= 0) {
     c = select_rule<NP>(x0, x1, x2, sa.step_sel + ((size_t)st0 * W + wave) * NP * 4, a.R) - 1;
-    if (!(NP == 6 && c >= 3))
+    if (STAGED) {
+      stage_load(a.elem + (size_t)e * a.Pe * 8, nchunk, lane, stg);
+      stage_store(my_rows, nchunk, lane, stg);
+    } else if (!(NP == 6 && c >= 3)) {
       load_rows01(a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, a.len[c], A0, B0, A1, B1);
+    }
   }
   for (int st = st0; st < st1; ++st) {
     const size_t nidx = (size_t)(st + 1) * W + wave;
     const int e_nxt =
         (st + 1 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[nidx]) : -1;
+    if (STAGED && e_nxt >= 0) stage_load(a.elem + (size_t)e_nxt * a.Pe * 8, nchunk, lane, stg);
     if (e >= 0) {
       const bool mid = NP == 6 && c >= 3;
       if (active && !mid) {
         double fr[NP], fi[NP], gr[NP], gi[NP];
 #pragma unroll
         for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
-        pair_quadrature_pre<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
-                                      phi + c * a.Qmax * NP, a.len[c], kr, ki, A0, B0, A1, B1,
-                                      fr, fi, gr, gi, csum);
+        if (STAGED)
+          pair_quadrature_lds<NP, CPLX>(x0, x1, x2, my_rows + (size_t)c * Qc * 8,
+                                        phi + c * a.Qmax * NP, Qc, kr, ki, fr, fi, gr, gi, csum);
+        else
+          pair_quadrature_pre<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
+                                        phi + c * a.Qmax * NP, a.len[c], kr, ki, A0, B0, A1, B1,
+                                        fr, fi, gr, gi, csum);
         const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
 #pragma unroll
         for (int s = 0; s < NP; ++s) {
@@ -312,47 +409,27 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
         }
       }
     }
-    // next element: rule selection and its first rows, in flight across the barrier
+    // next element: rule selection, and its rows (LDS: after this wave's reads above)
     int c_nxt = 0;
     if (e_nxt >= 0) {
       c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
-      if (!(NP == 6 && c_nxt >= 3))
+      if (STAGED)
+        stage_store(my_rows, nchunk, lane, stg);
+      else if (!(NP == 6 && c_nxt >= 3))
         load_rows01(a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8, a.len[c_nxt], A0, B0,
                     A1, B1);
     }
-    __syncthreads();
-    const int f1 = sa.flush_begin[st + 1];
-    for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
-      const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
-      const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
-      double* p = acc + (size_t)(ss & 0xffff) * 256 + lane;
-      const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
-      p[0] = 0.0;
-      p[64] = 0.0;
-      p[128] = 0.0;
-      p[192] = 0.0;
-      if (active) {
-        const size_t idx = 2 * ((size_t)i + (size_t)node * (size_t)a.M);
-        if (ss & kSharedBit) {
-          if (doF) {
-            unsafeAtomicAdd(a.F + idx, vr);
-            unsafeAtomicAdd(a.F + idx + 1, vi);
-          }
-          if (doG) {
-            unsafeAtomicAdd(a.G + idx, wr);
-            unsafeAtomicAdd(a.G + idx + 1, wi);
-          }
-        } else {
-          if (doF) *reinterpret_cast<double2*>(a.F + idx) = make_double2(vr, vi);
-          if (doG) *reinterpret_cast<double2*>(a.G + idx) = make_double2(wr, wi);
-        }
-      }
-    }
-    __syncthreads();
+    // the previous step's finished nodes: their slots were complete at the last barrier
+    if (st > st0) flush_step<W>(a, sa, st - 1, wave, lane, i, active, acc);
+    lds_barrier();
     e = e_nxt;
     c = c_nxt;
   }
-  if (NP == 6 && wave == 0) {  // the loop ended on a barrier: the LDS queue is complete
+  if (st1 > st0) {
+    flush_step<W>(a, sa, st1 - 1, wave, lane, i, active, acc);
+    lds_barrier();
+  }
+  if (NP == 6 && wave == 0) {  // after a barrier: the LDS queue is complete
     const unsigned n = min(*s_qn, (unsigned)kMidLds);
     if (n) {
       unsigned base = 0;
@@ -364,7 +441,7 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     if (lane == 0) *s_qn = 0u;
   }
   if (active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
-  __syncthreads();
+  lds_barrier();
 }
 
 template <int NP, bool CPLX, int W>
@@ -374,7 +451,9 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   double* acc = lds;                              // [slot][4][64]
   double* s_phi = lds + (size_t)sa.S * 256;
   const int nphi = a.R * a.Qmax * NP;
-  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  const int nstage = stage_doubles(a.len[0]);
+  double* s_rows = s_phi + phi_pad(nphi);
+  int2* s_q = reinterpret_cast<int2*>(s_rows + nstage);
   unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
   // XCD-aware tile order: consecutive block ids go to different XCDs, so give each XCD
   // a contiguous run of (row block, segment) tiles.
@@ -393,8 +472,11 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
   for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
-  __syncthreads();
-  slot_tile<NP, CPLX, W>(a, sa, rb, seg, acc, s_phi, s_q, s_qn);
+  lds_barrier();
+  if (nstage)
+    slot_tile<NP, CPLX, W, true>(a, sa, rb, seg, acc, s_phi, s_rows, s_q, s_qn);
+  else
+    slot_tile<NP, CPLX, W, false>(a, sa, rb, seg, acc, s_phi, s_rows, s_q, s_qn);
 }
 
 // Uneven batch: many meshes (same element kinds and rule set) in one persistent launch.
@@ -408,11 +490,12 @@ template <int NP, bool CPLX, int W>
 __global__ void __launch_bounds__(64 * W)
 k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ tiles,
                  int n_tiles, unsigned* __restrict__ counters, const double* __restrict__ phi,
-                 int nphi, int S) {
+                 int nphi, int nstage, int S) {
   extern __shared__ double lds[];
   double* acc = lds;
   double* s_phi = lds + (size_t)S * 256;
-  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  double* s_rows = s_phi + phi_pad(nphi);
+  int2* s_q = reinterpret_cast<int2*>(s_rows + nstage);
   unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
   int* s_tile = reinterpret_cast<int*>(s_qn + 1);
   for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = phi[t];
@@ -428,15 +511,20 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
         const unsigned got = atomicAdd(counters + x, 1u);
         *s_tile = (got < (unsigned)(end - begin)) ? begin + (int)got : -1;
       }
-      __syncthreads();
+      lds_barrier();
       const int t = __builtin_amdgcn_readfirstlane(*s_tile);
-      __syncthreads();  // every wave has read s_tile before thread 0 may write it again
+      lds_barrier();  // every wave has read s_tile before thread 0 may write it again
       if (t < 0) break;
       const int4 td = tiles[t];
       const int ii = __builtin_amdgcn_readfirstlane(td.x);
       const int rb = __builtin_amdgcn_readfirstlane(td.y);
       const int seg = __builtin_amdgcn_readfirstlane(td.z);
-      slot_tile<NP, CPLX, W>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_q, s_qn);
+      if (nstage)
+        slot_tile<NP, CPLX, W, true>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_rows, s_q,
+                                     s_qn);
+      else
+        slot_tile<NP, CPLX, W, false>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_rows, s_q,
+                                      s_qn);
     }
   }
 }
@@ -613,7 +701,8 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
   BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
-  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + (size_t)a.R * a.Qmax * a.NP) +
+  const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + phi_pad(a.R * a.Qmax * a.NP) +
+                                         stage_doubles(a.len[0])) +
                        sizeof(int2) * kMidLds + 16;
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
   const bool cplx = a.ki != 0.0;
@@ -655,15 +744,16 @@ void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_work, hi
   BIE_HIP_CHECK(hipGetLastError());
 }
 
-size_t batch_shmem(int S, int nphi) {
-  return sizeof(double) * ((size_t)S * 256 + (size_t)nphi) + sizeof(int2) * kMidLds + 16;
+size_t batch_shmem(int S, int nphi, int nstage) {
+  return sizeof(double) * ((size_t)S * 256 + phi_pad(nphi) + nstage) + sizeof(int2) * kMidLds +
+         16;
 }
 
 void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
-                           unsigned* d_counter, const double* d_phi, int nphi, int S, int NP,
-                           bool cplx, int grid, hipStream_t s) {
+                           unsigned* d_counter, const double* d_phi, int nphi, int nstage,
+                           int S, int NP, bool cplx, int grid, hipStream_t s) {
   if (n_tiles <= 0) return;
-  const size_t shmem = batch_shmem(S, nphi);
+  const size_t shmem = batch_shmem(S, nphi, nstage);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
   BIE_REQUIRE(grid >= 1, BIE_EINVAL, "empty persistent grid");
   hipLaunchKernelGGL(k_zero_u32_n, dim3(1), dim3(64), 0, s, d_counter, kXcds);
@@ -673,7 +763,7 @@ void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_t
     BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), shmem, s, d_inst, d_tiles,
-                       n_tiles, d_counter, d_phi, nphi, S);
+                       n_tiles, d_counter, d_phi, nphi, nstage, S);
   };
   if (NP == 3) {
     if (cplx)
@@ -690,8 +780,8 @@ void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_t
 }
 
 // Resident workgroups of the batch kernel per CU for slot count S (LDS-bound).
-int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi) {
-  const size_t shmem = batch_shmem(S, nphi);
+int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage) {
+  const size_t shmem = batch_shmem(S, nphi, nstage);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
   constexpr int W = kSlotWaves;
   int nb = 0;

@@ -80,9 +80,28 @@ struct AssemblyArgs {
 
 // Device view of a slot schedule plus the mid-edge pair queue.
 constexpr int kSlotWaves = 8;   // waves per workgroup of the slot kernel
-constexpr int kSlotCount = 64;  // LDS slots (64 rows x 4 doubles each)
+constexpr int kSlotCount = 64;  // LDS slots of the schedule check (64 rows x 4 doubles each)
 constexpr int kMidLds = 512;    // per-workgroup LDS queue of mid-edge pairs
 constexpr int kXcds = 8;        // XCDs of an MI355X; workgroup b is dispatched to XCD b mod 8
+constexpr int kLdsBytes = 160 * 1024;  // LDS per CU (one slot-kernel workgroup per CU)
+
+constexpr int kStageMaxQ = 16;  // vertex rules up to this length are staged in LDS per wave
+
+// LDS layout of the slot kernels, in doubles: S slots of 4 x 64 accumulators, the basis
+// table (padded to 4 doubles), then the per-wave staging of the three vertex rules' table
+// rows (W x 3 Qc points x 8 doubles, when Qc <= kStageMaxQ), then the mid-edge queue.
+__host__ __device__ inline int phi_pad(int nphi) { return (nphi + 3) & ~3; }
+__host__ __device__ inline int stage_doubles(int Qc) {
+  return Qc <= kStageMaxQ ? kSlotWaves * 3 * Qc * 8 : 0;
+}
+
+// LDS slots that fit beside `fixed_doubles` (padded basis table + staging) and the
+// mid-edge queue when each slot holds `per_slot` accumulators of 64 rows x 4 doubles.
+inline int fitting_slots(int fixed_doubles, int per_slot) {
+  const long fixed = (long)sizeof(double) * fixed_doubles + 8L * kMidLds + 16;
+  const long S = (kLdsBytes - fixed) / (256L * (long)sizeof(double) * per_slot);
+  return (int)(S < 120 ? S : 120);
+}
 
 struct SlotArgs {
   int W, S;
@@ -158,12 +177,12 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
 void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
 void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_work, hipStream_t s);
-size_t batch_shmem(int S, int nphi);
+size_t batch_shmem(int S, int nphi, int nstage);
 void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
-                           unsigned* d_counter, const double* d_phi, int nphi, int S, int NP,
-                           bool cplx, int grid, hipStream_t s);
+                           unsigned* d_counter, const double* d_phi, int nphi, int nstage,
+                           int S, int NP, bool cplx, int grid, hipStream_t s);
 void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, hipStream_t s);
-int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi);
+int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
@@ -230,7 +249,8 @@ struct bie_batch {
   std::vector<bie_dmesh*> meshes;
   // schedule configuration (chosen by bie_batch_tune)
   int elems_per_tile = 64;   // target elements per segment
-  int slots = 64;            // LDS slots per workgroup
+  int slots = 64;            // LDS slots per workgroup (0: as many as fit)
+  int slots_used = 64;       // the slot count the tile list was built with
   int order = 0;             // 0: longest tiles first, 1: instances interleaved
   int configured_key = -1;   // configuration the tile list was built for
   // device state

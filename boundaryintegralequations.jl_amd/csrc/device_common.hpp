@@ -8,6 +8,16 @@
 namespace bie {
 namespace {
 
+// Workgroup barrier for LDS hand-offs only: waits for this wave's LDS (and scalar-load)
+// operations, then s_barrier.  Unlike __syncthreads() it does not wait for outstanding
+// global loads and stores, so prefetched table rows and the column stores of F and G stay
+// in flight across it; nothing in global memory is exchanged between the waves of a
+// workgroup through these barriers.  The memory clobber keeps the compiler from moving
+// memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Julia Base n-ary hypot for Float64 (maxabs * sqrt(sum((x_i/maxabs)^2)), left to
 // right), with no fused multiply-add so that the result is bit-identical to the CPU.
 __device__ __noinline__ double julia_hypot3(double x, double y, double z) {

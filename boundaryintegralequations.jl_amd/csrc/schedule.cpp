@@ -4,14 +4,18 @@
 // element's contribution for target i.  A workgroup owns 64 target rows and one
 // *segment* of elements; it sweeps the segment in steps of W elements (one per wave) and
 // accumulates node contributions in LDS *slots*.  A node occupies a slot from the step
-// of its first incident element in the segment to the step of its last; after that step
-// the slot is flushed to F and G and reused.  Nodes whose elements all lie in one
-// segment ("owned") are written once with a plain store; nodes touched by several
-// segments ("shared") are added atomically into columns zeroed beforehand.
+// of its first incident element in the segment to the step of its last; the kernel
+// flushes it to F and G during the following step (overlapped with that step's
+// quadrature, one barrier per step), so the slot becomes free for a new node one step
+// after that: a node whose last step is t holds its slot through step t + 1 and the slot
+// is reused at step t + 2 at the earliest.  Nodes whose elements all lie in one segment
+// ("owned") are written once with a plain store; nodes touched by several segments
+// ("shared") are added atomically into columns zeroed beforehand.
 //
 // Segments come from recursive coordinate bisection of the element centroids; each is
-// swept along its longest axis, which keeps the number of live slots (the sweep front)
-// near the segment's width.  Segments are split until their front fits the slot budget.
+// swept along its longest axis, which keeps the number of occupied slots (the sweep
+// front) near the segment's width.  Segments are split until their front fits the slot
+// budget.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -44,8 +48,9 @@ void sort_along(std::vector<int>& idx, const double* cent, int ax) {
   });
 }
 
-// Largest number of nodes live at any step when idx (already in sweep order) is swept
-// in steps of W elements.
+// Largest number of slots occupied at any step when idx (already in sweep order) is
+// swept in steps of W elements: a node occupies its slot from its first step through
+// the step after its last.
 int front_width(const std::vector<int>& idx, const int* ptopo, int NP, int W,
                 std::vector<int>& first, std::vector<int>& last, std::vector<int>& touched) {
   touched.clear();
@@ -61,10 +66,10 @@ int front_width(const std::vector<int>& idx, const int* ptopo, int NP, int W,
       last[node] = step;
     }
   }
-  std::vector<int> delta(nsteps + 1, 0);
+  std::vector<int> delta(nsteps + 2, 0);
   for (int node : touched) {
     delta[first[node]] += 1;
-    delta[last[node] + 1] -= 1;
+    delta[last[node] + 2] -= 1;
   }
   int live = 0, best = 0;
   for (int t = 0; t < nsteps; ++t) {
@@ -151,8 +156,9 @@ SlotSchedule build_slot_schedule(const int* ptopo, int NP, int E, int N, const d
       starts[first[node]].push_back(node);
       ends[last[node]].push_back(node);
     }
-    // free slots, lowest index first
-    std::vector<int> freelist;
+    // free slots, lowest index first; slots of nodes that ended at the previous step are
+    // released after this step (they are flushed during it)
+    std::vector<int> freelist, pending;
     for (int q = S - 1; q >= 0; --q) freelist.push_back(q);
     int live = 0;
     for (int t = 0; t < nsteps; ++t) {
@@ -177,10 +183,12 @@ SlotSchedule build_slot_schedule(const int* ptopo, int NP, int E, int N, const d
         sc.flush_node.push_back(node);
         sc.flush_slot.push_back(node_slot[node] | (nseg[node] > 1 ? kSharedBit : 0));
       }
-      for (int node : ends[t]) {
-        freelist.push_back(node_slot[node]);
+      for (int sl : pending) {
+        freelist.push_back(sl);
         --live;
       }
+      pending.clear();
+      for (int node : ends[t]) pending.push_back(node_slot[node]);
       sc.flush_begin.push_back((int)sc.flush_node.size());
     }
     for (int node : touched) first[node] = last[node] = -1;
@@ -201,6 +209,7 @@ void validate_slot_schedule(const SlotSchedule& sc, const int* ptopo, int NP, in
   std::vector<int> seen(E, 0), nseg(N, 0), seg_mark(N, -1);
   std::vector<int> slot_node(sc.S, -1), node_slot(N, -1), flushed(N, -1);
   for (int g = 0; g < sc.n_segments; ++g) {
+    std::vector<int> pending;  // slots flushed during the current step, free after it
     for (int t = sc.seg_step_begin[g]; t < sc.seg_step_begin[g + 1]; ++t) {
       for (int w = 0; w < sc.W; ++w) {
         const int e = sc.step_elems[(size_t)t * sc.W + w];
@@ -217,7 +226,7 @@ void validate_slot_schedule(const SlotSchedule& sc, const int* ptopo, int NP, in
           }
           if (node_slot[node] < 0) {  // first touch in this segment
             if (flushed[node] == g) fail("node touched after its flush");
-            if (slot_node[sl] >= 0) fail("slot reused while live");
+            if (slot_node[sl] >= 0) fail("slot reused while occupied");
             slot_node[sl] = node;
             node_slot[node] = sl;
           } else if (node_slot[node] != sl) {
@@ -225,6 +234,9 @@ void validate_slot_schedule(const SlotSchedule& sc, const int* ptopo, int NP, in
           }
         }
       }
+      // the flush of step t - 1 runs during step t: its slots are free from step t + 1
+      for (int sl : pending) slot_node[sl] = -1;
+      pending.clear();
       for (int f = sc.flush_begin[t]; f < sc.flush_begin[t + 1]; ++f) {
         const int node = sc.flush_node[f];
         const int sl = sc.flush_slot[f] & 0xffff;
@@ -232,12 +244,13 @@ void validate_slot_schedule(const SlotSchedule& sc, const int* ptopo, int NP, in
         if (node_slot[node] != sl || slot_node[sl] != node) fail("flush of a slot not held");
         if (flushed[node] == g) fail("node flushed twice in a segment");
         flushed[node] = g;
-        slot_node[sl] = -1;
         node_slot[node] = -1;
+        pending.push_back(sl);
       }
     }
+    for (int sl : pending) slot_node[sl] = -1;
     for (int q = 0; q < sc.S; ++q)
-      if (slot_node[q] >= 0) fail("slot still live at segment end");
+      if (slot_node[q] >= 0) fail("slot still occupied at segment end");
   }
   for (int e = 0; e < E; ++e)
     if (seen[e] != 1) fail("element not swept");

@@ -10,7 +10,7 @@
 // replaces KB sincos evaluations by two plus KB-1 complex products; other blocks evaluate
 // each e^{i k_b r} directly, so any list of k is exact.  Accumulation uses the LDS-slot
 // schedule of the single-k kernel with KB accumulators per slot (the schedule is built
-// for kSlotCount / KB live slots).  The reference has no batched path: it calls
+// for as many slots of KB accumulators as fit in LDS).  The reference has no batched path: it calls
 // assemble_parallel! once per k (e.g. the Taylor sweep of src/3d/rosebem.jl:56-157
 // shares geometry across derivative orders in the same way).
 #include <hip/hip_runtime.h>
@@ -71,6 +71,52 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
   }
 }
 
+// Flush of one step's finished nodes for the KB wavenumbers of a block (see the
+// single-k kernel): slot contents to F_k, G_k columns, zeroing the slots.
+template <int KB, int W>
+__device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArgs& sa,
+                                            const SweepArgs& sw, int st, int wave, int lane,
+                                            int64_t i, bool active, int k0, double* acc) {
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  const int f1 = sa.flush_begin[st + 1];
+  for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
+    const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
+    const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
+#pragma unroll
+    for (int q2 = 0; q2 < KB; ++q2) {
+      double* p = acc + ((size_t)(ss & 0xffff) * KB + q2) * 256 + lane;
+      const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
+      p[0] = 0.0;
+      p[64] = 0.0;
+      p[128] = 0.0;
+      p[192] = 0.0;
+      const int kidx = k0 + q2;
+      if (active && kidx < sw.nk) {
+        const size_t idx = 2 * ((size_t)kidx * sw.stride + (size_t)i + (size_t)node * a.M);
+        if (ss & kSharedBit) {
+          if (doF) {
+            unsafeAtomicAdd(a.F + idx, vr);
+            unsafeAtomicAdd(a.F + idx + 1, vi);
+          }
+          if (doG) {
+            unsafeAtomicAdd(a.G + idx, wr);
+            unsafeAtomicAdd(a.G + idx + 1, wi);
+          }
+        } else {
+          if (doF) {
+            __builtin_nontemporal_store(vr, a.F + idx);
+            __builtin_nontemporal_store(vi, a.F + idx + 1);
+          }
+          if (doG) {
+            __builtin_nontemporal_store(wr, a.G + idx);
+            __builtin_nontemporal_store(wi, a.G + idx + 1);
+          }
+        }
+      }
+    }
+  }
+}
+
 template <int NP, int KB, int W>
 __global__ void __launch_bounds__(64 * W)
 k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
@@ -97,10 +143,9 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
   for (int t = threadIdx.x; t < sa.S * KB * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
-  __syncthreads();
+  lds_barrier();
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
-  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   const bool first_kb = kb == 0;
   double kv[KB];
   const int k0 = kb * KB;
@@ -188,41 +233,15 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         B0 = *reinterpret_cast<const double4*>(t0 + 4);
       }
     }
-    __syncthreads();
-    const int f1 = sa.flush_begin[st + 1];
-    for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
-      const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
-      const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
-#pragma unroll
-      for (int q2 = 0; q2 < KB; ++q2) {
-        double* p = acc + ((size_t)(ss & 0xffff) * KB + q2) * 256 + lane;
-        const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
-        p[0] = 0.0;
-        p[64] = 0.0;
-        p[128] = 0.0;
-        p[192] = 0.0;
-        const int kidx = k0 + q2;
-        if (active && kidx < sw.nk) {
-          const size_t idx = 2 * ((size_t)kidx * sw.stride + (size_t)i + (size_t)node * a.M);
-          if (ss & kSharedBit) {
-            if (doF) {
-              unsafeAtomicAdd(a.F + idx, vr);
-              unsafeAtomicAdd(a.F + idx + 1, vi);
-            }
-            if (doG) {
-              unsafeAtomicAdd(a.G + idx, wr);
-              unsafeAtomicAdd(a.G + idx + 1, wi);
-            }
-          } else {
-            if (doF) *reinterpret_cast<double2*>(a.F + idx) = make_double2(vr, vi);
-            if (doG) *reinterpret_cast<double2*>(a.G + idx) = make_double2(wr, wi);
-          }
-        }
-      }
-    }
-    __syncthreads();
+    // the previous step's finished nodes (their slots were complete at the last barrier)
+    if (st > st0) sweep_flush<KB, W>(a, sa, sw, st - 1, wave, lane, i, active, k0, acc);
+    lds_barrier();
     e = e_nxt;
     c = c_nxt;
+  }
+  if (st1 > st0) {
+    sweep_flush<KB, W>(a, sa, sw, st1 - 1, wave, lane, i, active, k0, acc);
+    lds_barrier();
   }
   if (NP == 6 && first_kb && wave == 0) {
     const unsigned n = min(*s_qn, (unsigned)kMidLds);
@@ -336,14 +355,13 @@ void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t
   BIE_HIP_CHECK(hipGetLastError());
 }
 
-size_t sweep_shmem(int NP, int nphi) {
+size_t sweep_shmem(int NP, int nphi, int S) {
   const int KB = sweep_kb(NP);
-  return sizeof(double) * ((size_t)(kSlotCount / KB) * KB * 256 + (size_t)nphi) +
-         sizeof(int2) * kMidLds + 16;
+  return sizeof(double) * ((size_t)S * KB * 256 + (size_t)nphi) + sizeof(int2) * kMidLds + 16;
 }
 
 void sweep_prepare_attributes(int NP, int nphi) {
-  const size_t shmem = sweep_shmem(NP, nphi);
+  const size_t shmem = sweep_shmem(NP, nphi, fitting_slots(nphi, sweep_kb(NP)));
   constexpr int W = kSlotWaves;
   const void* kern = NP == 3 ? reinterpret_cast<const void*>(k_sweep_slots<3, 4, W>)
                              : reinterpret_cast<const void*>(k_sweep_slots<6, 2, W>);
@@ -359,7 +377,7 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
                   hipStream_t s) {
   if (a.M == 0 || a.E == 0 || sw.nk == 0) return;
   const int KB = sweep_kb(a.NP);
-  BIE_REQUIRE(sa.W == kSlotWaves && sa.S * KB <= kSlotCount, BIE_EINVAL,
+  BIE_REQUIRE(sa.W == kSlotWaves && sa.S <= fitting_slots(a.R * a.Qmax * a.NP, KB), BIE_EINVAL,
               "sweep schedule does not match the kernel");
   const int64_t per_xcd = (sw.n_tiles + 7) / 8;
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many sweep tiles");
