@@ -14,7 +14,8 @@ PKG = os.path.dirname(HERE)
 CSRC = os.path.join(PKG, "csrc")
 LIB_PATH = os.path.join(HERE, "libbie_amd.so")
 
-SOURCES = ["rules.cpp", "schedule.cpp", "assemble.hip", "sweep.hip", "adjoint.hip", "apply.hip", "api.hip"]
+SOURCES = ["rules.cpp", "schedule.cpp", "assemble.hip", "sweep.hip", "adjoint.hip", "apply.hip",
+           "midsort.hip", "api.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

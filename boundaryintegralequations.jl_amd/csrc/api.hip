@@ -257,6 +257,7 @@ SlotArgs slot_args(bie_dmesh* dm, const bie_dmesh::Sched& sc, int64_t rowblocks)
   sa.mid_count = dm->d_mid_count;
   sa.mid_queue = dm->d_mid_queue;
   sa.mid_cap = dm->mid_cap;
+  sa.push_mid = 1;
   return sa;
 }
 
@@ -268,6 +269,29 @@ void ensure_mid_queue(bie_dmesh* dm, int64_t cap) {
   dm->mid_cap = 0;
   dm->d_mid_queue = dalloc<int2>((size_t)cap);
   dm->mid_cap = cap;
+}
+
+// Records the collocation mid-edge pairs that the slot kernel queued, sorted by (element,
+// target), and releases the queue (synchronises the stream once per mesh).
+void record_mid_pairs(bie_dmesh* dm, hipStream_t s) {
+  unsigned count = 0;
+  BIE_HIP_CHECK(hipMemcpyAsync(&count, dm->d_mid_count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  BIE_HIP_CHECK(hipStreamSynchronize(s));
+  const int64_t n = std::min<int64_t>((int64_t)count, dm->mid_cap);
+  unsigned long long* keys = dalloc<unsigned long long>((size_t)std::max<int64_t>(n, 1));
+  try {
+    launch_mid_keys(dm->d_mid_queue, n, keys, s);
+    sort_keys(keys, n, 62, s);
+  } catch (...) {
+    dfree(keys);
+    throw;
+  }
+  dfree(dm->d_mid_keys);
+  dm->d_mid_keys = keys;
+  dm->n_mid_keys = n;
+  dfree(dm->d_mid_queue);  // the M x E queue is no longer needed for collocation
+  dm->d_mid_queue = nullptr;
+  dm->mid_cap = 0;
 }
 
 AssemblyArgs make_args(bie_dmesh* dm, const double* d_targets, int64_t M, double kr,
@@ -341,13 +365,18 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       const int min_segments = (int)std::max<int64_t>(1, std::min<int64_t>(dm->E, (2048 + rowblocks - 1) / rowblocks));
       const bie_dmesh::Sched& sc =
           ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0]), min_segments, s);
-      if (dm->pk == 6) {
+      // collocation on quadratic physics: after the first call the mid-edge pairs come
+      // from the recorded sorted list instead of the slot kernel's queue
+      const bool colloc = d_targets == dm->d_sources;
+      const bool recorded = dm->pk == 6 && colloc && dm->n_mid_keys >= 0;
+      if (dm->pk == 6 && !recorded) {
         ensure_mid_queue(dm, M * dm->E);
         launch_fill_zero_u32(dm->d_mid_count, s);
       } else if (!dm->d_mid_count) {
         dm->d_mid_count = dalloc<unsigned>(1);
       }
       SlotArgs sa = slot_args(dm, sc, rowblocks);
+      sa.push_mid = recorded ? 0 : 1;
       // The slot kernel stores owned columns once; only shared/untouched columns need
       // zeros (a disabled output is zeroed whole above and never written).
       if (flags & BIE_F) launch_zero_columns(dF, M, sc.d_zero_cols, sc.zero_cols, s);
@@ -355,7 +384,12 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
       launch_assemble_slots(a, sa, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
-      launch_assemble_mid(a, sa, s);
+      if (recorded) {
+        launch_assemble_mid_sorted(a, dm->d_mid_keys, dm->n_mid_keys, s);
+      } else {
+        launch_assemble_mid(a, sa, s);
+        if (dm->pk == 6 && colloc) record_mid_pairs(dm, s);
+      }
     }
   } else if (timed) {
     BIE_HIP_CHECK(hipEventRecord(ev[1], s));
@@ -655,6 +689,7 @@ int bie_dmesh_destroy(bie_dmesh* dm) {
   dfree(dm->d_block_uniform);
   dfree(dm->d_mid_count);
   dfree(dm->d_mid_queue);
+  dfree(dm->d_mid_keys);
   delete dm;
   return BIE_OK;
 }

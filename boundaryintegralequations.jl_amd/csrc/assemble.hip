@@ -388,7 +388,7 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
           atomicAdd(p + 192, gi[s]);
         }
       }
-      if (NP == 6) {
+      if (NP == 6 && sa.push_mid) {
         const bool push = mid && active;
         const unsigned long long mask = __ballot(push);
         if (mask) {
@@ -529,43 +529,71 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
   }
 }
 
-// Mid-edge split-rule pairs from the queue (rotated_midpoint_triangular_quadpoints,
-// quadrature.jl:117-144): one thread per pair, results added atomically after the slot
-// kernel has written its columns.
+// One mid-edge split-rule pair (rotated_midpoint_triangular_quadpoints,
+// quadrature.jl:117-144): target i, element e, rule c (3..5), results added atomically
+// after the slot kernel has written its columns.
+__device__ __forceinline__ void mid_pair(const AssemblyArgs& a, int64_t i, int e, int c,
+                                         int cplx) {
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  const double x0 = a.targets[3 * i], x1 = a.targets[3 * i + 1], x2 = a.targets[3 * i + 2];
+  double fr[6], fi[6], gr[6], gi[6], csum = 0.0;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
+  const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
+  const double* ph = a.phi + c * a.Qmax * 6;
+  if (cplx)
+    pair_quadrature<6, true>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
+  else
+    pair_quadrature<6, false>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
+  const int* cols = a.ptopo + (size_t)e * 6;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const size_t idx = 2 * ((size_t)i + (size_t)cols[s] * (size_t)a.M);
+    if (doF) {
+      unsafeAtomicAdd(a.F + idx, fr[s]);
+      unsafeAtomicAdd(a.F + idx + 1, fi[s]);
+    }
+    if (doG) {
+      unsafeAtomicAdd(a.G + idx, gr[s]);
+      unsafeAtomicAdd(a.G + idx + 1, gi[s]);
+    }
+  }
+  if (a.flags & BIE_C) unsafeAtomicAdd(a.C + 2 * i, csum);
+}
+
+// Mid-edge pairs from the queue the slot kernel filled: one thread per pair.
 __device__ __forceinline__ void mid_pairs(const AssemblyArgs& a, const unsigned* count,
                                           const int2* queue, int64_t cap, int cplx,
                                           int64_t first, int64_t stride) {
   const int64_t n = min((int64_t)*count, cap);
-  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   for (int64_t t = first; t < n; t += stride) {
     const int2 en = queue[t];
-    const int64_t i = en.x;
-    const int e = en.y & 0x0fffffff;
-    const int c = 3 + (en.y >> 28);
-    const double x0 = a.targets[3 * i], x1 = a.targets[3 * i + 1], x2 = a.targets[3 * i + 2];
-    double fr[6], fi[6], gr[6], gi[6], csum = 0.0;
-#pragma unroll
-    for (int s = 0; s < 6; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
-    const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
-    const double* ph = a.phi + c * a.Qmax * 6;
-    if (cplx)
-      pair_quadrature<6, true>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
-    else
-      pair_quadrature<6, false>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
-    const int* cols = a.ptopo + (size_t)e * 6;
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-      const size_t idx = 2 * ((size_t)i + (size_t)cols[s] * (size_t)a.M);
-      if (doF) {
-        unsafeAtomicAdd(a.F + idx, fr[s]);
-        unsafeAtomicAdd(a.F + idx + 1, fi[s]);
-      }
-      if (doG) {
-        unsafeAtomicAdd(a.G + idx, gr[s]);
-        unsafeAtomicAdd(a.G + idx + 1, gi[s]);
-      }
-    }
-    if (a.flags & BIE_C) unsafeAtomicAdd(a.C + 2 * i, csum);
+    mid_pair(a, en.x, en.y & 0x0fffffff, 3 + (en.y >> 28), cplx);
+  }
+}
+
+// Sort key of a queued pair: element (bits 34..61), target (2..33), rule - 3 (0..1).
+__global__ void k_mid_keys(const int2* __restrict__ queue, int64_t n,
+                           unsigned long long* __restrict__ keys) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int2 en = queue[t];
+    keys[t] = ((unsigned long long)(en.y & 0x0fffffff) << 34) |
+              ((unsigned long long)(unsigned)en.x << 2) | (unsigned long long)(en.y >> 28);
+  }
+}
+
+// Mid-edge pairs from the recorded list sorted by (element, target): the lanes of a wave
+// mostly share one element, so its table rows are read once per wave and the column
+// atomics hit neighbouring rows.
+__global__ void __launch_bounds__(256)
+k_assemble_mid_sorted(const AssemblyArgs a, const unsigned long long* __restrict__ keys,
+                      int64_t n, int cplx) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
+    const unsigned long long key = keys[t];
+    mid_pair(a, (int64_t)((key >> 2) & 0xffffffffull), (int)(key >> 34), 3 + (int)(key & 3ull),
+             cplx);
   }
 }
 
@@ -803,6 +831,24 @@ int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage) {
       q(k_assemble_batch<6, false, W>);
   }
   return nb < 1 ? 1 : nb;
+}
+
+void launch_mid_keys(const int2* queue, int64_t n, unsigned long long* keys, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_mid_keys, dim3((unsigned)blocks), dim3(256), 0, s, queue, n, keys);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_assemble_mid_sorted(const AssemblyArgs& a, const unsigned long long* keys, int64_t n,
+                                hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_assemble_mid_sorted, dim3((unsigned)blocks), dim3(256), 0, s, a, keys, n,
+                     a.ki != 0.0 ? 1 : 0);
+  BIE_HIP_CHECK(hipGetLastError());
 }
 
 void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, hipStream_t s) {

@@ -206,18 +206,33 @@ def test_slot_and_atomic_kernels_agree(name, k, n, m, monkeypatch):
 
 
 def test_device_path_repeated_calls_reuse_caches():
-    """Second and third calls on one device mesh (cached rules, schedule, queue) agree,
-    also after switching rule orders and targets."""
+    """Second and later calls on one device mesh (cached rules, schedule, recorded
+    mid-edge pairs) agree with the oracle, also after switching rule orders and after a
+    call with explicit targets in between (which needs the pair queue again)."""
     mesh = MESHES["quad4"]()
     dm = DeviceMesh(mesh)
     N = mesh.n_sources
     bufs = [DevBuf(16 * N * N), DevBuf(16 * N * N), DevBuf(16 * N)]
+    pts = M.fibonacci_sphere(37, 1.02)
+    P = pts.shape[1]
+    tb = DevBuf(24 * P)
+    tb.upload(np.asfortranarray(pts).ravel(order="F"))
+    fb = [DevBuf(16 * P * N), DevBuf(16 * P * N), DevBuf(16 * P)]
     outs = []
-    for nm in [(3, 2), (4, 4), (3, 2)]:
+    for step, nm in enumerate([(3, 2), (4, 4), (3, 2), (3, 2)]):
+        if step == 3:
+            dm.assemble(6.0, fb[0].ptr, fb[1].ptr, fb[2].ptr, targets_ptr=tb.ptr, n_targets=P,
+                        n=3, m=2)
         dm.assemble(6.0, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n=nm[0], m=nm[1])
         _lib.check(_lib.lib().bie_device_synchronize())
         F = bufs[0].download(np.zeros(N * N, dtype=np.complex128)).reshape(N, N, order="F")
-        outs.append(F)
-    Fo32, _, _ = oracle.assemble(mesh, 6.0, n=3, m=2)
-    Fo44, _, _ = oracle.assemble(mesh, 6.0, n=4, m=4)
-    assert rel(outs[0], Fo32) < TOL and rel(outs[1], Fo44) < TOL and rel(outs[2], Fo32) < TOL
+        G = bufs[1].download(np.zeros(N * N, dtype=np.complex128)).reshape(N, N, order="F")
+        C = bufs[2].download(np.zeros(N, dtype=np.complex128))
+        outs.append((F, G, C))
+    ref = {nm: oracle.assemble(mesh, 6.0, n=nm[0], m=nm[1]) for nm in [(3, 2), (4, 4)]}
+    for (F, G, C), nm in zip(outs, [(3, 2), (4, 4), (3, 2), (3, 2)]):
+        Fo, Go, Co = ref[nm]
+        assert rel(F, Fo) < TOL and rel(G, Go) < TOL and rel(C, Co) < TOL
+    Ff = fb[0].download(np.zeros(P * N, dtype=np.complex128)).reshape(P, N, order="F")
+    Ffo, _, _ = oracle.assemble(mesh, 6.0, targets=pts, n=3, m=2)
+    assert rel(Ff, Ffo) < TOL
