@@ -1,0 +1,355 @@
+"""
+    BoundaryIntegralEquationsAMD
+
+Drop-in MI355X backend for `BoundaryIntegralEquations.assemble_parallel!`
+(src/3d/assembly_collocation.jl:259-268) on linear and quadratic triangles, calling
+`libbie_amd.so` (C ABI: include/bie_amd.h) through `ccall`.
+
+    using BoundaryIntegralEquations
+    import BoundaryIntegralEquationsAMD: assemble_parallel!
+    F, G, C = assemble_parallel!(mesh, k, mesh.sources; n=2, m=2)
+
+Supported meshes (TriangularLinear / TriangularQuadratic geometry with geometry or linear
+physics) run on the GPU; everything else (quads, discontinuous physics, `sparse=true`)
+is forwarded to the reference method unchanged, so the call stays a drop-in.  Errors from
+the library are thrown as `ArgumentError` (bad input) or `ErrorException` (HIP failure).
+
+This file is not executed in the build container (no Julia there); the same ABI is
+exercised by the Python ctypes harness.
+"""
+module BoundaryIntegralEquationsAMD
+
+import BoundaryIntegralEquations
+import LinearAlgebra
+const BIE = BoundaryIntegralEquations
+
+const libbie = Ref{String}(get(ENV, "BIE_AMD_LIB",
+                               joinpath(@__DIR__, "..", "..", "bie_amd", "libbie_amd.so")))
+
+# Mirror of `bie_mesh` in include/bie_amd.h (field order and widths must match).
+struct BieMesh
+    coordinates::Ptr{Float64}
+    topology::Ptr{Int64}
+    sources::Ptr{Float64}
+    physics_topology::Ptr{Int64}
+    n_coordinates::Int64
+    n_elements::Int64
+    n_sources::Int64
+    geometry_kind::Int32
+    physics_kind::Int32
+    index_base::Int32
+    reserved::Int32
+end
+
+element_kind(::BIE.TriangularLinear) = Int32(3)
+element_kind(::BIE.TriangularQuadratic) = Int32(6)
+element_kind(::Any) = Int32(0)
+
+last_error() = unsafe_string(ccall((:bie_last_error, libbie[]), Cstring, ()))
+
+function check(rc::Integer)
+    rc == 0 && return nothing
+    msg = last_error()
+    rc == 1 && throw(ArgumentError(msg))
+    rc == 2 && throw(ArgumentError("unsupported element family: " * msg))
+    throw(ErrorException("bie_amd error $rc: " * msg))
+end
+
+supported(mesh) = element_kind(mesh.shape_function) != 0 &&
+                  element_kind(mesh.physics_function) != 0 &&
+                  !(element_kind(mesh.shape_function) == 3 && element_kind(mesh.physics_function) == 6)
+
+"""
+    assemble_hip(mesh, k, in_sources; fOn=true, gOn=true, cOn=true, m=4, n=4)
+
+Dense F, G (M x N ComplexF64) and C (length M) on the GPU.
+"""
+function assemble_hip(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix;
+                      fOn=true, gOn=true, cOn=true, m=4, n=4)
+    coords = Matrix{Float64}(mesh.coordinates)
+    topo = Matrix{Int64}(mesh.topology)
+    srcs = Matrix{Float64}(mesh.sources)
+    ptopo = Matrix{Int64}(mesh.physics_topology)
+    collocation = in_sources === mesh.sources
+    targets = collocation ? srcs : Matrix{Float64}(in_sources)
+    M = size(targets, 2)
+    N = size(srcs, 2)
+    F = Matrix{ComplexF64}(undef, M, N)
+    G = Matrix{ComplexF64}(undef, M, N)
+    C = Vector{ComplexF64}(undef, M)
+    flags = UInt32((fOn ? 1 : 0) | (gOn ? 2 : 0) | (cOn ? 4 : 0))
+    kc = ComplexF64(k)
+    GC.@preserve coords topo srcs ptopo targets F G C begin
+        bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
+                     size(coords, 2), size(topo, 2), N,
+                     element_kind(mesh.shape_function), element_kind(mesh.physics_function),
+                     Int32(1), Int32(0))
+        rc = ccall((:bie_assemble, libbie[]), Cint,
+                   (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Int32, UInt32,
+                    Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                   bm, real(kc), imag(kc), pointer(targets), M, Int32(n), Int32(m), flags,
+                   F, G, C)
+        check(rc)
+    end
+    return F, G, C
+end
+
+"""
+    assemble_parallel!(mesh, k, in_sources; fOn, gOn, cOn, sparse=false, m=4, n=4, kwargs...)
+
+Same signature and defaults as the reference; routes supported dense assemblies to the
+GPU and everything else to `BoundaryIntegralEquations.assemble_parallel!`.
+"""
+function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true, cOn=true,
+                            sparse=false, m=4, n=4, progress=true, depth=2, offset=nothing)
+    if sparse || !supported(mesh)
+        return BIE.assemble_parallel!(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn,
+                                      sparse=sparse, m=m, n=n, progress=progress,
+                                      depth=depth, offset=offset)
+    end
+    return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
+end
+
+# ---------------------------------------------------------------------------------------
+# Device-resident meshes and the batched / adjoint / apply entry points
+# ---------------------------------------------------------------------------------------
+
+"""Device buffer from the library's allocator (freed by a finalizer)."""
+mutable struct DeviceBuffer
+    ptr::Ptr{Cvoid}
+    nbytes::Int
+    function DeviceBuffer(nbytes::Integer)
+        p = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:bie_device_malloc, libbie[]), Cint, (Ptr{Ptr{Cvoid}}, Int64), p,
+                    max(Int64(nbytes), 1)))
+        b = new(p[], Int(nbytes))
+        finalizer(x -> ccall((:bie_device_free, libbie[]), Cint, (Ptr{Cvoid},), x.ptr), b)
+        return b
+    end
+end
+
+function DeviceBuffer(a::Array)
+    b = DeviceBuffer(sizeof(a))
+    GC.@preserve a check(ccall((:bie_memcpy_h2d, libbie[]), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                               b.ptr, pointer(a), sizeof(a)))
+    return b
+end
+
+function download!(a::Array, b::DeviceBuffer)
+    check(ccall((:bie_device_synchronize, libbie[]), Cint, ()))
+    GC.@preserve a check(ccall((:bie_memcpy_d2h, libbie[]), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                               pointer(a), b.ptr, sizeof(a)))
+    return a
+end
+
+"""A mesh uploaded once (bie_dmesh), with its cached rule tables and schedules."""
+mutable struct DeviceMesh
+    handle::Ptr{Cvoid}
+    mesh::BIE.Mesh3d
+    function DeviceMesh(mesh::BIE.Mesh3d; device::Integer=0)
+        supported(mesh) || throw(ArgumentError("unsupported element family"))
+        coords = Matrix{Float64}(mesh.coordinates)
+        topo = Matrix{Int64}(mesh.topology)
+        srcs = Matrix{Float64}(mesh.sources)
+        ptopo = Matrix{Int64}(mesh.physics_topology)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve coords topo srcs ptopo begin
+            bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
+                         size(coords, 2), size(topo, 2), size(srcs, 2),
+                         element_kind(mesh.shape_function), element_kind(mesh.physics_function),
+                         Int32(1), Int32(0))
+            check(ccall((:bie_dmesh_create, libbie[]), Cint, (Ref{BieMesh}, Cint, Ptr{Ptr{Cvoid}}),
+                        bm, Cint(device), h))
+        end
+        dm = new(h[], mesh)
+        finalizer(x -> ccall((:bie_dmesh_destroy, libbie[]), Cint, (Ptr{Cvoid},), x.handle), dm)
+        return dm
+    end
+end
+
+n_sources(dm::DeviceMesh) = size(dm.mesh.sources, 2)
+flagbits(fOn, gOn, cOn) = UInt32((fOn ? 1 : 0) | (gOn ? 2 : 0) | (cOn ? 4 : 0))
+
+"""
+    assemble_sweep(dm, ks; n=4, m=4, fOn=true, gOn=true, cOn=true) -> (Fs, Gs, C)
+
+F_k, G_k for every real wavenumber in `ks` in one launch (bie_assemble_sweep_device);
+C does not depend on k.
+"""
+function assemble_sweep(dm::DeviceMesh, ks::AbstractVector{<:Real}; n=4, m=4, fOn=true,
+                        gOn=true, cOn=true)
+    N = n_sources(dm)
+    nk = length(ks)
+    kd = DeviceBuffer(Vector{Float64}(ks))
+    dF = DeviceBuffer(16 * N * N * nk)
+    dG = DeviceBuffer(16 * N * N * nk)
+    dC = DeviceBuffer(16 * N)
+    check(ccall((:bie_assemble_sweep_device, libbie[]), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64, Int32, Int32, UInt32,
+                 Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
+                dm.handle, kd.ptr, Int32(nk), C_NULL, N, Int32(n), Int32(m),
+                flagbits(fOn, gOn, cOn), dF.ptr, dG.ptr, Int64(N * N), dC.ptr, C_NULL))
+    Fall = download!(Array{ComplexF64}(undef, N, N, nk), dF)
+    Gall = download!(Array{ComplexF64}(undef, N, N, nk), dG)
+    C = download!(Vector{ComplexF64}(undef, N), dC)
+    return [Fall[:, :, q] for q in 1:nk], [Gall[:, :, q] for q in 1:nk], C
+end
+
+"""
+    batch_assemble(meshes, ks; n=4, m=4, tune=false) -> Vector of (F, G, C)
+
+Collocation F, G, C of several meshes (same element kinds), each at its own wavenumber, in
+one persistent launch (bie_batch_*); `tune=true` runs the schedule autotuner first.
+"""
+function batch_assemble(dms::Vector{DeviceMesh}, ks::AbstractVector; n=4, m=4, tune=false)
+    length(dms) == length(ks) || throw(ArgumentError("one wavenumber per mesh"))
+    handles = [dm.handle for dm in dms]
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:bie_batch_create, libbie[]), Cint, (Ptr{Ptr{Cvoid}}, Int32, Int32, Int32,
+                                                      Ptr{Ptr{Cvoid}}),
+                handles, Int32(length(dms)), Int32(n), Int32(m), b))
+    try
+        Ns = [n_sources(dm) for dm in dms]
+        bufs = [(DeviceBuffer(16 * N * N), DeviceBuffer(16 * N * N), DeviceBuffer(16 * N)) for N in Ns]
+        kr = Float64[real(k) for k in ks]
+        ki = Float64[imag(k) for k in ks]
+        pF = [x[1].ptr for x in bufs]; pG = [x[2].ptr for x in bufs]; pC = [x[3].ptr for x in bufs]
+        if tune
+            best = Ref{Float64}(0.0)
+            check(ccall((:bie_batch_tune, libbie[]), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, UInt32, Ptr{Ptr{Cvoid}},
+                         Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Float64}),
+                        b[], kr, ki, UInt32(7), pF, pG, pC, C_NULL, best))
+        else
+            check(ccall((:bie_batch_assemble, libbie[]), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, UInt32, Ptr{Ptr{Cvoid}},
+                         Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}),
+                        b[], kr, ki, UInt32(7), pF, pG, pC, C_NULL))
+        end
+        return [(download!(Matrix{ComplexF64}(undef, N, N), x[1]),
+                 download!(Matrix{ComplexF64}(undef, N, N), x[2]),
+                 download!(Vector{ComplexF64}(undef, N), x[3])) for (N, x) in zip(Ns, bufs)]
+    finally
+        ccall((:bie_batch_destroy, libbie[]), Cint, (Ptr{Cvoid},), b[])
+    end
+end
+
+"""
+    assembly_vjp(dm, k, alpha, p; beta=nothing, targets=nothing, target_node=nothing, n=4, m=4)
+
+Gradient of Re[alpha' * F * p + beta' * C] (plain transposes: alpha, beta are weights, not
+conjugated) with respect to the mesh coordinates (3 x Nc) and k, with the rule selection
+held fixed (bie_assembly_vjp_device).  `target_node[i]` (1-based) marks target i as sitting
+on mesh coordinate `target_node[i]` (it moves with the mesh); `targets === nothing` means
+collocation at the sources.
+"""
+function assembly_vjp(dm::DeviceMesh, k::Real, alpha::AbstractVector, p::AbstractVector;
+                      beta=nothing, targets=nothing, target_node=nothing, n=4, m=4)
+    mesh = dm.mesh
+    M = targets === nothing ? n_sources(dm) : size(targets, 2)
+    Nc = size(mesh.coordinates, 2)
+    dA = DeviceBuffer(Vector{ComplexF64}(alpha))
+    dP = DeviceBuffer(Vector{ComplexF64}(p))
+    dB = beta === nothing ? nothing : DeviceBuffer(Vector{ComplexF64}(beta))
+    dT = targets === nothing ? nothing : DeviceBuffer(Matrix{Float64}(targets))
+    dN = target_node === nothing ? nothing : DeviceBuffer(Vector{Int32}(target_node .- 1))
+    gX = DeviceBuffer(zeros(Float64, 3 * Nc))
+    gk = DeviceBuffer(zeros(Float64, 1))
+    ptr(x) = x === nothing ? C_NULL : x.ptr
+    check(ccall((:bie_assembly_vjp_device, libbie[]), Cint,
+                (Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Cvoid}, Int32, Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dm.handle, Float64(k), ptr(dT), Int64(M), ptr(dN), dA.ptr, ptr(dB), dP.ptr,
+                Int32(n), Int32(m), gX.ptr, gk.ptr, C_NULL))
+    g = download!(Vector{Float64}(undef, 3 * Nc), gX)
+    return reshape(g, 3, Nc), download!(Vector{Float64}(undef, 1), gk)[1]
+end
+
+plane_wave(x, k, d) = exp.(im * k .* vec(d' * x))
+
+"""
+    scattering_gradient(mesh, k, field_points; n=4, m=4, direction=[0, 0, 1]) -> (J, dJdX, dJdk)
+
+J = sum |u|^2 of the field pressure u = p_inc - F_field p at `field_points`, with p solving
+(F + diag(1 - C)) p = p_inc on the rigid body (examples/3d_rigid_sphere.jl:47-52,
+examples/3d_rosebem.jl:81), and its gradient with respect to the mesh coordinates and k by
+the adjoint method: A' lambda = F_field' u, then two vector-Jacobian products of the
+assembly (rule selection held fixed).
+"""
+function scattering_gradient(mesh::BIE.Mesh3d, k::Real, field_points::AbstractMatrix; n=4,
+                             m=4, direction=[0.0, 0.0, 1.0], dm=nothing)
+    dmesh = dm === nothing ? DeviceMesh(mesh) : dm
+    srcs = Matrix{Float64}(mesh.sources)
+    F, _, C = assemble_hip(mesh, k, mesh.sources; gOn=false, n=n, m=m)
+    Ff, _, _ = assemble_hip(mesh, k, Matrix{Float64}(field_points); gOn=false, cOn=false,
+                            n=n, m=m)
+    A = F + LinearAlgebra.Diagonal(1 .- C)
+    b = plane_wave(srcs, k, direction)
+    p = A \ b
+    pinc = plane_wave(field_points, k, direction)
+    u = pinc - Ff * p
+    J = real(sum(abs2, u))
+    lam = A' \ (Ff' * u)
+    # surface nodes are mesh coordinates: find each source's coordinate index
+    lut = Dict(Tuple(srcs[:, j]) => j for j in 1:size(srcs, 2))
+    coord_of = zeros(Int, size(srcs, 2))
+    for j in 1:size(mesh.coordinates, 2)
+        t = Tuple(mesh.coordinates[:, j])
+        haskey(lut, t) && (coord_of[lut[t]] = j)
+    end
+    all(>(0), coord_of) || throw(ArgumentError("collocation nodes are not mesh coordinates"))
+    gs, gks = assembly_vjp(dmesh, k, conj.(lam), p; beta=-conj.(lam) .* p,
+                           target_node=coord_of, n=n, m=m)
+    gf, gkf = assembly_vjp(dmesh, k, -conj.(u), p; targets=Matrix{Float64}(field_points),
+                           n=n, m=m)
+    grad = 2 .* (gs .+ gf)
+    dk = 2 * (gks + gkf)
+    d = Vector{Float64}(direction)
+    coef = real.(-conj.(lam) .* (im * k) .* b)
+    for j in 1:size(srcs, 2), c in 1:3
+        grad[c, coord_of[j]] += 2 * d[c] * coef[j]
+    end
+    dk += 2 * real(sum(-conj.(lam) .* im .* vec(d' * srcs) .* b))
+    dk += 2 * real(sum(conj.(u) .* im .* vec(d' * field_points) .* pinc))
+    return J, grad, dk
+end
+
+"""
+    apply(A, X) -> A * X
+
+Dense complex apply on FP64 MFMA (bie_zgemm_device), e.g. the assembled F or G times a
+block of right-hand sides.
+"""
+function apply(A::AbstractMatrix{ComplexF64}, X::AbstractVecOrMat{ComplexF64})
+    Ah = Matrix{ComplexF64}(A)
+    Xh = Matrix{ComplexF64}(reshape(X, size(X, 1), :))
+    M, K = size(Ah)
+    size(Xh, 1) == K || throw(DimensionMismatch("inner dimensions differ"))
+    nrhs = size(Xh, 2)
+    dA = DeviceBuffer(Ah); dX = DeviceBuffer(Xh); dY = DeviceBuffer(16 * M * nrhs)
+    check(ccall((:bie_zgemm_device, libbie[]), Cint,
+                (Int64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64,
+                 Ptr{Cvoid}),
+                M, nrhs, K, dA.ptr, max(M, 1), dX.ptr, max(K, 1), dY.ptr, max(M, 1), C_NULL))
+    Y = download!(Matrix{ComplexF64}(undef, M, nrhs), dY)
+    return X isa AbstractVector ? vec(Y) : Y
+end
+
+"""
+    scattering_objective(mesh, k, field_points; n=4, m=4, direction=[0, 0, 1]) -> J
+
+The objective of `scattering_gradient` alone.  With ChainRulesCore loaded, its `rrule`
+returns the adjoint gradient with respect to the mesh coordinates (as a tangent of
+`mesh.coordinates`) and k.
+"""
+function scattering_objective(mesh::BIE.Mesh3d, k::Real, field_points::AbstractMatrix; n=4,
+                              m=4, direction=[0.0, 0.0, 1.0])
+    F, _, C = assemble_hip(mesh, k, mesh.sources; gOn=false, n=n, m=m)
+    Ff, _, _ = assemble_hip(mesh, k, Matrix{Float64}(field_points); gOn=false, cOn=false,
+                            n=n, m=m)
+    p = (F + LinearAlgebra.Diagonal(1 .- C)) \ plane_wave(Matrix{Float64}(mesh.sources), k, direction)
+    u = plane_wave(field_points, k, direction) - Ff * p
+    return real(sum(abs2, u))
+end
+
+end # module
