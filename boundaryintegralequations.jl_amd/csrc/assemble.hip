@@ -200,27 +200,6 @@ __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double
   }
 }
 
-// Per-wave staging of an element's three vertex-rule tables (3 Qc rows of 64 bytes,
-// nchunk = 12 Qc chunks of 16 bytes, at most 3 per lane): global -> registers early,
-// registers -> the wave's LDS rows later, so no global load is pending in the point loop.
-__device__ __forceinline__ void stage_load(const double* __restrict__ elem_e, int nchunk,
-                                           int lane, double2 (&v)[3]) {
-#pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int j = lane + 64 * t;
-    v[t] = j < nchunk ? reinterpret_cast<const double2*>(elem_e)[j] : make_double2(0.0, 0.0);
-  }
-}
-
-__device__ __forceinline__ void stage_store(double* rows, int nchunk, int lane,
-                                            const double2 (&v)[3]) {
-#pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int j = lane + 64 * t;
-    if (j < nchunk) reinterpret_cast<double2*>(rows)[j] = v[t];
-  }
-}
-
 // One thread per target row; blockIdx.y selects a contiguous chunk of elements.
 // Every (target, element) contribution is added into zeroed F, G with hardware FP64
 // atomics; C is accumulated per thread and added once per chunk.  Used for small

@@ -91,5 +91,26 @@ __device__ __forceinline__ void load_rows01(const double* __restrict__ tab, int 
   B2 = *reinterpret_cast<const double4*>(tab + 8 * q1 + 4);
 }
 
+// Per-wave staging of an element's three vertex-rule tables (3 Qc rows of 64 bytes,
+// nchunk = 12 Qc chunks of 16 bytes, at most 3 per lane): global -> registers early,
+// registers -> the wave's LDS rows later, so no global load is pending in the point loop.
+__device__ __forceinline__ void stage_load(const double* __restrict__ elem_e, int nchunk,
+                                           int lane, double2 (&v)[3]) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int j = lane + 64 * t;
+    v[t] = j < nchunk ? reinterpret_cast<const double2*>(elem_e)[j] : make_double2(0.0, 0.0);
+  }
+}
+
+__device__ __forceinline__ void stage_store(double* rows, int nchunk, int lane,
+                                            const double2 (&v)[3]) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int j = lane + 64 * t;
+    if (j < nchunk) reinterpret_cast<double2*>(rows)[j] = v[t];
+  }
+}
+
 }  // namespace
 }  // namespace bie
