@@ -188,6 +188,32 @@ void free_schedules(bie_dmesh* dm) {
   for (auto& sc : dm->sched) free_schedule(sc);
 }
 
+// Segment count of the single-k slot kernel's schedule: at least 8 tiles (row block x
+// segment) per CU, and among counts up to twice that, the one whose tile count fills whole
+// rounds of resident workgroups (one per CU) best, so that the last round leaves few CUs
+// idle.  (The schedule may split further where a segment's front exceeds the slots.)
+int choose_segment_count(int device, int64_t rowblocks, int64_t E) {
+  static int cus_of[16] = {0};
+  int& cus = cus_of[device & 15];
+  if (cus == 0) {
+    BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (cus <= 0) cus = 1;
+  }
+  const int64_t hi_e = std::max<int64_t>(1, E);
+  const int64_t base = std::max<int64_t>(1, std::min<int64_t>(hi_e, (8LL * cus + rowblocks - 1) / rowblocks));
+  int64_t best = base;
+  double best_waste = 1e300;
+  for (int64_t p = base; p <= std::min<int64_t>(hi_e, 2 * base); ++p) {
+    const int64_t tiles = rowblocks * p;
+    const double waste = (double)((tiles + cus - 1) / cus * cus) / (double)tiles;
+    if (waste < best_waste - 1e-12) {
+      best_waste = waste;
+      best = p;
+    }
+  }
+  return (int)best;
+}
+
 // Schedule `which` (0: single-k kernel, 1: sweep kernel, 2: batch kernel) for S live
 // slots and at least min_segments segments.
 bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segments,
@@ -362,7 +388,7 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
     } else {
       const int64_t rowblocks = (M + 63) / 64;
-      const int min_segments = (int)std::max<int64_t>(1, std::min<int64_t>(dm->E, (2048 + rowblocks - 1) / rowblocks));
+      const int min_segments = choose_segment_count(dm->device, rowblocks, dm->E);
       const bie_dmesh::Sched& sc =
           ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0]), min_segments, s);
       // collocation on quadratic physics: after the first call the mid-edge pairs come

@@ -12,10 +12,10 @@
 // ("owned") are written once with a plain store; nodes touched by several segments
 // ("shared") are added atomically into columns zeroed beforehand.
 //
-// Segments come from recursive coordinate bisection of the element centroids; each is
-// swept along its longest axis, which keeps the number of occupied slots (the sweep
-// front) near the segment's width.  Segments are split until their front fits the slot
-// budget.
+// Segments come from recursive coordinate splitting of the element centroids into the
+// requested number of parts; each is swept along its longest axis, which keeps the number
+// of occupied slots (the sweep front) near the segment's width.  Segments are split
+// further until their front fits the slot budget.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -80,21 +80,27 @@ int front_width(const std::vector<int>& idx, const int* ptopo, int NP, int W,
   return best;
 }
 
-void bisect(std::vector<int> idx, const double* cent, const int* ptopo, int NP, int W, int S,
-            size_t max_size, std::vector<int>& first, std::vector<int>& last,
-            std::vector<int>& touched, std::vector<SegOrder>& out) {
+// Splits idx into `parts` segments of nearly equal size along longest axes (halves of
+// the part count at every level, so any count is reachable), then splits further any
+// segment whose front exceeds S slots.
+void split(std::vector<int> idx, int parts, const double* cent, const int* ptopo, int NP, int W,
+           int S, std::vector<int>& first, std::vector<int>& last, std::vector<int>& touched,
+           std::vector<SegOrder>& out) {
   const int ax = longest_axis(idx, cent);
   sort_along(idx, cent, ax);
-  if (idx.size() <= max_size || idx.size() <= 1) {
+  if ((int)idx.size() < parts) parts = std::max(1, (int)idx.size());
+  if (parts <= 1) {
     if (idx.size() <= 1 || front_width(idx, ptopo, NP, W, first, last, touched) <= S) {
       out.push_back(SegOrder{std::move(idx)});
       return;
     }
+    parts = 2;
   }
-  const size_t h = idx.size() / 2;
+  const int pa = parts / 2, pb = parts - pa;
+  const size_t h = idx.size() * (size_t)pa / (size_t)parts;
   std::vector<int> a(idx.begin(), idx.begin() + h), b(idx.begin() + h, idx.end());
-  bisect(std::move(a), cent, ptopo, NP, W, S, max_size, first, last, touched, out);
-  bisect(std::move(b), cent, ptopo, NP, W, S, max_size, first, last, touched, out);
+  split(std::move(a), pa, cent, ptopo, NP, W, S, first, last, touched, out);
+  split(std::move(b), pb, cent, ptopo, NP, W, S, first, last, touched, out);
 }
 
 }  // namespace
@@ -110,8 +116,8 @@ SlotSchedule build_slot_schedule(const int* ptopo, int NP, int E, int N, const d
   if (E > 0) {
     std::vector<int> all(E);
     std::iota(all.begin(), all.end(), 0);
-    const size_t max_size = std::max<size_t>(1, (E + min_segments - 1) / std::max(1, min_segments));
-    bisect(std::move(all), cent, ptopo, NP, W, S, max_size, first, last, touched, segs);
+    split(std::move(all), std::max(1, min_segments), cent, ptopo, NP, W, S, first, last, touched,
+          segs);
   }
   sc.n_segments = (int)segs.size();
   // how many segments touch each node

@@ -28,6 +28,8 @@ struct SlotSchedule {
   std::vector<int> zero_cols;       // columns to zero before the kernel
 };
 
+// min_segments: the number of segments to aim for (exactly that many unless a segment's
+// front exceeds S slots, which splits it further).
 SlotSchedule build_slot_schedule(const int* ptopo, int NP, int E, int N, const double* cent,
                                  int W, int S, int min_segments);
 
