@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB_PATH = os.path.join(HERE, "libbie_amd.so")
 
 SOURCES = ["rules.cpp", "schedule.cpp", "assemble.hip", "sweep.hip", "adjoint.hip", "apply.hip",
-           "midsort.hip", "api.hip"]
+           "api.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -297,24 +297,61 @@ void ensure_mid_queue(bie_dmesh* dm, int64_t cap) {
   dm->mid_cap = cap;
 }
 
-// Records the collocation mid-edge pairs that the slot kernel queued, sorted by (element,
-// target), and releases the queue (synchronises the stream once per mesh).
+// Records the collocation mid-edge pairs that the slot kernel queued: the pairs sorted by
+// (element, target), and the distinct (column, row) entries of F and G they add to, sorted
+// by (column, row), each with the list of (pair, node) contributions.  Releases the queue.
+// Host work once per mesh; synchronises the stream.
 void record_mid_pairs(bie_dmesh* dm, hipStream_t s) {
   unsigned count = 0;
   BIE_HIP_CHECK(hipMemcpyAsync(&count, dm->d_mid_count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   BIE_HIP_CHECK(hipStreamSynchronize(s));
   const int64_t n = std::min<int64_t>((int64_t)count, dm->mid_cap);
-  unsigned long long* keys = dalloc<unsigned long long>((size_t)std::max<int64_t>(n, 1));
-  try {
-    launch_mid_keys(dm->d_mid_queue, n, keys, s);
-    sort_keys(keys, n, 62, s);
-  } catch (...) {
-    dfree(keys);
-    throw;
+  BIE_REQUIRE(n < (int64_t)1 << 28, BIE_EINVAL, "too many mid-edge pairs to record");
+  std::vector<int2> q((size_t)n);
+  if (n)
+    BIE_HIP_CHECK(hipMemcpy(q.data(), dm->d_mid_queue, sizeof(int2) * n, hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> keys((size_t)n);
+  for (int64_t p = 0; p < n; ++p)
+    keys[p] = ((unsigned long long)(q[p].y & 0x0fffffff) << 34) |
+              ((unsigned long long)(unsigned)q[p].x << 2) | (unsigned long long)(q[p].y >> 28);
+  std::sort(keys.begin(), keys.end());
+  std::vector<std::pair<unsigned long long, int>> con((size_t)n * 6);
+  for (int64_t p = 0; p < n; ++p) {
+    const int e = (int)(keys[p] >> 34);
+    const unsigned row = (unsigned)((keys[p] >> 2) & 0xffffffffull);
+    for (int k = 0; k < 6; ++k) {
+      const unsigned col = (unsigned)dm->h_ptopo[(size_t)e * 6 + k];
+      con[(size_t)p * 6 + k] = {((unsigned long long)col << 32) | row, (int)(p * 6 + k)};
+    }
   }
-  dfree(dm->d_mid_keys);
-  dm->d_mid_keys = keys;
+  std::sort(con.begin(), con.end());
+  std::vector<unsigned long long> entries;
+  std::vector<int> off, refs(con.size());
+  for (size_t t = 0; t < con.size(); ++t) {
+    if (t == 0 || con[t].first != con[t - 1].first) {
+      entries.push_back(con[t].first);
+      off.push_back((int)t);
+    }
+    refs[t] = con[t].second;
+  }
+  off.push_back((int)con.size());
+  auto up = [&](auto& v) { return upload_vec(v, s); };
+  unsigned long long* d_keys = up(keys);
+  unsigned long long* d_entries = up(entries);
+  int* d_off = up(off);
+  int* d_refs = up(refs);
+  double* d_vals = dalloc<double>((size_t)std::max<int64_t>(n, 1) * 24);
+  BIE_HIP_CHECK(hipStreamSynchronize(s));
+  for (void* p : {(void*)dm->d_mid_keys, (void*)dm->d_mid_entries, (void*)dm->d_mid_off,
+                  (void*)dm->d_mid_refs, (void*)dm->d_mid_vals})
+    dfree(p);
+  dm->d_mid_keys = d_keys;
+  dm->d_mid_entries = d_entries;
+  dm->d_mid_off = d_off;
+  dm->d_mid_refs = d_refs;
+  dm->d_mid_vals = d_vals;
   dm->n_mid_keys = n;
+  dm->n_mid_entries = (int64_t)entries.size();
   dfree(dm->d_mid_queue);  // the M x E queue is no longer needed for collocation
   dm->d_mid_queue = nullptr;
   dm->mid_cap = 0;
@@ -411,7 +448,8 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       launch_assemble_slots(a, sa, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
       if (recorded) {
-        launch_assemble_mid_sorted(a, dm->d_mid_keys, dm->n_mid_keys, s);
+        launch_mid_recorded(a, dm->d_mid_keys, dm->n_mid_keys, dm->d_mid_vals, dm->d_mid_entries,
+                            dm->d_mid_off, dm->d_mid_refs, dm->n_mid_entries, s);
       } else {
         launch_assemble_mid(a, sa, s);
         if (dm->pk == 6 && colloc) record_mid_pairs(dm, s);
@@ -716,6 +754,10 @@ int bie_dmesh_destroy(bie_dmesh* dm) {
   dfree(dm->d_mid_count);
   dfree(dm->d_mid_queue);
   dfree(dm->d_mid_keys);
+  dfree(dm->d_mid_vals);
+  dfree(dm->d_mid_entries);
+  dfree(dm->d_mid_off);
+  dfree(dm->d_mid_refs);
   delete dm;
   return BIE_OK;
 }

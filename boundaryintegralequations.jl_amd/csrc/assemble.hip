@@ -509,13 +509,14 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
 }
 
 // One mid-edge split-rule pair (rotated_midpoint_triangular_quadpoints,
-// quadrature.jl:117-144): target i, element e, rule c (3..5), results added atomically
-// after the slot kernel has written its columns.
-__device__ __forceinline__ void mid_pair(const AssemblyArgs& a, int64_t i, int e, int c,
-                                         int cplx) {
-  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+// quadrature.jl:117-144): target i, element e, rule c (3..5): the F and G contributions of
+// the element's 6 nodes and the C integrand.
+__device__ __forceinline__ void mid_pair_values(const AssemblyArgs& a, int64_t i, int e, int c,
+                                                int cplx, double (&fr)[6], double (&fi)[6],
+                                                double (&gr)[6], double (&gi)[6],
+                                                double& csum) {
   const double x0 = a.targets[3 * i], x1 = a.targets[3 * i + 1], x2 = a.targets[3 * i + 2];
-  double fr[6], fi[6], gr[6], gi[6], csum = 0.0;
+  csum = 0.0;
 #pragma unroll
   for (int s = 0; s < 6; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
   const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
@@ -524,6 +525,14 @@ __device__ __forceinline__ void mid_pair(const AssemblyArgs& a, int64_t i, int e
     pair_quadrature<6, true>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
   else
     pair_quadrature<6, false>(x0, x1, x2, tab, ph, a.len[c], a.kr, a.ki, fr, fi, gr, gi, csum);
+}
+
+// A mid-edge pair added atomically after the slot kernel has written its columns.
+__device__ __forceinline__ void mid_pair(const AssemblyArgs& a, int64_t i, int e, int c,
+                                         int cplx) {
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+  double fr[6], fi[6], gr[6], gi[6], csum;
+  mid_pair_values(a, i, e, c, cplx, fr, fi, gr, gi, csum);
   const int* cols = a.ptopo + (size_t)e * 6;
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
@@ -551,28 +560,57 @@ __device__ __forceinline__ void mid_pairs(const AssemblyArgs& a, const unsigned*
   }
 }
 
-// Sort key of a queued pair: element (bits 34..61), target (2..33), rule - 3 (0..1).
-__global__ void k_mid_keys(const int2* __restrict__ queue, int64_t n,
-                           unsigned long long* __restrict__ keys) {
+// Recorded collocation mid-edge pairs, part 1: one thread per pair (key: element bits
+// 34..61, target 2..33, rule - 3 0..1; sorted, so a wave mostly shares one element) writes
+// its 6 x {F re, F im, G re, G im} to vals[24 p ..) with plain stores; C is added here.
+__global__ void __launch_bounds__(256)
+k_mid_compute(const AssemblyArgs a, const unsigned long long* __restrict__ keys, int64_t n,
+              double* __restrict__ vals, int cplx) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
-    const int2 en = queue[t];
-    keys[t] = ((unsigned long long)(en.y & 0x0fffffff) << 34) |
-              ((unsigned long long)(unsigned)en.x << 2) | (unsigned long long)(en.y >> 28);
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += stride) {
+    const unsigned long long key = keys[p];
+    const int64_t i = (int64_t)((key >> 2) & 0xffffffffull);
+    double fr[6], fi[6], gr[6], gi[6], csum;
+    mid_pair_values(a, i, (int)(key >> 34), 3 + (int)(key & 3ull), cplx, fr, fi, gr, gi, csum);
+    double4* out = reinterpret_cast<double4*>(vals + (size_t)p * 24);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) out[s] = make_double4(fr[s], fi[s], gr[s], gi[s]);
+    if (a.flags & BIE_C) unsafeAtomicAdd(a.C + 2 * i, csum);
   }
 }
 
-// Mid-edge pairs from the recorded list sorted by (element, target): the lanes of a wave
-// mostly share one element, so its table rows are read once per wave and the column
-// atomics hit neighbouring rows.
+// Part 2: one thread per distinct (column, row) entry (sorted by column, then row) sums its
+// pairs' contributions (refs[off[t] .. off[t+1]) = pair * 6 + node) and adds them to F and G
+// with a plain read-modify-write: no two threads share an entry, and the slot kernel's
+// writes precede this kernel on the stream.
 __global__ void __launch_bounds__(256)
-k_assemble_mid_sorted(const AssemblyArgs a, const unsigned long long* __restrict__ keys,
-                      int64_t n, int cplx) {
+k_mid_scatter(const AssemblyArgs a, const unsigned long long* __restrict__ entries,
+              const int* __restrict__ off, const int* __restrict__ refs, int64_t n,
+              const double* __restrict__ vals) {
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
-    const unsigned long long key = keys[t];
-    mid_pair(a, (int64_t)((key >> 2) & 0xffffffffull), (int)(key >> 34), 3 + (int)(key & 3ull),
-             cplx);
+    const unsigned long long u = entries[t];
+    const size_t col = (size_t)(u >> 32), row = (size_t)(u & 0xffffffffull);
+    double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0;
+    for (int r = off[t]; r < off[t + 1]; ++r) {
+      const double4 v = *reinterpret_cast<const double4*>(vals + (size_t)refs[r] * 4);
+      fr += v.x;
+      fi += v.y;
+      gr += v.z;
+      gi += v.w;
+    }
+    const size_t idx = row + col * (size_t)a.M;
+    if (doF) {
+      double2* f = reinterpret_cast<double2*>(a.F) + idx;
+      const double2 o = *f;
+      *f = make_double2(o.x + fr, o.y + fi);
+    }
+    if (doG) {
+      double2* g = reinterpret_cast<double2*>(a.G) + idx;
+      const double2 o = *g;
+      *g = make_double2(o.x + gr, o.y + gi);
+    }
   }
 }
 
@@ -812,21 +850,20 @@ int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage) {
   return nb < 1 ? 1 : nb;
 }
 
-void launch_mid_keys(const int2* queue, int64_t n, unsigned long long* keys, hipStream_t s) {
-  if (n <= 0) return;
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_mid_keys, dim3((unsigned)blocks), dim3(256), 0, s, queue, n, keys);
-  BIE_HIP_CHECK(hipGetLastError());
-}
-
-void launch_assemble_mid_sorted(const AssemblyArgs& a, const unsigned long long* keys, int64_t n,
-                                hipStream_t s) {
-  if (n <= 0) return;
-  int64_t blocks = (n + 255) / 256;
+void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
+                         double* vals, const unsigned long long* entries, const int* off,
+                         const int* refs, int64_t n_entries, hipStream_t s) {
+  if (n_pairs <= 0) return;
+  int64_t blocks = (n_pairs + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_assemble_mid_sorted, dim3((unsigned)blocks), dim3(256), 0, s, a, keys, n,
-                     a.ki != 0.0 ? 1 : 0);
+  hipLaunchKernelGGL(k_mid_compute, dim3((unsigned)blocks), dim3(256), 0, s, a, keys, n_pairs,
+                     vals, a.ki != 0.0 ? 1 : 0);
+  BIE_HIP_CHECK(hipGetLastError());
+  if (n_entries <= 0 || !(a.flags & (BIE_F | BIE_G))) return;
+  blocks = (n_entries + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_mid_scatter, dim3((unsigned)blocks), dim3(256), 0, s, a, entries, off, refs,
+                     n_entries, vals);
   BIE_HIP_CHECK(hipGetLastError());
 }
 

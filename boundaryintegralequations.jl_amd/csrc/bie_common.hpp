@@ -183,12 +183,9 @@ void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_t
                            unsigned* d_counter, const double* d_phi, int nphi, int nstage,
                            int S, int NP, bool cplx, int grid, hipStream_t s);
 void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, hipStream_t s);
-void launch_mid_keys(const int2* queue, int64_t n, unsigned long long* keys, hipStream_t s);
-void launch_assemble_mid_sorted(const AssemblyArgs& a, const unsigned long long* keys, int64_t n,
-                                hipStream_t s);
-// midsort.hip: keys sorted ascending (radix sort on the low `bits` bits); temporary storage
-// is allocated and freed inside
-void sort_keys(unsigned long long* keys, int64_t n, int bits, hipStream_t s);
+void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
+                         double* vals, const unsigned long long* entries, const int* off,
+                         const int* refs, int64_t n_entries, hipStream_t s);
 int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
@@ -245,10 +242,16 @@ struct bie_dmesh {
   unsigned* d_mid_count = nullptr;
   int2* d_mid_queue = nullptr;
   int64_t mid_cap = 0;
-  // mid-edge pairs of collocation (targets = sources), sorted by (element, target): they
-  // depend on the geometry only, so the first collocation assembly records them
+  // mid-edge pairs of collocation (targets = sources): they depend on the geometry only,
+  // so the first collocation assembly records them, sorted by (element, target), with the
+  // distinct (column, row) entries they add to (CSR over pair * 6 + node)
   unsigned long long* d_mid_keys = nullptr;
   int64_t n_mid_keys = -1;  // -1: not recorded yet
+  double* d_mid_vals = nullptr;  // 24 per pair
+  unsigned long long* d_mid_entries = nullptr;
+  int* d_mid_off = nullptr;
+  int* d_mid_refs = nullptr;
+  int64_t n_mid_entries = 0;
   std::mutex mu;
 };
 
