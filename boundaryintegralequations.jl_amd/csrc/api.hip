@@ -639,9 +639,9 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
     bi.zero_cols = sc.d_zero_cols;
     bi.n_zero = sc.zero_cols;
     bi.pad = 0;
-    max_work = std::max<int64_t>(max_work, dm->N * std::max<int64_t>(1, sc.zero_cols));
+    max_work = std::max<int64_t>(max_work, sc.zero_cols);  // columns to zero per output
     if ((!(flags & BIE_F) && dF[ii]) || (!(flags & BIE_G) && dG[ii]))
-      max_work = std::max<int64_t>(max_work, dm->N * dm->N);
+      max_work = std::max<int64_t>(max_work, dm->N);
   }
   BIE_HIP_CHECK(hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(BatchInst) * n_inst,
                                hipMemcpyHostToDevice, s));

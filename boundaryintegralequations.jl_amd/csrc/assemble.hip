@@ -636,40 +636,33 @@ k_assemble_mid_batch(const BatchInst* __restrict__ inst, int cplx) {
 __global__ void __launch_bounds__(256) k_batch_prep(const BatchInst* __restrict__ inst) {
   const BatchInst& b = inst[blockIdx.y];
   const AssemblyArgs& a = b.a;
-  const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const double2 z = make_double2(0.0, 0.0);
-  if (first == 0 && b.sa.mid_count) *b.sa.mid_count = 0u;
-  if (a.C)
-    for (int64_t t = first; t < a.M; t += stride) reinterpret_cast<double2*>(a.C)[t] = z;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && b.sa.mid_count) *b.sa.mid_count = 0u;
+  if (a.C && blockIdx.x == 0)
+    for (int64_t r = threadIdx.x; r < a.M; r += blockDim.x) reinterpret_cast<double2*>(a.C)[r] = z;
   double* outs[2] = {a.F, a.G};
   const unsigned bits[2] = {BIE_F, BIE_G};
   for (int o = 0; o < 2; ++o) {
     double2* X = reinterpret_cast<double2*>(outs[o]);
     if (!X) continue;
-    if (a.flags & bits[o]) {
-      const int64_t total = (int64_t)b.n_zero * a.M;
-      for (int64_t t = first; t < total; t += stride) {
-        const int64_t c = t / a.M;
-        const int64_t r = t - c * a.M;
-        X[(size_t)b.zero_cols[c] * a.M + r] = z;
-      }
-    } else {
-      const int64_t total = a.N * a.M;
-      for (int64_t t = first; t < total; t += stride) X[t] = z;
+    // workgroups stride over the columns to zero, threads over the rows of a column
+    const bool whole = !(a.flags & bits[o]);
+    const int64_t ncols = whole ? a.N : b.n_zero;
+    for (int64_t c = blockIdx.x; c < ncols; c += gridDim.x) {
+      double2* col = X + (size_t)(whole ? c : b.zero_cols[c]) * a.M;
+      for (int64_t r = threadIdx.x; r < a.M; r += blockDim.x) col[r] = z;
     }
   }
 }
 
-// Zero whole columns j of a column-major M x N complex matrix, j from a list.
+// Zero whole columns j of a column-major M x N complex matrix, j from a list: workgroups
+// stride over the listed columns, threads over the rows of a column.
 __global__ void k_zero_columns(double2* __restrict__ X, int64_t M, const int* __restrict__ cols,
                                int ncols) {
-  const int64_t total = (int64_t)ncols * M;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += stride) {
-    const int64_t c = t / M;
-    const int64_t r = t - c * M;
-    X[(size_t)cols[c] * M + r] = make_double2(0.0, 0.0);
+  const double2 z = make_double2(0.0, 0.0);
+  for (int c = blockIdx.x; c < ncols; c += gridDim.x) {
+    double2* col = X + (size_t)cols[c] * M;
+    for (int64_t r = threadIdx.x; r < M; r += blockDim.x) col[r] = z;
   }
 }
 
@@ -732,8 +725,7 @@ void launch_select(const double* targets, int64_t M, const double* sel, int NP, 
 
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s) {
   if (ncols <= 0 || M <= 0) return;
-  int64_t blocks = ((int64_t)ncols * M + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
+  const int blocks = ncols < 8192 ? ncols : 8192;
   hipLaunchKernelGGL(k_zero_columns, dim3((unsigned)blocks), dim3(256), 0, s,
                      reinterpret_cast<double2*>(X), M, cols, ncols);
   BIE_HIP_CHECK(hipGetLastError());
@@ -778,10 +770,10 @@ void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t 
   BIE_HIP_CHECK(hipGetLastError());
 }
 
-void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_work, hipStream_t s) {
+void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_cols, hipStream_t s) {
   if (n_inst <= 0) return;
   BIE_REQUIRE(n_inst <= 65535, BIE_EINVAL, "too many batch instances");
-  int64_t blocks = (max_work + 255) / 256;
+  int64_t blocks = max_cols;
   if (blocks < 1) blocks = 1;
   if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_batch_prep, dim3((unsigned)blocks, (unsigned)n_inst), dim3(256), 0, s,

@@ -177,7 +177,7 @@ void launch_assemble(const AssemblyArgs& a, hipStream_t s);
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
-void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_work, hipStream_t s);
+void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_cols, hipStream_t s);
 size_t batch_shmem(int S, int nphi, int nstage);
 void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
                            unsigned* d_counter, const double* d_phi, int nphi, int nstage,
