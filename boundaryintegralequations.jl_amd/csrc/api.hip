@@ -430,7 +430,7 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
           ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0]), min_segments, s);
       // collocation on quadratic physics: after the first call the mid-edge pairs come
       // from the recorded sorted list instead of the slot kernel's queue
-      const bool colloc = d_targets == dm->d_sources;
+      const bool colloc = d_targets == dm->d_sources && M == dm->N;
       const bool recorded = dm->pk == 6 && colloc && dm->n_mid_keys >= 0;
       if (dm->pk == 6 && !recorded) {
         ensure_mid_queue(dm, M * dm->E);

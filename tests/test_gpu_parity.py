@@ -236,3 +236,23 @@ def test_device_path_repeated_calls_reuse_caches():
     Ff = fb[0].download(np.zeros(P * N, dtype=np.complex128)).reshape(P, N, order="F")
     Ffo, _, _ = oracle.assemble(mesh, 6.0, targets=pts, n=3, m=2)
     assert rel(Ff, Ffo) < TOL
+
+
+def test_sources_prefix_as_targets_does_not_record_collocation_pairs():
+    """The device sources pointer with fewer targets than sources is not collocation: the
+    recorded mid-edge pairs of a later full collocation call must still be complete."""
+    mesh = MESHES["quad4"]()
+    dm = DeviceMesh(mesh)
+    N = mesh.n_sources
+    P = N // 3
+    fb = [DevBuf(16 * P * N), DevBuf(16 * P * N), DevBuf(16 * P)]
+    dm.assemble(4.0, fb[0].ptr, fb[1].ptr, fb[2].ptr, targets_ptr=dm.sources_ptr, n_targets=P,
+                n=3, m=2)
+    bufs = [DevBuf(16 * N * N), DevBuf(16 * N * N), DevBuf(16 * N)]
+    for _ in range(2):   # first call records, second uses the record
+        dm.assemble(4.0, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n=3, m=2)
+    _lib.check(_lib.lib().bie_device_synchronize())
+    F = bufs[0].download(np.zeros(N * N, dtype=np.complex128)).reshape(N, N, order="F")
+    Fp = fb[0].download(np.zeros(P * N, dtype=np.complex128)).reshape(P, N, order="F")
+    Fo, _, _ = oracle.assemble(mesh, 4.0, n=3, m=2)
+    assert rel(F, Fo) < TOL and rel(Fp, Fo[:P]) < TOL
