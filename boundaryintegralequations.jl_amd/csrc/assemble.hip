@@ -264,10 +264,10 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
                                            int wave, int lane, int64_t i, bool active,
                                            double* acc) {
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
-  const int f1 = sa.flush_begin[st + 1];
-  for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
-    const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
-    const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
+  const int f1 = sload(sa.flush_begin + st + 1);
+  for (int f = sload(sa.flush_begin + st) + wave; f < f1; f += W) {
+    const int node = sload(sa.flush_node + f);
+    const int ss = sload(sa.flush_slot + f);
     double* p = acc + (size_t)(ss & 0xffff) * 256 + lane;
     const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
     p[0] = 0.0;
@@ -326,8 +326,8 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   const int nchunk = 12 * Qc;
   double* my_rows = STAGED ? s_rows + (size_t)wave * 24 * Qc : nullptr;
   double2 stg[3];
-  const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
-  int e = (st0 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st0 * W + wave]) : -1;
+  const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
+  int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * W + wave) : -1;
   int c = 0;
   double4 A0 = make_double4(0, 0, 0, 0), B0 = A0, A1 = A0, B1 = A0;
   if (e >= 0) {
@@ -342,8 +342,11 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   for (int st = st0; st < st1; ++st) {
     const size_t nidx = (size_t)(st + 1) * W + wave;
     const int e_nxt =
-        (st + 1 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[nidx]) : -1;
-    if (STAGED && e_nxt >= 0) stage_load(a.elem + (size_t)e_nxt * a.Pe * 8, nchunk, lane, stg);
+        (st + 1 < st1) ? sload(sa.step_elems + nidx) : -1;
+    // unconditional (element 0 stands in at the segment's end): a path that skipped the
+    // LDS store below would leave these loads outstanding at the loop head, where the
+    // compiler would wait for them, and with them for the previous step's column stores
+    if (STAGED) stage_load(a.elem + (size_t)max(e_nxt, 0) * a.Pe * 8, nchunk, lane, stg);
     if (e >= 0) {
       const bool mid = NP == 6 && c >= 3;
       if (active && !mid) {
@@ -360,7 +363,7 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
         const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
 #pragma unroll
         for (int s = 0; s < NP; ++s) {
-          double* p = acc + (size_t)so[s] * 256 + lane;
+          double* p = acc + (size_t)sload(so + s) * 256 + lane;
           atomicAdd(p, fr[s]);
           atomicAdd(p + 64, fi[s]);
           atomicAdd(p + 128, gr[s]);
@@ -392,12 +395,11 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     int c_nxt = 0;
     if (e_nxt >= 0) {
       c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
-      if (STAGED)
-        stage_store(my_rows, nchunk, lane, stg);
-      else if (!(NP == 6 && c_nxt >= 3))
+      if (!STAGED && !(NP == 6 && c_nxt >= 3))
         load_rows01(a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8, a.len[c_nxt], A0, B0,
                     A1, B1);
     }
+    if (STAGED) stage_store(my_rows, nchunk, lane, stg);  // only this wave reads my_rows
     // the previous step's finished nodes: their slots were complete at the last barrier
     if (st > st0) flush_step<W>(a, sa, st - 1, wave, lane, i, active, acc);
     lds_barrier();

@@ -18,6 +18,18 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Wave-uniform read of read-only data (the host-built slot schedules) through the scalar
+// data cache: the load goes to an SGPR and is counted by lgkmcnt.  A vector load of the
+// same value would be counted by vmcnt, which on gfx9-family GPUs retires loads and stores
+// in issue order, so waiting for it would also wait for every column store issued before
+// it.  Only for addresses that are the same in all active lanes and memory the kernel
+// never writes.
+template <class T>
+__device__ __forceinline__ T sload(const T* p) {
+  return *reinterpret_cast<const __attribute__((address_space(4))) T*>(
+      reinterpret_cast<uintptr_t>(p));
+}
+
 // Julia Base n-ary hypot for Float64 (maxabs * sqrt(sum((x_i/maxabs)^2)), left to
 // right), with no fused multiply-add so that the result is bit-identical to the CPU.
 __device__ __noinline__ double julia_hypot3(double x, double y, double z) {
@@ -94,12 +106,19 @@ __device__ __forceinline__ void load_rows01(const double* __restrict__ tab, int 
 // Per-wave staging of an element's three vertex-rule tables (3 Qc rows of 64 bytes,
 // nchunk = 12 Qc chunks of 16 bytes, at most 3 per lane): global -> registers early,
 // registers -> the wave's LDS rows later, so no global load is pending in the point loop.
+// The loads are buffer loads whose record count is the element's nchunk chunks, so lanes
+// past the end read zeros from the bounds check: no exec mask and no register zeroing
+// around them, which would otherwise make the compiler wait for every older vector-memory
+// operation (the previous step's column stores included) before issuing them.
 __device__ __forceinline__ void stage_load(const double* __restrict__ elem_e, int nchunk,
                                            int lane, double2 (&v)[3]) {
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double*>(elem_e), (short)0, nchunk * 16, 0x00020000);
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
-    const int j = lane + 64 * t;
-    v[t] = j < nchunk ? reinterpret_cast<const double2*>(elem_e)[j] : make_double2(0.0, 0.0);
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const u4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, (lane + 64 * t) * 16, 0, 0);
+    v[t] = __builtin_bit_cast(double2, w);
   }
 }
 

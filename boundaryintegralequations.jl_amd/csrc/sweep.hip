@@ -78,10 +78,10 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
                                             const SweepArgs& sw, int st, int wave, int lane,
                                             int64_t i, bool active, int k0, double* acc) {
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
-  const int f1 = sa.flush_begin[st + 1];
-  for (int f = sa.flush_begin[st] + wave; f < f1; f += W) {
-    const int node = __builtin_amdgcn_readfirstlane(sa.flush_node[f]);
-    const int ss = __builtin_amdgcn_readfirstlane(sa.flush_slot[f]);
+  const int f1 = sload(sa.flush_begin + st + 1);
+  for (int f = sload(sa.flush_begin + st) + wave; f < f1; f += W) {
+    const int node = sload(sa.flush_node + f);
+    const int ss = sload(sa.flush_slot + f);
 #pragma unroll
     for (int q2 = 0; q2 < KB; ++q2) {
       double* p = acc + ((size_t)(ss & 0xffff) * KB + q2) * 256 + lane;
@@ -154,8 +154,8 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const bool rec = sw.block_uniform[kb] != 0;
   const double dk = KB > 1 ? kv[1] - kv[0] : 0.0;
   double csum = 0.0;
-  const int st0 = sa.seg_step_begin[seg], st1 = sa.seg_step_begin[seg + 1];
-  int e = (st0 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[(size_t)st0 * W + wave]) : -1;
+  const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
+  int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * W + wave) : -1;
   int c = 0;
   double4 A0 = make_double4(0, 0, 0, 0), B0 = A0;
   if (e >= 0) {
@@ -169,7 +169,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   for (int st = st0; st < st1; ++st) {
     const size_t nidx = (size_t)(st + 1) * W + wave;
     const int e_nxt =
-        (st + 1 < st1) ? __builtin_amdgcn_readfirstlane(sa.step_elems[nidx]) : -1;
+        (st + 1 < st1) ? sload(sa.step_elems + nidx) : -1;
     if (e >= 0) {
       const bool mid = NP == 6 && c >= 3;
       if (active && !mid) {
@@ -196,7 +196,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         for (int s = 0; s < NP; ++s)
 #pragma unroll
           for (int q2 = 0; q2 < KB; ++q2) {
-            double* p = acc + ((size_t)so[s] * KB + q2) * 256 + lane;
+            double* p = acc + ((size_t)sload(so + s) * KB + q2) * 256 + lane;
             atomicAdd(p, fr[q2][s]);
             atomicAdd(p + 64, fi[q2][s]);
             atomicAdd(p + 128, gr[q2][s]);
