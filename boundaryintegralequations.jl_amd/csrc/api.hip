@@ -104,11 +104,21 @@ void ensure_rules(bie_dmesh* dm, int n, int m, hipStream_t s) {
   dm->cn = dm->cm = -1;
   dm->geom_valid = false;
   dm->d_pts = dalloc<double>(rs.pts.size());
-  dm->d_phi = dalloc<double>(rs.phi.size());
+  // basis table, padded to phi_pad doubles, then the trig table of the slot kernels
+  const int nphi = rs.R * rs.Qmax * dm->pk;
+  BIE_REQUIRE((size_t)nphi == rs.phi.size(), BIE_EINVAL, "internal: basis table size mismatch");
+  std::vector<double> phi_trig((size_t)phi_pad(nphi) + kTrigDoubles, 0.0);
+  std::copy(rs.phi.begin(), rs.phi.end(), phi_trig.begin());
+  for (int j = 0; j < kTrigN; ++j) {  // (cos, sin)(j pi / 64), rounded from long double
+    const long double th = (long double)j * (3.14159265358979323846264338327950288L / 64);
+    phi_trig[phi_pad(nphi) + 2 * j] = (double)cosl(th);
+    phi_trig[phi_pad(nphi) + 2 * j + 1] = (double)sinl(th);
+  }
+  dm->d_phi = dalloc<double>(phi_trig.size());
   dm->d_elem = dalloc<double>((size_t)std::max<int64_t>(dm->E, 1) * rs.Pe * 8);
   BIE_HIP_CHECK(hipMemcpyAsync(dm->d_pts, rs.pts.data(), rs.pts.size() * sizeof(double),
                                hipMemcpyHostToDevice, s));
-  BIE_HIP_CHECK(hipMemcpyAsync(dm->d_phi, rs.phi.data(), rs.phi.size() * sizeof(double),
+  BIE_HIP_CHECK(hipMemcpyAsync(dm->d_phi, phi_trig.data(), phi_trig.size() * sizeof(double),
                                hipMemcpyHostToDevice, s));
   BIE_HIP_CHECK(hipStreamSynchronize(s));
   dm->rs = std::move(rs);
@@ -151,10 +161,10 @@ bool use_atomic_kernel() {
 }
 
 // LDS slots per workgroup of the single-k slot kernel: as many as fit beside the basis
-// table, the staged table rows and the mid-edge queue (one workgroup per CU);
+// and trig tables, the staged table rows and the mid-edge queue (one workgroup per CU);
 // BIE_AMD_SLOTS=<n> asks for fewer.
 int plain_slot_count(int nphi, int Qc) {
-  int S = fitting_slots(phi_pad(nphi) + stage_doubles(Qc), 1);
+  int S = fitting_slots(phi_pad(nphi) + kTrigDoubles + stage_doubles(Qc), 1);
   if (const char* v = std::getenv("BIE_AMD_SLOTS")) {
     const int x = std::atoi(v);
     if (x >= 8 && x < S) S = x;
@@ -536,7 +546,7 @@ void batch_configure(bie_batch* b, hipStream_t s) {
   bie_dmesh* d0 = b->meshes[0];
   const int S = b->slots ? b->slots
                          : fitting_slots(phi_pad(d0->rs.R * d0->rs.Qmax * d0->pk) +
-                                             stage_doubles(d0->rs.len[0]),
+                                             kTrigDoubles + stage_doubles(d0->rs.len[0]),
                                          1);
   bool current = b->configured_key == key && b->d_tiles != nullptr && b->slots_used == S;
   // a mesh's batch schedule may have been rebuilt by another batch that shares it

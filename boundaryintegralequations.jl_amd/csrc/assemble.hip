@@ -95,12 +95,14 @@ k_select(const double* __restrict__ targets, int64_t M, const double* __restrict
 // ph[q * NP + s]: accumulates the F and G contributions of the NP element nodes and the
 // C integrand.  Single-layer g = e^{ikr}/(4 pi r), double-layer
 // f = e^{ikr}(ikr - 1)(n.d)/(4 pi r^3) and k0 = -(n.d)/(4 pi r^3) (kernels.jl:13-55) share
-// one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].
-template <int NP, bool CPLX>
+// one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].  With TAB, e^{ikr}
+// comes from the LDS trig table trig (sincos_tab), else from sincos_f64.
+template <int NP, bool CPLX, bool TAB = false>
 __device__ __forceinline__ void point_kernel(double x0, double x1, double x2, const double4 A,
                                              const double4 B, const double* __restrict__ p,
                                              double kr, double ki, double* fr, double* fi,
-                                             double* gr, double* gi, double& csum) {
+                                             double* gr, double* gi, double& csum,
+                                             const double2* trig = nullptr) {
   const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
   const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
   const double rinv = rsqrt_nr(r2);
@@ -110,7 +112,10 @@ __device__ __forceinline__ void point_kernel(double x0, double x1, double x2, co
   const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
   csum -= ak;
   double sn, cs;
-  sincos_f64(kr * r, &sn, &cs);
+  if (TAB)
+    sincos_tab(kr * r, trig, &sn, &cs);
+  else
+    sincos_f64(kr * r, &sn, &cs);
   double Fr, Fi;
   if (CPLX) {
     const double ex = exp(-ki * r);
@@ -164,39 +169,45 @@ __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
 
 // As pair_quadrature, with the first two table rows (points 0 and 1, or 0 twice when
 // Q == 1) already loaded by the caller, so their latency overlaps earlier work.
-template <int NP, bool CPLX>
+template <int NP, bool CPLX, bool TAB = false>
 __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double x2,
                                                     const double* __restrict__ tab,
                                                     const double* __restrict__ ph, int Q,
                                                     double kr, double ki, double4 A, double4 B,
                                                     double4 A2, double4 B2, double* fr,
                                                     double* fi, double* gr, double* gi,
-                                                    double& csum) {
+                                                    double& csum, const double2* trig = nullptr) {
   int q = 0;
   for (; q + 1 < Q; q += 2) {
-    point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum,
+                                trig);
     const int q2 = (q + 2 < Q) ? q + 2 : q + 1;
     A = *reinterpret_cast<const double4*>(tab + 8 * q2);
     B = *reinterpret_cast<const double4*>(tab + 8 * q2 + 4);
-    point_kernel<NP, CPLX>(x0, x1, x2, A2, B2, ph + (q + 1) * NP, kr, ki, fr, fi, gr, gi, csum);
+    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A2, B2, ph + (q + 1) * NP, kr, ki, fr, fi, gr, gi,
+                                csum, trig);
     const int q3 = (q + 3 < Q) ? q + 3 : q + 1;
     A2 = *reinterpret_cast<const double4*>(tab + 8 * q3);
     B2 = *reinterpret_cast<const double4*>(tab + 8 * q3 + 4);
   }
-  if (q < Q) point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+  if (q < Q)
+    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum,
+                                trig);
 }
 
 // As pair_quadrature over table rows that this wave staged in LDS (rows[8 q .. 8 q + 8)).
-template <int NP, bool CPLX>
+template <int NP, bool CPLX, bool TAB = false>
 __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double x2,
                                                     const double* rows, const double* ph, int Q,
                                                     double kr, double ki, double* fr, double* fi,
-                                                    double* gr, double* gi, double& csum) {
+                                                    double* gr, double* gi, double& csum,
+                                                    const double2* trig = nullptr) {
 #pragma unroll 2
   for (int q = 0; q < Q; ++q) {
     const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
     const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
-    point_kernel<NP, CPLX>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum);
+    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum,
+                                trig);
   }
 }
 
@@ -313,7 +324,8 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
 template <int NP, bool CPLX, int W, bool STAGED>
 __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
                                           int seg, double* acc, const double* phi,
-                                          double* s_rows, int2* s_q, unsigned* s_qn) {
+                                          const double2* trig, double* s_rows, int2* s_q,
+                                          unsigned* s_qn) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = (int64_t)rb * 64 + lane;
@@ -354,12 +366,13 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
 #pragma unroll
         for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
         if (STAGED)
-          pair_quadrature_lds<NP, CPLX>(x0, x1, x2, my_rows + (size_t)c * Qc * 8,
-                                        phi + c * a.Qmax * NP, Qc, kr, ki, fr, fi, gr, gi, csum);
+          pair_quadrature_lds<NP, CPLX, true>(x0, x1, x2, my_rows + (size_t)c * Qc * 8,
+                                              phi + c * a.Qmax * NP, Qc, kr, ki, fr, fi, gr, gi,
+                                              csum, trig);
         else
-          pair_quadrature_pre<NP, CPLX>(x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8,
-                                        phi + c * a.Qmax * NP, a.len[c], kr, ki, A0, B0, A1, B1,
-                                        fr, fi, gr, gi, csum);
+          pair_quadrature_pre<NP, CPLX, true>(
+              x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, phi + c * a.Qmax * NP,
+              a.len[c], kr, ki, A0, B0, A1, B1, fr, fi, gr, gi, csum, trig);
         const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
 #pragma unroll
         for (int s = 0; s < NP; ++s) {
@@ -433,7 +446,8 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   double* s_phi = lds + (size_t)sa.S * 256;
   const int nphi = a.R * a.Qmax * NP;
   const int nstage = stage_doubles(a.len[0]);
-  double* s_rows = s_phi + phi_pad(nphi);
+  const double2* s_trig = reinterpret_cast<const double2*>(s_phi + phi_pad(nphi));
+  double* s_rows = s_phi + phi_pad(nphi) + kTrigDoubles;
   int2* s_q = reinterpret_cast<int2*>(s_rows + nstage);
   unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
   // XCD-aware tile order: consecutive block ids go to different XCDs, so give each XCD
@@ -450,14 +464,15 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
     rb = (int)(tile / sa.n_segments);
     seg = (int)(tile - (int64_t)rb * sa.n_segments);
   }
-  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
+  // basis table and trig table: one buffer of phi_pad(nphi) + kTrigDoubles doubles
+  for (int t = threadIdx.x; t < phi_pad(nphi) + kTrigDoubles; t += blockDim.x) s_phi[t] = a.phi[t];
   for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
   lds_barrier();
   if (nstage)
-    slot_tile<NP, CPLX, W, true>(a, sa, rb, seg, acc, s_phi, s_rows, s_q, s_qn);
+    slot_tile<NP, CPLX, W, true>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
   else
-    slot_tile<NP, CPLX, W, false>(a, sa, rb, seg, acc, s_phi, s_rows, s_q, s_qn);
+    slot_tile<NP, CPLX, W, false>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
 }
 
 // Uneven batch: many meshes (same element kinds and rule set) in one persistent launch.
@@ -475,11 +490,12 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
   extern __shared__ double lds[];
   double* acc = lds;
   double* s_phi = lds + (size_t)S * 256;
-  double* s_rows = s_phi + phi_pad(nphi);
+  const double2* s_trig = reinterpret_cast<const double2*>(s_phi + phi_pad(nphi));
+  double* s_rows = s_phi + phi_pad(nphi) + kTrigDoubles;
   int2* s_q = reinterpret_cast<int2*>(s_rows + nstage);
   unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
   int* s_tile = reinterpret_cast<int*>(s_qn + 1);
-  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = phi[t];
+  for (int t = threadIdx.x; t < phi_pad(nphi) + kTrigDoubles; t += blockDim.x) s_phi[t] = phi[t];
   for (int t = threadIdx.x; t < S * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
   const int x0 = blockIdx.x & (kXcds - 1);
@@ -501,11 +517,11 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
       const int rb = __builtin_amdgcn_readfirstlane(td.y);
       const int seg = __builtin_amdgcn_readfirstlane(td.z);
       if (nstage)
-        slot_tile<NP, CPLX, W, true>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_rows, s_q,
-                                     s_qn);
+        slot_tile<NP, CPLX, W, true>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_trig,
+                                     s_rows, s_q, s_qn);
       else
-        slot_tile<NP, CPLX, W, false>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_rows, s_q,
-                                      s_qn);
+        slot_tile<NP, CPLX, W, false>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_trig,
+                                      s_rows, s_q, s_qn);
     }
   }
 }
@@ -741,7 +757,7 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
   const dim3 grid((unsigned)(per_xcd * 8));
   BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
   const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + phi_pad(a.R * a.Qmax * a.NP) +
-                                         stage_doubles(a.len[0])) +
+                                         kTrigDoubles + stage_doubles(a.len[0])) +
                        sizeof(int2) * kMidLds + 16;
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
   const bool cplx = a.ki != 0.0;
@@ -784,8 +800,8 @@ void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_cols, hi
 }
 
 size_t batch_shmem(int S, int nphi, int nstage) {
-  return sizeof(double) * ((size_t)S * 256 + phi_pad(nphi) + nstage) + sizeof(int2) * kMidLds +
-         16;
+  return sizeof(double) * ((size_t)S * 256 + phi_pad(nphi) + kTrigDoubles + nstage) +
+         sizeof(int2) * kMidLds + 16;
 }
 
 void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,

@@ -87,9 +87,16 @@ constexpr int kLdsBytes = 160 * 1024;  // LDS per CU (one slot-kernel workgroup 
 
 constexpr int kStageMaxQ = 16;  // vertex rules up to this length are staged in LDS per wave
 
+// Table of (cos, sin)(j pi / 64), j < kTrigN, for the table-driven sincos of the slot
+// kernels.  It follows the basis table in the same device buffer (at phi_pad(nphi)
+// doubles) and in LDS.
+constexpr int kTrigN = 128;
+constexpr int kTrigDoubles = 2 * kTrigN;
+
 // LDS layout of the slot kernels, in doubles: S slots of 4 x 64 accumulators, the basis
-// table (padded to 4 doubles), then the per-wave staging of the three vertex rules' table
-// rows (W x 3 Qc points x 8 doubles, when Qc <= kStageMaxQ), then the mid-edge queue.
+// table (padded to 4 doubles), the trig table (kTrigDoubles), then the per-wave staging of
+// the three vertex rules' table rows (W x 3 Qc points x 8 doubles, when Qc <=
+// kStageMaxQ), then the mid-edge queue.
 __host__ __device__ inline int phi_pad(int nphi) { return (nphi + 3) & ~3; }
 __host__ __device__ inline int stage_doubles(int Qc) {
   return Qc <= kStageMaxQ ? kSlotWaves * 3 * Qc * 8 : 0;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bie_common.hpp"
+
 namespace bie {
 
 // sin and cos of x for |x| < 2^26 by Cody-Waite reduction modulo pi/2 with two fused
@@ -43,6 +45,36 @@ __device__ __forceinline__ void sincos_f64(double x, double* s, double* c) {
   if ((q + 1) & 2) cc = -cc;
   *s = ss;
   *c = cc;
+}
+
+// sin and cos of x for |x| < 2^26 from the table tab[j] = (cos, sin)(j pi / 64), j < 128,
+// in LDS: x = n pi / 64 + d with n = rint(64 x / pi) and the two-term reduction above
+// (|d| <= pi / 128), j = n mod 128, then sin x = s_j cos d + c_j sin d and
+// cos x = c_j cos d - s_j sin d with the Taylor polynomials sin d = d - d^3/3! + d^5/5! -
+// d^7/7! and cos d = 1 - d^2/2! + d^4/4! - d^6/6! (truncation below 9e-21 and 3.3e-18 on
+// |d| <= pi / 128).  Absolute error ~2e-16 with the table entries correctly rounded; no
+// quadrant selection and three polynomial constants per function.  A NaN x reads entry 0
+// and returns NaN; larger arguments take the library sincos.
+__device__ __forceinline__ void sincos_tab(double x, const double2* __restrict__ tab,
+                                           double* s, double* c) {
+  if (fabs(x) >= 67108864.0) {  // 2^26
+    sincos(x, s, c);
+    return;
+  }
+  constexpr double kInvStep = 20.371832715762602978;   // 64 / pi
+  constexpr double kStepHi = 1.5707963267948965580 / 32;   // pi/2 rounded, / 32 (exact)
+  constexpr double kStepLo = 6.1232339957367658e-17 / 32;  // its remainder, / 32 (exact)
+  const double n = rint(x * kInvStep);
+  double d = fma(-n, kStepHi, x);
+  d = fma(-n, kStepLo, d);
+  const double2 t = tab[(int)n & (kTrigN - 1)];
+  const double d2 = d * d;
+  const double sp = fma(d2, fma(d2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0);
+  const double sd = fma(d * d2, sp, d);
+  const double cp = fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5);
+  const double cd = fma(d2, cp, 1.0);
+  *s = fma(t.y, cd, t.x * sd);
+  *c = fma(t.x, cd, -(t.y * sd));
 }
 
 // 1/sqrt(x) from the hardware estimate refined by two Newton steps.
