@@ -498,7 +498,8 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   BIE_REQUIRE(sizeof(double) * (size_t)dm->rs.R * dm->rs.Qmax * dm->pk <= 24 * 1024,
               BIE_EUNSUPPORTED, "sweep supports quadrature orders up to n*m' = 36");
   const bie_dmesh::Sched& sc =
-      ensure_schedule(dm, 1, fitting_slots(dm->rs.R * dm->rs.Qmax * dm->pk, KB), 1, s);
+      ensure_schedule(dm, 1, fitting_slots(sweep_fixed_doubles(dm->rs.R * dm->rs.Qmax * dm->pk), KB),
+                      1, s);
   if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
   if (dm->pk == 6) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
   const int nkb = (nk + KB - 1) / KB;

@@ -98,6 +98,10 @@ constexpr int kTrigDoubles = 2 * kTrigN;
 // the three vertex rules' table rows (W x 3 Qc points x 8 doubles, when Qc <=
 // kStageMaxQ), then the mid-edge queue.
 __host__ __device__ inline int phi_pad(int nphi) { return (nphi + 3) & ~3; }
+// The sweep kernel's fixed LDS (doubles): basis and trig tables, no staging.
+__host__ __device__ inline int sweep_fixed_doubles(int nphi) {
+  return phi_pad(nphi) + kTrigDoubles;
+}
 __host__ __device__ inline int stage_doubles(int Qc) {
   return Qc <= kStageMaxQ ? kSlotWaves * 3 * Qc * 8 : 0;
 }

@@ -25,13 +25,14 @@
 namespace bie {
 namespace {
 
-template <int NP, int KB>
+// With TAB, e^{ikr} comes from the LDS trig table trig (sincos_tab), else from sincos_f64.
+template <int NP, int KB, bool TAB>
 __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, const double4 A,
                                             const double4 B, const double* __restrict__ p,
                                             const double* kv, double dk, bool rec,
                                             double (&fr)[KB][NP], double (&fi)[KB][NP],
                                             double (&gr)[KB][NP], double (&gi)[KB][NP],
-                                            double& csum) {
+                                            double& csum, const double2* trig) {
   const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
   const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
   const double rinv = rsqrt_nr(r2);
@@ -41,8 +42,13 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
   const double ak = nd * rinv * rinv * g;
   csum -= ak;
   double sn, cs, zs = 0.0, zc = 1.0;
-  sincos_f64(kv[0] * r, &sn, &cs);
-  if (rec) sincos_f64(dk * r, &zs, &zc);
+  if (TAB) {
+    sincos_tab(kv[0] * r, trig, &sn, &cs);
+    if (rec) sincos_tab(dk * r, trig, &zs, &zc);
+  } else {
+    sincos_f64(kv[0] * r, &sn, &cs);
+    if (rec) sincos_f64(dk * r, &zs, &zc);
+  }
   double ps[NP];
 #pragma unroll
   for (int s = 0; s < NP; ++s) ps[s] = p[s];
@@ -54,7 +60,10 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
         sn = fma(cs, zs, sn * zc);
         cs = c2;
       } else {
-        sincos_f64(kv[b] * r, &sn, &cs);
+        if (TAB)
+          sincos_tab(kv[b] * r, trig, &sn, &cs);
+        else
+          sincos_f64(kv[b] * r, &sn, &cs);
       }
     }
     const double krr = kv[b] * r;
@@ -124,7 +133,8 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   double* acc = lds;  // [slot][b][4][64]
   const int nphi = a.R * a.Qmax * NP;
   double* s_phi = lds + (size_t)sa.S * KB * 256;
-  int2* s_q = reinterpret_cast<int2*>(s_phi + nphi);
+  const double2* s_trig = reinterpret_cast<const double2*>(s_phi + phi_pad(nphi));
+  int2* s_q = reinterpret_cast<int2*>(s_phi + phi_pad(nphi) + kTrigDoubles);
   unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
   const int b = blockIdx.x;
   const int64_t per_xcd = (sw.n_tiles + 7) / 8;
@@ -140,7 +150,8 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = (int64_t)rb * 64 + lane;
   const bool active = i < a.M;
-  for (int t = threadIdx.x; t < nphi; t += blockDim.x) s_phi[t] = a.phi[t];
+  // basis table and trig table: one buffer of phi_pad(nphi) + kTrigDoubles doubles
+  for (int t = threadIdx.x; t < phi_pad(nphi) + kTrigDoubles; t += blockDim.x) s_phi[t] = a.phi[t];
   for (int t = threadIdx.x; t < sa.S * KB * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
   lds_barrier();
@@ -183,12 +194,13 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         const int Q = a.len[c];
         double cs_local = 0.0;
         // point 0 was fetched before the barrier; later rows are loaded in turn
-        sweep_point<NP, KB>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi, cs_local);
+        sweep_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi, cs_local,
+                                  s_trig);
         for (int q = 1; q < Q; ++q) {
           const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
           const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-          sweep_point<NP, KB>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, fr, fi, gr, gi,
-                              cs_local);
+          sweep_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, fr, fi, gr, gi,
+                                    cs_local, s_trig);
         }
         csum += cs_local;
         const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
@@ -279,7 +291,8 @@ k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
     for (int q = 0; q < a.len[c]; ++q) {
       const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
       const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-      sweep_point<6, 1>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi, csum);
+      sweep_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi,
+                               csum, nullptr);
     }
     const int* cols = a.ptopo + (size_t)e * 6;
 #pragma unroll
@@ -357,11 +370,13 @@ void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t
 
 size_t sweep_shmem(int NP, int nphi, int S) {
   const int KB = sweep_kb(NP);
-  return sizeof(double) * ((size_t)S * KB * 256 + (size_t)nphi) + sizeof(int2) * kMidLds + 16;
+  return sizeof(double) * ((size_t)S * KB * 256 + sweep_fixed_doubles(nphi)) +
+         sizeof(int2) * kMidLds + 16;
 }
 
 void sweep_prepare_attributes(int NP, int nphi) {
-  const size_t shmem = sweep_shmem(NP, nphi, fitting_slots(nphi, sweep_kb(NP)));
+  const size_t shmem =
+      sweep_shmem(NP, nphi, fitting_slots(sweep_fixed_doubles(nphi), sweep_kb(NP)));
   constexpr int W = kSlotWaves;
   const void* kern = NP == 3 ? reinterpret_cast<const void*>(k_sweep_slots<3, 4, W>)
                              : reinterpret_cast<const void*>(k_sweep_slots<6, 2, W>);
@@ -377,14 +392,14 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
                   hipStream_t s) {
   if (a.M == 0 || a.E == 0 || sw.nk == 0) return;
   const int KB = sweep_kb(a.NP);
-  BIE_REQUIRE(sa.W == kSlotWaves && sa.S <= fitting_slots(a.R * a.Qmax * a.NP, KB), BIE_EINVAL,
-              "sweep schedule does not match the kernel");
+  const int nphi = a.R * a.Qmax * a.NP;
+  BIE_REQUIRE(sa.W == kSlotWaves && sa.S <= fitting_slots(sweep_fixed_doubles(nphi), KB),
+              BIE_EINVAL, "sweep schedule does not match the kernel");
   const int64_t per_xcd = (sw.n_tiles + 7) / 8;
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many sweep tiles");
-  const size_t shmem = sizeof(double) * ((size_t)sa.S * KB * 256 + (size_t)a.R * a.Qmax * a.NP) +
-                       sizeof(int2) * kMidLds + 16;
+  const size_t shmem = sweep_shmem(a.NP, nphi, sa.S);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "sweep kernel LDS request exceeds 160 KiB");
-  sweep_prepare_attributes(a.NP, a.R * a.Qmax * a.NP);
+  sweep_prepare_attributes(a.NP, nphi);
   constexpr int W = kSlotWaves;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * 8)), dim3(64 * W), shmem, s, a, sa, sw);
