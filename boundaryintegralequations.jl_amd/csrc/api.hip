@@ -545,10 +545,10 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
 void batch_configure(bie_batch* b, hipStream_t s) {
   const int key = b->elems_per_tile * 1000 + b->slots * 10 + b->order;
   bie_dmesh* d0 = b->meshes[0];
-  const int S = b->slots ? b->slots
-                         : fitting_slots(phi_pad(d0->rs.R * d0->rs.Qmax * d0->pk) +
-                                             kTrigDoubles + stage_doubles(d0->rs.len[0]),
-                                         1);
+  // slots = 32 or 64 asks for that many, capped at what fits beside the tables and staging
+  const int fit = fitting_slots(
+      phi_pad(d0->rs.R * d0->rs.Qmax * d0->pk) + kTrigDoubles + stage_doubles(d0->rs.len[0]), 1);
+  const int S = b->slots ? std::min(b->slots, fit) : fit;
   bool current = b->configured_key == key && b->d_tiles != nullptr && b->slots_used == S;
   // a mesh's batch schedule may have been rebuilt by another batch that shares it
   for (size_t ii = 0; current && ii < b->meshes.size(); ++ii) {

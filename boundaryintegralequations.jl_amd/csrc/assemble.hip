@@ -336,19 +336,24 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   double csum = 0.0;
   const int Qc = a.len[0];
   const int nchunk = 12 * Qc;
-  double* my_rows = STAGED ? s_rows + (size_t)wave * 24 * Qc : nullptr;
-  double2 stg[3];
+  double* my_rows = STAGED ? s_rows + (size_t)wave * stage_wave_doubles(Qc) : nullptr;
+  double* my_sel = STAGED ? my_rows + 24 * Qc : nullptr;  // the element's selection nodes
+  constexpr int nsel = NP * 2;                             // 16-byte chunks of them
+  double2 stg[3], stg_sel;
   const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
   int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * W + wave) : -1;
   int c = 0;
   double4 A0 = make_double4(0, 0, 0, 0), B0 = A0, A1 = A0, B1 = A0;
   if (e >= 0) {
-    c = select_rule<NP>(x0, x1, x2, sa.step_sel + ((size_t)st0 * W + wave) * NP * 4, a.R) - 1;
+    const double* sel = sa.step_sel + ((size_t)st0 * W + wave) * NP * 4;
     if (STAGED) {
-      stage_load(a.elem + (size_t)e * a.Pe * 8, nchunk, lane, stg);
-      stage_store(my_rows, nchunk, lane, stg);
-    } else if (!(NP == 6 && c >= 3)) {
-      load_rows01(a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, a.len[c], A0, B0, A1, B1);
+      stage_load(a.elem + (size_t)e * a.Pe * 8, nchunk, sel, nsel, lane, stg, stg_sel);
+      stage_store(my_rows, nchunk, my_sel, nsel, lane, stg, stg_sel);
+      c = select_rule<NP>(x0, x1, x2, my_sel, a.R) - 1;
+    } else {
+      c = select_rule<NP>(x0, x1, x2, sel, a.R) - 1;
+      if (!(NP == 6 && c >= 3))
+        load_rows01(a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, a.len[c], A0, B0, A1, B1);
     }
   }
   for (int st = st0; st < st1; ++st) {
@@ -358,7 +363,10 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     // unconditional (element 0 stands in at the segment's end): a path that skipped the
     // LDS store below would leave these loads outstanding at the loop head, where the
     // compiler would wait for them, and with them for the previous step's column stores
-    if (STAGED) stage_load(a.elem + (size_t)max(e_nxt, 0) * a.Pe * 8, nchunk, lane, stg);
+    if (STAGED)
+      stage_load(a.elem + (size_t)max(e_nxt, 0) * a.Pe * 8, nchunk,
+                 sa.step_sel + (e_nxt >= 0 ? nidx : 0) * NP * 4, e_nxt >= 0 ? nsel : 0, lane,
+                 stg, stg_sel);
     if (e >= 0) {
       const bool mid = NP == 6 && c >= 3;
       if (active && !mid) {
@@ -406,13 +414,15 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     }
     // next element: rule selection, and its rows (LDS: after this wave's reads above)
     int c_nxt = 0;
-    if (e_nxt >= 0) {
+    if (STAGED) {  // only this wave reads my_rows and my_sel, and its reads above are done
+      stage_store(my_rows, nchunk, my_sel, nsel, lane, stg, stg_sel);
+      if (e_nxt >= 0) c_nxt = select_rule<NP>(x0, x1, x2, my_sel, a.R) - 1;
+    } else if (e_nxt >= 0) {
       c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
-      if (!STAGED && !(NP == 6 && c_nxt >= 3))
+      if (!(NP == 6 && c_nxt >= 3))
         load_rows01(a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8, a.len[c_nxt], A0, B0,
                     A1, B1);
     }
-    if (STAGED) stage_store(my_rows, nchunk, lane, stg);  // only this wave reads my_rows
     // the previous step's finished nodes: their slots were complete at the last barrier
     if (st > st0) flush_step<W>(a, sa, st - 1, wave, lane, i, active, acc);
     lds_barrier();

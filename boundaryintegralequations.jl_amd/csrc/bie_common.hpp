@@ -94,16 +94,18 @@ constexpr int kTrigN = 128;
 constexpr int kTrigDoubles = 2 * kTrigN;
 
 // LDS layout of the slot kernels, in doubles: S slots of 4 x 64 accumulators, the basis
-// table (padded to 4 doubles), the trig table (kTrigDoubles), then the per-wave staging of
-// the three vertex rules' table rows (W x 3 Qc points x 8 doubles, when Qc <=
-// kStageMaxQ), then the mid-edge queue.
+// table (padded to 4 doubles), the trig table (kTrigDoubles), then the per-wave staging
+// (when Qc <= kStageMaxQ) of the element's three vertex rules' table rows (3 Qc points x
+// 8 doubles) and its selection nodes (kStageSel doubles), then the mid-edge queue.
+constexpr int kStageSel = 24;  // NP x 4 doubles of selection-node data, NP <= 6
 __host__ __device__ inline int phi_pad(int nphi) { return (nphi + 3) & ~3; }
+__host__ __device__ inline int stage_wave_doubles(int Qc) { return 3 * Qc * 8 + kStageSel; }
 // The sweep kernel's fixed LDS (doubles): basis and trig tables, no staging.
 __host__ __device__ inline int sweep_fixed_doubles(int nphi) {
   return phi_pad(nphi) + kTrigDoubles;
 }
 __host__ __device__ inline int stage_doubles(int Qc) {
-  return Qc <= kStageMaxQ ? kSlotWaves * 3 * Qc * 8 : 0;
+  return Qc <= kStageMaxQ ? kSlotWaves * stage_wave_doubles(Qc) : 0;
 }
 
 // LDS slots that fit beside `fixed_doubles` (padded basis table + staging) and the

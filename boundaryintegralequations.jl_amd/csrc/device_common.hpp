@@ -104,31 +104,39 @@ __device__ __forceinline__ void load_rows01(const double* __restrict__ tab, int 
 }
 
 // Per-wave staging of an element's three vertex-rule tables (3 Qc rows of 64 bytes,
-// nchunk = 12 Qc chunks of 16 bytes, at most 3 per lane): global -> registers early,
-// registers -> the wave's LDS rows later, so no global load is pending in the point loop.
-// The loads are buffer loads whose record count is the element's nchunk chunks, so lanes
-// past the end read zeros from the bounds check: no exec mask and no register zeroing
-// around them, which would otherwise make the compiler wait for every older vector-memory
-// operation (the previous step's column stores included) before issuing them.
+// nchunk = 12 Qc chunks of 16 bytes, at most 3 per lane) and of its nsel chunks of
+// selection-node data (one per lane, nsel <= 12): global -> registers early, registers ->
+// the wave's LDS rows later, so no global load is pending in the point loop or in the
+// rule selection.  The loads are buffer loads whose record counts are nchunk and nsel
+// chunks, so lanes past the end read zeros from the bounds check (nsel = 0 reads nothing):
+// no exec mask and no register zeroing around them, which would otherwise make the
+// compiler wait for every older vector-memory operation (the previous step's column
+// stores included) before issuing them.
 __device__ __forceinline__ void stage_load(const double* __restrict__ elem_e, int nchunk,
-                                           int lane, double2 (&v)[3]) {
+                                           const double* __restrict__ sel, int nsel, int lane,
+                                           double2 (&v)[3], double2& vs) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<double*>(elem_e), (short)0, nchunk * 16, 0x00020000);
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
-    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
     const u4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, (lane + 64 * t) * 16, 0, 0);
     v[t] = __builtin_bit_cast(double2, w);
   }
+  const __amdgpu_buffer_rsrc_t rsel = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double*>(sel), (short)0, nsel * 16, 0x00020000);
+  vs = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rsel, lane * 16, 0, 0));
 }
 
-__device__ __forceinline__ void stage_store(double* rows, int nchunk, int lane,
-                                            const double2 (&v)[3]) {
+__device__ __forceinline__ void stage_store(double* rows, int nchunk, double* sel_rows,
+                                            int nsel, int lane, const double2 (&v)[3],
+                                            const double2& vs) {
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
     const int j = lane + 64 * t;
     if (j < nchunk) reinterpret_cast<double2*>(rows)[j] = v[t];
   }
+  if (lane < nsel) reinterpret_cast<double2*>(sel_rows)[lane] = vs;
 }
 
 }  // namespace

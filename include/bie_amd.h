@@ -136,8 +136,9 @@ int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_
                      bie_batch** out);
 int bie_batch_destroy(bie_batch* b);
 /* Schedule of the persistent launch: target elements per segment (>= 8), LDS slots per
- * workgroup (32 or 64: two or one resident workgroups per CU) and tile order (0: longest
- * tiles first, 1: interleaved across instances).  Takes effect at the next call. */
+ * workgroup (32 or 64, capped at what fits beside the basis tables and the staged rows;
+ * 0: as many as fit) and tile order (0: longest tiles first, 1: interleaved across
+ * instances).  Takes effect at the next call. */
 int bie_batch_configure(bie_batch* b, int32_t elems_per_tile, int32_t slots, int32_t order);
 int bie_batch_info(bie_batch* b, int32_t* elems_per_tile, int32_t* slots, int32_t* order,
                    int64_t* n_tiles);
