@@ -12,11 +12,14 @@
 //   D[i][j] in lane j + 16 (i % 4), element i / 4.
 // A complex product uses four real MFMAs (Re += Ar Xr + (-Ai) Xi, Im += Ar Xi + Ai Xr).
 // Each wave owns 32 rows x 32 columns (2 x 2 tiles); a workgroup of 4 waves owns 128
-// rows; K is split over blockIdx.y and the partial sums are added into a zeroed Y with
-// FP64 atomics.  X is first repacked k-major (K x ncp, ncp = nrhs rounded up to 32) so
-// the B operand is a coalesced 16-byte load per lane.
+// rows; K is split over blockIdx.y (as many splits as fill one round of resident
+// workgroups) and the partial sums are added into a zeroed Y with FP64 atomics.  X is
+// first repacked k-major (K x ncp, ncp = nrhs rounded up to 32) so the B operand is a
+// coalesced 16-byte load per lane.  Each wave keeps kRing k steps of operands in flight
+// (bounds-checked buffer loads, so the compiler counts them instead of draining them).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "bie_common.hpp"
@@ -25,6 +28,8 @@ namespace bie {
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kRing = 4;  // k steps (of 4) of operands in flight per wave
 
 __device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -71,40 +76,59 @@ k_zgemm_mfma(const double2* __restrict__ A, int64_t lda, const double2* __restri
       re[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
       im[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
-  const int64_t row0 = r0 + li, row1 = r0 + 16 + li;
-  const bool ok0 = row0 < M, ok1 = row1 < M;
+  // Operand loads are bounds-checked buffer loads over this split's k range [kb, ke): k >= ke
+  // reads zeros with no lane mask or branch, so the compiler can count the loads in flight
+  // instead of waiting for all of them.  Rows past M read in-range values (columns kb..ke-1
+  // of A) or zeros, and their results are never stored (each output row depends on its own
+  // A row only).  The host keeps (klen + 64) 16 max(lda, ncp) below 2^31.
+  const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double2*>(A + kb * lda), (short)0, (int)((ke - kb) * lda * 16), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double2*>(Xp + kb * ncp), (short)0, (int)((ke - kb) * ncp * 16), 0x00020000);
+  const int64_t row0 = r0 + li;
+  auto ld = [](__amdgpu_buffer_rsrc_t rs, int64_t idx) {
+    return __builtin_bit_cast(double2,
+                              __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(idx * 16), 0, 0));
+  };
   auto loadA = [&](int64_t k, double2& a0, double2& a1) {
-    const bool kk = k < ke;
-    a0 = (kk && ok0) ? A[row0 + k * lda] : make_double2(0.0, 0.0);
-    a1 = (kk && ok1) ? A[row1 + k * lda] : make_double2(0.0, 0.0);
+    a0 = ld(rsA, (k - kb) * lda + row0);
+    a1 = ld(rsA, (k - kb) * lda + row0 + 16);
   };
   auto loadB = [&](int64_t k, double2& b0, double2& b1) {
-    const bool kk = k < ke;
-    b0 = kk ? Xp[k * ncp + c0 + li] : make_double2(0.0, 0.0);
-    b1 = kk ? Xp[k * ncp + c0 + 16 + li] : make_double2(0.0, 0.0);
+    b0 = ld(rsB, (k - kb) * ncp + c0 + li);
+    b1 = ld(rsB, (k - kb) * ncp + c0 + 16 + li);
   };
-  double2 a0, a1, b0, b1;
-  loadA(kb + lk, a0, a1);
-  loadB(kb + lk, b0, b1);
-  for (int64_t k0 = kb; k0 < ke; k0 += 4) {
-    double2 na0, na1, nb0, nb1;
-    loadA(k0 + 4 + lk, na0, na1);
-    loadB(k0 + 4 + lk, nb0, nb1);
-    const double ar[2] = {a0.x, a1.x}, ai[2] = {a0.y, a1.y};
-    const double br[2] = {b0.x, b1.x}, bi[2] = {b0.y, b1.y};
+  // ring of kRing k steps of A and X operands in flight: HBM latency is several k steps of
+  // MFMA work, so one step of prefetch left the matrix pipe waiting
+  double2 ra[kRing][2], rb[kRing][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+  for (int p = 0; p < kRing; ++p) {
+    loadA(kb + 4 * p + lk, ra[p][0], ra[p][1]);
+    loadB(kb + 4 * p + lk, rb[p][0], rb[p][1]);
+  }
+  for (int64_t k0 = kb; k0 < ke; k0 += 4 * kRing) {
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        re[a][b] = mfma4(ar[a], br[b], re[a][b]);
-        re[a][b] = mfma4(-ai[a], bi[b], re[a][b]);
-        im[a][b] = mfma4(ar[a], bi[b], im[a][b]);
-        im[a][b] = mfma4(ai[a], br[b], im[a][b]);
-      }
-    a0 = na0;
-    a1 = na1;
-    b0 = nb0;
-    b1 = nb1;
+    for (int p = 0; p < kRing; ++p) {
+      const double ar[2] = {ra[p][0].x, ra[p][1].x}, ai[2] = {ra[p][0].y, ra[p][1].y};
+      const double br[2] = {rb[p][0].x, rb[p][1].x}, bi[2] = {rb[p][0].y, rb[p][1].y};
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          re[a][b] = mfma4(ar[a], br[b], re[a][b]);
+          im[a][b] = mfma4(ar[a], bi[b], im[a][b]);
+        }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          re[a][b] = mfma4(-ai[a], bi[b], re[a][b]);
+          im[a][b] = mfma4(ai[a], br[b], im[a][b]);
+        }
+      const int64_t kn = k0 + 4 * (p + kRing) + lk;  // refill this ring entry
+      loadA(kn, ra[p][0], ra[p][1]);
+      loadB(kn, rb[p][0], rb[p][1]);
+    }
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -122,6 +146,21 @@ k_zgemm_mfma(const double2* __restrict__ A, int64_t lda, const double2* __restri
         }
       }
     }
+}
+
+// Workgroups of k_zgemm_mfma resident at once on a device: CUs x occupancy (cached).
+int resident_workgroups(int dev) {
+  static std::mutex mu;
+  static int cached[16] = {};
+  std::lock_guard<std::mutex> lk(mu);
+  if (cached[dev] == 0) {
+    int cus = 0, per_cu = 0;
+    BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BIE_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_zgemm_mfma), 256, 0));
+    cached[dev] = std::max(1, cus * std::max(1, per_cu));
+  }
+  return cached[dev];
 }
 
 struct Workspace {
@@ -181,14 +220,23 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
   }
   const int64_t rowblocks = (M + 127) / 128;
   const int colblocks = ncp / 32;
-  int64_t splits = (1024 + rowblocks * colblocks - 1) / (rowblocks * colblocks);
+  // K splits: one round of resident workgroups, each with as long a K range as fits (more
+  // splits than fill one round add a mostly empty round, or more partial-sum atomics)
+  int64_t splits = resident_workgroups(dev) / (rowblocks * colblocks);
   const int64_t kmax_splits = (K + 255) / 256;
   if (splits > kmax_splits) splits = kmax_splits;
   if (splits < 1) splits = 1;
   if (splits > 65535) splits = 65535;
+  // 32-bit buffer offsets: (klen + 64) 16 max(lda, ncp) < 2^31
+  const int64_t wide = std::max<int64_t>(lda, ncp) * 16;
+  BIE_REQUIRE(wide * (64 + 4 * kRing) < (1LL << 31), BIE_EINVAL,
+              "apply: leading dimension too large (lda <= 1.6e6)");
+  const int64_t klen_max = ((1LL << 31) - 1) / wide - 64;
   int64_t klen = (K + splits - 1) / splits;
-  klen = (klen + 3) / 4 * 4;
+  klen = (klen + 4 * kRing - 1) / (4 * kRing) * (4 * kRing);  // whole rings but in the last split
+  if (klen > klen_max) klen = klen_max / (4 * kRing) * (4 * kRing);
   splits = (K + klen - 1) / klen;
+  BIE_REQUIRE(splits <= 65535, BIE_EINVAL, "apply: too many K splits");
   BIE_REQUIRE(rowblocks < 2147483647LL && colblocks <= 65535, BIE_EINVAL, "apply too large");
   hipLaunchKernelGGL(k_zgemm_mfma, dim3((unsigned)rowblocks, (unsigned)splits, colblocks),
                      dim3(256), 0, s, reinterpret_cast<const double2*>(dA), lda, Xp, ncp,
