@@ -38,9 +38,10 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
   const double rinv = rsqrt_nr(r2);
   const double r = r2 * rinv;
   const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
-  const double g = B.z * rinv;
-  const double ak = nd * rinv * rinv * g;
-  csum -= ak;
+  const double g = B.z * rinv;          // J w / (4 pi r)
+  const double t = nd * rinv * rinv;     // (n.d) / r^2: f = t (ikr - 1) g e^{ikr}
+  csum -= t * g;
+  // the recurrence runs on g e^{ik_b r} (= G_b): one complex product per further k
   double sn, cs, zs = 0.0, zc = 1.0;
   if (TAB) {
     sincos_tab(kv[0] * r, trig, &sn, &cs);
@@ -49,6 +50,7 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
     sincos_f64(kv[0] * r, &sn, &cs);
     if (rec) sincos_f64(dk * r, &zs, &zc);
   }
+  double Gr = g * cs, Gi = g * sn;
   double ps[NP];
 #pragma unroll
   for (int s = 0; s < NP; ++s) ps[s] = p[s];
@@ -56,20 +58,21 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
   for (int b = 0; b < KB; ++b) {
     if (b > 0) {
       if (rec) {
-        const double c2 = fma(cs, zc, -sn * zs);
-        sn = fma(cs, zs, sn * zc);
-        cs = c2;
+        const double c2 = fma(Gr, zc, -Gi * zs);
+        Gi = fma(Gr, zs, Gi * zc);
+        Gr = c2;
       } else {
         if (TAB)
           sincos_tab(kv[b] * r, trig, &sn, &cs);
         else
           sincos_f64(kv[b] * r, &sn, &cs);
+        Gr = g * cs;
+        Gi = g * sn;
       }
     }
     const double krr = kv[b] * r;
-    const double Fr = -ak * fma(krr, sn, cs);
-    const double Fi = ak * fma(krr, cs, -sn);
-    const double Gr = g * cs, Gi = g * sn;
+    const double Fr = -t * fma(krr, Gi, Gr);
+    const double Fi = t * fma(krr, Gr, -Gi);
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
       fr[b][s] = fma(Fr, ps[s], fr[b][s]);

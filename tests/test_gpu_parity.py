@@ -126,6 +126,20 @@ def test_assembly_parity(name, k, n, m):
     assert np.isfortran(F) and F.shape == (mesh.n_sources, mesh.n_sources)
 
 
+@pytest.mark.parametrize("k,tol", [(60.0, TOL), (5.0e7, 1e-6)])
+def test_assembly_parity_large_kr(k, tol):
+    """e^{ikr} at kr up to 120 (table-driven sincos well away from small arguments) and at
+    kr up to 1e8, where points on both sides of 2^26 take the table path and the library
+    sincos.  There the phase can only agree to the rounding of r itself, |d(kr)| ~ kr 4e-16
+    ~ 4e-8, hence the looser tolerance; C does not depend on k."""
+    mesh = MESHES["lin8"]()
+    print(f"[case] assembly lin8 N={mesh.n_sources} k={k} (n,m)=(2,2)", flush=True)
+    F, G, C = assemble_parallel(mesh, k, mesh.sources, n=2, m=2)
+    Fo, Go, Co = oracle.assemble(mesh, k, n=2, m=2)
+    assert np.isfinite(F).all() and np.isfinite(G).all()
+    assert rel(F, Fo) < tol and rel(G, Go) < tol and rel(C, Co) < TOL
+
+
 def test_field_points_and_flags():
     mesh = MESHES["quad4"]()
     rng = np.random.default_rng(3)
