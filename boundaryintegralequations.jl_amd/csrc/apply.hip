@@ -180,6 +180,11 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
   BIE_REQUIRE(nrhs <= (1 << 20), BIE_EINVAL, "too many right-hand sides");
   if (M == 0 || nrhs == 0) return;
   BIE_REQUIRE(dA && dX && dY, BIE_EINVAL, "NULL pointer");
+  const int ncp = (int)((nrhs + 31) / 32 * 32);
+  // 32-bit buffer offsets in the kernel: (klen + 64) 16 max(lda, ncp) < 2^31 with klen >= 16
+  const int64_t wide = std::max<int64_t>(lda, ncp) * 16;
+  BIE_REQUIRE(wide * (64 + 4 * kRing) < (1LL << 31), BIE_EINVAL,
+              "apply: leading dimension too large (lda <= 1.6e6)");
   {
     int64_t blocks = (M * nrhs + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -187,7 +192,6 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
                        reinterpret_cast<double2*>(dY), ldy, M, (int)nrhs);
   }
   if (K == 0) return;
-  const int ncp = (int)((nrhs + 31) / 32 * 32);
   int dev = 0;
   BIE_HIP_CHECK(hipGetDevice(&dev));
   BIE_REQUIRE(dev >= 0 && dev < 16, BIE_EINVAL, "device ordinal out of range");
@@ -227,10 +231,6 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
   if (splits > kmax_splits) splits = kmax_splits;
   if (splits < 1) splits = 1;
   if (splits > 65535) splits = 65535;
-  // 32-bit buffer offsets: (klen + 64) 16 max(lda, ncp) < 2^31
-  const int64_t wide = std::max<int64_t>(lda, ncp) * 16;
-  BIE_REQUIRE(wide * (64 + 4 * kRing) < (1LL << 31), BIE_EINVAL,
-              "apply: leading dimension too large (lda <= 1.6e6)");
   const int64_t klen_max = ((1LL << 31) - 1) / wide - 64;
   int64_t klen = (K + splits - 1) / splits;
   klen = (klen + 4 * kRing - 1) / (4 * kRing) * (4 * kRing);  // whole rings but in the last split

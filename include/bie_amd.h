@@ -174,7 +174,8 @@ int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, in
  * Replaces the BLAS ZGEMM behind `F*X` / `G*vs` in the reference's callers
  * (examples/3d_pulsating_sphere.jl:42,62), batched like the FMM operators' 32-RHS path
  * (src/3d/fast_multipole_method.jl:249-287).  Uses a cached per-device workspace of
- * 16 * K * ceil(nrhs/32)*32 bytes. */
+ * 16 * K * ceil(nrhs/32)*32 bytes.  The kernel addresses each K split with 32-bit buffer
+ * offsets: lda and ceil(nrhs/32)*32 up to 1.6e6 (BIE_EINVAL beyond). */
 int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream);
 

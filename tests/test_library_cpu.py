@@ -91,6 +91,16 @@ def test_python_mirror_argument_errors():
         assemble_parallel(mesh, 1.0, np.zeros((2, 5)))
 
 
+def test_apply_rejects_leading_dimension_beyond_buffer_offsets():
+    # checked on the host before any HIP call: the apply kernel addresses a K split with
+    # 32-bit buffer offsets, so lda is limited to ~1.6e6 rows (the pointers are never used)
+    p = ctypes.c_void_p(0x1000)
+    lib = _lib.lib()
+    rc = lib.bie_zgemm_device(10, 1, 4, p, 2_000_000, p, 4, p, 2_000_000, None)
+    assert rc == _lib.BIE_EINVAL
+    assert b"leading dimension too large" in lib.bie_last_error()
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
