@@ -260,7 +260,7 @@ def main():
     ap.add_argument("--k", type=float, default=10.0)
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="target rows in the CPU baseline sample (0: all rows)")
-    ap.add_argument("--cpu-repeats", type=int, default=3,
+    ap.add_argument("--cpu-repeats", type=int, default=5,
                     help="timed CPU runs after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
