@@ -12,8 +12,9 @@ quadrature point with sqrt, reciprocal, sin, cos counted once each, plus 10 n_s 
 (target, element) pair for rule selection; n_s = 6) over its average duration measured
 with HIP events on the launch stream, against the MI355X FP64 vector peak 78.6 TFLOP/s.
 
-CPU baseline: the oracle's C++/OpenMP restatement of assemble_parallel! on a bounded
-sample of rows of the same workload, all host threads.
+CPU baseline: the oracle's C++/OpenMP restatement of assemble_parallel! on the same
+workload (all rows by default, about 2 s per run; --cpu-rows takes a sample), all host
+threads, median of 5 timed runs after a warm-up.
 """
 import argparse
 import ctypes
