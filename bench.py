@@ -178,6 +178,35 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
     return res
 
 
+def measure_adjoint(torch, reps=2):
+    """Config 4: 2562-node linear sphere with radial N(0, 0.01^2) perturbation (seed 1), k = 5,
+    100 Fibonacci field points on r = 3, (n,m) = (4,4).  Wall time of one evaluation of
+    J = sum |u|^2 (two GPU assemblies, the LU solve on the host) and of J with its adjoint
+    gradient (7686 coordinates and k: two assembly VJP launches besides the forward and
+    adjoint solves); central differences would take 2 x 7687 evaluations."""
+    from bie_amd import mesh as M
+    from bie_amd.adjoint import scattering_gradient, scattering_objective
+    mesh = M.perturb_radial(M.icosphere(16, "linear"), 0.01, 1)
+    pts = M.fibonacci_sphere(100, 3.0)
+    k = 5.0
+    t_obj, t_grad = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        J, _ = scattering_objective(mesh, k, pts, n=4, m=4)
+        torch.cuda.synchronize()
+        t_obj.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        Jg, grad, dk = scattering_gradient(mesh, k, pts, n=4, m=4)
+        torch.cuda.synchronize()
+        t_grad.append(time.perf_counter() - t0)
+    n_coord = int(grad.size)
+    return {"objective_s": min(t_obj), "objective_and_gradient_s": min(t_grad),
+            "n_coordinates": n_coord, "J": J,
+            "central_difference_estimate_s": 2 * (n_coord + 1) * min(t_obj),
+            "gradient_speedup_vs_central_differences":
+                2 * (n_coord + 1) * min(t_obj) / min(t_grad)}
+
+
 def measure_batch(torch, bie_amd, steps=3, n=4, m=4):
     """Config 5: 48 ellipsoids (semi-axes U[0.5, 2], N log-uniform in [500, 8000] rounded to
     10 f^2 + 2, k U[1, 20], seed 3), complex128 F, G, C each, (n,m)=(4,4).  One tuned
@@ -362,6 +391,7 @@ def main():
         torch.cuda.empty_cache()
         extra["sweep"] = measure_sweep(torch, bie_amd)
         extra["uneven_batch"] = measure_batch(torch, bie_amd)
+        extra["adjoint"] = measure_adjoint(torch)
         G = torch.empty(2 * N * N, dtype=torch.float64, device=dev)
         run(n, m, 1, 0)
 
