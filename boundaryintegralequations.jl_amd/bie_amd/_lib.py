@@ -115,6 +115,13 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build the HIP library first "
                 "(python -c 'import __graft_entry__ as g; g.build()')")
+        # PyTorch (when installed) is imported before the library is loaded: with the
+        # library loaded first, PyTorch's later device initialisation in the same process
+        # fails ("No HIP GPUs are available"); in this order both work.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -24,7 +24,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .assembly import DeviceMesh, assemble_parallel
+from .assembly import DeviceMesh
 from .mesh import Mesh3d
 
 
@@ -84,38 +84,96 @@ def plane_wave(x, k, direction=(0.0, 0.0, 1.0)):
     return np.exp(1j * k * (d @ x))
 
 
+def _forward(torch, dm: DeviceMesh, k: float, field_points, n, m, direction):
+    """J and the forward quantities, on the device of dm: the surface and field operators
+    assembled into device buffers, A p = b and the field pressure by device LU solves."""
+    mesh = dm.mesh
+    dev = torch.device("cuda", dm.device)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    pts = np.asarray(field_points, dtype=np.float64)
+    N, P = mesh.n_sources, pts.shape[1]
+    cplx = torch.complex128
+    Fm = torch.empty((N, N), dtype=cplx, device=dev)        # column-major F: Fm = F^T
+    Ffm = torch.empty((N, P), dtype=cplx, device=dev)       # column-major F_field
+    Gs = torch.empty(N * max(N, P), dtype=cplx, device=dev)  # G is not needed (zero-filled)
+    C = torch.empty(N, dtype=cplx, device=dev)
+    Cf = torch.empty(P, dtype=cplx, device=dev)
+    tp = torch.as_tensor(np.asfortranarray(pts).ravel(order="F"), device=dev)
+    dm.assemble(k, Fm.data_ptr(), Gs.data_ptr(), C.data_ptr(), n=n, m=m, gOn=False,
+                stream=stream)
+    dm.assemble(k, Ffm.data_ptr(), Gs.data_ptr(), Cf.data_ptr(), targets_ptr=tp.data_ptr(),
+                n_targets=P, n=n, m=m, gOn=False, cOn=False, stream=stream)
+    del Gs
+    d = torch.as_tensor(np.asarray(direction, dtype=np.float64), device=dev)
+    xs = torch.as_tensor(np.ascontiguousarray(mesh.sources), device=dev)
+    xf = torch.as_tensor(np.ascontiguousarray(pts), device=dev)
+    b = torch.exp(1j * k * (d @ xs))
+    pf = torch.exp(1j * k * (d @ xf))
+    A = Fm.T + torch.diag(1.0 - C)
+    p = torch.linalg.solve(A, b)
+    Ff = Ffm.T
+    u = pf - Ff @ p
+    J = float(torch.sum(u.real ** 2 + u.imag ** 2))
+    return J, dict(A=A, Ff=Ff, p=p, u=u, b=b, pf=pf, d=d, xs=xs, xf=xf, tp=tp,
+                   stream=stream, dev=dev)
+
+
 def scattering_objective(mesh: Mesh3d, k: float, field_points, *, n=4, m=4,
-                         direction=(0.0, 0.0, 1.0)):
-    """J and the forward quantities (p, u, A, F_field) on the GPU-assembled operators."""
-    F, _, C = assemble_parallel(mesh, k, mesh.sources, n=n, m=m, gOn=False)
-    Ff, _, _ = assemble_parallel(mesh, k, field_points, n=n, m=m, gOn=False, cOn=False)
-    A = F + np.diag(1.0 - C)
-    b = plane_wave(mesh.sources, k, direction)
-    p = np.linalg.solve(A, b)
-    u = plane_wave(field_points, k, direction) - Ff @ p
-    return float(np.real(np.vdot(u, u))), dict(A=A, Ff=Ff, p=p, u=u, b=b)
+                         direction=(0.0, 0.0, 1.0), dmesh: DeviceMesh | None = None):
+    """J and the forward quantities (p, u, A, F_field as numpy arrays), computed on the GPU."""
+    import torch
+    dm = dmesh if dmesh is not None else DeviceMesh(mesh)
+    J, fw = _forward(torch, dm, float(k), field_points, n, m, direction)
+    return J, {key: fw[key].cpu().numpy() for key in ("A", "Ff", "p", "u", "b")}
+
+
+def _vjp_device(torch, dm: DeviceMesh, k, alpha, p, beta, targets_ptr, n_targets, target_node,
+                n, m, stream, dev):
+    """assembly_vjp on device tensors (complex128 alpha, p, beta; int32 target_node)."""
+    gX = torch.zeros(3 * dm.mesh.n_coordinates, dtype=torch.float64, device=dev)
+    gk = torch.zeros(1, dtype=torch.float64, device=dev)
+    # the library reads raw memory: materialise lazy conjugate / negative views first
+    def raw(t):
+        return t.resolve_conj().resolve_neg().contiguous()
+
+    alpha, p = raw(alpha), raw(p)
+    beta = raw(beta) if beta is not None else None
+    _lib.check(_lib.lib().bie_assembly_vjp_device(
+        dm.handle, float(k), ctypes.c_void_p(targets_ptr or None), int(n_targets),
+        ctypes.c_void_p(target_node.data_ptr() if target_node is not None else None),
+        ctypes.c_void_p(alpha.data_ptr()),
+        ctypes.c_void_p(beta.data_ptr() if beta is not None else None),
+        ctypes.c_void_p(p.data_ptr()), int(n), int(m), ctypes.c_void_p(gX.data_ptr()),
+        ctypes.c_void_p(gk.data_ptr()), ctypes.c_void_p(stream or None)))
+    return gX.view(dm.mesh.n_coordinates, 3).T, gk[0]
 
 
 def scattering_gradient(mesh: Mesh3d, k: float, field_points, *, n=4, m=4,
                         direction=(0.0, 0.0, 1.0), dmesh: DeviceMesh | None = None):
-    """J, dJ/d(mesh coordinates) (3 x Nc) and dJ/dk by the adjoint method."""
-    J, fw = scattering_objective(mesh, k, field_points, n=n, m=m, direction=direction)
-    A, Ff, p, u, b = fw["A"], fw["Ff"], fw["p"], fw["u"], fw["b"]
-    lam = np.linalg.solve(A.conj().T, Ff.conj().T @ u)
+    """J, dJ/d(mesh coordinates) (3 x Nc numpy array) and dJ/dk by the adjoint method.  The
+    operators, both solves and both vector-Jacobian products stay on the GPU; only J, the
+    gradient and dJ/dk come back."""
+    import torch
     dm = dmesh if dmesh is not None else DeviceMesh(mesh)
-    gs, gks = assembly_vjp(dm, k, np.conj(lam), p, beta=-np.conj(lam) * p,
-                           target_node=source_nodes(mesh), n=n, m=m)
-    gf, gkf = assembly_vjp(dm, k, -np.conj(u), p, targets=np.asarray(field_points), n=n, m=m)
+    k = float(k)
+    J, fw = _forward(torch, dm, k, field_points, n, m, direction)
+    A, Ff, p, u, b = fw["A"], fw["Ff"], fw["p"], fw["u"], fw["b"]
+    stream, dev = fw["stream"], fw["dev"]
+    lam = torch.linalg.solve(A.conj().T, Ff.conj().T @ u)
+    nodes = torch.as_tensor(source_nodes(mesh), device=dev)
+    P = u.shape[0]
+    lam_c = torch.conj_physical(lam)
+    gs, gks = _vjp_device(torch, dm, k, lam_c, p, -lam_c * p, 0, mesh.n_sources, nodes, n, m,
+                          stream, dev)
+    gf, gkf = _vjp_device(torch, dm, k, -torch.conj_physical(u), p, None, fw["tp"].data_ptr(),
+                          P, None, n, m, stream, dev)
     grad = 2.0 * (gs + gf)
     dk = 2.0 * (gks + gkf)
-    d = np.asarray(direction, dtype=np.float64)
+    d, xs, xf, pf = fw["d"], fw["xs"], fw["xf"], fw["pf"]
     # -lambda^H db with b_i = e^{ik d.x_i}: d b_i/d x_i = i k d b_i, d b_i/dk = i (d.x_i) b_i
-    nodes = source_nodes(mesh)
-    coef = np.real(-np.conj(lam) * 1j * k * b)
-    for c in range(3):
-        np.add.at(grad[c], nodes, 2.0 * d[c] * coef)
-    dk += 2.0 * np.real(np.sum(-np.conj(lam) * 1j * (d @ mesh.sources) * b))
+    coef = (-lam_c * 1j * k * b).real
+    grad = grad.index_add(1, nodes.long(), 2.0 * d[:, None] * coef[None, :])
+    dk = dk + 2.0 * (-lam_c * 1j * (d @ xs) * b).sum().real
     # u^H dp_inc,field/dk
-    pf = plane_wave(field_points, k, direction)
-    dk += 2.0 * np.real(np.sum(np.conj(u) * 1j * (d @ field_points) * pf))
-    return J, grad, dk
+    dk = dk + 2.0 * (u.conj() * 1j * (d @ xf) * pf).sum().real
+    return J, grad.cpu().numpy(), float(dk)

@@ -89,6 +89,7 @@ class DeviceMesh:
         _lib.check(_lib.lib().bie_dmesh_create(ctypes.byref(hm.struct), int(device),
                                                ctypes.byref(h)))
         self.handle = h
+        self.device = int(device)
         self.n_sources = mesh.n_sources
         self.n_elements = mesh.n_elements
 
