@@ -3,7 +3,7 @@
 The reference solves its rigid-sphere system A p = p_inc, A = F + diag(1 - C)
 (examples/3d_rigid_sphere.jl:47-52), with a direct or Krylov solver from the Julia
 ecosystem (IterativeSolvers.gmres in its examples and tests).  This is the same
-iteration: Arnoldi with modified Gram-Schmidt (one reorthogonalisation pass), Givens
+iteration: Arnoldi with classical Gram-Schmidt applied twice (CGS2), Givens
 rotations for the small least-squares problem, restart after `restart` steps, stopping at
 ||b - A x|| <= tol ||b||.  Device tensors are torch complex128; the operator is any
 callable x -> A x on them, so the assembled matrix stays where the assembly wrote it.
@@ -45,13 +45,17 @@ def gmres(torch, A, b, *, tol=1e-10, restart=60, maxiter=2000, x0=None):
         k_used = 0
         for j in range(m):
             w = A(V[j])
-            for _ in range(2):                     # MGS with one reorthogonalisation
-                h = torch.mv(V[: j + 1].conj(), w)
-                w = w - torch.mv(V[: j + 1].t(), h)
-                hv = h.cpu().tolist()
-                for i in range(j + 1):
-                    H[i][j] += hv[i]
-            hn = float(torch.linalg.vector_norm(w))
+            # classical Gram-Schmidt with one reorthogonalisation pass, both on the device;
+            # the coefficients and the norm come to the host in one transfer per step
+            h = torch.mv(V[: j + 1].conj(), w)
+            w = w - torch.mv(V[: j + 1].t(), h)
+            h2 = torch.mv(V[: j + 1].conj(), w)
+            w = w - torch.mv(V[: j + 1].t(), h2)
+            hv = torch.cat([h + h2, torch.linalg.vector_norm(w).reshape(1).to(h.dtype)])
+            hv = hv.cpu().tolist()
+            for i in range(j + 1):
+                H[i][j] += hv[i]
+            hn = float(hv[j + 1].real)
             H[j + 1][j] = complex(hn)
             if hn > 0.0:
                 V[j + 1] = w / hn
