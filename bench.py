@@ -151,23 +151,20 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
         err = max(err, float(torch.linalg.norm(F[sl] - Fp[sl]) / torch.linalg.norm(Fp[sl])),
                   float(torch.linalg.norm(G[sl] - Gp[sl]) / torch.linalg.norm(Gp[sl])))
     g.close()
-    # per-k GMRES (tol 1e-10) for the rigid sphere under p_inc = e^{ikz}: A_k = F_k + diag(1 - C)
-    from bie_amd.solvers import collocation_operator, gmres
+    # GMRES (tol 1e-10, restart 60) for the rigid sphere under p_inc = e^{ikz},
+    # A_k = F_k + diag(1 - C), all 64 systems advanced in lockstep
+    from bie_amd.solvers import collocation_operator_batched, gmres_batched
     Fc = torch.view_as_complex(F.view(nk, N, N, 2))
     Cc = torch.view_as_complex(C.view(N, 2))
     z = torch.as_tensor(mesh.sources[2], device=dev)
-    its, rrs = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for q in range(nk):
-        b = torch.exp(1j * float(ks[q]) * z)
-        _, it, rr = gmres(torch, collocation_operator(torch, Fc[q], Cc), b, tol=1e-10)
-        its.append(it)
-        rrs.append(rr)
+    Bt = torch.exp(1j * kd[:, None] * z[None, :])
+    _, its, rrs = gmres_batched(torch, collocation_operator_batched(torch, Fc, Cc), Bt, tol=1e-10)
     torch.cuda.synchronize()
-    res["gmres"] = {"tol": 1e-10, "total_s": time.perf_counter() - t0,
-                    "iterations_min_mean_max": [min(its), sum(its) / nk, max(its)],
-                    "max_rel_residual": max(rrs)}
+    res["gmres"] = {"tol": 1e-10, "total_s": time.perf_counter() - t0, "batched": True,
+                    "iterations_min_mean_max": [int(its.min()), float(its.mean()), int(its.max())],
+                    "max_rel_residual": float(rrs.max())}
     res["speedup"] = res["per_k_launches_ms"] / res["batched_graph_ms"]
     res["max_rel_err_batched_vs_per_k"] = err
     res["workload"] = (f"linear geodesic sphere N={N}, E={mesh.n_elements}, {nk} k uniform in "

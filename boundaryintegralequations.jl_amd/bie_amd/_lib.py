@@ -63,6 +63,10 @@ SIGNATURES = {
     "bie_assembly_vjp_device": (ctypes.c_int, [_dp, ctypes.c_double, _dp, ctypes.c_int64, _dp,
                                                 _dp, _dp, _dp, ctypes.c_int32, ctypes.c_int32,
                                                 _dp, _dp, _dp]),
+    "bie_zgemv_batched_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _dp,
+                                                 ctypes.c_int64, ctypes.c_int64, _dp,
+                                                 ctypes.c_int64, _dp, ctypes.c_int64, _dp,
+                                                 ctypes.c_int32, _dp]),
     "bie_zgemm_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp]),

@@ -17,6 +17,16 @@ def zgemm_device(M, nrhs, K, A_ptr, lda, X_ptr, ldx, Y_ptr, ldy, stream=0):
         int(ldx), ctypes.c_void_p(Y_ptr), int(ldy), ctypes.c_void_p(stream or None)))
 
 
+def zgemv_batched_device(M, K, A_ptr, strideA, lda, X_ptr, ldx, Y_ptr, ldy, idx_ptr, n_idx,
+                         stream=0):
+    """Asynchronous y_q += A_q x_q for q in the device int32 array idx (complex128; strides
+    and leading dimensions in complex entries)."""
+    _lib.check(_lib.lib().bie_zgemv_batched_device(
+        int(M), int(K), ctypes.c_void_p(A_ptr), int(strideA), int(lda), ctypes.c_void_p(X_ptr),
+        int(ldx), ctypes.c_void_p(Y_ptr), int(ldy), ctypes.c_void_p(idx_ptr), int(n_idx),
+        ctypes.c_void_p(stream or None)))
+
+
 class _Dev:
     def __init__(self, nbytes):
         p = ctypes.c_void_p()

@@ -956,6 +956,15 @@ int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64
   });
 }
 
+int bie_zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA,
+                             int64_t lda, const double* dX, int64_t ldx, double* dY, int64_t ldy,
+                             const int32_t* d_idx, int32_t n_idx, void* stream) {
+  return guarded([&] {
+    zgemv_batched_device(M, K, dA, strideA, lda, dX, ldx, dY, ldy, d_idx, n_idx,
+                         static_cast<hipStream_t>(stream));
+  });
+}
+
 int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
                               const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
                               uint32_t flags, double* dF, double* dG, int64_t stride,

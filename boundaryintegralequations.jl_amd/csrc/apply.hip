@@ -23,6 +23,7 @@
 #include <mutex>
 
 #include "bie_common.hpp"
+#include "device_common.hpp"
 
 namespace bie {
 namespace {
@@ -163,6 +164,51 @@ int resident_workgroups(int dev) {
   return cached[dev];
 }
 
+// y_q += A_q x_q for the systems q = idx[blockIdx.z] of a strided batch of column-major
+// complex matrices (A_q at A + q strideA, leading dimension lda), x_q at X + q ldx, y_q at
+// Y + q ldy: the matrix-vector product of Krylov solves over a frequency sweep's operators.
+// One row per thread, so a wave reads 1 KB of a column at a time; kCols columns are loaded
+// before use to keep enough bytes in flight; x is wave-uniform and comes through scalar
+// loads; K is split over blockIdx.y with FP64 atomics into y.  Memory-bound: 16 bytes of
+// A per complex multiply-add.
+constexpr int kGemvCols = 8;
+
+__global__ void __launch_bounds__(256)
+k_zgemv_batched(const double2* __restrict__ A, int64_t strideA, int64_t lda,
+                const double2* __restrict__ X, int64_t ldx, double2* __restrict__ Y,
+                int64_t ldy, const int* __restrict__ idx, int64_t M, int64_t K, int64_t klen) {
+  const int q = sload(idx + blockIdx.z);
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t kb = (int64_t)blockIdx.y * klen;
+  const int64_t ke = min(K, kb + klen);
+  if (r >= M || kb >= ke) return;
+  const double2* __restrict__ a = A + (size_t)q * strideA + r;
+  const double* xq = reinterpret_cast<const double*>(X + (size_t)q * ldx);
+  auto xload = [&](int64_t jj) { return make_double2(sload(xq + 2 * jj), sload(xq + 2 * jj + 1)); };
+  double yr = 0.0, yi = 0.0;
+  int64_t j = kb;
+  for (; j + kGemvCols <= ke; j += kGemvCols) {
+    double2 av[kGemvCols];
+#pragma unroll
+    for (int t = 0; t < kGemvCols; ++t) av[t] = a[(j + t) * lda];
+#pragma unroll
+    for (int t = 0; t < kGemvCols; ++t) {
+      const double2 x = xload(j + t);
+      yr = fma(av[t].x, x.x, fma(-av[t].y, x.y, yr));
+      yi = fma(av[t].x, x.y, fma(av[t].y, x.x, yi));
+    }
+  }
+  for (; j < ke; ++j) {
+    const double2 av = a[j * lda];
+    const double2 x = xload(j);
+    yr = fma(av.x, x.x, fma(-av.y, x.y, yr));
+    yi = fma(av.x, x.y, fma(av.y, x.x, yi));
+  }
+  double* y = reinterpret_cast<double*>(Y + (size_t)q * ldy + r);
+  unsafeAtomicAdd(y, yr);
+  unsafeAtomicAdd(y + 1, yi);
+}
+
 struct Workspace {
   std::mutex mu;
   double2* p[16] = {};
@@ -241,6 +287,34 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
   hipLaunchKernelGGL(k_zgemm_mfma, dim3((unsigned)rowblocks, (unsigned)splits, colblocks),
                      dim3(256), 0, s, reinterpret_cast<const double2*>(dA), lda, Xp, ncp,
                      reinterpret_cast<double2*>(dY), ldy, M, K, (int)nrhs, klen);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA, int64_t lda,
+                          const double* dX, int64_t ldx, double* dY, int64_t ldy,
+                          const int32_t* d_idx, int32_t n_idx, hipStream_t s) {
+  BIE_REQUIRE(M >= 0 && K >= 0 && n_idx >= 0, BIE_EINVAL, "negative size");
+  BIE_REQUIRE(n_idx <= 65535, BIE_EINVAL, "more than 65535 systems in one call");
+  BIE_REQUIRE(lda >= (M > 0 ? M : 1) && ldx >= (K > 0 ? K : 1) && ldy >= (M > 0 ? M : 1),
+              BIE_EINVAL, "leading dimension too small");
+  BIE_REQUIRE(strideA >= lda * K, BIE_EINVAL, "matrix stride smaller than one matrix");
+  if (M == 0 || K == 0 || n_idx == 0) return;
+  BIE_REQUIRE(dA && dX && dY && d_idx, BIE_EINVAL, "NULL pointer");
+  int dev = 0;
+  BIE_HIP_CHECK(hipGetDevice(&dev));
+  int cus = 0;
+  BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t rowblocks = (M + 255) / 256;
+  // enough workgroups for the memory system (8 per CU), but columns of at least 64
+  int64_t splits = (8LL * cus + rowblocks * n_idx - 1) / (rowblocks * n_idx);
+  splits = std::max<int64_t>(1, std::min<int64_t>(splits, (K + 63) / 64));
+  int64_t klen = (K + splits - 1) / splits;
+  splits = (K + klen - 1) / klen;
+  BIE_REQUIRE(rowblocks < 2147483647LL && splits <= 65535, BIE_EINVAL, "gemv too large");
+  hipLaunchKernelGGL(k_zgemv_batched, dim3((unsigned)rowblocks, (unsigned)splits, (unsigned)n_idx),
+                     dim3(256), 0, s, reinterpret_cast<const double2*>(dA), strideA, lda,
+                     reinterpret_cast<const double2*>(dX), ldx, reinterpret_cast<double2*>(dY),
+                     ldy, reinterpret_cast<const int*>(d_idx), M, K, klen);
   BIE_HIP_CHECK(hipGetLastError());
 }
 

@@ -200,6 +200,9 @@ void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, 
                          double* vals, const unsigned long long* entries, const int* off,
                          const int* refs, int64_t n_entries, hipStream_t s);
 int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage);
+void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA, int64_t lda,
+                          const double* dX, int64_t ldx, double* dY, int64_t ldy,
+                          const int32_t* d_idx, int32_t n_idx, hipStream_t s);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,

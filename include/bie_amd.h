@@ -179,6 +179,18 @@ int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, in
 int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream);
 
+/* Batched matrix-vector product for Krylov solves over a frequency sweep's operators:
+ *   y_q += A_q x_q   for q = d_idx[0 .. n_idx)  (device int32 array, n_idx <= 65535)
+ * A_q column-major M x K complex128 at dA + 2 q strideA doubles (leading dimension lda),
+ * x_q at dX + 2 q ldx doubles (K entries), y_q at dY + 2 q ldy doubles (M entries, added
+ * to: initialise y first).  Strides and leading dimensions count complex entries; the
+ * batched sweep's F_k (stride N*N) are the intended A_q.  Asynchronous on stream.  No
+ * counterpart in the reference, which solves one system at a time
+ * (examples/3d_rigid_sphere.jl:47-52). */
+int bie_zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA,
+                             int64_t lda, const double* dX, int64_t ldx, double* dY,
+                             int64_t ldy, const int32_t* d_idx, int32_t n_idx, void* stream);
+
 /* ---- Kernel timing ------------------------------------------------------------------
  * While enabled, every device assembly records four HIP events on its launch stream:
  * call start, before the dense assembly kernel, between it and the mid-edge kernel (the

@@ -83,7 +83,8 @@ def test_sweep_then_gmres_rigid_sphere():
     oracle dense solve and the analytic series (theta measured from -z for +z incidence)."""
     import torch
     import analytic
-    from bie_amd.solvers import collocation_operator, gmres
+    from bie_amd.solvers import (collocation_operator, collocation_operator_batched, gmres,
+                                 gmres_batched)
 
     mesh = M.icosphere(8, "linear")
     N = mesh.n_sources
@@ -100,6 +101,7 @@ def test_sweep_then_gmres_rigid_sphere():
     torch.cuda.synchronize()
     z = mesh.sources[2]
     th = np.arccos(np.clip(-z, -1, 1))
+    sols = []
     for q, k in enumerate(ks):
         b = torch.as_tensor(np.exp(1j * k * z), device=dev)
         x, it, rr = gmres(torch, collocation_operator(torch, F[q], C), b, tol=1e-10)
@@ -110,3 +112,11 @@ def test_sweep_then_gmres_rigid_sphere():
         assert rel(p, pd) < 1e-8
         pa = analytic.plane_wave_scattering_sphere(k, 1.0, 1.0, th, 1e-8)
         assert rel(p, pa) < 0.0045 * k + 0.001
+        sols.append((p, it))
+    # the same six systems in one lockstep batched solve
+    B = torch.as_tensor(np.exp(1j * ks[:, None] * z[None, :]), device=dev)
+    X, its, rrs = gmres_batched(torch, collocation_operator_batched(torch, F, C), B, tol=1e-10)
+    for q in range(len(ks)):
+        p, it = sols[q]
+        assert rrs[q] <= 1e-10 and abs(int(its[q]) - it) <= 1
+        assert rel(X[q].cpu().numpy(), p) < 1e-9
