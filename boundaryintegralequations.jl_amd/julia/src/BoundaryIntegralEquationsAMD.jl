@@ -336,6 +336,26 @@ function apply(A::AbstractMatrix{ComplexF64}, X::AbstractVecOrMat{ComplexF64})
 end
 
 """
+    gemv_batched!(dY, dF, N, dX, ldx, active) -> dY
+
+y_q += F_q x_q on the device (bie_zgemv_batched_device) for the systems `active` (0-based)
+of a batched sweep's output `dF` (F_q, N x N column-major, at byte offset 16 q N^2), with
+x_q at entry offset q ldx of `dX` and y_q at q N of `dY`: the operator of Krylov solves over
+all wavenumbers at once.  Waits for the device before returning.
+"""
+function gemv_batched!(dY::DeviceBuffer, dF::DeviceBuffer, N::Integer, dX::DeviceBuffer,
+                       ldx::Integer, active::AbstractVector{<:Integer})
+    dI = DeviceBuffer(Vector{Int32}(active))
+    check(ccall((:bie_zgemv_batched_device, libbie[]), Cint,
+                (Int64, Int64, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64,
+                 Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                N, N, dF.ptr, Int64(N)^2, N, dX.ptr, ldx, dY.ptr, N, dI.ptr,
+                Int32(length(active)), C_NULL))
+    check(ccall((:bie_device_synchronize, libbie[]), Cint, ()))
+    return dY
+end
+
+"""
     scattering_objective(mesh, k, field_points; n=4, m=4, direction=[0, 0, 1]) -> J
 
 The objective of `scattering_gradient` alone.  With ChainRulesCore loaded, its `rrule`
