@@ -189,7 +189,7 @@ def measure_adjoint(torch, reps=2):
     t_obj, t_grad = [], []
     for _ in range(reps):
         t0 = time.perf_counter()
-        J, _ = scattering_objective(mesh, k, pts, n=4, m=4)
+        J, _ = scattering_objective(mesh, k, pts, n=4, m=4, forward=False)
         torch.cuda.synchronize()
         t_obj.append(time.perf_counter() - t0)
         t0 = time.perf_counter()

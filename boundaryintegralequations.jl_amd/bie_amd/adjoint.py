@@ -119,11 +119,15 @@ def _forward(torch, dm: DeviceMesh, k: float, field_points, n, m, direction):
 
 
 def scattering_objective(mesh: Mesh3d, k: float, field_points, *, n=4, m=4,
-                         direction=(0.0, 0.0, 1.0), dmesh: DeviceMesh | None = None):
-    """J and the forward quantities (p, u, A, F_field as numpy arrays), computed on the GPU."""
+                         direction=(0.0, 0.0, 1.0), dmesh: DeviceMesh | None = None,
+                         forward=True):
+    """J and the forward quantities (p, u, A, F_field as numpy arrays), computed on the GPU;
+    with forward=False only J (nothing else leaves the device)."""
     import torch
     dm = dmesh if dmesh is not None else DeviceMesh(mesh)
     J, fw = _forward(torch, dm, float(k), field_points, n, m, direction)
+    if not forward:
+        return J, None
     return J, {key: fw[key].cpu().numpy() for key in ("A", "Ff", "p", "u", "b")}
 
 
