@@ -161,6 +161,48 @@ def test_empty_targets():
     assert F.shape == (0, mesh.n_sources) and C.shape == (0,)
 
 
+def _submesh(mesh, elems):
+    """The elements `elems` of a mesh as a mesh of their own (nodes re-indexed)."""
+    topo = mesh.topology[:, elems]
+    ptopo = mesh.physics_topology[:, elems]
+    g_nodes, g_inv = np.unique(topo, return_inverse=True)
+    p_nodes, p_inv = np.unique(ptopo, return_inverse=True)
+    return M.Mesh3d(mesh.coordinates[:, g_nodes], g_inv.reshape(topo.shape),
+                    mesh.sources[:, p_nodes], p_inv.reshape(ptopo.shape),
+                    mesh.geometry_kind, mesh.physics_kind)
+
+
+@pytest.mark.parametrize("kind,n_el", [("linear", 1), ("linear", 3), ("quadratic", 2)])
+def test_fewer_elements_than_waves_per_step(kind, n_el):
+    """Meshes of 1-3 elements: most of the 8 waves of every slot-kernel step have no
+    element; collocation and field targets (some on nodes, one far away) against the
+    oracle."""
+    full = M.icosphere(1, kind)
+    # elements sharing nodes with element 0, so the patch is connected
+    first = set(full.physics_topology[:, 0].tolist())
+    near = [e for e in range(full.n_elements)
+            if first & set(full.physics_topology[:, e].tolist())]
+    mesh = _submesh(full, near[:n_el])
+    assert mesh.n_elements == n_el
+    rng = np.random.default_rng(n_el)
+    X = np.concatenate([mesh.sources[:, :2], rng.standard_normal((3, 5)),
+                        np.array([[30.0], [-4.0], [2.0]])], axis=1)
+    for k in (0.0, 3.0):
+        print(f"[case] {kind} patch of {n_el} element(s) k={k}", flush=True)
+        for targets in (mesh.sources, X):
+            F, G, C = assemble_parallel(mesh, k, targets, n=3, m=2)
+            Fo, Go, Co = oracle.assemble(mesh, k, targets=None if targets is mesh.sources
+                                         else targets, n=3, m=2)
+            assert rel(G, Go) < TOL
+            # on one flat triangle every collocation point lies in the element's plane, so
+            # n.(x - y) = 0 and F, C vanish exactly: both sides are round-off there, and
+            # their error is measured against the size of G (same integral scale on this
+            # unit-sphere patch) instead of against themselves
+            scale = max(np.linalg.norm(Fo), np.linalg.norm(Co), np.linalg.norm(Go))
+            assert np.linalg.norm(F - Fo) < TOL * scale
+            assert np.linalg.norm(C - Co) < TOL * scale
+
+
 def test_reference_tolerance_table_on_gpu_output():
     # test/3d/assembly_test.jl:12-44 with the GPU's F, C
     k = 2 * np.pi * 100.0 / 340.0
