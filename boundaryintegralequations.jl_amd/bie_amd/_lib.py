@@ -78,6 +78,13 @@ SIGNATURES = {
                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
                                               _dp, _dp, ctypes.c_int64, _dp,
                                               ctypes.POINTER(_dp)]),
+    "bie_assemble_taylor_device": (ctypes.c_int, [_dp, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_int32, _dp, ctypes.c_int64,
+                                                   ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.c_uint32, _dp, _dp, ctypes.c_int64,
+                                                   _dp, _dp]),
+    "bie_taylor_expand_device": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _dp,
+                                                 ctypes.c_int64, _dp, _dp, _dp]),
     "bie_graph_launch": (ctypes.c_int, [_dp, _dp]),
     "bie_graph_destroy": (ctypes.c_int, [_dp]),
     "bie_batch_create": (ctypes.c_int, [ctypes.POINTER(_dp), ctypes.c_int32, ctypes.c_int32,

@@ -476,16 +476,20 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
   }
 }
 
-// Batched sweep over nk real wavenumbers (d_k on the device).  prepare_only builds every
+// Batched sweep over nk real wavenumbers (d_k on the device), or with taylor the k
+// derivatives of orders 0..nk-1 at k_re + i k_im (d_k unused).  prepare_only builds every
 // cache (rules, schedule, queues) without launching, so that a later call can be
 // captured into a graph.
 void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d_targets,
                        int64_t M, int n, int m, uint32_t flags, double* dF, double* dG,
-                       int64_t stride, double* dC, hipStream_t s, bool prepare_only) {
+                       int64_t stride, double* dC, hipStream_t s, bool prepare_only,
+                       bool taylor = false, double k_re = 0.0, double k_im = 0.0) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
   BIE_REQUIRE(nk >= 0 && M >= 0, BIE_EINVAL, "negative size");
   BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
-  BIE_REQUIRE(nk == 0 || d_k, BIE_EINVAL, "k list is NULL");
+  BIE_REQUIRE(taylor || nk == 0 || d_k, BIE_EINVAL, "k list is NULL");
+  BIE_REQUIRE(!taylor || (std::isfinite(k_re) && std::isfinite(k_im)), BIE_EINVAL,
+              "wavenumber is not finite");
   BIE_REQUIRE(dF && dG && dC, BIE_EINVAL, "output pointer is NULL");
   BIE_REQUIRE(stride >= M * dm->N, BIE_EINVAL, "stride smaller than M * N");
   if (d_targets == nullptr) {
@@ -523,16 +527,19 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   }
   if (M == 0 || dm->E == 0 || nk == 0) return;
   refresh_geometry(dm, s);
-  launch_block_uniform(d_k, nk, KB, dm->d_block_uniform, s);
+  if (!taylor) launch_block_uniform(d_k, nk, KB, dm->d_block_uniform, s);
   const int64_t rowblocks = (M + 63) / 64;
   AssemblyArgs a = make_args(dm, d_targets, M, 0.0, 0.0, flags, dF, dG, dC);
   SlotArgs sa = slot_args(dm, sc, rowblocks);
   SweepArgs sw{};
-  sw.k = d_k;
+  sw.k = taylor ? nullptr : d_k;
   sw.nk = nk;
-  sw.block_uniform = dm->d_block_uniform;
+  sw.block_uniform = taylor ? nullptr : dm->d_block_uniform;
   sw.stride = stride;
   sw.n_tiles = (int64_t)nkb * sc.segments * rowblocks;
+  sw.taylor = taylor ? 1 : 0;
+  sw.k_re = k_re;
+  sw.k_im = k_im;
   launch_sweep(a, sa, sw, s);
 }
 
@@ -977,6 +984,27 @@ int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
   });
 }
 
+int bie_assemble_taylor_device(bie_dmesh* dm, double k_re, double k_im, int32_t n_orders,
+                               const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
+                               uint32_t flags, double* dF, double* dG, int64_t stride,
+                               double* dC, void* stream) {
+  return guarded([&] {
+    BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+    BIE_REQUIRE(n_orders >= 0 && n_orders <= kTaylorMaxOrders, BIE_EINVAL,
+                "n_orders must be in 0..64");
+    std::lock_guard<std::mutex> lk(dm->mu);
+    sweep_device_impl(dm, nullptr, n_orders, d_targets, n_targets, n, m, flags, dF, dG, stride,
+                      dC, static_cast<hipStream_t>(stream), false, true, k_re, k_im);
+  });
+}
+
+int bie_taylor_expand_device(int32_t n_terms, int64_t n, const double* dA, int64_t strideA,
+                             const double* coef, double* dY, void* stream) {
+  return guarded([&] {
+    taylor_expand_device(n_terms, n, dA, strideA, coef, dY, static_cast<hipStream_t>(stream));
+  });
+}
+
 struct bie_graph {
   int device = 0;
   hipGraph_t graph = nullptr;
@@ -1000,7 +1028,7 @@ int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
       // build every cache and set kernel attributes outside the capture
       sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC, cs,
                         true);
-      sweep_prepare_attributes(dm->pk, dm->rs.R * dm->rs.Qmax * dm->pk);
+      sweep_prepare_attributes(dm->pk, dm->rs.R * dm->rs.Qmax * dm->pk, false);
       BIE_HIP_CHECK(hipStreamSynchronize(cs));
       BIE_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {

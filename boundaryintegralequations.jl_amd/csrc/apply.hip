@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 
 #include "bie_common.hpp"
@@ -315,6 +316,58 @@ void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t stride
                      dim3(256), 0, s, reinterpret_cast<const double2*>(dA), strideA, lda,
                      reinterpret_cast<const double2*>(dX), ldx, reinterpret_cast<double2*>(dY),
                      ldy, reinterpret_cast<const int*>(d_idx), M, K, klen);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+
+struct TaylorCoefs {
+  double c[2 * kTaylorMaxOrders];  // interleaved complex coefficients (1 KiB of kernel arguments)
+};
+
+// One complex entry per thread and pass: the n_terms loads are independent, so a wave keeps
+// several in flight; Y may be A_0 itself (each entry is read before it is written, by the
+// same thread).
+__global__ void __launch_bounds__(256)
+k_taylor_expand(int nt, int64_t n, const double2* A, int64_t strideA, const TaylorCoefs cf,
+                double2* Y) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += step) {
+    double yr = 0.0, yi = 0.0;
+#pragma unroll 4
+    for (int t = 0; t < nt; ++t) {
+      const double2 v = A[(size_t)t * strideA + e];
+      const double cr = cf.c[2 * t], ci = cf.c[2 * t + 1];
+      yr = fma(cr, v.x, fma(-ci, v.y, yr));
+      yi = fma(cr, v.y, fma(ci, v.x, yi));
+    }
+    Y[e] = make_double2(yr, yi);
+  }
+}
+
+}  // namespace
+
+void taylor_expand_device(int n_terms, int64_t n, const double* dA, int64_t strideA,
+                          const double* coef, double* dY, hipStream_t s) {
+  BIE_REQUIRE(n_terms >= 1 && n_terms <= kTaylorMaxOrders, BIE_EINVAL,
+              "n_terms must be in 1..64");
+  BIE_REQUIRE(n >= 0, BIE_EINVAL, "negative size");
+  BIE_REQUIRE(n_terms == 1 || strideA >= n, BIE_EINVAL, "stride smaller than one array");
+  BIE_REQUIRE(coef != nullptr, BIE_EINVAL, "coefficients are NULL");
+  if (n == 0) return;
+  BIE_REQUIRE(dA && dY, BIE_EINVAL, "NULL pointer");
+  TaylorCoefs cf{};
+  for (int t = 0; t < 2 * n_terms; ++t) {
+    BIE_REQUIRE(std::isfinite(coef[t]), BIE_EINVAL, "coefficient is not finite");
+    cf.c[t] = coef[t];
+  }
+  int dev = 0, cus = 0;
+  BIE_HIP_CHECK(hipGetDevice(&dev));
+  BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16LL * cus));
+  hipLaunchKernelGGL(k_taylor_expand, dim3((unsigned)blocks), dim3(256), 0, s, n_terms, n,
+                     reinterpret_cast<const double2*>(dA), strideA, cf,
+                     reinterpret_cast<double2*>(dY));
   BIE_HIP_CHECK(hipGetLastError());
 }
 

@@ -153,6 +153,8 @@ struct SweepArgs {
   const int* block_uniform;  // per k block: 1 when its k are uniformly spaced
   int64_t stride;            // complex elements between consecutive F_k (and G_k)
   int64_t n_tiles;           // k blocks x segments x row blocks
+  int taylor;                // 1: the nk matrices are d^q/dk^q, q = 0..nk-1, at k_re + i k_im
+  double k_re, k_im;
 };
 
 // Vector-Jacobian product of the assembly (adjoint.hip).
@@ -203,6 +205,11 @@ int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage);
 void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA, int64_t lda,
                           const double* dX, int64_t ldx, double* dY, int64_t ldy,
                           const int32_t* d_idx, int32_t n_idx, hipStream_t s);
+// Y = sum_t coef_t A_t over n_terms arrays of n complex entries spaced strideA apart
+// (apply_taylor_expansion, src/3d/rosebem.jl:164-199); coef on the host.
+constexpr int kTaylorMaxOrders = 64;
+void taylor_expand_device(int n_terms, int64_t n, const double* dA, int64_t strideA,
+                          const double* coef, double* dY, hipStream_t s);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
@@ -210,7 +217,7 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
 int sweep_kb(int NP);
 void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw,
                   hipStream_t s);
-void sweep_prepare_attributes(int NP, int nphi);
+void sweep_prepare_attributes(int NP, int nphi, bool taylor);
 void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s);
 void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,

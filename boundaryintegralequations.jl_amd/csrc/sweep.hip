@@ -83,6 +83,69 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
   }
 }
 
+// k-derivatives of orders m0 .. m0+KB-1 at one complex k = kr + i ki (taylor_greens3d!,
+// taylor_freens3d!, src/3d/kernels.jl:94-118, as taylor_assemble! integrates them,
+// src/3d/rosebem.jl:7-44):
+//   G_m = (i r)^m e^{ikr} J w / (4 pi r),   F_m = (i k r - 1 + m) (n.d / r^2) G_m,
+// so one e^{ikr} serves every order and each further order is one product by i r.
+template <int NP, int KB, bool TAB>
+__device__ __forceinline__ void taylor_point(double x0, double x1, double x2, const double4 A,
+                                             const double4 B, const double* __restrict__ p,
+                                             double kr, double ki, int m0,
+                                             double (&fr)[KB][NP], double (&fi)[KB][NP],
+                                             double (&gr)[KB][NP], double (&gi)[KB][NP],
+                                             double& csum, const double2* trig) {
+  const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rinv = rsqrt_nr(r2);
+  const double r = r2 * rinv;
+  const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
+  const double g = B.z * rinv;       // J w / (4 pi r)
+  const double t = nd * rinv * rinv;  // (n.d) / r^2
+  csum -= t * g;
+  double sn, cs;
+  if (TAB)
+    sincos_tab(kr * r, trig, &sn, &cs);
+  else
+    sincos_f64(kr * r, &sn, &cs);
+  double gm = g;
+  if (ki != 0.0) gm *= exp(-ki * r);
+  for (int j = 0; j < m0; ++j) gm *= r;  // r^m0; m0 is the same for the whole wave
+  double Gr = gm * cs, Gi = gm * sn;
+  if (m0 & 1) {  // times i^m0
+    const double t2 = -Gi;
+    Gi = Gr;
+    Gr = t2;
+  }
+  if (m0 & 2) {
+    Gr = -Gr;
+    Gi = -Gi;
+  }
+  const double krr = kr * r;
+  const double a0 = (double)(m0 - 1) - ki * r;  // Re(i k r - 1 + m0)
+  double ps[NP];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) ps[s] = p[s];
+#pragma unroll
+  for (int b = 0; b < KB; ++b) {
+    if (b > 0) {  // G_{m+1} = i r G_m
+      const double t2 = -r * Gi;
+      Gi = r * Gr;
+      Gr = t2;
+    }
+    const double am = a0 + (double)b;
+    const double Fr = t * fma(am, Gr, -krr * Gi);
+    const double Fi = t * fma(am, Gi, krr * Gr);
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      fr[b][s] = fma(Fr, ps[s], fr[b][s]);
+      fi[b][s] = fma(Fi, ps[s], fi[b][s]);
+      gr[b][s] = fma(Gr, ps[s], gr[b][s]);
+      gi[b][s] = fma(Gi, ps[s], gi[b][s]);
+    }
+  }
+}
+
 // Flush of one step's finished nodes for the KB wavenumbers of a block (see the
 // single-k kernel): slot contents to F_k, G_k columns, zeroing the slots.
 template <int KB, int W>
@@ -129,7 +192,9 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
   }
 }
 
-template <int NP, int KB, int W>
+// With TAY the KB accumulator sets of a block hold the k-derivative orders
+// kb*KB .. kb*KB+KB-1 at the one wavenumber sw.k_re + i sw.k_im instead of KB wavenumbers.
+template <int NP, int KB, int W, bool TAY>
 __global__ void __launch_bounds__(64 * W)
 k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   extern __shared__ double lds[];
@@ -163,10 +228,14 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const bool first_kb = kb == 0;
   double kv[KB];
   const int k0 = kb * KB;
+  bool rec = false;
+  double dk = 0.0;
+  if constexpr (!TAY) {
 #pragma unroll
-  for (int q = 0; q < KB; ++q) kv[q] = sw.k[min(k0 + q, sw.nk - 1)];
-  const bool rec = sw.block_uniform[kb] != 0;
-  const double dk = KB > 1 ? kv[1] - kv[0] : 0.0;
+    for (int q = 0; q < KB; ++q) kv[q] = sw.k[min(k0 + q, sw.nk - 1)];
+    rec = sw.block_uniform[kb] != 0;
+    dk = KB > 1 ? kv[1] - kv[0] : 0.0;
+  }
   double csum = 0.0;
   const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
   int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * W + wave) : -1;
@@ -197,13 +266,24 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         const int Q = a.len[c];
         double cs_local = 0.0;
         // point 0 was fetched before the barrier; later rows are loaded in turn
-        sweep_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi, cs_local,
-                                  s_trig);
-        for (int q = 1; q < Q; ++q) {
-          const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
-          const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-          sweep_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, fr, fi, gr, gi,
+        if constexpr (TAY) {
+          taylor_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, sw.k_re, sw.k_im, k0, fr, fi, gr,
+                                     gi, cs_local, s_trig);
+          for (int q = 1; q < Q; ++q) {
+            const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
+            const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+            taylor_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, sw.k_re, sw.k_im, k0, fr,
+                                       fi, gr, gi, cs_local, s_trig);
+          }
+        } else {
+          sweep_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi,
                                     cs_local, s_trig);
+          for (int q = 1; q < Q; ++q) {
+            const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
+            const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+            sweep_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, fr, fi, gr,
+                                      gi, cs_local, s_trig);
+          }
         }
         csum += cs_local;
         const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
@@ -271,7 +351,8 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   if (first_kb && active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
 }
 
-// Mid-edge pairs for every k: one thread per (pair, k).
+// Mid-edge pairs for every k (with TAY: every derivative order): one thread per (pair, k).
+template <bool TAY>
 __global__ void __launch_bounds__(256)
 k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const int64_t n = min((int64_t)*sa.mid_count, sa.mid_cap);
@@ -290,12 +371,16 @@ k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
     for (int s = 0; s < 6; ++s) fr[0][s] = fi[0][s] = gr[0][s] = gi[0][s] = 0.0;
     const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
     const double* ph = a.phi + c * a.Qmax * 6;
-    const double kk = sw.k[kidx];
+    const double kk = TAY ? sw.k_re : sw.k[kidx];
     for (int q = 0; q < a.len[c]; ++q) {
       const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
       const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-      sweep_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi,
-                               csum, nullptr);
+      if constexpr (TAY)
+        taylor_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, kk, sw.k_im, kidx, fr, fi, gr,
+                                  gi, csum, nullptr);
+      else
+        sweep_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi,
+                                 csum, nullptr);
     }
     const int* cols = a.ptopo + (size_t)e * 6;
 #pragma unroll
@@ -360,7 +445,16 @@ __global__ void k_block_uniform(const double* __restrict__ k, int nk, int KB, in
   out[kb] = ok;
 }
 
-size_t g_sweep_attr[2] = {0, 0};
+size_t g_sweep_attr[4] = {0, 0, 0, 0};
+
+template <int W>
+const void* sweep_kernel(int NP, bool taylor) {
+  if (NP == 3)
+    return taylor ? reinterpret_cast<const void*>(k_sweep_slots<3, 4, W, true>)
+                  : reinterpret_cast<const void*>(k_sweep_slots<3, 4, W, false>);
+  return taylor ? reinterpret_cast<const void*>(k_sweep_slots<6, 2, W, true>)
+                : reinterpret_cast<const void*>(k_sweep_slots<6, 2, W, false>);
+}
 
 }  // namespace
 
@@ -377,13 +471,12 @@ size_t sweep_shmem(int NP, int nphi, int S) {
          sizeof(int2) * kMidLds + 16;
 }
 
-void sweep_prepare_attributes(int NP, int nphi) {
+void sweep_prepare_attributes(int NP, int nphi, bool taylor) {
   const size_t shmem =
       sweep_shmem(NP, nphi, fitting_slots(sweep_fixed_doubles(nphi), sweep_kb(NP)));
   constexpr int W = kSlotWaves;
-  const void* kern = NP == 3 ? reinterpret_cast<const void*>(k_sweep_slots<3, 4, W>)
-                             : reinterpret_cast<const void*>(k_sweep_slots<6, 2, W>);
-  size_t& done = g_sweep_attr[NP == 3 ? 0 : 1];
+  const void* kern = sweep_kernel<W>(NP, taylor);
+  size_t& done = g_sweep_attr[(NP == 3 ? 0 : 1) + (taylor ? 2 : 0)];
   if (shmem > done) {
     BIE_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)std::max(shmem, (size_t)(144 * 1024))));
@@ -402,18 +495,29 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many sweep tiles");
   const size_t shmem = sweep_shmem(a.NP, nphi, sa.S);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "sweep kernel LDS request exceeds 160 KiB");
-  sweep_prepare_attributes(a.NP, nphi);
+  const bool tay = sw.taylor != 0;
+  sweep_prepare_attributes(a.NP, nphi, tay);
   constexpr int W = kSlotWaves;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * 8)), dim3(64 * W), shmem, s, a, sa, sw);
   };
-  if (a.NP == 3)
-    go(k_sweep_slots<3, 4, W>);
-  else
-    go(k_sweep_slots<6, 2, W>);
+  if (a.NP == 3) {
+    if (tay)
+      go(k_sweep_slots<3, 4, W, true>);
+    else
+      go(k_sweep_slots<3, 4, W, false>);
+  } else {
+    if (tay)
+      go(k_sweep_slots<6, 2, W, true>);
+    else
+      go(k_sweep_slots<6, 2, W, false>);
+  }
   BIE_HIP_CHECK(hipGetLastError());
   if (a.NP == 6) {
-    hipLaunchKernelGGL(k_sweep_mid, dim3(4096), dim3(256), 0, s, a, sa, sw);
+    if (tay)
+      hipLaunchKernelGGL(k_sweep_mid<true>, dim3(4096), dim3(256), 0, s, a, sa, sw);
+    else
+      hipLaunchKernelGGL(k_sweep_mid<false>, dim3(4096), dim3(256), 0, s, a, sa, sw);
     BIE_HIP_CHECK(hipGetLastError());
   }
 }

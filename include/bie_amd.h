@@ -119,6 +119,31 @@ int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
 int bie_graph_launch(bie_graph* g, void* stream);
 int bie_graph_destroy(bie_graph* g);
 
+/* ---- Wavenumber derivatives (Taylor expansion in k) ------------------------------------
+ * d^q F / dk^q and d^q G / dk^q for q = 0 .. n_orders-1 (n_orders <= 64) at one complex
+ * wavenumber k_re + i k_im: replaces taylor_assemble! (src/3d/rosebem.jl:56-157, kernels
+ * taylor_greens3d! / taylor_freens3d!, src/3d/kernels.jl:94-118) with U = I.  Order q goes
+ * to dF + 2*q*stride doubles (stride >= n_targets * n_sources complex elements), column-
+ * major n_targets x n_sources: the reference's F[:, :, q+1].  Likewise G; C (order
+ * independent) once to dC.  Order 0 equals bie_assemble_device.  The reference defines the
+ * method for linear triangles; quadratic meshes are accepted as well.  Same rule selection,
+ * tables and kernel schedule as the frequency sweep (one e^{ikr} per point serves every
+ * order).  Asynchronous on stream. */
+int bie_assemble_taylor_device(bie_dmesh* dm, double k_re, double k_im, int32_t n_orders,
+                               const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
+                               uint32_t flags, double* dF, double* dG, int64_t stride,
+                               double* dC, void* stream);
+
+/* Y = sum_t coef_t A_t for t = 0 .. n_terms-1 (1 <= n_terms <= 64): A_t are n complex128
+ * entries at dA + 2*t*strideA doubles (strideA >= n when n_terms > 1), coef 2*n_terms
+ * doubles (interleaved complex) in host memory, read during the call.  With coef_t =
+ * (k - k0)^t / t! this is apply_taylor_expansion (src/3d/rosebem.jl:164-199) on the
+ * derivative matrices of bie_assemble_taylor_device (or on right-hand-side vectors).  dY may
+ * equal dA (the order-0 array is then overwritten); no other overlap.  Asynchronous on
+ * stream. */
+int bie_taylor_expand_device(int32_t n_terms, int64_t n, const double* dA, int64_t strideA,
+                             const double* coef, double* dY, void* stream);
+
 /* ---- Uneven batch ---------------------------------------------------------------------
  * Collocation assembly (targets = each mesh's sources) of many meshes of different sizes,
  * each with its own wavenumber, in one persistent kernel launch: every workgroup stages

@@ -101,6 +101,29 @@ def test_apply_rejects_leading_dimension_beyond_buffer_offsets():
     assert b"leading dimension too large" in lib.bie_last_error()
 
 
+def test_taylor_entry_points_validate_arguments_on_the_host():
+    # every check below runs before any HIP call (the pointers are never used)
+    p = ctypes.c_void_p(0x1000)
+    lib = _lib.lib()
+    coef = (ctypes.c_double * 4)(1.0, 0.0, 0.5, 0.0)
+    assert lib.bie_taylor_expand_device(0, 10, p, 10, coef, p, None) == _lib.BIE_EINVAL
+    assert lib.bie_taylor_expand_device(65, 10, p, 10, coef, p, None) == _lib.BIE_EINVAL
+    assert lib.bie_taylor_expand_device(2, 10, p, 9, coef, p, None) == _lib.BIE_EINVAL
+    assert b"stride" in lib.bie_last_error()
+    assert lib.bie_taylor_expand_device(2, 10, p, 10, None, p, None) == _lib.BIE_EINVAL
+    bad = (ctypes.c_double * 4)(1.0, 0.0, float("nan"), 0.0)
+    assert lib.bie_taylor_expand_device(2, 10, p, 10, bad, p, None) == _lib.BIE_EINVAL
+    assert lib.bie_assemble_taylor_device(None, 1.0, 0.0, 3, None, 4, 3, 3, 7, p, p, 16, p,
+                                          None) == _lib.BIE_EINVAL
+    from bie_amd.taylor import taylor_assemble, taylor_coefficients
+    with pytest.raises(ValueError):
+        taylor_assemble(M.icosphere(1, "linear"), 1.0, M=64)
+    c = taylor_coefficients(1.5 + 0.5j, 1.0, 25)
+    dk = 0.5 + 0.5j
+    assert c[0] == 1.0 and abs(c[1] - dk) < 1e-16
+    assert abs(c[24] - dk ** 24 / 620448401733239439360000) < 1e-16 * abs(c[24])
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
