@@ -204,6 +204,68 @@ def measure_adjoint(torch, reps=2):
                 2 * (n_coord + 1) * min(t_obj) / min(t_grad)}
 
 
+def measure_taylor(torch, bie_amd, steps=3, f=20, M=25, n=2, m=2):
+    """k-derivative operators as the reference's ROSEBEM example forms them
+    (examples/3d_rosebem.jl:53-61: k0 = 2 pi 225 / 343, M = 25, (n,m) = (2,2), gOn = false)
+    on the 4002-node linear sphere: one bie_assemble_taylor_device call for the 26 orders of
+    F, against one single-k F assembly; then apply_taylor_expansion of the 26 matrices at a
+    new k (one streaming pass), checked against the assembly at that k."""
+    from bie_amd.taylor import assemble_taylor_device, taylor_coefficients, taylor_expand_device
+    mesh = bie_amd.icosphere(f, "linear")
+    N = mesh.n_sources
+    dm = bie_amd.DeviceMesh(mesh, 0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    k0 = 2 * 3.141592653589793 * 225 / 343
+    k1 = 2 * 3.141592653589793 * 250 / 343
+    nord = M + 1
+    stride = N * N
+    Fq = torch.empty(nord * 2 * stride, dtype=torch.float64, device=dev)
+    C = torch.empty(2 * N, dtype=torch.float64, device=dev)
+    F1 = torch.empty(2 * stride, dtype=torch.float64, device=dev)
+    G1 = torch.empty_like(F1)  # the single-k entry takes a G buffer even with gOn = false
+    Y = torch.empty_like(F1)
+    coef = taylor_coefficients(k1, k0, nord)
+
+    def orders():
+        assemble_taylor_device(dm, k0, nord, Fq.data_ptr(), 0, stride, C.data_ptr(), n=n, m=m,
+                               gOn=False, stream=stream)
+
+    def single():
+        dm.assemble(k1, F1.data_ptr(), G1.data_ptr(), C.data_ptr(), n=n, m=m, gOn=False,
+                    stream=stream)
+
+    def expand():
+        taylor_expand_device(nord, stride, Fq.data_ptr(), stride, coef, Y.data_ptr(), stream)
+
+    res = {}
+    for name, fn in (("orders", orders), ("single_k", single), ("expand", expand)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name + "_ms"] = e0.elapsed_time(e1) / steps
+    # F1 holds the assembly at k1 (single() ran last before expand), Y the expansion at k1
+    res["max_rel_err_expansion_vs_assembly"] = float(torch.linalg.norm(Y - F1) /
+                                                     torch.linalg.norm(F1))
+    res["per_order_ms"] = res["orders_ms"] / nord
+    res["orders_entries_per_s"] = nord * N * N / (res["orders_ms"] * 1e-3)
+    gbs = (nord + 1) * 16 * N * N / (res["expand_ms"] * 1e-3) / 1e9
+    res["expand_GB_per_s"] = gbs
+    res["expand_frac_of_hbm_peak"] = gbs / 8000.0
+    res["workload"] = (f"linear geodesic sphere N={N}, E={mesh.n_elements}, k0 = 2 pi 225/343, "
+                       f"orders 0..{M} of F (gOn=false), (n,m)=({n},{m}); expansion at "
+                       f"k = 2 pi 250/343")
+    del Fq
+    torch.cuda.empty_cache()
+    return res
+
+
 def measure_batch(torch, bie_amd, steps=3, n=4, m=4):
     """Config 5: 48 ellipsoids (semi-axes U[0.5, 2], N log-uniform in [500, 8000] rounded to
     10 f^2 + 2, k U[1, 20], seed 3), complex128 F, G, C each, (n,m)=(4,4).  One tuned
@@ -291,12 +353,14 @@ def main():
                     help="timed CPU runs after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--only", choices=["sweep", "batch", "adjoint", "taylor"],
+                    help="run only this secondary workload and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3 times (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
-                                   "row blocks is described in DESIGN.md but not launched"}))
+        print(json.dumps({"error": "this build runs on one GPU; sharding the row blocks "
+                                   "across GPUs is designed but not launched"}))
         sys.exit(2)
 
     import numpy as np
@@ -313,6 +377,13 @@ def main():
     import bie_amd
     from bie_amd import _lib
     from bie_amd.rules import rule_size
+    if args.only:
+        legs = {"sweep": lambda: measure_sweep(torch, bie_amd),
+                "batch": lambda: measure_batch(torch, bie_amd),
+                "adjoint": lambda: measure_adjoint(torch),
+                "taylor": lambda: measure_taylor(torch, bie_amd)}
+        print(json.dumps({"only": args.only, **legs[args.only]()}), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -389,6 +460,7 @@ def main():
         extra["sweep"] = measure_sweep(torch, bie_amd)
         extra["uneven_batch"] = measure_batch(torch, bie_amd)
         extra["adjoint"] = measure_adjoint(torch)
+        extra["taylor"] = measure_taylor(torch, bie_amd)
         G = torch.empty(2 * N * N, dtype=torch.float64, device=dev)
         run(n, m, 1, 0)
 

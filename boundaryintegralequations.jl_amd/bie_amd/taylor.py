@@ -95,14 +95,16 @@ def taylor_assemble(mesh: Mesh3d, k, in_sources=None, *, M=0, fOn=True, gOn=True
             raise ValueError("U must be n_sources x r, with the collocation targets")
     L = _lib.lib()
     dm = DeviceMesh(mesh, device)
-    dF = _Dev(16 * nt * N * n_orders)
-    dG = _Dev(16 * nt * N * n_orders)
+    # disabled outputs get no device buffer; they are returned as zeros (rosebem.jl:98-100)
+    dF = _Dev(16 * nt * N * n_orders) if fOn else None
+    dG = _Dev(16 * nt * N * n_orders) if gOn else None
     dC = _Dev(16 * nt)
     dX = None
     if targets is not None:
         dX = _Dev(targets.nbytes)
         _lib.check(L.bie_memcpy_h2d(ctypes.c_void_p(dX.ptr), _lib._ptr(targets), targets.nbytes))
-    assemble_taylor_device(dm, k, n_orders, dF.ptr, dG.ptr, nt * N, dC.ptr,
+    assemble_taylor_device(dm, k, n_orders, dF.ptr if dF else 0, dG.ptr if dG else 0,
+                           nt * N, dC.ptr,
                            targets_ptr=dX.ptr if dX else 0, n_targets=nt, n=n, m=m, fOn=fOn,
                            gOn=gOn, cOn=cOn)
     _lib.check(L.bie_device_synchronize())
@@ -114,13 +116,16 @@ def taylor_assemble(mesh: Mesh3d, k, in_sources=None, *, M=0, fOn=True, gOn=True
         dU, dUH, dT = _Dev(U.nbytes), _Dev(UH.nbytes), _Dev(16 * nt * r)
         _lib.check(L.bie_memcpy_h2d(ctypes.c_void_p(dU.ptr), _lib._ptr(U), U.nbytes))
         _lib.check(L.bie_memcpy_h2d(ctypes.c_void_p(dUH.ptr), _lib._ptr(UH), UH.nbytes))
-        return (_reduce(L, dF, nt, N, n_orders, U, dU, dUH, dT),
-                _reduce(L, dG, nt, N, n_orders, U, dU, dUH, dT), C)
-    F = np.empty((nt, N, n_orders), dtype=np.complex128, order="F")
-    G = np.empty((nt, N, n_orders), dtype=np.complex128, order="F")
-    _lib.check(L.bie_memcpy_d2h(_lib._ptr(F), ctypes.c_void_p(dF.ptr), F.nbytes))
-    _lib.check(L.bie_memcpy_d2h(_lib._ptr(G), ctypes.c_void_p(dG.ptr), G.nbytes))
-    return F, G, C
+        zero = np.zeros((r, r, n_orders), dtype=np.complex128, order="F")
+        return (_reduce(L, dF, nt, N, n_orders, U, dU, dUH, dT) if dF else zero,
+                _reduce(L, dG, nt, N, n_orders, U, dU, dUH, dT) if dG else zero.copy(), C)
+    out = []
+    for d in (dF, dG):
+        X = np.zeros((nt, N, n_orders), dtype=np.complex128, order="F")
+        if d:
+            _lib.check(L.bie_memcpy_d2h(_lib._ptr(X), ctypes.c_void_p(d.ptr), X.nbytes))
+        out.append(X)
+    return out[0], out[1], C
 
 
 def _expand_host(basis, k, k0):

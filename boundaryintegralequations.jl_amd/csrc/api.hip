@@ -490,7 +490,9 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   BIE_REQUIRE(taylor || nk == 0 || d_k, BIE_EINVAL, "k list is NULL");
   BIE_REQUIRE(!taylor || (std::isfinite(k_re) && std::isfinite(k_im)), BIE_EINVAL,
               "wavenumber is not finite");
-  BIE_REQUIRE(dF && dG && dC, BIE_EINVAL, "output pointer is NULL");
+  // a disabled F or G may be NULL (then nothing is written there)
+  BIE_REQUIRE(((flags & BIE_F) == 0 || dF) && ((flags & BIE_G) == 0 || dG) && dC, BIE_EINVAL,
+              "output pointer is NULL");
   BIE_REQUIRE(stride >= M * dm->N, BIE_EINVAL, "stride smaller than M * N");
   if (d_targets == nullptr) {
     d_targets = dm->d_sources;
@@ -522,7 +524,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
     const bool on = (flags & (which == 0 ? BIE_F : BIE_G)) != 0;
     if (on)
       launch_zero_columns_multi(X, stride, nk, M, sc.d_zero_cols, sc.zero_cols, s);
-    else
+    else if (X)
       for (int q = 0; q < nk; ++q) launch_fill_zero(X + 2 * (size_t)q * stride, 2 * M * dm->N, s);
   }
   if (M == 0 || dm->E == 0 || nk == 0) return;

@@ -105,7 +105,8 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
  * at dF + 2*q*stride doubles (stride >= n_targets * n_sources complex elements); likewise
  * G_q.  C is k-independent and written once to dC.  Everything k-independent (rule
  * selection, geometry, table loads, 1/r, n.d) is shared by the wavenumbers of a block, and
- * uniformly spaced blocks replace sincos by a recurrence.  Asynchronous on stream. */
+ * uniformly spaced blocks replace sincos by a recurrence.  A disabled F / G (flag off) may
+ * be NULL.  Asynchronous on stream. */
 int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
                               const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
                               uint32_t flags, double* dF, double* dG, int64_t stride,
@@ -128,7 +129,7 @@ int bie_graph_destroy(bie_graph* g);
  * independent) once to dC.  Order 0 equals bie_assemble_device.  The reference defines the
  * method for linear triangles; quadratic meshes are accepted as well.  Same rule selection,
  * tables and kernel schedule as the frequency sweep (one e^{ikr} per point serves every
- * order).  Asynchronous on stream. */
+ * order).  A disabled F / G (flag off) may be NULL.  Asynchronous on stream. */
 int bie_assemble_taylor_device(bie_dmesh* dm, double k_re, double k_im, int32_t n_orders,
                                const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
                                uint32_t flags, double* dF, double* dG, int64_t stride,
