@@ -91,7 +91,7 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
 template <int NP, int KB, bool TAB>
 __device__ __forceinline__ void taylor_point(double x0, double x1, double x2, const double4 A,
                                              const double4 B, const double* __restrict__ p,
-                                             double kr, double ki, int m0,
+                                             double kr, double ki, int m0, bool doG,
                                              double (&fr)[KB][NP], double (&fi)[KB][NP],
                                              double (&gr)[KB][NP], double (&gi)[KB][NP],
                                              double& csum, const double2* trig) {
@@ -110,7 +110,12 @@ __device__ __forceinline__ void taylor_point(double x0, double x1, double x2, co
     sincos_f64(kr * r, &sn, &cs);
   double gm = g;
   if (ki != 0.0) gm *= exp(-ki * r);
-  for (int j = 0; j < m0; ++j) gm *= r;  // r^m0; m0 is the same for the whole wave
+  // r^m0 by binary powering (m0 is the same for the whole wave, m0 < 64)
+  double rb = r;
+  for (int e = m0; e != 0; e >>= 1) {
+    if (e & 1) gm *= rb;
+    rb *= rb;
+  }
   double Gr = gm * cs, Gi = gm * sn;
   if (m0 & 1) {  // times i^m0
     const double t2 = -Gi;
@@ -140,8 +145,13 @@ __device__ __forceinline__ void taylor_point(double x0, double x1, double x2, co
     for (int s = 0; s < NP; ++s) {
       fr[b][s] = fma(Fr, ps[s], fr[b][s]);
       fi[b][s] = fma(Fi, ps[s], fi[b][s]);
-      gr[b][s] = fma(Gr, ps[s], gr[b][s]);
-      gi[b][s] = fma(Gi, ps[s], gi[b][s]);
+    }
+    if (doG) {  // the same for every lane: F-only stacks (gOn = false) skip this work
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        gr[b][s] = fma(Gr, ps[s], gr[b][s]);
+        gi[b][s] = fma(Gi, ps[s], gi[b][s]);
+      }
     }
   }
 }
@@ -226,6 +236,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   const bool first_kb = kb == 0;
+  const bool doG = (a.flags & BIE_G) != 0;
   double kv[KB];
   const int k0 = kb * KB;
   bool rec = false;
@@ -267,13 +278,13 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         double cs_local = 0.0;
         // point 0 was fetched before the barrier; later rows are loaded in turn
         if constexpr (TAY) {
-          taylor_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, sw.k_re, sw.k_im, k0, fr, fi, gr,
-                                     gi, cs_local, s_trig);
+          taylor_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, sw.k_re, sw.k_im, k0, doG, fr, fi,
+                                     gr, gi, cs_local, s_trig);
           for (int q = 1; q < Q; ++q) {
             const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
             const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-            taylor_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, sw.k_re, sw.k_im, k0, fr,
-                                       fi, gr, gi, cs_local, s_trig);
+            taylor_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, sw.k_re, sw.k_im, k0, doG,
+                                       fr, fi, gr, gi, cs_local, s_trig);
           }
         } else {
           sweep_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi,
@@ -294,8 +305,10 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
             double* p = acc + ((size_t)sload(so + s) * KB + q2) * 256 + lane;
             atomicAdd(p, fr[q2][s]);
             atomicAdd(p + 64, fi[q2][s]);
-            atomicAdd(p + 128, gr[q2][s]);
-            atomicAdd(p + 192, gi[q2][s]);
+            if (!TAY || doG) {
+              atomicAdd(p + 128, gr[q2][s]);
+              atomicAdd(p + 192, gi[q2][s]);
+            }
           }
       }
       if (NP == 6 && first_kb) {  // mid-edge pairs are queued once, for every k
@@ -376,8 +389,8 @@ k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
       const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
       const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
       if constexpr (TAY)
-        taylor_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, kk, sw.k_im, kidx, fr, fi, gr,
-                                  gi, csum, nullptr);
+        taylor_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, kk, sw.k_im, kidx, doG, fr, fi,
+                                  gr, gi, csum, nullptr);
       else
         sweep_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi,
                                  csum, nullptr);
