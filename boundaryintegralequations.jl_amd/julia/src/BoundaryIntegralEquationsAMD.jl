@@ -372,4 +372,81 @@ function scattering_objective(mesh::BIE.Mesh3d, k::Real, field_points::AbstractM
     return real(sum(abs2, u))
 end
 
+# ---------------------------------------------------------------------------------------
+# Wavenumber derivatives and their Taylor expansion (src/3d/rosebem.jl)
+# ---------------------------------------------------------------------------------------
+
+"""
+    taylor_assemble!(mesh, k, in_sources, shape_function; M=0, fOn=true, gOn=true, cOn=true,
+                     m=3, n=3, progress=true, U=I) -> (From, Grom, C)
+
+Same signature and defaults as the reference's `taylor_assemble!` (src/3d/rosebem.jl:56-157):
+`From[:, :, q+1]` is the q-th k-derivative of F for q = 0..M, all orders from one device call
+(bie_assemble_taylor_device), reduced to `U' * F_q * U` on the host when a basis U is given.
+Unsupported meshes and M > 63 go to the reference method.
+"""
+function taylor_assemble!(mesh::BIE.Mesh3d, k, in_sources, shape_function; M=0, fOn=true,
+                          gOn=true, cOn=true, m=3, n=3, progress=true, U=LinearAlgebra.I)
+    if !supported(mesh) || M > 63
+        return BIE.taylor_assemble!(mesh, k, in_sources, shape_function; M=M, fOn=fOn,
+                                    gOn=gOn, cOn=cOn, m=m, n=n, progress=progress, U=U)
+    end
+    dm = DeviceMesh(mesh)
+    N = n_sources(dm)
+    dX = DeviceBuffer(Matrix{Float64}(in_sources))
+    nt = size(in_sources, 2)
+    nord = M + 1
+    dF = fOn ? DeviceBuffer(16 * nt * N * nord) : nothing
+    dG = gOn ? DeviceBuffer(16 * nt * N * nord) : nothing
+    dC = DeviceBuffer(16 * nt)
+    kc = ComplexF64(k)
+    check(ccall((:bie_assemble_taylor_device, libbie[]), Cint,
+                (Ptr{Cvoid}, Cdouble, Cdouble, Int32, Ptr{Cvoid}, Int64, Int32, Int32, UInt32,
+                 Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
+                dm.handle, real(kc), imag(kc), Int32(nord), dX.ptr, nt, Int32(n), Int32(m),
+                flagbits(fOn, gOn, cOn), fOn ? dF.ptr : C_NULL, gOn ? dG.ptr : C_NULL,
+                Int64(nt) * N, dC.ptr, C_NULL))
+    F = fOn ? download!(Array{ComplexF64}(undef, nt, N, nord), dF) : zeros(ComplexF64, nt, N, nord)
+    G = gOn ? download!(Array{ComplexF64}(undef, nt, N, nord), dG) : zeros(ComplexF64, nt, N, nord)
+    C = download!(Vector{ComplexF64}(undef, nt), dC)
+    U isa LinearAlgebra.UniformScaling && return F, G, C
+    From = cat((U' * F[:, :, q] * U for q in 1:nord)...; dims=3)
+    Grom = cat((U' * G[:, :, q] * U for q in 1:nord)...; dims=3)
+    return From, Grom, C
+end
+
+taylor_weight(k, k0, i) = i > 20 ? ComplexF64((k - k0)^i / factorial(Int128(i))) :
+                                   ComplexF64((k - k0)^i / factorial(i))
+
+function expand_device(basis::AbstractArray, k, k0)
+    B = Array{ComplexF64}(basis)
+    nt = size(B, ndims(B))
+    n = length(B) ÷ nt
+    coef = ComplexF64[taylor_weight(k, k0, i) for i in 0:nt-1]
+    dA = DeviceBuffer(B)
+    dY = DeviceBuffer(16 * n)
+    GC.@preserve coef check(ccall((:bie_taylor_expand_device, libbie[]), Cint,
+                                  (Int32, Int64, Ptr{Cvoid}, Int64, Ptr{ComplexF64}, Ptr{Cvoid},
+                                   Ptr{Cvoid}),
+                                  Int32(nt), n, dA.ptr, n, coef, dY.ptr, C_NULL))
+    return download!(Array{ComplexF64}(undef, size(B)[1:end-1]...), dY)
+end
+
+"""
+    apply_taylor_expansion(Abasis, k, k0[, bbasis])
+
+The reference's `apply_taylor_expansion` (src/3d/rosebem.jl:164-199) as one streaming pass
+on the device (bie_taylor_expand_device): sum over i of (k - k0)^i / i! Abasis[:, :, i+1]
+(and the same for the vectors bbasis).  More than 64 terms go to the reference method.
+"""
+function apply_taylor_expansion(Abasis, k, k0)
+    size(Abasis, 3) > 64 && return BIE.apply_taylor_expansion(Abasis, k, k0)
+    return expand_device(Abasis, k, k0)
+end
+
+function apply_taylor_expansion(Abasis, bbasis, k, k0)
+    size(Abasis, 3) > 64 && return BIE.apply_taylor_expansion(Abasis, bbasis, k, k0)
+    return expand_device(Abasis, k, k0), expand_device(bbasis, k, k0)
+end
+
 end # module
