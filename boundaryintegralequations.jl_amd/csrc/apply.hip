@@ -326,7 +326,8 @@ struct TaylorCoefs {
 };
 
 // One complex entry per thread and pass: the n_terms loads are independent, so a wave keeps
-// several in flight; Y may be A_0 itself (each entry is read before it is written, by the
+// several in flight; every byte is touched once, so loads and stores bypass the caches
+// (non-temporal).  Y may be A_0 itself (each entry is read before it is written, by the
 // same thread).
 __global__ void __launch_bounds__(256)
 k_taylor_expand(int nt, int64_t n, const double2* A, int64_t strideA, const TaylorCoefs cf,
@@ -336,12 +337,15 @@ k_taylor_expand(int nt, int64_t n, const double2* A, int64_t strideA, const Tayl
     double yr = 0.0, yi = 0.0;
 #pragma unroll 4
     for (int t = 0; t < nt; ++t) {
-      const double2 v = A[(size_t)t * strideA + e];
+      const double* pa = reinterpret_cast<const double*>(A + (size_t)t * strideA + e);
+      const double vx = __builtin_nontemporal_load(pa), vy = __builtin_nontemporal_load(pa + 1);
       const double cr = cf.c[2 * t], ci = cf.c[2 * t + 1];
-      yr = fma(cr, v.x, fma(-ci, v.y, yr));
-      yi = fma(cr, v.y, fma(ci, v.x, yi));
+      yr = fma(cr, vx, fma(-ci, vy, yr));
+      yi = fma(cr, vy, fma(ci, vx, yi));
     }
-    Y[e] = make_double2(yr, yi);
+    double* py = reinterpret_cast<double*>(Y + e);
+    __builtin_nontemporal_store(yr, py);
+    __builtin_nontemporal_store(yi, py + 1);
   }
 }
 
@@ -364,7 +368,7 @@ void taylor_expand_device(int n_terms, int64_t n, const double* dA, int64_t stri
   int dev = 0, cus = 0;
   BIE_HIP_CHECK(hipGetDevice(&dev));
   BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16LL * cus));
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 64LL * cus));
   hipLaunchKernelGGL(k_taylor_expand, dim3((unsigned)blocks), dim3(256), 0, s, n_terms, n,
                      reinterpret_cast<const double2*>(dA), strideA, cf,
                      reinterpret_cast<double2*>(dY));
