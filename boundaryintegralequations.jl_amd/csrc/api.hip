@@ -1030,7 +1030,8 @@ int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
       // build every cache and set kernel attributes outside the capture
       sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC, cs,
                         true);
-      sweep_prepare_attributes(dm->pk, dm->rs.R * dm->rs.Qmax * dm->pk, false);
+      sweep_prepare_attributes(dm->pk, dm->rs.R * dm->rs.Qmax * dm->pk, false,
+                               (flags & BIE_G) != 0);
       BIE_HIP_CHECK(hipStreamSynchronize(cs));
       BIE_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {

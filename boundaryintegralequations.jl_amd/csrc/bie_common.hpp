@@ -217,7 +217,7 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
 int sweep_kb(int NP);
 void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw,
                   hipStream_t s);
-void sweep_prepare_attributes(int NP, int nphi, bool taylor);
+void sweep_prepare_attributes(int NP, int nphi, bool taylor, bool gon);
 void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s);
 void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,

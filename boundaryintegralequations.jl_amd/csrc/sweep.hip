@@ -29,7 +29,7 @@ namespace {
 template <int NP, int KB, bool TAB>
 __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, const double4 A,
                                             const double4 B, const double* __restrict__ p,
-                                            const double* kv, double dk, bool rec,
+                                            const double* kv, double dk, bool rec, bool doG,
                                             double (&fr)[KB][NP], double (&fi)[KB][NP],
                                             double (&gr)[KB][NP], double (&gi)[KB][NP],
                                             double& csum, const double2* trig) {
@@ -77,8 +77,13 @@ __device__ __forceinline__ void sweep_point(double x0, double x1, double x2, con
     for (int s = 0; s < NP; ++s) {
       fr[b][s] = fma(Fr, ps[s], fr[b][s]);
       fi[b][s] = fma(Fi, ps[s], fi[b][s]);
-      gr[b][s] = fma(Gr, ps[s], gr[b][s]);
-      gi[b][s] = fma(Gi, ps[s], gi[b][s]);
+    }
+    if (doG) {  // the same for every lane: F-only sweeps (gOn = false) skip this work
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        gr[b][s] = fma(Gr, ps[s], gr[b][s]);
+        gi[b][s] = fma(Gi, ps[s], gi[b][s]);
+      }
     }
   }
 }
@@ -204,7 +209,9 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
 
 // With TAY the KB accumulator sets of a block hold the k-derivative orders
 // kb*KB .. kb*KB+KB-1 at the one wavenumber sw.k_re + i sw.k_im instead of KB wavenumbers.
-template <int NP, int KB, int W, bool TAY>
+// GON is whether G is formed (the BIE_G flag): F-only launches skip the G accumulation and
+// its LDS atomics in a separate instance, leaving the F-and-G code as it is.
+template <int NP, int KB, int W, bool TAY, bool GON>
 __global__ void __launch_bounds__(64 * W)
 k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   extern __shared__ double lds[];
@@ -236,7 +243,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const int64_t ic = active ? i : (a.M - 1);
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   const bool first_kb = kb == 0;
-  const bool doG = (a.flags & BIE_G) != 0;
+  constexpr bool doG = GON;
   double kv[KB];
   const int k0 = kb * KB;
   bool rec = false;
@@ -287,13 +294,13 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
                                        fr, fi, gr, gi, cs_local, s_trig);
           }
         } else {
-          sweep_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, kv, dk, rec, fr, fi, gr, gi,
+          sweep_point<NP, KB, true>(x0, x1, x2, A0, B0, ph, kv, dk, rec, doG, fr, fi, gr, gi,
                                     cs_local, s_trig);
           for (int q = 1; q < Q; ++q) {
             const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
             const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
-            sweep_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, fr, fi, gr,
-                                      gi, cs_local, s_trig);
+            sweep_point<NP, KB, true>(x0, x1, x2, A, B, ph + q * NP, kv, dk, rec, doG, fr, fi,
+                                      gr, gi, cs_local, s_trig);
           }
         }
         csum += cs_local;
@@ -305,7 +312,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
             double* p = acc + ((size_t)sload(so + s) * KB + q2) * 256 + lane;
             atomicAdd(p, fr[q2][s]);
             atomicAdd(p + 64, fi[q2][s]);
-            if (!TAY || doG) {
+            if (doG) {
               atomicAdd(p + 128, gr[q2][s]);
               atomicAdd(p + 192, gi[q2][s]);
             }
@@ -392,8 +399,8 @@ k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         taylor_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, kk, sw.k_im, kidx, doG, fr, fi,
                                   gr, gi, csum, nullptr);
       else
-        sweep_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, fr, fi, gr, gi,
-                                 csum, nullptr);
+        sweep_point<6, 1, false>(x0, x1, x2, A, B, ph + q * 6, &kk, 0.0, false, doG, fr, fi,
+                                 gr, gi, csum, nullptr);
     }
     const int* cols = a.ptopo + (size_t)e * 6;
 #pragma unroll
@@ -458,15 +465,20 @@ __global__ void k_block_uniform(const double* __restrict__ k, int nk, int KB, in
   out[kb] = ok;
 }
 
-size_t g_sweep_attr[4] = {0, 0, 0, 0};
+size_t g_sweep_attr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+template <int NP, int KB, int W>
+const void* sweep_kernel_np(bool taylor, bool gon) {
+  if (taylor)
+    return gon ? reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, true, true>)
+               : reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, true, false>);
+  return gon ? reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, false, true>)
+             : reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, false, false>);
+}
 
 template <int W>
-const void* sweep_kernel(int NP, bool taylor) {
-  if (NP == 3)
-    return taylor ? reinterpret_cast<const void*>(k_sweep_slots<3, 4, W, true>)
-                  : reinterpret_cast<const void*>(k_sweep_slots<3, 4, W, false>);
-  return taylor ? reinterpret_cast<const void*>(k_sweep_slots<6, 2, W, true>)
-                : reinterpret_cast<const void*>(k_sweep_slots<6, 2, W, false>);
+const void* sweep_kernel(int NP, bool taylor, bool gon) {
+  return NP == 3 ? sweep_kernel_np<3, 4, W>(taylor, gon) : sweep_kernel_np<6, 2, W>(taylor, gon);
 }
 
 }  // namespace
@@ -484,12 +496,12 @@ size_t sweep_shmem(int NP, int nphi, int S) {
          sizeof(int2) * kMidLds + 16;
 }
 
-void sweep_prepare_attributes(int NP, int nphi, bool taylor) {
+void sweep_prepare_attributes(int NP, int nphi, bool taylor, bool gon) {
   const size_t shmem =
       sweep_shmem(NP, nphi, fitting_slots(sweep_fixed_doubles(nphi), sweep_kb(NP)));
   constexpr int W = kSlotWaves;
-  const void* kern = sweep_kernel<W>(NP, taylor);
-  size_t& done = g_sweep_attr[(NP == 3 ? 0 : 1) + (taylor ? 2 : 0)];
+  const void* kern = sweep_kernel<W>(NP, taylor, gon);
+  size_t& done = g_sweep_attr[(NP == 3 ? 0 : 1) + (taylor ? 2 : 0) + (gon ? 4 : 0)];
   if (shmem > done) {
     BIE_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)std::max(shmem, (size_t)(144 * 1024))));
@@ -509,23 +521,16 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
   const size_t shmem = sweep_shmem(a.NP, nphi, sa.S);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "sweep kernel LDS request exceeds 160 KiB");
   const bool tay = sw.taylor != 0;
-  sweep_prepare_attributes(a.NP, nphi, tay);
+  const bool gon = (a.flags & BIE_G) != 0;
+  sweep_prepare_attributes(a.NP, nphi, tay, gon);
   constexpr int W = kSlotWaves;
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * 8)), dim3(64 * W), shmem, s, a, sa, sw);
-  };
-  if (a.NP == 3) {
-    if (tay)
-      go(k_sweep_slots<3, 4, W, true>);
-    else
-      go(k_sweep_slots<3, 4, W, false>);
-  } else {
-    if (tay)
-      go(k_sweep_slots<6, 2, W, true>);
-    else
-      go(k_sweep_slots<6, 2, W, false>);
-  }
-  BIE_HIP_CHECK(hipGetLastError());
+  const void* kern = sweep_kernel<W>(a.NP, tay, gon);
+  AssemblyArgs a_ = a;
+  SlotArgs sa_ = sa;
+  SweepArgs sw_ = sw;
+  void* args[] = {&a_, &sa_, &sw_};
+  BIE_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)(per_xcd * 8)), dim3(64 * W), args, shmem,
+                                s));
   if (a.NP == 6) {
     if (tay)
       hipLaunchKernelGGL(k_sweep_mid<true>, dim3(4096), dim3(256), 0, s, a, sa, sw);
