@@ -97,7 +97,7 @@ k_select(const double* __restrict__ targets, int64_t M, const double* __restrict
 // f = e^{ikr}(ikr - 1)(n.d)/(4 pi r^3) and k0 = -(n.d)/(4 pi r^3) (kernels.jl:13-55) share
 // one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].  With TAB, e^{ikr}
 // comes from the LDS trig table trig (sincos_tab), else from sincos_f64.
-template <int NP, bool CPLX, bool TAB = false>
+template <int NP, bool CPLX, bool TAB = false, bool GON = true>
 __device__ __forceinline__ void point_kernel(double x0, double x1, double x2, const double4 A,
                                              const double4 B, const double* __restrict__ p,
                                              double kr, double ki, double* fr, double* fi,
@@ -135,8 +135,10 @@ __device__ __forceinline__ void point_kernel(double x0, double x1, double x2, co
     const double ps = p[s];
     fr[s] = fma(Fr, ps, fr[s]);
     fi[s] = fma(Fi, ps, fi[s]);
-    gr[s] = fma(Gr, ps, gr[s]);
-    gi[s] = fma(Gi, ps, gi[s]);
+    if constexpr (GON) {  // without G (gOn = false) its accumulation is left out
+      gr[s] = fma(Gr, ps, gr[s]);
+      gi[s] = fma(Gi, ps, gi[s]);
+    }
   }
 }
 
@@ -169,7 +171,7 @@ __device__ __forceinline__ void pair_quadrature(double x0, double x1, double x2,
 
 // As pair_quadrature, with the first two table rows (points 0 and 1, or 0 twice when
 // Q == 1) already loaded by the caller, so their latency overlaps earlier work.
-template <int NP, bool CPLX, bool TAB = false>
+template <int NP, bool CPLX, bool TAB = false, bool GON = true>
 __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double x2,
                                                     const double* __restrict__ tab,
                                                     const double* __restrict__ ph, int Q,
@@ -179,24 +181,24 @@ __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double
                                                     double& csum, const double2* trig = nullptr) {
   int q = 0;
   for (; q + 1 < Q; q += 2) {
-    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum,
-                                trig);
+    point_kernel<NP, CPLX, TAB, GON>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi,
+                                     csum, trig);
     const int q2 = (q + 2 < Q) ? q + 2 : q + 1;
     A = *reinterpret_cast<const double4*>(tab + 8 * q2);
     B = *reinterpret_cast<const double4*>(tab + 8 * q2 + 4);
-    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A2, B2, ph + (q + 1) * NP, kr, ki, fr, fi, gr, gi,
-                                csum, trig);
+    point_kernel<NP, CPLX, TAB, GON>(x0, x1, x2, A2, B2, ph + (q + 1) * NP, kr, ki, fr, fi, gr,
+                                     gi, csum, trig);
     const int q3 = (q + 3 < Q) ? q + 3 : q + 1;
     A2 = *reinterpret_cast<const double4*>(tab + 8 * q3);
     B2 = *reinterpret_cast<const double4*>(tab + 8 * q3 + 4);
   }
   if (q < Q)
-    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum,
-                                trig);
+    point_kernel<NP, CPLX, TAB, GON>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi,
+                                     csum, trig);
 }
 
 // As pair_quadrature over table rows that this wave staged in LDS (rows[8 q .. 8 q + 8)).
-template <int NP, bool CPLX, bool TAB = false>
+template <int NP, bool CPLX, bool TAB = false, bool GON = true>
 __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double x2,
                                                     const double* rows, const double* ph, int Q,
                                                     double kr, double ki, double* fr, double* fi,
@@ -206,8 +208,8 @@ __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double
   for (int q = 0; q < Q; ++q) {
     const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
     const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
-    point_kernel<NP, CPLX, TAB>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi, csum,
-                                trig);
+    point_kernel<NP, CPLX, TAB, GON>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi,
+                                     csum, trig);
   }
 }
 
@@ -321,7 +323,7 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
 // its element's vertex-rule rows in its LDS rows (s_rows + wave * 24 Qc) and loads the
 // next element's rows into registers a whole pair ahead: the global loads never wait
 // behind the flush's stores (vector-memory counters are in order).  Ends on a barrier.
-template <int NP, bool CPLX, int W, bool STAGED>
+template <int NP, bool CPLX, int W, bool STAGED, bool GON = true>
 __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
                                           int seg, double* acc, const double* phi,
                                           const double2* trig, double* s_rows, int2* s_q,
@@ -374,11 +376,11 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
 #pragma unroll
         for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
         if (STAGED)
-          pair_quadrature_lds<NP, CPLX, true>(x0, x1, x2, my_rows + (size_t)c * Qc * 8,
-                                              phi + c * a.Qmax * NP, Qc, kr, ki, fr, fi, gr, gi,
-                                              csum, trig);
+          pair_quadrature_lds<NP, CPLX, true, GON>(x0, x1, x2, my_rows + (size_t)c * Qc * 8,
+                                                   phi + c * a.Qmax * NP, Qc, kr, ki, fr, fi,
+                                                   gr, gi, csum, trig);
         else
-          pair_quadrature_pre<NP, CPLX, true>(
+          pair_quadrature_pre<NP, CPLX, true, GON>(
               x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, phi + c * a.Qmax * NP,
               a.len[c], kr, ki, A0, B0, A1, B1, fr, fi, gr, gi, csum, trig);
         const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
@@ -387,8 +389,10 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
           double* p = acc + (size_t)sload(so + s) * 256 + lane;
           atomicAdd(p, fr[s]);
           atomicAdd(p + 64, fi[s]);
-          atomicAdd(p + 128, gr[s]);
-          atomicAdd(p + 192, gi[s]);
+          if constexpr (GON) {
+            atomicAdd(p + 128, gr[s]);
+            atomicAdd(p + 192, gi[s]);
+          }
         }
       }
       if (NP == 6 && sa.push_mid) {
@@ -448,7 +452,8 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   lds_barrier();
 }
 
-template <int NP, bool CPLX, int W>
+// GON is whether G is formed (the BIE_G flag): F-only launches use their own instance.
+template <int NP, bool CPLX, int W, bool GON>
 __global__ void __launch_bounds__(64 * W)
 k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   extern __shared__ double lds[];
@@ -480,9 +485,9 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   if (threadIdx.x == 0) *s_qn = 0u;
   lds_barrier();
   if (nstage)
-    slot_tile<NP, CPLX, W, true>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
+    slot_tile<NP, CPLX, W, true, GON>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
   else
-    slot_tile<NP, CPLX, W, false>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
+    slot_tile<NP, CPLX, W, false, GON>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
 }
 
 // Uneven batch: many meshes (same element kinds and rule set) in one persistent launch.
@@ -777,16 +782,17 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     hipLaunchKernelGGL(kern, grid, dim3(64 * W), shmem, s, a, sa);
   };
+  const bool gon = (a.flags & BIE_G) != 0;
   if (a.NP == 3) {
     if (cplx)
-      go(k_assemble_slots<3, true, W>);
+      gon ? go(k_assemble_slots<3, true, W, true>) : go(k_assemble_slots<3, true, W, false>);
     else
-      go(k_assemble_slots<3, false, W>);
+      gon ? go(k_assemble_slots<3, false, W, true>) : go(k_assemble_slots<3, false, W, false>);
   } else {
     if (cplx)
-      go(k_assemble_slots<6, true, W>);
+      gon ? go(k_assemble_slots<6, true, W, true>) : go(k_assemble_slots<6, true, W, false>);
     else
-      go(k_assemble_slots<6, false, W>);
+      gon ? go(k_assemble_slots<6, false, W, true>) : go(k_assemble_slots<6, false, W, false>);
   }
   BIE_HIP_CHECK(hipGetLastError());
 }
