@@ -157,10 +157,13 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
     Fc = torch.view_as_complex(F.view(nk, N, N, 2))
     Cc = torch.view_as_complex(C.view(N, 2))
     z = torch.as_tensor(mesh.sources[2], device=dev)
+    op = collocation_operator_batched(torch, Fc, Cc)
+    # untimed warm-up (three Arnoldi steps): the first calls load the library kernels
+    gmres_batched(torch, op, torch.exp(1j * kd[:, None] * z[None, :]), tol=1e-10, maxiter=3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     Bt = torch.exp(1j * kd[:, None] * z[None, :])
-    _, its, rrs = gmres_batched(torch, collocation_operator_batched(torch, Fc, Cc), Bt, tol=1e-10)
+    _, its, rrs = gmres_batched(torch, op, Bt, tol=1e-10)
     torch.cuda.synchronize()
     res["gmres"] = {"tol": 1e-10, "total_s": time.perf_counter() - t0, "batched": True,
                     "iterations_min_mean_max": [int(its.min()), float(its.mean()), int(its.max())],
