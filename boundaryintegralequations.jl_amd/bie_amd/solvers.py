@@ -97,6 +97,22 @@ def gmres(torch, A, b, *, tol=1e-10, restart=60, maxiter=2000, x0=None):
     return x, it, beta / bnorm
 
 
+def lu_solve(torch, F_storage, C, B, *, overwrite=False):
+    """X with (F + diag(1 - C)) X = B by LU on the device (torch.linalg.solve, i.e.
+    rocSOLVER getrf/getrs): the direct solve of the rigid-sphere system
+    (examples/3d_rigid_sphere.jl:50-52, test/3d/assembly_test.jl:37) with F left where the
+    assembly wrote it.  F_storage is the (N, N) torch view of the column-major F (F
+    transposed), C the length-N free term, B (N,) or (N, nrhs).  With overwrite=True the
+    diagonal term is added into F_storage in place (no second N x N buffer; F is then A)."""
+    d = 1.0 - C
+    if overwrite:
+        F_storage.diagonal().add_(d)
+        A = F_storage.t()
+    else:
+        A = F_storage.t() + torch.diag(d)
+    return torch.linalg.solve(A, B)
+
+
 def collocation_operator_batched(torch, F_storage, C):
     """(X, active) -> Y with Y[q] = (F_q + diag(1 - C)) X[q] for the systems q marked in the
     boolean numpy array `active` (the others get only the diagonal term, which

@@ -360,6 +360,16 @@ void taylor_expand_device(int n_terms, int64_t n, const double* dA, int64_t stri
   BIE_REQUIRE(coef != nullptr, BIE_EINVAL, "coefficients are NULL");
   if (n == 0) return;
   BIE_REQUIRE(dA && dY, BIE_EINVAL, "NULL pointer");
+  {  // dY may be A_0 itself, but must not overlap any other term
+    const uintptr_t y0 = reinterpret_cast<uintptr_t>(dY), y1 = y0 + 16 * (uintptr_t)n;
+    for (int t = 0; t < n_terms; ++t) {
+      const uintptr_t a0 = reinterpret_cast<uintptr_t>(dA) + 16 * (uintptr_t)t * strideA;
+      const uintptr_t a1 = a0 + 16 * (uintptr_t)n;
+      const bool same = t == 0 && a0 == y0;
+      BIE_REQUIRE(same || y1 <= a0 || a1 <= y0, BIE_EINVAL,
+                  "output overlaps a term other than the first");
+    }
+  }
   TaylorCoefs cf{};
   for (int t = 0; t < 2 * n_terms; ++t) {
     BIE_REQUIRE(std::isfinite(coef[t]), BIE_EINVAL, "coefficient is not finite");

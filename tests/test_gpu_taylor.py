@@ -93,3 +93,22 @@ def test_expansion_reproduces_the_assembly_at_a_nearby_wavenumber():
     Fa, Ga, Ca = assemble_parallel(mesh, k0 + dk, n=3, m=3)
     assert rel(Fk, Fa) < 1e-11 and rel(Gk, Ga) < 1e-11
     assert rel(C, Ca) < TOL
+
+
+def test_expansion_in_place_on_the_first_term():
+    """dY = dA: the order-0 array is overwritten with the sum (each entry is read before it
+    is written, by the same thread)."""
+    from bie_amd.taylor import taylor_expand_device
+    from gpu_helpers import DevBuf, sync
+    rng = np.random.default_rng(4)
+    n, nt = 100003, 7
+    A = rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))
+    c = taylor_coefficients(2.2, 2.0, nt)
+    buf = DevBuf(A.nbytes)
+    buf.upload(np.ascontiguousarray(A))
+    print("[case] taylor expansion in place", flush=True)
+    taylor_expand_device(nt, n, buf.ptr, n, c, buf.ptr)
+    sync()
+    out = buf.download(np.zeros(nt * n, dtype=np.complex128))
+    assert rel(out[:n], c @ A) < 1e-14
+    assert np.array_equal(out[n:].reshape(nt - 1, n), A[1:])

@@ -111,6 +111,13 @@ def test_taylor_entry_points_validate_arguments_on_the_host():
     assert lib.bie_taylor_expand_device(2, 10, p, 9, coef, p, None) == _lib.BIE_EINVAL
     assert b"stride" in lib.bie_last_error()
     assert lib.bie_taylor_expand_device(2, 10, p, 10, None, p, None) == _lib.BIE_EINVAL
+    # the output may be the first term but no other (fake addresses: rejected before use)
+    a0 = 0x100000
+    assert lib.bie_taylor_expand_device(2, 10, ctypes.c_void_p(a0), 10, coef,
+                                        ctypes.c_void_p(a0 + 160), None) == _lib.BIE_EINVAL
+    assert b"overlaps" in lib.bie_last_error()
+    assert lib.bie_taylor_expand_device(2, 10, ctypes.c_void_p(a0), 10, coef,
+                                        ctypes.c_void_p(a0 + 80), None) == _lib.BIE_EINVAL
     bad = (ctypes.c_double * 4)(1.0, 0.0, float("nan"), 0.0)
     assert lib.bie_taylor_expand_device(2, 10, p, 10, bad, p, None) == _lib.BIE_EINVAL
     assert lib.bie_assemble_taylor_device(None, 1.0, 0.0, 3, None, 4, 3, 3, 7, p, p, 16, p,
