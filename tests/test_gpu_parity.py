@@ -203,6 +203,46 @@ def test_fewer_elements_than_waves_per_step(kind, n_el):
             assert np.linalg.norm(C - Co) < TOL * scale
 
 
+def _golden():
+    import sys
+    if GOLD not in sys.path:
+        sys.path.insert(0, GOLD)
+    import make_fixtures
+    return make_fixtures, np.load(os.path.join(GOLD, "oracle_vectors.npz"))
+
+
+@pytest.mark.parametrize("name", ["coarse_quad", "coarse_lin", "ico2_lin", "ico1_quad"])
+def test_gpu_matches_golden_vectors(name):
+    """The committed golden vectors (the reference's coarsest COMSOL sphere, quadratic and
+    linear, and two small geodesic spheres at k = 0, 1, 5 + 0.3i): rule indices bit-exact,
+    F, G, C within the parity tolerance."""
+    mf, gv = _golden()
+    mesh = mf.meshes()[name]
+    print(f"[case] golden vectors {name}", flush=True)
+    assert np.array_equal(gpu_rules(mesh), gv[f"{name}_rules"])
+    for j, k in enumerate(mf.KS):
+        if f"{name}_k{j}_F" not in gv:
+            continue
+        F, G, C = assemble_parallel(mesh, k, n=mf.NM[0], m=mf.NM[1])
+        assert rel(F, gv[f"{name}_k{j}_F"]) < TOL
+        assert rel(G, gv[f"{name}_k{j}_G"]) < TOL
+        assert rel(C, gv[f"{name}_k{j}_C"]) < TOL
+
+
+@pytest.mark.parametrize("i", range(3))
+def test_gpu_far_entries_match_mpmath(i):
+    """Single flat elements, far targets, (n,m) = (12,12): the HIP path against mpmath's
+    adaptive quadrature directly (no oracle in between)."""
+    mf, gv = _golden()
+    one = mf.single_element(mf.meshes()["coarse_lin"], int(gv[f"far_{i}_e"]))
+    x = gv[f"far_{i}_x"].reshape(3, 1)
+    k = complex(gv[f"far_{i}_k"])
+    print(f"[case] far entry {i} vs mpmath (12,12)", flush=True)
+    F, G, _ = assemble_parallel(one, k, x, n=12, m=12)
+    assert rel(F[0], gv[f"far_{i}_F"]) < 1e-13
+    assert rel(G[0], gv[f"far_{i}_G"]) < 1e-13
+
+
 def test_reference_tolerance_table_on_gpu_output():
     # test/3d/assembly_test.jl:12-44 with the GPU's F, C
     k = 2 * np.pi * 100.0 / 340.0
