@@ -161,9 +161,11 @@ __device__ __forceinline__ void taylor_point(double x0, double x1, double x2, co
   }
 }
 
-// Flush of one step's finished nodes for the KB wavenumbers of a block (see the
-// single-k kernel): slot contents to F_k, G_k columns, zeroing the slots.
-template <int KB, int W>
+// Flush of one step's finished nodes for the KBS slot groups of a block (see the
+// single-k kernel): slot contents to F_k, G_k columns, zeroing the slots.  With PACK a
+// group holds F of two consecutive matrices (k0 + 2 q2 at +0/+64, k0 + 2 q2 + 1 at
+// +128/+192) and no G.
+template <int KBS, int W, bool PACK>
 __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArgs& sa,
                                             const SweepArgs& sw, int st, int wave, int lane,
                                             int64_t i, bool active, int k0, double* acc) {
@@ -173,13 +175,33 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
     const int node = sload(sa.flush_node + f);
     const int ss = sload(sa.flush_slot + f);
 #pragma unroll
-    for (int q2 = 0; q2 < KB; ++q2) {
-      double* p = acc + ((size_t)(ss & 0xffff) * KB + q2) * 256 + lane;
+    for (int q2 = 0; q2 < KBS; ++q2) {
+      double* p = acc + ((size_t)(ss & 0xffff) * KBS + q2) * 256 + lane;
       const double vr = p[0], vi = p[64], wr = p[128], wi = p[192];
       p[0] = 0.0;
       p[64] = 0.0;
       p[128] = 0.0;
       p[192] = 0.0;
+      if constexpr (PACK) {
+        const int ka = k0 + 2 * q2;
+        if (active && doF) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (ka + h >= sw.nk) break;
+            const double xr = h ? wr : vr, xi = h ? wi : vi;
+            const size_t idx =
+                2 * ((size_t)(ka + h) * sw.stride + (size_t)i + (size_t)node * a.M);
+            if (ss & kSharedBit) {
+              unsafeAtomicAdd(a.F + idx, xr);
+              unsafeAtomicAdd(a.F + idx + 1, xi);
+            } else {
+              __builtin_nontemporal_store(xr, a.F + idx);
+              __builtin_nontemporal_store(xi, a.F + idx + 1);
+            }
+          }
+        }
+        continue;
+      }
       const int kidx = k0 + q2;
       if (active && kidx < sw.nk) {
         const size_t idx = 2 * ((size_t)kidx * sw.stride + (size_t)i + (size_t)node * a.M);
@@ -211,13 +233,19 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
 // kb*KB .. kb*KB+KB-1 at the one wavenumber sw.k_re + i sw.k_im instead of KB wavenumbers.
 // GON is whether G is formed (the BIE_G flag): F-only launches skip the G accumulation and
 // its LDS atomics in a separate instance, leaving the F-and-G code as it is.
+// In the derivative mode without G (PACK) the KB orders of a block fill KB / 2 slot groups,
+// two orders' F per group (see sweep_flush), so a block carries twice the orders that the
+// slot schedule built for KB / 2 groups would otherwise allow.
 template <int NP, int KB, int W, bool TAY, bool GON>
 __global__ void __launch_bounds__(64 * W)
 k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
+  constexpr bool PACK = TAY && !GON;
+  constexpr int KBS = PACK ? KB / 2 : KB;  // slot groups per slot
+  static_assert(!PACK || KB % 2 == 0, "packed blocks hold an even number of orders");
   extern __shared__ double lds[];
-  double* acc = lds;  // [slot][b][4][64]
+  double* acc = lds;  // [slot][group][4][64]
   const int nphi = a.R * a.Qmax * NP;
-  double* s_phi = lds + (size_t)sa.S * KB * 256;
+  double* s_phi = lds + (size_t)sa.S * KBS * 256;
   const double2* s_trig = reinterpret_cast<const double2*>(s_phi + phi_pad(nphi));
   int2* s_q = reinterpret_cast<int2*>(s_phi + phi_pad(nphi) + kTrigDoubles);
   unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
@@ -237,7 +265,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   const bool active = i < a.M;
   // basis table and trig table: one buffer of phi_pad(nphi) + kTrigDoubles doubles
   for (int t = threadIdx.x; t < phi_pad(nphi) + kTrigDoubles; t += blockDim.x) s_phi[t] = a.phi[t];
-  for (int t = threadIdx.x; t < sa.S * KB * 256; t += blockDim.x) acc[t] = 0.0;
+  for (int t = threadIdx.x; t < sa.S * KBS * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
   lds_barrier();
   const int64_t ic = active ? i : (a.M - 1);
@@ -309,12 +337,19 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
         for (int s = 0; s < NP; ++s)
 #pragma unroll
           for (int q2 = 0; q2 < KB; ++q2) {
-            double* p = acc + ((size_t)sload(so + s) * KB + q2) * 256 + lane;
-            atomicAdd(p, fr[q2][s]);
-            atomicAdd(p + 64, fi[q2][s]);
-            if (doG) {
-              atomicAdd(p + 128, gr[q2][s]);
-              atomicAdd(p + 192, gi[q2][s]);
+            if constexpr (PACK) {
+              double* p = acc + ((size_t)sload(so + s) * KBS + q2 / 2) * 256 + lane +
+                          (q2 & 1) * 128;
+              atomicAdd(p, fr[q2][s]);
+              atomicAdd(p + 64, fi[q2][s]);
+            } else {
+              double* p = acc + ((size_t)sload(so + s) * KB + q2) * 256 + lane;
+              atomicAdd(p, fr[q2][s]);
+              atomicAdd(p + 64, fi[q2][s]);
+              if (doG) {
+                atomicAdd(p + 128, gr[q2][s]);
+                atomicAdd(p + 192, gi[q2][s]);
+              }
             }
           }
       }
@@ -349,13 +384,13 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
       }
     }
     // the previous step's finished nodes (their slots were complete at the last barrier)
-    if (st > st0) sweep_flush<KB, W>(a, sa, sw, st - 1, wave, lane, i, active, k0, acc);
+    if (st > st0) sweep_flush<KBS, W, PACK>(a, sa, sw, st - 1, wave, lane, i, active, k0, acc);
     lds_barrier();
     e = e_nxt;
     c = c_nxt;
   }
   if (st1 > st0) {
-    sweep_flush<KB, W>(a, sa, sw, st1 - 1, wave, lane, i, active, k0, acc);
+    sweep_flush<KBS, W, PACK>(a, sa, sw, st1 - 1, wave, lane, i, active, k0, acc);
     lds_barrier();
   }
   if (NP == 6 && first_kb && wave == 0) {
@@ -448,6 +483,10 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
 
 int sweep_kb(int NP) { return NP == 3 ? 4 : 2; }
 
+int sweep_block_size(int NP, bool taylor, bool gon) {
+  return (taylor && !gon) ? 2 * sweep_kb(NP) : sweep_kb(NP);
+}
+
 namespace {
 
 __global__ void k_block_uniform(const double* __restrict__ k, int nk, int KB, int* out) {
@@ -471,7 +510,7 @@ template <int NP, int KB, int W>
 const void* sweep_kernel_np(bool taylor, bool gon) {
   if (taylor)
     return gon ? reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, true, true>)
-               : reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, true, false>);
+               : reinterpret_cast<const void*>(k_sweep_slots<NP, 2 * KB, W, true, false>);
   return gon ? reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, false, true>)
              : reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, false, false>);
 }

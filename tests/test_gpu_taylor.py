@@ -112,3 +112,24 @@ def test_expansion_in_place_on_the_first_term():
     out = buf.download(np.zeros(nt * n, dtype=np.complex128))
     assert rel(out[:n], c @ A) < 1e-14
     assert np.array_equal(out[n:].reshape(nt - 1, n), A[1:])
+
+
+@pytest.mark.parametrize("kind,f", [("linear", 2), ("quadratic", 1)])
+@pytest.mark.parametrize("Mo", [9, 12])
+def test_taylor_without_g_matches_oracle(kind, f, Mo):
+    """gOn = false: the kernel packs two orders' F into each slot group and carries twice
+    the orders per block; 10 and 13 orders leave partial blocks (13: an odd count, so the
+    last group's second half is past the end).  Collocation and field targets, complex k."""
+    mesh = M.icosphere(f, kind)
+    rng = np.random.default_rng(Mo)
+    targets = np.concatenate([1.7 * rng.standard_normal((3, 30)), mesh.sources[:, :4]], axis=1)
+    k = 2.5 + 0.2j
+    for tg in (None, targets):
+        print(f"[case] taylor without G {kind} orders 0..{Mo} field={tg is not None}", flush=True)
+        F, G, C = taylor_assemble(mesh, k, tg, M=Mo, gOn=False, n=3, m=3)
+        assert not G.any()
+        for q in range(Mo + 1):
+            Fo, _ = oracle.assemble_taylor(mesh, k, q, targets=tg, n=3, m=3)
+            assert rel(F[:, :, q], Fo) < TOL, q
+        _, _, Co = oracle.assemble(mesh, k, targets=tg, n=3, m=3)
+        assert rel(C, Co) < TOL
