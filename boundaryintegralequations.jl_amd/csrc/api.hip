@@ -530,7 +530,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   }
   if (M == 0 || dm->E == 0 || nk == 0) return;
   refresh_geometry(dm, s);
-  if (!taylor) launch_block_uniform(d_k, nk, KB, dm->d_block_uniform, s);
+  if (!taylor) launch_block_uniform(d_k, nk, KBk, dm->d_block_uniform, s);
   const int64_t rowblocks = (M + 63) / 64;
   AssemblyArgs a = make_args(dm, d_targets, M, 0.0, 0.0, flags, dF, dG, dC);
   SlotArgs sa = slot_args(dm, sc, rowblocks);

@@ -215,7 +215,7 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
                                int ncols, hipStream_t s);
 int sweep_kb(int NP);  // slot groups (wavenumbers) per k block of the sweep kernel
-// k (or derivative-order) matrices per block: twice sweep_kb for derivative stacks without G
+// k (or derivative-order) matrices per block: twice sweep_kb without G
 int sweep_block_size(int NP, bool taylor, bool gon);
 void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw,
                   hipStream_t s);

@@ -233,13 +233,13 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
 // kb*KB .. kb*KB+KB-1 at the one wavenumber sw.k_re + i sw.k_im instead of KB wavenumbers.
 // GON is whether G is formed (the BIE_G flag): F-only launches skip the G accumulation and
 // its LDS atomics in a separate instance, leaving the F-and-G code as it is.
-// In the derivative mode without G (PACK) the KB orders of a block fill KB / 2 slot groups,
-// two orders' F per group (see sweep_flush), so a block carries twice the orders that the
+// Without G (PACK) the KB wavenumbers or orders of a block fill KB / 2 slot groups, two
+// matrices' F per group (see sweep_flush), so a block carries twice the matrices that the
 // slot schedule built for KB / 2 groups would otherwise allow.
 template <int NP, int KB, int W, bool TAY, bool GON>
 __global__ void __launch_bounds__(64 * W)
 k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
-  constexpr bool PACK = TAY && !GON;
+  constexpr bool PACK = !GON;
   constexpr int KBS = PACK ? KB / 2 : KB;  // slot groups per slot
   static_assert(!PACK || KB % 2 == 0, "packed blocks hold an even number of orders");
   extern __shared__ double lds[];
@@ -484,7 +484,8 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
 int sweep_kb(int NP) { return NP == 3 ? 4 : 2; }
 
 int sweep_block_size(int NP, bool taylor, bool gon) {
-  return (taylor && !gon) ? 2 * sweep_kb(NP) : sweep_kb(NP);
+  (void)taylor;
+  return gon ? sweep_kb(NP) : 2 * sweep_kb(NP);
 }
 
 namespace {
@@ -512,7 +513,7 @@ const void* sweep_kernel_np(bool taylor, bool gon) {
     return gon ? reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, true, true>)
                : reinterpret_cast<const void*>(k_sweep_slots<NP, 2 * KB, W, true, false>);
   return gon ? reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, false, true>)
-             : reinterpret_cast<const void*>(k_sweep_slots<NP, KB, W, false, false>);
+             : reinterpret_cast<const void*>(k_sweep_slots<NP, 2 * KB, W, false, false>);
 }
 
 template <int W>

@@ -16,7 +16,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def run_sweep(mesh, ks, n, m, graph_replays=0):
+def run_sweep(mesh, ks, n, m, graph_replays=0, gOn=True):
     dm = DeviceMesh(mesh)
     N = mesh.n_sources
     nk = len(ks)
@@ -36,7 +36,7 @@ def run_sweep(mesh, ks, n, m, graph_replays=0):
             sync()
         g.close()
     else:
-        dm.assemble_sweep(kb.ptr, nk, F.ptr, G.ptr, stride, C.ptr, n=n, m=m)
+        dm.assemble_sweep(kb.ptr, nk, F.ptr, G.ptr, stride, C.ptr, n=n, m=m, gOn=gOn)
         sync()
         outs.append(read(F, G, C, N, nk, stride))
     return outs
@@ -66,6 +66,25 @@ def test_sweep_matches_single_k(name, ks, n, m):
     assert rel(c, C1) < TOL
     Fo, Go, Co = oracle.assemble(mesh, ks[-1], n=n, m=m)
     assert rel(Fs[-1], Fo) < TOL and rel(Gs[-1], Go) < TOL
+
+
+@pytest.mark.parametrize("name,ks", [
+    ("lin", np.linspace(0.5, 30.0, 19)),           # two uniform blocks of 8 + a partial one
+    ("lin", [1.0, 2.5, 2.6, 7.0, 11.0, 11.5, 3.0, 4.0, 9.0]),  # non-uniform: direct path
+    ("quad", np.linspace(1.0, 12.0, 11)),          # quadratic: uniform block of 4 + partial
+])
+def test_sweep_without_g_matches_single_k(name, ks):
+    """gOn = false: two wavenumbers' F per slot group, twice the wavenumbers per block, the
+    uniform-spacing recurrence over the longer blocks; every F_k against a single-k F
+    assembly."""
+    mesh = M.icosphere(4, "linear") if name == "lin" else M.icosphere(3, "quadratic")
+    print(f"[case] sweep without G {name} nk={len(ks)}", flush=True)
+    (Fs, Gs, c), = run_sweep(mesh, ks, 3, 2, gOn=False)
+    for q, k in enumerate(ks):
+        F1, _, C1 = assemble_parallel(mesh, k, mesh.sources, n=3, m=2, gOn=False)
+        assert rel(Fs[q], F1) < TOL, (q, k)
+        assert not Gs[q].any()
+    assert rel(c, C1) < TOL
 
 
 def test_sweep_graph_replays():
