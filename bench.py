@@ -107,7 +107,8 @@ def measure_apply(torch, F, N, steps, stream, nrhs=32):
 def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
     """Config 3: 4002-node linear sphere, 64 wavenumbers uniform in [0.5, 30], (n,m)=(4,4).
     Time of 64 single-k device assemblies vs one replay of the captured batched sweep,
-    each writing all 64 (F_k, G_k) pairs; parity between the two on every k."""
+    each writing all 64 (F_k, G_k) pairs; parity between the two on every k.  The same with
+    F only ("f_only"), then the 64 rigid-sphere GMRES solves on those F_k."""
     mesh = bie_amd.icosphere(f, "linear")
     N = mesh.n_sources
     dm = bie_amd.DeviceMesh(mesh, 0)
@@ -151,6 +152,38 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
         err = max(err, float(torch.linalg.norm(F[sl] - Fp[sl]) / torch.linalg.norm(Fp[sl])),
                   float(torch.linalg.norm(G[sl] - Gp[sl]) / torch.linalg.norm(Gp[sl])))
     g.close()
+
+    # the same 64 k with F only (gOn = false, all the GMRES below needs): per-k launches
+    # against one sweep launch
+    def plain_f():
+        for q in range(nk):
+            dm.assemble(float(ks[q]), Fp.data_ptr() + 16 * q * stride,
+                        Gp.data_ptr() + 16 * q * stride, C.data_ptr(), n=n, m=m, gOn=False,
+                        stream=stream)
+
+    def batched_f():
+        dm.assemble_sweep(kd.data_ptr(), nk, F.data_ptr(), 0, stride, C.data_ptr(), n=n, m=m,
+                          gOn=False, stream=stream)
+
+    fo = {}
+    for name, fn in (("per_k_launches", plain_f), ("batched", batched_f)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        fo[name + "_ms"] = e0.elapsed_time(e1) / steps
+    fo["speedup"] = fo["per_k_launches_ms"] / fo["batched_ms"]
+    fo["max_rel_err_batched_vs_per_k"] = max(
+        float(torch.linalg.norm(F[2 * q * stride:2 * (q + 1) * stride] -
+                                Fp[2 * q * stride:2 * (q + 1) * stride]) /
+              torch.linalg.norm(Fp[2 * q * stride:2 * (q + 1) * stride])) for q in range(nk))
+    res["f_only"] = fo
+    # F now holds the F-only sweep's F_k and C: the same matrices the GMRES below needs
     # GMRES (tol 1e-10, restart 60) for the rigid sphere under p_inc = e^{ikz},
     # A_k = F_k + diag(1 - C), all 64 systems advanced in lockstep
     from bie_amd.solvers import collocation_operator_batched, gmres_batched
