@@ -454,18 +454,17 @@ k_sweep_mid(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   }
 }
 
-// Zero listed columns of nk column-major M x N complex matrices spaced stride apart.
+// Zero listed columns of nk column-major M x N complex matrices spaced stride apart:
+// workgroups stride over the (matrix, column) pairs, threads over the rows of a column.
 __global__ void k_zero_columns_multi(double2* __restrict__ X, int64_t stride, int nk, int64_t M,
                                      const int* __restrict__ cols, int ncols) {
-  const int64_t per = (int64_t)ncols * M;
-  const int64_t total = per * nk;
-  const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += step) {
-    const int64_t kq = t / per;
-    const int64_t u = t - kq * per;
-    const int64_t c = u / M;
-    const int64_t r = u - c * M;
-    X[kq * stride + (size_t)cols[c] * M + r] = make_double2(0.0, 0.0);
+  const double2 z = make_double2(0.0, 0.0);
+  const int64_t pairs = (int64_t)nk * ncols;
+  for (int64_t pc = blockIdx.x; pc < pairs; pc += gridDim.x) {
+    const int kq = (int)(pc / ncols);
+    const int c = (int)(pc - (int64_t)kq * ncols);
+    double2* col = X + (size_t)kq * stride + (size_t)cols[c] * M;
+    for (int64_t r = threadIdx.x; r < M; r += blockDim.x) col[r] = z;
   }
 }
 
@@ -474,7 +473,7 @@ __global__ void k_zero_columns_multi(double2* __restrict__ X, int64_t stride, in
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
                                int ncols, hipStream_t s) {
   if (ncols <= 0 || M <= 0 || nk <= 0) return;
-  int64_t blocks = ((int64_t)ncols * M * nk + 255) / 256;
+  int64_t blocks = (int64_t)ncols * nk;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_zero_columns_multi, dim3((unsigned)blocks), dim3(256), 0, s,
                      reinterpret_cast<double2*>(X), stride, nk, M, cols, ncols);
