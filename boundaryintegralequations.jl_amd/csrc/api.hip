@@ -508,7 +508,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
                       1, s);
   if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
   if (dm->pk == 6) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
-  const int KBk = sweep_block_size(dm->pk, taylor, (flags & BIE_G) != 0);
+  const int KBk = sweep_block_size(dm->pk, (flags & BIE_G) != 0);
   const int nkb = (nk + KBk - 1) / KBk;
   if (dm->block_uniform_cap < std::max(nkb, 1)) {
     dfree(dm->d_block_uniform);

@@ -216,7 +216,7 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
                                int ncols, hipStream_t s);
 int sweep_kb(int NP);  // slot groups (wavenumbers) per k block of the sweep kernel
 // k (or derivative-order) matrices per block: twice sweep_kb without G
-int sweep_block_size(int NP, bool taylor, bool gon);
+int sweep_block_size(int NP, bool gon);
 void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw,
                   hipStream_t s);
 void sweep_prepare_attributes(int NP, int nphi, bool taylor, bool gon);

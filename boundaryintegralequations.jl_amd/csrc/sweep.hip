@@ -482,10 +482,7 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
 
 int sweep_kb(int NP) { return NP == 3 ? 4 : 2; }
 
-int sweep_block_size(int NP, bool taylor, bool gon) {
-  (void)taylor;
-  return gon ? sweep_kb(NP) : 2 * sweep_kb(NP);
-}
+int sweep_block_size(int NP, bool gon) { return gon ? sweep_kb(NP) : 2 * sweep_kb(NP); }
 
 namespace {
 
