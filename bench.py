@@ -503,7 +503,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle  # the CPU baseline and checker
+        import oracle  # the CPU baseline, and the comparison of its rows
         nrows = N if args.cpu_rows <= 0 else min(N, args.cpu_rows)
         rows = np.linspace(0, N - 1, nrows).astype(np.int64)
         threads = len(os.sched_getaffinity(0))

@@ -1,8 +1,8 @@
 // TEST INFRASTRUCTURE — CPU oracle for the dense 3D collocation assembly.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
-// library, and only as the checker (or the timed CPU baseline).  The product path never
-// calls it.
+// library, and only to compare results with (or as the timed CPU baseline).  The product
+// path never calls it.
 //
 // A restatement, in C++ with OpenMP threads over target rows, of the reference's
 // assemble_parallel! on linear and quadratic triangles:

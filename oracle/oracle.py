@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE — ctypes front end of the CPU oracle (oracle/liboracle.so).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
-module, and only as the checker.  The oracle restates the reference's
+module, and only to compare results with.  The oracle restates the reference's
 assemble_parallel! (src/3d/assembly_collocation.jl:259-479); see bie_oracle.cpp for the
 per-function citations.
 """

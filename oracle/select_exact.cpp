@@ -1,5 +1,5 @@
 // TEST INFRASTRUCTURE — part of the CPU oracle.  Only tests/, __graft_entry__.smoke()
-// and bench.py's cpu_baseline leg may load it, and only as the checker.
+// and bench.py's cpu_baseline leg may load it, and only to compare results with.
 //
 // Rule selection of the reference's dense collocation assembly, restated in IEEE-754
 // double arithmetic with a fixed operation order.  This translation unit is compiled
