@@ -87,6 +87,17 @@ def test_sweep_without_g_matches_single_k(name, ks):
     assert rel(c, C1) < TOL
 
 
+@pytest.mark.parametrize("gOn", [True, False])
+def test_sweep_of_one_wavenumber(gOn):
+    """nk = 1: a single partial block, with and without G."""
+    mesh = M.icosphere(3, "quadratic")
+    print(f"[case] sweep of one k gOn={gOn}", flush=True)
+    (Fs, Gs, c), = run_sweep(mesh, [4.5], 3, 2, gOn=gOn)
+    F1, G1, C1 = assemble_parallel(mesh, 4.5, mesh.sources, n=3, m=2, gOn=gOn)
+    assert rel(Fs[0], F1) < TOL and rel(c, C1) < TOL
+    assert (rel(Gs[0], G1) < TOL) if gOn else not Gs[0].any()
+
+
 def test_sweep_graph_replays():
     mesh = M.icosphere(3, "linear")
     ks = np.linspace(2.0, 9.0, 8)

@@ -133,3 +133,18 @@ def test_taylor_without_g_matches_oracle(kind, f, Mo):
             assert rel(F[:, :, q], Fo) < TOL, q
         _, _, Co = oracle.assemble(mesh, k, targets=tg, n=3, m=3)
         assert rel(C, Co) < TOL
+
+
+def test_taylor_edge_cases():
+    """One order (M = 0) equals the plain assembly; no targets gives empty outputs; an empty
+    expansion is a no-op."""
+    from bie_amd.taylor import taylor_expand_device
+    mesh = M.icosphere(2, "quadratic")
+    print("[case] taylor edge cases", flush=True)
+    F, G, C = taylor_assemble(mesh, 3.0 + 0.1j, M=0, n=3, m=3)
+    Fa, Ga, Ca = assemble_parallel(mesh, 3.0 + 0.1j, n=3, m=3)
+    assert F.shape[2] == 1
+    assert rel(F[:, :, 0], Fa) < TOL and rel(G[:, :, 0], Ga) < TOL and rel(C, Ca) < TOL
+    F, G, C = taylor_assemble(mesh, 2.0, np.zeros((3, 0)), M=3)
+    assert F.shape == (0, mesh.n_sources, 4) and C.shape == (0,)
+    taylor_expand_device(3, 0, 0, 0, taylor_coefficients(1.0, 0.5, 3), 0)
