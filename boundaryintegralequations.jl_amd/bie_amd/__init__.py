@@ -5,8 +5,9 @@ The compute path is the HIP library libbie_amd.so (built for gfx950); this packa
 its host-side mirror: mesh data model and generators, ctypes binding, and the
 ``assemble_parallel`` entry point with the reference's keyword interface.
 """
-from .mesh import Mesh3d, TRI3, TRI6, icosphere, ellipsoid, load_comsol, read_comsol  # noqa: F401
+from .mesh import (Mesh3d, TRI3, TRI6, icosphere, ellipsoid, load_comsol, read_comsol,  # noqa: F401
+                   load3d_triangular_mesh, read_mesh_file)
 from .assembly import assemble_parallel, Batch, DeviceMesh  # noqa: F401
 
 __all__ = ["Mesh3d", "TRI3", "TRI6", "icosphere", "ellipsoid", "load_comsol", "read_comsol",
-           "assemble_parallel", "DeviceMesh"]
+           "load3d_triangular_mesh", "read_mesh_file", "assemble_parallel", "DeviceMesh"]

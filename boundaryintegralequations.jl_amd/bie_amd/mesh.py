@@ -262,6 +262,106 @@ def load_comsol(path: str, geometry_order: str = "quadratic",
     raise ValueError("physics_order must be 'geometry' or 'linear'")
 
 
+def _read_obj(path):
+    verts, faces = [], []
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                verts.append([float(t) for t in tok[1:4]])
+            elif tok[0] == "f":
+                idx = []
+                for t in tok[1:]:
+                    i = int(t.split("/")[0])          # "v", "v/vt", "v//vn", "v/vt/vn"
+                    idx.append(i - 1 if i > 0 else len(verts) + i)   # 1-based or relative
+                faces.append(idx)
+    return verts, faces
+
+
+def _read_off(path):
+    with open(path) as fh:
+        tok = [t for line in fh for t in line.split("#")[0].split()]
+    if not tok or not tok[0].endswith("OFF"):
+        raise ValueError(f"{path}: not an OFF file")
+    nv, nf = int(tok[1]), int(tok[2])
+    p = 4
+    verts = [[float(tok[p + 3 * i + j]) for j in range(3)] for i in range(nv)]
+    p += 3 * nv
+    faces = []
+    for _ in range(nf):
+        n = int(tok[p])
+        faces.append([int(t) for t in tok[p + 1:p + 1 + n]])
+        p += 1 + n
+    return verts, faces
+
+
+def _read_ply(path):
+    with open(path, "rb") as fh:
+        lines = fh.read().decode("latin-1").splitlines()
+    if not lines or lines[0].strip() != "ply":
+        raise ValueError(f"{path}: not a PLY file")
+    i, nv, nf, vprops, fmt, cur = 1, 0, 0, [], None, None
+    while lines[i].strip() != "end_header":
+        tok = lines[i].split()
+        if tok[0] == "format":
+            fmt = tok[1]
+        elif tok[0] == "element":
+            cur = tok[1]
+            if cur == "vertex":
+                nv = int(tok[2])
+            elif cur == "face":
+                nf = int(tok[2])
+        elif tok[0] == "property" and cur == "vertex":
+            vprops.append(tok[-1])
+        i += 1
+    if fmt != "ascii":
+        raise NotImplementedError(f"{path}: only ASCII PLY files are read")
+    ix = [vprops.index(c) for c in ("x", "y", "z")]
+    body = lines[i + 1:]
+    verts = [[float(body[k].split()[j]) for j in ix] for k in range(nv)]
+    faces = []
+    for k in range(nv, nv + nf):
+        tok = body[k].split()
+        faces.append([int(t) for t in tok[1:1 + int(tok[0])]])
+    return verts, faces
+
+
+def read_mesh_file(path: str):
+    """(coordinates 3 x Nc, topology 3 x E, 0-based) of a triangle mesh in Wavefront OBJ,
+    OFF or ASCII PLY: the formats ``load_mesh_file`` reads through MeshIO
+    (src/3d/meshing.jl:8-34), restated from the published file formats.  Faces must be
+    triangles (the linear-triangle path takes the first three nodes of every face)."""
+    ext = os.path.splitext(path)[1].lower()
+    readers = {".obj": _read_obj, ".off": _read_off, ".ply": _read_ply}
+    if ext not in readers:
+        raise NotImplementedError(f"{ext} meshes are not read here (OBJ, OFF, ASCII PLY are)")
+    verts, faces = readers[ext](path)
+    if any(len(f) != 3 for f in faces):
+        raise ValueError(f"{path}: only triangular faces are supported")
+    coords = np.asarray(verts, dtype=np.float64).reshape(-1, 3).T.copy()
+    topo = np.asarray(faces, dtype=np.int64).reshape(-1, 3).T.copy()
+    if topo.size and (topo.min() < 0 or topo.max() >= coords.shape[1]):
+        raise ValueError(f"{path}: face index out of range")
+    return coords, topo
+
+
+def load3d_triangular_mesh(path: str, geometry_order: str = "linear",
+                           physics_order: str = "geometry") -> Mesh3d:
+    """Restatement of ``load3dTriangularMesh`` (src/3d/meshing.jl:48-106) for linear
+    geometry (the only order the reference accepts there) with geometry or linear physics:
+    unused nodes removed, collocation at the used geometry nodes."""
+    if geometry_order != "linear":
+        raise ValueError("Currently only linear element supported")
+    if physics_order not in ("geometry", "linear"):
+        raise NotImplementedError("discontinuous physics is not part of this library")
+    coords0, topo0 = read_mesh_file(path)
+    topo, used = _remove_unused(topo0)
+    coords = coords0[:, used]
+    return Mesh3d(coords, topo, coords[:, np.unique(topo)].copy(), topo.copy(), TRI3, TRI3)
+
+
 def with_coordinates(mesh: Mesh3d, coords) -> Mesh3d:
     """The same mesh with new coordinates; collocation nodes that are coordinates move
     with them (physics = geometry, or the corner nodes for linear physics)."""
