@@ -12,7 +12,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <vector>
 
 #include "bie_common.hpp"
 #include "schedule.hpp"
@@ -837,12 +839,80 @@ int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, doub
   });
 }
 
+namespace {
+
+// The host-array entry point keeps the device form of the last mesh it was given (upload,
+// rule tables, slot schedule, recorded mid-edge pairs), so a caller that assembles one
+// mesh at many wavenumbers - a frequency loop calling assemble_parallel! once per k - pays
+// that setup once.  A mesh is matched byte for byte against a copy of its fields and
+// arrays.  The cache is never destroyed (device memory is released with the process, not
+// by a static destructor that could run after the HIP runtime has gone).
+struct HostMeshCache {
+  std::mutex mu;
+  std::vector<unsigned char> key;
+  std::shared_ptr<bie_dmesh> dm;
+};
+
+HostMeshCache& host_mesh_cache() {
+  static HostMeshCache* c = new HostMeshCache;
+  return *c;
+}
+
+void append_bytes(std::vector<unsigned char>& key, const void* p, size_t n) {
+  const unsigned char flag = p ? 1 : 0;
+  key.push_back(flag);
+  if (p && n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    key.insert(key.end(), b, b + n);
+  }
+}
+
+std::vector<unsigned char> mesh_key(const bie_mesh* m) {
+  std::vector<unsigned char> key;
+  const int64_t scal[7] = {m->n_coordinates, m->n_elements, m->n_sources, m->geometry_kind,
+                           m->physics_kind, m->index_base, 0};
+  append_bytes(key, scal, sizeof(scal));
+  append_bytes(key, m->coordinates, sizeof(double) * 3 * (size_t)m->n_coordinates);
+  append_bytes(key, m->topology, sizeof(int64_t) * (size_t)m->geometry_kind * m->n_elements);
+  append_bytes(key, m->sources, sizeof(double) * 3 * (size_t)m->n_sources);
+  append_bytes(key, m->physics_topology,
+               sizeof(int64_t) * (size_t)m->physics_kind * m->n_elements);
+  return key;
+}
+
+int host_mesh_handle(const bie_mesh* mesh, std::shared_ptr<bie_dmesh>* out) {
+  std::vector<unsigned char> key;
+  int rc = guarded([&] {
+    validate_mesh(mesh);
+    key = mesh_key(mesh);
+  });
+  if (rc != BIE_OK) return rc;
+  HostMeshCache& c = host_mesh_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.dm && c.key == key) {
+    *out = c.dm;
+    return BIE_OK;
+  }
+  bie_dmesh* raw = nullptr;
+  rc = bie_dmesh_create(mesh, 0, &raw);
+  if (rc != BIE_OK) return rc;
+  // a replaced handle is destroyed when its last caller releases it
+  std::shared_ptr<bie_dmesh> sp(raw, [](bie_dmesh* d) { bie_dmesh_destroy(d); });
+  c.dm = sp;
+  c.key = std::move(key);
+  *out = std::move(sp);
+  return BIE_OK;
+}
+
+}  // namespace
+
 int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* targets,
                  int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
                  double* G, double* C) {
-  bie_dmesh* dm = nullptr;
-  int rc = bie_dmesh_create(mesh, 0, &dm);
+  std::shared_ptr<bie_dmesh> handle;
+  int rc = host_mesh_handle(mesh, &handle);
   if (rc != BIE_OK) return rc;
+  bie_dmesh* dm = handle.get();
   double* dT = nullptr;
   double* dF = nullptr;
   double* dG = nullptr;
@@ -874,9 +944,6 @@ int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* t
   dfree(dF);
   dfree(dG);
   dfree(dC);
-  std::string keep = g_last_error;
-  bie_dmesh_destroy(dm);
-  g_last_error = keep;
   return rc;
 }
 

@@ -60,7 +60,11 @@ int bie_version(void);
  * LossyGlobalMatrix.jl:119).  targets: 3 x M (NULL = mesh sources).  F, G: M x N complex,
  * C: M complex, caller-allocated; disabled outputs are zero-filled (as :308-310).
  * Rule selection for any target reads mesh sources (defined behaviour; the reference
- * indexes the targets there, :325/:436, which is only valid for collocation). */
+ * indexes the targets there, :325/:436, which is only valid for collocation).
+ * The device form of the last mesh (upload, rule tables, schedule, recorded mid-edge pairs)
+ * is kept for the next call: a call whose mesh matches it byte for byte skips that setup
+ * (a frequency loop calling assemble_parallel! once per k); device memory it holds is
+ * released when another mesh replaces it or the process ends. */
 int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* targets,
                  int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
                  double* G, double* C);

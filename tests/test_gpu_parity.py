@@ -352,3 +352,24 @@ def test_sources_prefix_as_targets_does_not_record_collocation_pairs():
     Fp = fb[0].download(np.zeros(P * N, dtype=np.complex128)).reshape(P, N, order="F")
     Fo, _, _ = oracle.assemble(mesh, 4.0, n=3, m=2)
     assert rel(F, Fo) < TOL and rel(Fp, Fo[:P]) < TOL
+
+
+def test_host_calls_reuse_the_mesh_across_wavenumbers_rules_and_targets():
+    """The host-array entry point keeps the last mesh's device form: repeated calls on one
+    mesh at other k and (n,m), with field targets in between, then another mesh, then the
+    first again (its arrays copied, so a new but identical mesh object), all against the
+    oracle."""
+    a = M.icosphere(3, "quadratic")
+    b = M.icosphere(4, "linear")
+    X = 1.8 * np.random.default_rng(2).standard_normal((3, 25))
+    seq = [(a, 2.0, None, 3, 2), (a, 5.0 + 0.2j, None, 3, 2), (a, 3.0, None, 4, 4),
+           (a, 3.0, X, 3, 2), (a, 1.0, None, 3, 2), (b, 2.0, None, 3, 3),
+           (M.Mesh3d(a.coordinates.copy(), a.topology.copy(), a.sources.copy(),
+                     a.physics_topology.copy(), a.geometry_kind, a.physics_kind),
+            4.0, None, 3, 2)]
+    for mesh, k, tg, n, m in seq:
+        print(f"[case] host call N={mesh.n_sources} k={k} targets={tg is not None} ({n},{m})",
+              flush=True)
+        F, G, C = assemble_parallel(mesh, k, mesh.sources if tg is None else tg, n=n, m=m)
+        Fo, Go, Co = oracle.assemble(mesh, k, targets=tg, n=n, m=m)
+        assert rel(F, Fo) < TOL and rel(G, Go) < TOL and rel(C, Co) < TOL
