@@ -14,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "bie_common.hpp"
@@ -906,6 +907,91 @@ int host_mesh_handle(const bie_mesh* mesh, std::shared_ptr<bie_dmesh>* out) {
 
 }  // namespace
 
+namespace {
+
+// Device-to-host copy of large outputs into caller (pageable) memory: DMA into one of two
+// pinned staging buffers while host threads copy the other one out, instead of the
+// runtime's single-threaded pageable path.  The buffers are allocated once per process and
+// shared by all callers under a mutex; without them (allocation failed) the plain copy
+// is used.
+struct HostStaging {
+  std::mutex mu;
+  bool tried = false;
+  char* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipStream_t s = nullptr;
+};
+constexpr size_t kStageBytes = size_t(128) << 20;
+
+HostStaging& host_staging() {
+  static HostStaging* h = new HostStaging;  // never destroyed (see host_mesh_cache)
+  return *h;
+}
+
+void parallel_copy(char* dst, const char* src, size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n >> 23));
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t part = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; ++t) {
+    const size_t b = t * part, e = std::min(n, b + part);
+    if (b < e) th.emplace_back([=] { std::memcpy(dst + b, src + b, e - b); });
+  }
+  std::memcpy(dst, src, std::min(n, part));
+  for (auto& x : th) x.join();
+}
+
+void copy_to_host(void* dst, const void* src, size_t n) {
+  if (n == 0) return;
+  HostStaging& h = host_staging();
+  std::lock_guard<std::mutex> lk(h.mu);
+  if (!h.tried) {
+    h.tried = true;
+    bool ok = hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking) == hipSuccess;
+    for (int i = 0; ok && i < 2; ++i) {
+      ok = hipHostMalloc(reinterpret_cast<void**>(&h.buf[i]), kStageBytes,
+                         hipHostMallocDefault) == hipSuccess &&
+           hipEventCreateWithFlags(&h.ev[i], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+      for (int i = 0; i < 2; ++i) {
+        if (h.buf[i]) (void)hipHostFree(h.buf[i]);
+        if (h.ev[i]) (void)hipEventDestroy(h.ev[i]);
+        h.buf[i] = nullptr;
+        h.ev[i] = nullptr;
+      }
+      if (h.s) (void)hipStreamDestroy(h.s);
+      h.s = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  if (!h.s || n < 2 * kStageBytes) {
+    BIE_HIP_CHECK(hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+    return;
+  }
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  const size_t nchunk = (n + kStageBytes - 1) / kStageBytes;
+  for (size_t c = 0; c <= nchunk; ++c) {
+    if (c < nchunk) {  // chunk c into buffer c % 2 (its previous contents were copied out)
+      const size_t off = c * kStageBytes, len = std::min(kStageBytes, n - off);
+      BIE_HIP_CHECK(hipMemcpyAsync(h.buf[c & 1], s + off, len, hipMemcpyDeviceToHost, h.s));
+      BIE_HIP_CHECK(hipEventRecord(h.ev[c & 1], h.s));
+    }
+    if (c > 0) {  // chunk c - 1 out of its buffer while chunk c is in flight
+      const size_t off = (c - 1) * kStageBytes, len = std::min(kStageBytes, n - off);
+      BIE_HIP_CHECK(hipEventSynchronize(h.ev[(c - 1) & 1]));
+      parallel_copy(d + off, h.buf[(c - 1) & 1], len);
+    }
+  }
+}
+
+}  // namespace
+
 int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* targets,
                  int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
                  double* G, double* C) {
@@ -935,8 +1021,8 @@ int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* t
                          nullptr);
     BIE_HIP_CHECK(hipDeviceSynchronize());
     if (fg) {
-      BIE_HIP_CHECK(hipMemcpy(F, dF, fg * sizeof(double), hipMemcpyDeviceToHost));
-      BIE_HIP_CHECK(hipMemcpy(G, dG, fg * sizeof(double), hipMemcpyDeviceToHost));
+      copy_to_host(F, dF, fg * sizeof(double));
+      copy_to_host(G, dG, fg * sizeof(double));
     }
     if (M) BIE_HIP_CHECK(hipMemcpy(C, dC, 16 * M, hipMemcpyDeviceToHost));
   });

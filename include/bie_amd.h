@@ -64,7 +64,9 @@ int bie_version(void);
  * The device form of the last mesh (upload, rule tables, schedule, recorded mid-edge pairs)
  * is kept for the next call: a call whose mesh matches it byte for byte skips that setup
  * (a frequency loop calling assemble_parallel! once per k); device memory it holds is
- * released when another mesh replaces it or the process ends. */
+ * released when another mesh replaces it or the process ends.  Outputs of 256 MiB or more
+ * are copied back through two pinned 128 MiB host buffers (allocated on first use, kept for
+ * the process) with host threads. */
 int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* targets,
                  int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
                  double* G, double* C);
