@@ -26,6 +26,22 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "boundaryintegralequations.jl_amd")
 sys.path.insert(0, PKG)
+# the CPU baseline's OpenMP threads are pinned one per core, next to each other; the
+# OpenMP runtime reads these when it starts, before anything here loads it
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def cpu_model():
+    """The host CPU's model name (first "model name" line of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 METRIC = ("FP64 F/G/C entries/s (+% FP64 peak) at ≤1e-12 rel err; "
           "batched-sweep vs per-k speedup")
@@ -186,19 +202,24 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
     # F now holds the F-only sweep's F_k and C: the same matrices the GMRES below needs
     # GMRES (tol 1e-10, restart 60) for the rigid sphere under p_inc = e^{ikz},
     # A_k = F_k + diag(1 - C), all 64 systems advanced in lockstep
-    from bie_amd.solvers import collocation_operator_batched, gmres_batched
-    Fc = torch.view_as_complex(F.view(nk, N, N, 2))
-    Cc = torch.view_as_complex(C.view(N, 2))
+    # the library's batched GMRES (bie_gmres_batched_device) on the sweep's F_k in place
+    from bie_amd import solvers
     z = torch.as_tensor(mesh.sources[2], device=dev)
-    op = collocation_operator_batched(torch, Fc, Cc)
-    # untimed warm-up (three Arnoldi steps): the first calls load the library kernels
-    gmres_batched(torch, op, torch.exp(1j * kd[:, None] * z[None, :]), tol=1e-10, maxiter=3)
+    Bt = torch.exp(1j * kd[:, None] * z[None, :]).contiguous()
+    Xt = torch.empty_like(Bt)
+
+    def solve(maxiter):
+        return solvers.gmres_batched(N, nk, F.data_ptr(), N * N, N, C.data_ptr(),
+                                     Bt.data_ptr(), N, Xt.data_ptr(), N, tol=1e-10,
+                                     maxiter=maxiter, stream=stream)
+
+    solve(3)   # untimed warm-up (three Arnoldi steps): the first calls load the kernels
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    Bt = torch.exp(1j * kd[:, None] * z[None, :])
-    _, its, rrs = gmres_batched(torch, op, Bt, tol=1e-10)
+    its, rrs = solve(2000)
     torch.cuda.synchronize()
     res["gmres"] = {"tol": 1e-10, "total_s": time.perf_counter() - t0, "batched": True,
+                    "solver": "bie_gmres_batched_device",
                     "iterations_min_mean_max": [int(its.min()), float(its.mean()), int(its.max())],
                     "max_rel_residual": float(rrs.max())}
     res["speedup"] = res["per_k_launches_ms"] / res["batched_graph_ms"]
@@ -214,9 +235,9 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
 def measure_adjoint(torch, reps=2):
     """Config 4: 2562-node linear sphere with radial N(0, 0.01^2) perturbation (seed 1), k = 5,
     100 Fibonacci field points on r = 3, (n,m) = (4,4).  Wall time of one evaluation of
-    J = sum |u|^2 (two GPU assemblies, the LU solve on the host) and of J with its adjoint
-    gradient (7686 coordinates and k: two assembly VJP launches besides the forward and
-    adjoint solves); central differences would take 2 x 7687 evaluations."""
+    J = sum |u|^2 (two GPU assemblies, the LU factorisation and solve on the device) and of J
+    with its adjoint gradient (7686 coordinates and k: two assembly VJP launches besides the
+    forward and adjoint solves); central differences would take 2 x 7687 evaluations."""
     from bie_amd import mesh as M
     from bie_amd.adjoint import scattering_gradient, scattering_objective
     mesh = M.perturb_radial(M.icosphere(16, "linear"), 0.01, 1)
@@ -527,7 +548,9 @@ def main():
                   np.linalg.norm(Gg - Go) / np.linalg.norm(Go),
                   np.linalg.norm(Cg - Co) / np.linalg.norm(Co))
         cpu = {"value": (2 * len(rows) * N + len(rows)) / t_cpu, "unit": "entries/s",
-               "cores": threads, "kind": "port",
+               "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+               "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+                       "OMP_PLACES": os.environ.get("OMP_PLACES")},
                "sample": f"{len(rows)} of {N} target rows of the same workload (all {E} "
                          f"elements), median of {max(1, len(times) - 1)} runs after a warm-up, "
                          f"{t_cpu:.2f} s wall each on {threads} threads; GPU vs CPU rel "
@@ -552,7 +575,7 @@ def main():
         "config": {"workload": f"quadratic geodesic sphere r=1, N={N} nodes, E={E} tri6, "
                                f"k={args.k}, collocation, (n,m)=({n},{m}), complex128 F/G/C",
                    "n": n, "m": m, "k": args.k, "n_nodes": N, "n_elements": E},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP64,
                      "traffic": measured_traffic(N, n, m, args.k),
                      "kernel": kernel_name, "kernel_ms": main_res["kernel_ms"],

@@ -30,7 +30,20 @@ class BieMesh(ctypes.Structure):
                 ("physics_topology", _dp), ("n_coordinates", ctypes.c_int64),
                 ("n_elements", ctypes.c_int64), ("n_sources", ctypes.c_int64),
                 ("geometry_kind", ctypes.c_int32), ("physics_kind", ctypes.c_int32),
-                ("index_base", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("index_base", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+# bie_mesh.flags: which n-ary hypot rule selection compares distances with
+BIE_MESH_SCALED_HYPOT = 1
+HYPOT_FLAGS = {"julia1.9": 0, "scaled": BIE_MESH_SCALED_HYPOT}
+
+
+def hypot_flag(hypot: str) -> int:
+    """bie_mesh.flags for the hypot formula of rule selection: "julia1.9" (the reference's
+    CI Julia: maxabs * sqrt(sum((x/maxabs)^2))) or "scaled" (later Julia releases:
+    power-of-two scaled sqrt(x^2 + y^2 + z^2))."""
+    if hypot not in HYPOT_FLAGS:
+        raise ValueError(f"hypot must be one of {sorted(HYPOT_FLAGS)}")
+    return HYPOT_FLAGS[hypot]
 
 
 class BieError(RuntimeError):
@@ -70,6 +83,19 @@ SIGNATURES = {
     "bie_zgemm_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp]),
+    "bie_host_cache_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64)] * 4),
+    "bie_lu_factor_device": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, _dp, _dp, _dp]),
+    "bie_lu_solve_device": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, _dp,
+                                            ctypes.c_int32, _dp, ctypes.c_int64, ctypes.c_int64,
+                                            _dp]),
+    "bie_zgemv_device": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _dp,
+                                         ctypes.c_int64, _dp, _dp, _dp]),
+    "bie_gmres_batched_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _dp,
+                                                 ctypes.c_int64, ctypes.c_int64, _dp, _dp,
+                                                 ctypes.c_int64, _dp, ctypes.c_int64,
+                                                 ctypes.c_double, ctypes.c_int32,
+                                                 ctypes.c_int32, _dp, _dp, _dp]),
+    "bie_host_cache_release": (ctypes.c_int, []),
     "bie_assemble_sweep_device": (ctypes.c_int, [_dp, _dp, ctypes.c_int32, _dp, ctypes.c_int64,
                                                   ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.c_uint32, _dp, _dp, ctypes.c_int64,
@@ -104,6 +130,23 @@ SIGNATURES = {
                                       ctypes.POINTER(_dp), ctypes.POINTER(_dp),
                                       ctypes.POINTER(_dp), _dp,
                                       ctypes.POINTER(ctypes.c_double)]),
+    "bie_batch_dfdk_device": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp,
+                                             _dp]),
+    "bie_sparse_count": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int64)]),
+    "bie_sparse_assemble": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_double,
+                                           ctypes.c_double, _dp, ctypes.c_int64,
+                                           ctypes.c_int32, ctypes.c_double, ctypes.c_uint32,
+                                           ctypes.c_int64, _dp, _dp, _dp, _dp]),
+    "bie_sparse_create": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_double,
+                                         ctypes.POINTER(_dp)]),
+    "bie_sparse_destroy": (ctypes.c_int, [_dp]),
+    "bie_sparse_nnz": (ctypes.c_int64, [_dp]),
+    "bie_sparse_pattern": (ctypes.c_int, [_dp, ctypes.c_int32, _dp, _dp]),
+    "bie_sparse_assemble_device": (ctypes.c_int, [_dp, ctypes.c_double, ctypes.c_double, _dp,
+                                                   ctypes.c_uint32, _dp, _dp, _dp]),
     "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),
@@ -155,11 +198,11 @@ def _ptr(a):
 class HostMesh:
     """Keeps Fortran-ordered copies alive while a bie_mesh struct points at them."""
 
-    def __init__(self, m: "_mesh.Mesh3d"):
+    def __init__(self, m: "_mesh.Mesh3d", flags: int = 0):
         self.coords = np.asfortranarray(m.coordinates, dtype=np.float64)
         self.topo = np.asfortranarray(m.topology, dtype=np.int64)
         self.sources = np.asfortranarray(m.sources, dtype=np.float64)
         self.ptopo = np.asfortranarray(m.physics_topology, dtype=np.int64)
         self.struct = BieMesh(_ptr(self.coords), _ptr(self.topo), _ptr(self.sources),
                               _ptr(self.ptopo), m.n_coordinates, m.n_elements, m.n_sources,
-                              m.geometry_kind, m.physics_kind, 0, 0)
+                              m.geometry_kind, m.physics_kind, 0, flags)

@@ -29,15 +29,24 @@ from .mesh import Mesh3d
 
 
 class _Dev:
-    def __init__(self, arr: np.ndarray):
-        arr = np.ascontiguousarray(arr)
+    """Device buffer from the library's allocator (bie_device_malloc), optionally filled
+    from a host array."""
+
+    def __init__(self, arr: np.ndarray | None = None, nbytes: int = 0):
+        if arr is not None:
+            arr = np.ascontiguousarray(arr)
+            nbytes = arr.nbytes
         p = ctypes.c_void_p()
-        _lib.check(_lib.lib().bie_device_malloc(ctypes.byref(p), int(max(arr.nbytes, 8))))
+        _lib.check(_lib.lib().bie_device_malloc(ctypes.byref(p), int(max(nbytes, 8))))
         self.ptr = p.value
-        self.nbytes = arr.nbytes
-        if arr.nbytes:
+        self.nbytes = nbytes
+        if arr is not None and arr.nbytes:
             _lib.check(_lib.lib().bie_memcpy_h2d(ctypes.c_void_p(self.ptr), _lib._ptr(arr),
                                                  arr.nbytes))
+
+    @classmethod
+    def empty(cls, nbytes: int) -> "_Dev":
+        return cls(None, nbytes)
 
     def get(self, out: np.ndarray) -> np.ndarray:
         _lib.check(_lib.lib().bie_memcpy_d2h(_lib._ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
@@ -84,100 +93,88 @@ def plane_wave(x, k, direction=(0.0, 0.0, 1.0)):
     return np.exp(1j * k * (d @ x))
 
 
-def _forward(torch, dm: DeviceMesh, k: float, field_points, n, m, direction):
-    """J and the forward quantities, on the device of dm: the surface and field operators
-    assembled into device buffers, A p = b and the field pressure by device LU solves."""
+class _Forward:
+    """Device state of one objective evaluation: A = F + diag(1 - C) factored in place, the
+    field operator F_field, p and the small host vectors."""
+
+
+def _forward(dm: DeviceMesh, k: float, field_points, n, m, direction, stream=0):
+    """J and the forward quantities: the surface and field operators assembled into device
+    buffers, A p = b by LU on the device (A overwrites F), u = p_inc - F_field p; only
+    vectors of N and P entries come to the host."""
+    from . import solvers
     mesh = dm.mesh
-    dev = torch.device("cuda", dm.device)
-    stream = torch.cuda.current_stream(dev).cuda_stream
-    pts = np.asarray(field_points, dtype=np.float64)
+    pts = np.asfortranarray(field_points, dtype=np.float64)
     N, P = mesh.n_sources, pts.shape[1]
-    cplx = torch.complex128
-    Fm = torch.empty((N, N), dtype=cplx, device=dev)        # column-major F: Fm = F^T
-    Ffm = torch.empty((N, P), dtype=cplx, device=dev)       # column-major F_field
-    Gs = torch.empty(N * max(N, P), dtype=cplx, device=dev)  # G is not needed (zero-filled)
-    C = torch.empty(N, dtype=cplx, device=dev)
-    Cf = torch.empty(P, dtype=cplx, device=dev)
-    tp = torch.as_tensor(np.asfortranarray(pts).ravel(order="F"), device=dev)
-    dm.assemble(k, Fm.data_ptr(), Gs.data_ptr(), C.data_ptr(), n=n, m=m, gOn=False,
-                stream=stream)
-    dm.assemble(k, Ffm.data_ptr(), Gs.data_ptr(), Cf.data_ptr(), targets_ptr=tp.data_ptr(),
-                n_targets=P, n=n, m=m, gOn=False, cOn=False, stream=stream)
-    del Gs
-    d = torch.as_tensor(np.asarray(direction, dtype=np.float64), device=dev)
-    xs = torch.as_tensor(np.ascontiguousarray(mesh.sources), device=dev)
-    xf = torch.as_tensor(np.ascontiguousarray(pts), device=dev)
-    b = torch.exp(1j * k * (d @ xs))
-    pf = torch.exp(1j * k * (d @ xf))
-    A = Fm.T + torch.diag(1.0 - C)
-    p = torch.linalg.solve(A, b)
-    Ff = Ffm.T
-    u = pf - Ff @ p
-    J = float(torch.sum(u.real ** 2 + u.imag ** 2))
-    return J, dict(A=A, Ff=Ff, p=p, u=u, b=b, pf=pf, d=d, xs=xs, xf=xf, tp=tp,
-                   stream=stream, dev=dev)
+    fw = _Forward()
+    fw.dF = _Dev.empty(16 * N * N)               # F, then A and its LU factors
+    fw.dFf = _Dev.empty(16 * P * N)              # F_field, P x N column-major
+    dG = _Dev.empty(16 * N * max(N, P))          # G is not needed (zero-filled)
+    dC = _Dev.empty(16 * N)
+    dCf = _Dev.empty(16 * P)
+    fw.dpts = _Dev(pts.ravel(order="F"))
+    dm.assemble(k, fw.dF.ptr, dG.ptr, dC.ptr, n=n, m=m, gOn=False, stream=stream)
+    dm.assemble(k, fw.dFf.ptr, dG.ptr, dCf.ptr, targets_ptr=fw.dpts.ptr, n_targets=P, n=n,
+                m=m, gOn=False, cOn=False, stream=stream)
+    del dG
+    fw.ipiv = _Dev.empty(4 * N)
+    solvers.lu_factor(N, fw.dF.ptr, N, fw.ipiv.ptr, C_ptr=dC.ptr, stream=stream)
+    fw.b = plane_wave(mesh.sources, k, direction)
+    fw.pf = plane_wave(pts, k, direction)
+    dp = _Dev(fw.b)
+    solvers.lu_solve(N, fw.dF.ptr, N, fw.ipiv.ptr, dp.ptr, N, 1, stream=stream)
+    dfp = _Dev.empty(16 * P)
+    solvers.zgemv(P, N, fw.dFf.ptr, P, dp.ptr, dfp.ptr, stream=stream)
+    _lib.check(_lib.lib().bie_device_synchronize())
+    fw.p = dp.get(np.empty(N, dtype=np.complex128))
+    fw.u = fw.pf - dfp.get(np.empty(P, dtype=np.complex128))
+    fw.dp = dp
+    fw.pts = pts
+    J = float(np.sum(fw.u.real ** 2 + fw.u.imag ** 2))
+    return J, fw
 
 
 def scattering_objective(mesh: Mesh3d, k: float, field_points, *, n=4, m=4,
                          direction=(0.0, 0.0, 1.0), dmesh: DeviceMesh | None = None,
                          forward=True):
-    """J and the forward quantities (p, u, A, F_field as numpy arrays), computed on the GPU;
-    with forward=False only J (nothing else leaves the device)."""
-    import torch
+    """J and the forward quantities (p, u, b as numpy arrays), computed on the GPU; with
+    forward=False only J."""
     dm = dmesh if dmesh is not None else DeviceMesh(mesh)
-    J, fw = _forward(torch, dm, float(k), field_points, n, m, direction)
+    J, fw = _forward(dm, float(k), field_points, n, m, direction)
     if not forward:
         return J, None
-    return J, {key: fw[key].cpu().numpy() for key in ("A", "Ff", "p", "u", "b")}
-
-
-def _vjp_device(torch, dm: DeviceMesh, k, alpha, p, beta, targets_ptr, n_targets, target_node,
-                n, m, stream, dev):
-    """assembly_vjp on device tensors (complex128 alpha, p, beta; int32 target_node)."""
-    gX = torch.zeros(3 * dm.mesh.n_coordinates, dtype=torch.float64, device=dev)
-    gk = torch.zeros(1, dtype=torch.float64, device=dev)
-    # the library reads raw memory: materialise lazy conjugate / negative views first
-    def raw(t):
-        return t.resolve_conj().resolve_neg().contiguous()
-
-    alpha, p = raw(alpha), raw(p)
-    beta = raw(beta) if beta is not None else None
-    _lib.check(_lib.lib().bie_assembly_vjp_device(
-        dm.handle, float(k), ctypes.c_void_p(targets_ptr or None), int(n_targets),
-        ctypes.c_void_p(target_node.data_ptr() if target_node is not None else None),
-        ctypes.c_void_p(alpha.data_ptr()),
-        ctypes.c_void_p(beta.data_ptr() if beta is not None else None),
-        ctypes.c_void_p(p.data_ptr()), int(n), int(m), ctypes.c_void_p(gX.data_ptr()),
-        ctypes.c_void_p(gk.data_ptr()), ctypes.c_void_p(stream or None)))
-    return gX.view(dm.mesh.n_coordinates, 3).T, gk[0]
+    return J, {"p": fw.p, "u": fw.u, "b": fw.b}
 
 
 def scattering_gradient(mesh: Mesh3d, k: float, field_points, *, n=4, m=4,
                         direction=(0.0, 0.0, 1.0), dmesh: DeviceMesh | None = None):
     """J, dJ/d(mesh coordinates) (3 x Nc numpy array) and dJ/dk by the adjoint method.  The
-    operators, both solves and both vector-Jacobian products stay on the GPU; only J, the
-    gradient and dJ/dk come back."""
-    import torch
+    operators, their LU factors, both solves and both vector-Jacobian products stay on the
+    GPU; vectors of N and P entries, J, the gradient and dJ/dk come back."""
+    from . import solvers
     dm = dmesh if dmesh is not None else DeviceMesh(mesh)
     k = float(k)
-    J, fw = _forward(torch, dm, k, field_points, n, m, direction)
-    A, Ff, p, u, b = fw["A"], fw["Ff"], fw["p"], fw["u"], fw["b"]
-    stream, dev = fw["stream"], fw["dev"]
-    lam = torch.linalg.solve(A.conj().T, Ff.conj().T @ u)
-    nodes = torch.as_tensor(source_nodes(mesh), device=dev)
-    P = u.shape[0]
-    lam_c = torch.conj_physical(lam)
-    gs, gks = _vjp_device(torch, dm, k, lam_c, p, -lam_c * p, 0, mesh.n_sources, nodes, n, m,
-                          stream, dev)
-    gf, gkf = _vjp_device(torch, dm, k, -torch.conj_physical(u), p, None, fw["tp"].data_ptr(),
-                          P, None, n, m, stream, dev)
+    J, fw = _forward(dm, k, field_points, n, m, direction)
+    N, P = mesh.n_sources, fw.pts.shape[1]
+    # lambda = A^-H F_field^H u on the device
+    du = _Dev(fw.u)
+    dlam = _Dev.empty(16 * N)
+    solvers.zgemv(P, N, fw.dFf.ptr, P, du.ptr, dlam.ptr, trans="C")
+    solvers.lu_solve(N, fw.dF.ptr, N, fw.ipiv.ptr, dlam.ptr, N, 1, trans="C")
+    _lib.check(_lib.lib().bie_device_synchronize())
+    lam = dlam.get(np.empty(N, dtype=np.complex128))
+    lam_c = np.conj(lam)
+    p, u, b, pf = fw.p, fw.u, fw.b, fw.pf
+    nodes = source_nodes(mesh)
+    gs, gks = assembly_vjp(dm, k, lam_c, p, beta=-lam_c * p, target_node=nodes, n=n, m=m)
+    gf, gkf = assembly_vjp(dm, k, -np.conj(u), p, targets=fw.pts, n=n, m=m)
     grad = 2.0 * (gs + gf)
     dk = 2.0 * (gks + gkf)
-    d, xs, xf, pf = fw["d"], fw["xs"], fw["xf"], fw["pf"]
+    d = np.asarray(direction, dtype=np.float64)
     # -lambda^H db with b_i = e^{ik d.x_i}: d b_i/d x_i = i k d b_i, d b_i/dk = i (d.x_i) b_i
     coef = (-lam_c * 1j * k * b).real
-    grad = grad.index_add(1, nodes.long(), 2.0 * d[:, None] * coef[None, :])
-    dk = dk + 2.0 * (-lam_c * 1j * (d @ xs) * b).sum().real
+    np.add.at(grad, (slice(None), nodes), 2.0 * d[:, None] * coef[None, :])
+    dk += 2.0 * (-lam_c * 1j * (d @ mesh.sources) * b).sum().real
     # u^H dp_inc,field/dk
-    dk = dk + 2.0 * (u.conj() * 1j * (d @ xf) * pf).sum().real
-    return J, grad.cpu().numpy(), float(dk)
+    dk += 2.0 * (np.conj(u) * 1j * (d @ fw.pts) * pf).sum().real
+    return J, grad, float(dk)

@@ -26,14 +26,19 @@ def _flags(fOn, gOn, cOn):
 
 
 def assemble_parallel(mesh: Mesh3d, k, in_sources=None, *, fOn=True, gOn=True, cOn=True,
-                      sparse=False, m=4, n=4, progress=True, depth=2, offset=None):
-    """Dense F, G, C on linear/quadratic triangles (drop-in for assemble_parallel!)."""
+                      sparse=False, m=4, n=4, progress=True, depth=2, offset=None,
+                      hypot="julia1.9"):
+    """Dense F, G, C on linear/quadratic triangles (drop-in for assemble_parallel!).
+
+    ``hypot`` (an addition to the reference's keywords) names the n-ary hypot whose
+    rounding decides the rule of exactly tied (target, element) pairs: "julia1.9" (the
+    reference's CI Julia) or "scaled" (later Julia releases)."""
     if sparse:
         raise NotImplementedError(
             "sparse=true (the near-field assembly behind the fast operators) is not part of "
             "this library; call the reference's sparse_assemble_parallel!")
     del progress, depth, offset  # accepted for signature parity; unused by dense assembly
-    hm = _lib.HostMesh(mesh)
+    hm = _lib.HostMesh(mesh, _lib.hypot_flag(hypot))
     if in_sources is None or in_sources is mesh.sources:
         targets = None
         M = mesh.n_sources
@@ -82,9 +87,9 @@ class Graph:
 class DeviceMesh:
     """A mesh resident on one GPU (bie_dmesh), with its cached rule tables."""
 
-    def __init__(self, mesh: Mesh3d, device: int = 0):
+    def __init__(self, mesh: Mesh3d, device: int = 0, hypot: str = "julia1.9"):
         self.mesh = mesh
-        hm = _lib.HostMesh(mesh)
+        hm = _lib.HostMesh(mesh, _lib.hypot_flag(hypot))
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().bie_dmesh_create(ctypes.byref(hm.struct), int(device),
                                                ctypes.byref(h)))

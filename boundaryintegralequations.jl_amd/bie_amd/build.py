@@ -15,7 +15,10 @@ CSRC = os.path.join(PKG, "csrc")
 LIB_PATH = os.path.join(HERE, "libbie_amd.so")
 
 SOURCES = ["rules.cpp", "schedule.cpp", "assemble.hip", "sweep.hip", "adjoint.hip", "apply.hip",
-           "api.hip"]
+           "solve.hip", "sparse.hip", "api.hip"]
+# rocSOLVER (LU) and rocBLAS (GEMV) from the ROCm install, found at run time by rpath
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+LINK_LIBS = ["-L" + ROCM_LIB, "-Wl,-rpath," + ROCM_LIB, "-lrocsolver", "-lrocblas"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -50,7 +53,7 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB_PATH + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + LINK_LIBS
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH

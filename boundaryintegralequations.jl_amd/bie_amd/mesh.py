@@ -281,62 +281,170 @@ def _read_obj(path):
 
 
 def _read_off(path):
+    """OFF (Object File Format): a header line "OFF", then "nv nf ne", nv vertex lines and nf
+    face lines "n i_1 .. i_n [colour]".  Comments (#) and blank lines are skipped; the
+    colour, normal and texture variants (COFF, NOFF, STOFF, ...) carry extra values per
+    vertex, of which the first three (x, y, z) are used; values after a face's n indices
+    (a face colour) are ignored.  4OFF / nOFF (other dimensions) are rejected."""
     with open(path) as fh:
-        tok = [t for line in fh for t in line.split("#")[0].split()]
-    if not tok or not tok[0].endswith("OFF"):
+        rows = [r for r in (line.split("#")[0].split() for line in fh) if r]
+    if not rows:
         raise ValueError(f"{path}: not an OFF file")
-    nv, nf = int(tok[1]), int(tok[2])
-    p = 4
-    verts = [[float(tok[p + 3 * i + j]) for j in range(3)] for i in range(nv)]
-    p += 3 * nv
+    head = rows[0][0]
+    if not head.endswith("OFF") or not set(head[:-3]) <= set("STCN"):
+        raise ValueError(f"{path}: not an OFF file (header {head!r}; only 3D OFF variants "
+                         "such as OFF, COFF, NOFF, STOFF are read)")
+    rest = rows[0][1:]
+    r = 1
+    if not rest:                       # counts on their own line
+        rest, r = rows[1], 2
+    nv, nf = int(rest[0]), int(rest[1])
+    if len(rows) < r + nv + nf:
+        raise ValueError(f"{path}: expected {nv} vertices and {nf} faces")
+    verts = [[float(t) for t in rows[r + i][:3]] for i in range(nv)]
     faces = []
-    for _ in range(nf):
-        n = int(tok[p])
-        faces.append([int(t) for t in tok[p + 1:p + 1 + n]])
-        p += 1 + n
+    for row in rows[r + nv:r + nv + nf]:
+        n = int(row[0])
+        if len(row) < 1 + n:
+            raise ValueError(f"{path}: face with fewer than {n} indices")
+        faces.append([int(t) for t in row[1:1 + n]])
     return verts, faces
 
 
+_PLY_TYPES = {"char": "i1", "int8": "i1", "uchar": "u1", "uint8": "u1", "short": "i2",
+              "int16": "i2", "ushort": "u2", "uint16": "u2", "int": "i4", "int32": "i4",
+              "uint": "u4", "uint32": "u4", "float": "f4", "float32": "f4", "double": "f8",
+              "float64": "f8"}
+
+
 def _read_ply(path):
+    """PLY, ASCII or binary (little or big endian): the elements are read in header order;
+    x, y, z of "vertex" and the index list of "face" (vertex_indices or vertex_index) are
+    kept, every other element and property is skipped."""
     with open(path, "rb") as fh:
-        lines = fh.read().decode("latin-1").splitlines()
-    if not lines or lines[0].strip() != "ply":
+        data = fh.read()
+    end = data.find(b"end_header")
+    if not data.startswith(b"ply") or end < 0:
         raise ValueError(f"{path}: not a PLY file")
-    i, nv, nf, vprops, fmt, cur = 1, 0, 0, [], None, None
-    while lines[i].strip() != "end_header":
-        tok = lines[i].split()
+    body_at = data.index(b"\n", end) + 1
+    header = [ln.split() for ln in data[:end].decode("latin-1").splitlines()]
+    fmt, elems = None, []                       # elems: [name, count, [props]]
+    for tok in header[1:]:
+        if not tok or tok[0] in ("comment", "obj_info"):
+            continue
         if tok[0] == "format":
             fmt = tok[1]
         elif tok[0] == "element":
-            cur = tok[1]
-            if cur == "vertex":
-                nv = int(tok[2])
-            elif cur == "face":
-                nf = int(tok[2])
-        elif tok[0] == "property" and cur == "vertex":
-            vprops.append(tok[-1])
-        i += 1
-    if fmt != "ascii":
-        raise NotImplementedError(f"{path}: only ASCII PLY files are read")
-    ix = [vprops.index(c) for c in ("x", "y", "z")]
-    body = lines[i + 1:]
-    verts = [[float(body[k].split()[j]) for j in ix] for k in range(nv)]
-    faces = []
-    for k in range(nv, nv + nf):
-        tok = body[k].split()
-        faces.append([int(t) for t in tok[1:1 + int(tok[0])]])
+            elems.append([tok[1], int(tok[2]), []])
+        elif tok[0] == "property":
+            if not elems:
+                raise ValueError(f"{path}: property before any element")
+            if tok[1] == "list":                # property list <count type> <item type> name
+                elems[-1][2].append(("list", tok[2], tok[3], tok[4]))
+            else:
+                elems[-1][2].append(("scalar", tok[1], None, tok[2]))
+    if fmt not in ("ascii", "binary_little_endian", "binary_big_endian"):
+        raise ValueError(f"{path}: unknown PLY format {fmt!r}")
+    verts, faces = None, None
+    if fmt == "ascii":
+        toks = data[body_at:].decode("latin-1").split()
+        p = 0
+        for name, count, props in elems:
+            rows = []
+            for _ in range(count):
+                vals = {}
+                for kind, t1, t2, pname in props:
+                    if kind == "scalar":
+                        vals[pname] = float(toks[p])
+                        p += 1
+                    else:
+                        n = int(toks[p])
+                        vals[pname] = [int(float(t)) for t in toks[p + 1:p + 1 + n]]
+                        p += 1 + n
+                rows.append(vals)
+            if name == "vertex":
+                verts = [[r["x"], r["y"], r["z"]] for r in rows]
+            elif name == "face":
+                key = "vertex_indices" if rows and "vertex_indices" in rows[0] else "vertex_index"
+                faces = [r[key] for r in rows]
+    else:
+        bo = "<" if fmt == "binary_little_endian" else ">"
+        p = body_at
+        for name, count, props in elems:
+            if all(k == "scalar" for k, *_ in props):   # fixed-size records: one read
+                dt = np.dtype([(pn, bo + _PLY_TYPES[t1]) for _, t1, _, pn in props])
+                rec = np.frombuffer(data, dtype=dt, count=count, offset=p)
+                p += dt.itemsize * count
+                if name == "vertex":
+                    verts = np.stack([rec["x"], rec["y"], rec["z"]], 1).astype(np.float64).tolist()
+                continue
+            rows = []
+            for _ in range(count):
+                vals = {}
+                for kind, t1, t2, pname in props:
+                    if kind == "scalar":
+                        dt = np.dtype(bo + _PLY_TYPES[t1])
+                        vals[pname] = np.frombuffer(data, dtype=dt, count=1, offset=p)[0]
+                        p += dt.itemsize
+                    else:
+                        dc, di = np.dtype(bo + _PLY_TYPES[t1]), np.dtype(bo + _PLY_TYPES[t2])
+                        n = int(np.frombuffer(data, dtype=dc, count=1, offset=p)[0])
+                        p += dc.itemsize
+                        vals[pname] = np.frombuffer(data, dtype=di, count=n, offset=p).astype(int).tolist()
+                        p += di.itemsize * n
+                rows.append(vals)
+            if name == "face":
+                key = "vertex_indices" if rows and "vertex_indices" in rows[0] else "vertex_index"
+                faces = [r[key] for r in rows]
+            elif name == "vertex":
+                verts = [[float(r["x"]), float(r["y"]), float(r["z"])] for r in rows]
+    if verts is None or faces is None:
+        raise ValueError(f"{path}: PLY file without vertex or face element")
+    return verts, faces
+
+
+def _read_stl(path):
+    """STL, ASCII ("solid ... facet ... vertex x y z ... endsolid") or binary (80-byte header,
+    uint32 count, 50-byte facets).  STL stores every facet's three corners separately; corners
+    with identical coordinates are merged into one node (in order of first appearance), so
+    neighbouring facets share nodes as collocation needs."""
+    with open(path, "rb") as fh:
+        data = fh.read()
+    corners = None
+    if len(data) >= 84:
+        nfac = int(np.frombuffer(data, dtype="<u4", count=1, offset=80)[0])
+        if len(data) == 84 + 50 * nfac:
+            rec = np.frombuffer(data, dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)),
+                                                      ("a", "<u2")]), count=nfac, offset=84)
+            corners = rec["v"].reshape(-1, 3).astype(np.float64)
+    if corners is None:
+        text = data.decode("latin-1")
+        if not text.lstrip().startswith("solid"):
+            raise ValueError(f"{path}: not an STL file")
+        tok = text.split()
+        corners = np.array([[float(tok[i + 1]), float(tok[i + 2]), float(tok[i + 3])]
+                            for i, t in enumerate(tok) if t == "vertex"], dtype=np.float64)
+        if corners.shape[0] % 3:
+            raise ValueError(f"{path}: facets without three vertices")
+    uniq, first, inv = np.unique(corners, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)                  # nodes numbered by first appearance
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    verts = uniq[order].tolist()
+    faces = rank[inv.reshape(-1)].reshape(-1, 3).tolist()
     return verts, faces
 
 
 def read_mesh_file(path: str):
     """(coordinates 3 x Nc, topology 3 x E, 0-based) of a triangle mesh in Wavefront OBJ,
-    OFF or ASCII PLY: the formats ``load_mesh_file`` reads through MeshIO
+    OFF, PLY (ASCII or binary) or STL (ASCII or binary): the formats ``load_mesh_file`` reads
+    through MeshIO
     (src/3d/meshing.jl:8-34), restated from the published file formats.  Faces must be
     triangles (the linear-triangle path takes the first three nodes of every face)."""
     ext = os.path.splitext(path)[1].lower()
-    readers = {".obj": _read_obj, ".off": _read_off, ".ply": _read_ply}
+    readers = {".obj": _read_obj, ".off": _read_off, ".ply": _read_ply, ".stl": _read_stl}
     if ext not in readers:
-        raise NotImplementedError(f"{ext} meshes are not read here (OBJ, OFF, ASCII PLY are)")
+        raise NotImplementedError(f"{ext} meshes are not read here (OBJ, OFF, PLY, STL are)")
     verts, faces = readers[ext](path)
     if any(len(f) != 3 for f in faces):
         raise ValueError(f"{path}: only triangular faces are supported")

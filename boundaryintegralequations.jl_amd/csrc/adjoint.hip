@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
   const int e0 = blockIdx.y * ech;
   const int e1 = min(a.E, e0 + ech);
   for (int e = e0; e < e1; ++e) {
-    const int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
+    const int c =
+        select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R, a.hyp) - 1;
     double X[NG][3];
 #pragma unroll
     for (int q = 0; q < NG; ++q) {
@@ -157,7 +158,86 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
   if ((threadIdx.x & 63) == 0 && a.gradk) unsafeAtomicAdd(a.gradk, gks);
 }
 
+// Derivative in k of the assembly, contracted: w = alpha^T (dF/dk) p for one instance of an
+// uneven batch (blockIdx.x = tile: instance, 256-row block, element chunk).  With the
+// element tables of the assembly (y, the reference's normal n, J w / (4 pi) at every rule
+// point), d/dk of the double-layer integrand e^{ikr}(ikr - 1)(n.d) J w / (4 pi r^3)
+// (kernels.jl:29-37; its k-derivative is taylor_freens3d! at order 1, kernels.jl:104-118)
+// is -k e^{ikr} (n.d) J w / (4 pi r), holomorphic in k, so complex wavenumbers (the lossy
+// callers) are covered.  Every (target, element) pair is included, mid-edge rules too, with
+// the assembly's rule selection.  One thread per target row; the wave's partial sums are
+// added into w[instance] with one atomic per wave.
+template <int NP>
+__global__ void __launch_bounds__(256)
+k_dfdk_batch(const DfdkInst* __restrict__ inst, const int4* __restrict__ tiles, int ech) {
+  const int4 td = tiles[blockIdx.x];
+  const DfdkInst& a = inst[td.x];
+  const int64_t i = (int64_t)td.y * 256 + threadIdx.x;
+  const bool active = i < a.M;
+  const int64_t ic = active ? i : (a.M - 1);
+  const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
+  const double kr = a.kr, ki = a.ki;
+  double sr = 0.0, si = 0.0;  // sum over pairs and points of (n.d) Jw/(4 pi r) e^{ikr} p_h
+  const int e0 = td.z * ech, e1 = min(a.E, e0 + ech);
+  for (int e = e0; e < e1; ++e) {
+    const int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R, a.hyp) - 1;
+    double pr[NP], pim[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      const double2 v = a.p[a.ptopo[(size_t)e * NP + s]];
+      pr[s] = v.x;
+      pim[s] = v.y;
+    }
+    const double* tab = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
+    const double* ph = a.phi + (size_t)c * a.Qmax * NP;
+    const int Q = a.len[c];
+    for (int q = 0; q < Q; ++q) {
+      const double4 A = *reinterpret_cast<const double4*>(tab + 8 * q);
+      const double4 B = *reinterpret_cast<const double4*>(tab + 8 * q + 4);
+      const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
+      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      const double rinv = rsqrt_nr(r2);
+      const double r = r2 * rinv;
+      const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
+      double sn, cs;
+      sincos_f64(kr * r, &sn, &cs);
+      double amp = nd * B.z * rinv;  // (n.d) J w / (4 pi r)
+      if (ki != 0.0) amp *= exp(-ki * r);
+      double phr = 0.0, phi_ = 0.0;
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        const double b = ph[q * NP + s];
+        phr = fma(b, pr[s], phr);
+        phi_ = fma(b, pim[s], phi_);
+      }
+      // amp e^{i kr r} (phr + i phi)
+      sr = fma(amp, cs * phr - sn * phi_, sr);
+      si = fma(amp, cs * phi_ + sn * phr, si);
+    }
+  }
+  // w += alpha (-k) S
+  const double2 al = active ? a.alpha[ic] : make_double2(0.0, 0.0);
+  const double tr = -(kr * sr - ki * si), ti = -(kr * si + ki * sr);
+  const double wr = wave_sum(al.x * tr - al.y * ti), wi = wave_sum(al.x * ti + al.y * tr);
+  if ((threadIdx.x & 63) == 0) {
+    unsafeAtomicAdd(a.w, wr);
+    unsafeAtomicAdd(a.w + 1, wi);
+  }
+}
+
 }  // namespace
+
+void launch_dfdk_batch(const DfdkInst* d_inst, const int4* d_tiles, int n_tiles, int ech, int NP,
+                       hipStream_t s) {
+  if (n_tiles <= 0) return;
+  if (NP == 3)
+    hipLaunchKernelGGL(k_dfdk_batch<3>, dim3((unsigned)n_tiles), dim3(256), 0, s, d_inst, d_tiles,
+                       ech);
+  else
+    hipLaunchKernelGGL(k_dfdk_batch<6>, dim3((unsigned)n_tiles), dim3(256), 0, s, d_inst, d_tiles,
+                       ech);
+  BIE_HIP_CHECK(hipGetLastError());
+}
 
 void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;

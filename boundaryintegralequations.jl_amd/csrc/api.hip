@@ -71,6 +71,7 @@ void validate_mesh(const bie_mesh* m) {
   BIE_REQUIRE(!(m->geometry_kind == BIE_TRI3 && m->physics_kind == BIE_TRI6), BIE_EUNSUPPORTED,
               "quadratic physics on linear geometry is not a reference configuration");
   BIE_REQUIRE(m->index_base == 0 || m->index_base == 1, BIE_EINVAL, "index_base must be 0 or 1");
+  BIE_REQUIRE((m->flags & ~BIE_MESH_SCALED_HYPOT) == 0, BIE_EINVAL, "unknown mesh flag bits");
   BIE_REQUIRE(m->n_elements >= 0 && m->n_coordinates >= 0 && m->n_sources >= 0, BIE_EINVAL,
               "negative mesh size");
   BIE_REQUIRE(m->n_elements <= (1LL << 28) && m->n_coordinates < (1LL << 31) &&
@@ -166,8 +167,21 @@ bool use_atomic_kernel() {
 // LDS slots per workgroup of the single-k slot kernel: as many as fit beside the basis
 // and trig tables, the staged table rows and the mid-edge queue (one workgroup per CU);
 // BIE_AMD_SLOTS=<n> asks for fewer.
-int plain_slot_count(int nphi, int Qc) {
-  int S = fitting_slots(phi_pad(nphi) + kTrigDoubles + stage_doubles(Qc), 1);
+int slot_occupancy() {
+  const char* v = std::getenv("BIE_AMD_OCC");
+  return (v && std::atoi(v) == 2) ? 2 : 1;
+}
+
+int slot_split() {
+  const char* v = std::getenv("BIE_AMD_SPLIT");
+  return (v && std::atoi(v) == 2) ? 2 : 1;
+}
+
+int plain_slot_count(int nphi, int Qc, int occ = 1, int split = 1) {
+  // LDS of one workgroup when occ of them share a CU (allocation granularity left spare)
+  const long budget = occ == 2 ? kLdsBytes / 2 - 1024 : kLdsBytes;
+  const int stage = split == 2 ? split_stage_doubles(Qc, 2) : stage_doubles(Qc);
+  int S = fitting_slots(phi_pad(nphi) + kTrigDoubles + stage, 1, budget);
   if (const char* v = std::getenv("BIE_AMD_SLOTS")) {
     const int x = std::atoi(v);
     if (x >= 8 && x < S) S = x;
@@ -205,20 +219,21 @@ void free_schedules(bie_dmesh* dm) {
 // segment) per CU, and among counts up to twice that, the one whose tile count fills whole
 // rounds of resident workgroups (one per CU) best, so that the last round leaves few CUs
 // idle.  (The schedule may split further where a segment's front exceeds the slots.)
-int choose_segment_count(int device, int64_t rowblocks, int64_t E) {
+int choose_segment_count(int device, int64_t rowblocks, int64_t E, int per_cu = 1) {
   static int cus_of[16] = {0};
   int& cus = cus_of[device & 15];
   if (cus == 0) {
     BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (cus <= 0) cus = 1;
   }
+  const int64_t round = (int64_t)cus * per_cu;  // workgroups resident at once
   const int64_t hi_e = std::max<int64_t>(1, E);
-  const int64_t base = std::max<int64_t>(1, std::min<int64_t>(hi_e, (8LL * cus + rowblocks - 1) / rowblocks));
+  const int64_t base = std::max<int64_t>(1, std::min<int64_t>(hi_e, (8LL * round + rowblocks - 1) / rowblocks));
   int64_t best = base;
   double best_waste = 1e300;
   for (int64_t p = base; p <= std::min<int64_t>(hi_e, 2 * base); ++p) {
     const int64_t tiles = rowblocks * p;
-    const double waste = (double)((tiles + cus - 1) / cus * cus) / (double)tiles;
+    const double waste = (double)((tiles + round - 1) / round * round) / (double)tiles;
     if (waste < best_waste - 1e-12) {
       best_waste = waste;
       best = p;
@@ -389,6 +404,7 @@ AssemblyArgs make_args(bie_dmesh* dm, const double* d_targets, int64_t M, double
     a.len[c] = dm->rs.len[c];
   }
   a.R = dm->rs.R;
+  a.hyp = dm->hyp;
   a.kr = kr;
   a.ki = ki;
   a.flags = flags;
@@ -398,9 +414,53 @@ AssemblyArgs make_args(bie_dmesh* dm, const double* d_targets, int64_t M, double
   return a;
 }
 
+// Whether stream s is capturing a graph (work enqueued there is ordered by the graph's
+// later launches, not now).
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  BIE_HIP_CHECK(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Calls on one handle share its device scratch (element tables, schedules, the mid-edge
+// queue, counter and values), so a call's kernels must not overlap the previous call's:
+// order_after_previous makes stream s wait for the last call on the handle when that call
+// ran on another stream, and mark_done records this call's end.  Called under dm->mu.
+void order_after_previous(bie_dmesh* dm, hipStream_t s) {
+  if (!dm->done_valid || dm->done_stream == s || capturing(s)) return;
+  BIE_HIP_CHECK(hipStreamWaitEvent(s, dm->done, 0));
+}
+
+void mark_done(bie_dmesh* dm, hipStream_t s) {
+  if (capturing(s)) return;
+  BIE_HIP_CHECK(hipEventRecord(dm->done, s));
+  dm->done_stream = s;
+  dm->done_valid = true;
+}
+
+// Largest distance from the mesh centre to any of n points (3 x n, column-major).
+double radius_about(const double* c, const double* x, int64_t n) {
+  double r2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double a = x[3 * i] - c[0], b = x[3 * i + 1] - c[1], d = x[3 * i + 2] - c[2];
+    r2 = std::max(r2, a * a + b * b + d * d);
+  }
+  return std::sqrt(r2);
+}
+
+// Whether every argument k r of the slot kernel's sincos stays below kTabFastMaxArg: r is
+// at most |x - c| + |y - c| for a target x and an element point y, and y = sum N_k X_k with
+// sum |N_k| <= 2 on linear and quadratic triangles, so |y - c| <= 2 coord_radius (taken
+// twice again for margin).  target_radius < 0: unknown (device targets), never fast.
+bool fast_trig_ok(const bie_dmesh* dm, double kr, double ki, double target_radius) {
+  if (ki != 0.0 || !(target_radius >= 0.0)) return false;
+  const double reach = target_radius + 4.0 * dm->coord_radius;
+  return std::isfinite(reach) && std::fabs(kr) * reach < 0.5 * kTabFastMaxArg;
+}
+
 void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_targets,
                           int64_t M, int n, int m, uint32_t flags, double* dF, double* dG,
-                          double* dC, hipStream_t s) {
+                          double* dC, hipStream_t s, double target_radius = -1.0) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
   BIE_REQUIRE(M >= 0, BIE_EINVAL, "negative number of targets");
   BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
@@ -409,9 +469,11 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
   if (d_targets == nullptr) {
     d_targets = dm->d_sources;
     BIE_REQUIRE(M == dm->N, BIE_EINVAL, "collocation (NULL targets) needs n_targets == n_sources");
+    target_radius = dm->source_radius;
   }
   BIE_HIP_CHECK(hipSetDevice(dm->device));
   std::lock_guard<std::mutex> lk(dm->mu);
+  order_after_previous(dm, s);
   ensure_rules(dm, n, m, s);
   std::array<hipEvent_t, 4> ev{};
   bool timed = false;
@@ -438,9 +500,13 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
     } else {
       const int64_t rowblocks = (M + 63) / 64;
-      const int min_segments = choose_segment_count(dm->device, rowblocks, dm->E);
-      const bie_dmesh::Sched& sc =
-          ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0]), min_segments, s);
+      const bool fast = fast_trig_ok(dm, kr, ki, target_radius);
+      const bool qc_inst = fast && (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16);
+      const int occ = qc_inst ? slot_occupancy() : 1;
+      const int split = qc_inst && occ == 1 ? slot_split() : 1;
+      const int min_segments = choose_segment_count(dm->device, rowblocks, dm->E, occ);
+      const bie_dmesh::Sched& sc = ensure_schedule(
+          dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0], occ, split), min_segments, s);
       // collocation on quadratic physics: after the first call the mid-edge pairs come
       // from the recorded sorted list instead of the slot kernel's queue
       const bool colloc = d_targets == dm->d_sources && M == dm->N;
@@ -453,12 +519,14 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       }
       SlotArgs sa = slot_args(dm, sc, rowblocks);
       sa.push_mid = recorded ? 0 : 1;
+      sa.occupancy = occ;
+      sa.split = split;
       // The slot kernel stores owned columns once; only shared/untouched columns need
       // zeros (a disabled output is zeroed whole above and never written).
       if (flags & BIE_F) launch_zero_columns(dF, M, sc.d_zero_cols, sc.zero_cols, s);
       if (flags & BIE_G) launch_zero_columns(dG, M, sc.d_zero_cols, sc.zero_cols, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
-      launch_assemble_slots(a, sa, s);
+      launch_assemble_slots(a, sa, fast, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
       if (recorded) {
         launch_mid_recorded(a, dm->d_mid_keys, dm->n_mid_keys, dm->d_mid_vals, dm->d_mid_entries,
@@ -477,6 +545,7 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
     std::lock_guard<std::mutex> tl(g_timing.mu);
     g_timing.rec.push_back(ev);
   }
+  mark_done(dm, s);
 }
 
 // Batched sweep over nk real wavenumbers (d_k on the device), or with taylor the k
@@ -521,6 +590,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
     dm->block_uniform_cap = std::max(nkb, 1);
   }
   if (prepare_only) return;
+  order_after_previous(dm, s);
   launch_fill_zero(dC, 2 * M, s);
   launch_fill_zero_u32(dm->d_mid_count, s);
   for (int which = 0; which < 2; ++which) {
@@ -531,7 +601,10 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
     else if (X)
       for (int q = 0; q < nk; ++q) launch_fill_zero(X + 2 * (size_t)q * stride, 2 * M * dm->N, s);
   }
-  if (M == 0 || dm->E == 0 || nk == 0) return;
+  if (M == 0 || dm->E == 0 || nk == 0) {
+    mark_done(dm, s);
+    return;
+  }
   refresh_geometry(dm, s);
   if (!taylor) launch_block_uniform(d_k, nk, KBk, dm->d_block_uniform, s);
   const int64_t rowblocks = (M + 63) / 64;
@@ -547,6 +620,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   sw.k_re = k_re;
   sw.k_im = k_im;
   launch_sweep(a, sa, sw, s);
+  mark_done(dm, s);
 }
 
 
@@ -641,6 +715,7 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
   std::lock_guard<std::mutex> bl(b->mu);
   MeshLocks locks(b->meshes);
   for (bie_dmesh* dm : b->meshes) {
+    order_after_previous(dm, s);
     ensure_rules(dm, b->n, b->m, s);
     if (!dm->geom_valid) refresh_geometry(dm, s);
     if (dm->pk == 6)
@@ -683,6 +758,80 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
                           nstage, b->slots_used, b->pk, cplx, grid, s);
   }
   if (b->pk == 6) launch_assemble_mid_batch(b->d_inst, n_inst, cplx, s);
+  for (bie_dmesh* dm : b->meshes) mark_done(dm, s);
+}
+
+// w_i = alpha_i^T (dF_i/dk) p_i for every instance of the batch, in one launch.
+void batch_dfdk_impl(bie_batch* b, const double* k_re, const double* k_im,
+                     const double* const* d_alpha, const double* const* d_p, double* d_w,
+                     hipStream_t s) {
+  BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
+  BIE_REQUIRE(k_re && d_alpha && d_p && d_w, BIE_EINVAL, "NULL argument array");
+  const int n_inst = (int)b->meshes.size();
+  for (int ii = 0; ii < n_inst; ++ii) {
+    const double ki = k_im ? k_im[ii] : 0.0;
+    BIE_REQUIRE(std::isfinite(k_re[ii]) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+    BIE_REQUIRE(d_alpha[ii] && d_p[ii], BIE_EINVAL, "alpha or p pointer is NULL");
+  }
+  BIE_HIP_CHECK(hipSetDevice(b->device));
+  std::lock_guard<std::mutex> bl(b->mu);
+  MeshLocks locks(b->meshes);
+  std::vector<DfdkInst> h(n_inst);
+  std::vector<int4> tiles;
+  constexpr int ech = 64;  // elements per tile
+  for (int ii = 0; ii < n_inst; ++ii) {
+    bie_dmesh* dm = b->meshes[ii];
+    order_after_previous(dm, s);
+    ensure_rules(dm, b->n, b->m, s);
+    if (!dm->geom_valid) refresh_geometry(dm, s);
+    DfdkInst& a = h[ii];
+    a = DfdkInst{};
+    a.targets = dm->d_sources;
+    a.M = dm->N;
+    a.E = (int)dm->E;
+    a.Pe = dm->rs.Pe;
+    a.elem = dm->d_elem;
+    a.selnodes = dm->d_selnodes;
+    a.ptopo = dm->d_ptopo;
+    a.phi = dm->d_phi;
+    a.Qmax = dm->rs.Qmax;
+    a.R = dm->rs.R;
+    a.hyp = dm->hyp;
+    for (int c = 0; c < 6; ++c) {
+      a.off[c] = dm->rs.off[c];
+      a.len[c] = dm->rs.len[c];
+    }
+    a.kr = k_re[ii];
+    a.ki = k_im ? k_im[ii] : 0.0;
+    a.alpha = reinterpret_cast<const double2*>(d_alpha[ii]);
+    a.p = reinterpret_cast<const double2*>(d_p[ii]);
+    a.w = d_w + 2 * (size_t)ii;
+    const int64_t rbs = (dm->N + 255) / 256, chunks = (dm->E + ech - 1) / ech;
+    for (int64_t rb = 0; rb < rbs; ++rb)
+      for (int64_t ch = 0; ch < chunks; ++ch) tiles.push_back(make_int4(ii, (int)rb, (int)ch, 0));
+  }
+  BIE_REQUIRE(tiles.size() < (size_t)INT32_MAX, BIE_EINVAL, "batch has too many tiles");
+  DfdkInst* d_inst = dalloc<DfdkInst>(h.size());
+  int4* d_tiles = nullptr;
+  try {
+    d_tiles = dalloc<int4>(tiles.size());
+    BIE_HIP_CHECK(hipMemcpyAsync(d_inst, h.data(), sizeof(DfdkInst) * h.size(),
+                                 hipMemcpyHostToDevice, s));
+    if (!tiles.empty())
+      BIE_HIP_CHECK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * tiles.size(),
+                                   hipMemcpyHostToDevice, s));
+    launch_fill_zero(d_w, 2 * (int64_t)n_inst, s);
+    launch_dfdk_batch(d_inst, d_tiles, (int)tiles.size(), ech, b->pk, s);
+    // the per-call tables are freed once the launch has read them
+    BIE_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    dfree(d_inst);
+    dfree(d_tiles);
+    throw;
+  }
+  dfree(d_inst);
+  dfree(d_tiles);
+  for (bie_dmesh* dm : b->meshes) mark_done(dm, s);
 }
 
 void batch_destroy_impl(bie_batch* b) {
@@ -733,11 +882,24 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     dm->h_coords.assign(mesh->coordinates, mesh->coordinates + 3 * mesh->n_coordinates);
     dm->h_sources.assign(mesh->sources, mesh->sources + 3 * mesh->n_sources);
     dm->device = device;
+    dm->hyp = (mesh->flags & BIE_MESH_SCALED_HYPOT) ? 1 : 0;
     dm->gk = mesh->geometry_kind;
     dm->pk = mesh->physics_kind;
     dm->Nc = mesh->n_coordinates;
     dm->E = mesh->n_elements;
     dm->N = mesh->n_sources;
+    if (dm->Nc > 0) {  // bounding-box centre of the coordinates
+      for (int d = 0; d < 3; ++d) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int64_t v = 0; v < dm->Nc; ++v) {
+          lo = std::min(lo, dm->h_coords[3 * v + d]);
+          hi = std::max(hi, dm->h_coords[3 * v + d]);
+        }
+        dm->center[d] = 0.5 * (lo + hi);
+      }
+    }
+    dm->coord_radius = radius_about(dm->center, dm->h_coords.data(), dm->Nc);
+    dm->source_radius = radius_about(dm->center, dm->h_sources.data(), dm->N);
     try {
       dm->d_coords = dalloc<double>((size_t)3 * dm->Nc);
       dm->d_sources = dalloc<double>((size_t)3 * dm->N);
@@ -753,6 +915,7 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
       if (!ptopo.empty())
         BIE_HIP_CHECK(hipMemcpy(dm->d_ptopo, ptopo.data(), ptopo.size() * sizeof(int), hipMemcpyHostToDevice));
       launch_gather_selnodes(dm->d_sources, dm->d_ptopo, dm->pk, (int)dm->E, dm->d_selnodes, nullptr);
+      BIE_HIP_CHECK(hipEventCreateWithFlags(&dm->done, hipEventDisableTiming));
       BIE_HIP_CHECK(hipDeviceSynchronize());
     } catch (...) {
       bie_dmesh_destroy(dm.release());
@@ -765,6 +928,10 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
 int bie_dmesh_destroy(bie_dmesh* dm) {
   if (!dm) return BIE_OK;
   (void)hipSetDevice(dm->device);
+  if (dm->done) {  // the handle's last call may still be running
+    (void)hipEventSynchronize(dm->done);
+    (void)hipEventDestroy(dm->done);
+  }
   dfree(dm->d_coords);
   dfree(dm->d_topo);
   dfree(dm->d_sources);
@@ -810,7 +977,7 @@ int bie_select_rules_device(bie_dmesh* dm, const double* d_targets, int64_t n_ta
     }
     BIE_HIP_CHECK(hipSetDevice(dm->device));
     launch_select(d_targets, n_targets, dm->d_selnodes, dm->pk, (int)dm->E,
-                  dm->gk == 6 ? 6 : 3, d_rules, static_cast<hipStream_t>(stream));
+                  dm->gk == 6 ? 6 : 3, dm->hyp, d_rules, static_cast<hipStream_t>(stream));
   });
 }
 
@@ -820,12 +987,14 @@ int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, doub
     BIE_REQUIRE(dm && y && normals && jw, BIE_EINVAL, "NULL argument");
     BIE_HIP_CHECK(hipSetDevice(dm->device));
     std::lock_guard<std::mutex> lk(dm->mu);
+    order_after_previous(dm, nullptr);
     ensure_rules(dm, n, m, nullptr);
     BIE_REQUIRE(rule >= 1 && rule <= dm->rs.R, BIE_EINVAL, "rule index out of range");
     const RuleSet& rs = dm->rs;
     refresh_geometry(dm, nullptr);
     std::vector<double> h((size_t)dm->E * rs.Pe * 8);
     BIE_HIP_CHECK(hipMemcpy(h.data(), dm->d_elem, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    mark_done(dm, nullptr);
     const int Q = rs.len[rule - 1], off = rs.off[rule - 1];
     for (int64_t e = 0; e < dm->E; ++e)
       for (int q = 0; q < Q; ++q) {
@@ -852,6 +1021,7 @@ struct HostMeshCache {
   std::mutex mu;
   std::vector<unsigned char> key;
   std::shared_ptr<bie_dmesh> dm;
+  int64_t hits = 0, builds = 0;
 };
 
 HostMeshCache& host_mesh_cache() {
@@ -871,7 +1041,7 @@ void append_bytes(std::vector<unsigned char>& key, const void* p, size_t n) {
 std::vector<unsigned char> mesh_key(const bie_mesh* m) {
   std::vector<unsigned char> key;
   const int64_t scal[7] = {m->n_coordinates, m->n_elements, m->n_sources, m->geometry_kind,
-                           m->physics_kind, m->index_base, 0};
+                           m->physics_kind, m->index_base, m->flags};
   append_bytes(key, scal, sizeof(scal));
   append_bytes(key, m->coordinates, sizeof(double) * 3 * (size_t)m->n_coordinates);
   append_bytes(key, m->topology, sizeof(int64_t) * (size_t)m->geometry_kind * m->n_elements);
@@ -892,11 +1062,13 @@ int host_mesh_handle(const bie_mesh* mesh, std::shared_ptr<bie_dmesh>* out) {
   std::lock_guard<std::mutex> lk(c.mu);
   if (c.dm && c.key == key) {
     *out = c.dm;
+    ++c.hits;
     return BIE_OK;
   }
   bie_dmesh* raw = nullptr;
   rc = bie_dmesh_create(mesh, 0, &raw);
   if (rc != BIE_OK) return rc;
+  ++c.builds;
   // a replaced handle is destroyed when its last caller releases it
   std::shared_ptr<bie_dmesh> sp(raw, [](bie_dmesh* d) { bie_dmesh_destroy(d); });
   c.dm = sp;
@@ -999,6 +1171,12 @@ int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* t
   int rc = host_mesh_handle(mesh, &handle);
   if (rc != BIE_OK) return rc;
   bie_dmesh* dm = handle.get();
+  // targets equal to the mesh sources value for value are collocation (the reference's own
+  // call passes mesh.sources as in_sources): the collocation path then applies (recorded
+  // mid-edge pairs, no M x E pair queue)
+  if (targets && n_targets == mesh->n_sources &&
+      (n_targets == 0 || std::memcmp(targets, mesh->sources, 24 * (size_t)n_targets) == 0))
+    targets = nullptr;
   double* dT = nullptr;
   double* dF = nullptr;
   double* dG = nullptr;
@@ -1018,19 +1196,310 @@ int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* t
       if (M) BIE_HIP_CHECK(hipMemcpy(dT, targets, 24 * M, hipMemcpyHostToDevice));
     }
     assemble_device_impl(dm, k_re, k_im, targets ? dT : nullptr, M, n, m, flags, dF, dG, dC,
-                         nullptr);
+                         nullptr, targets ? radius_about(dm->center, targets, M) : -1.0);
     BIE_HIP_CHECK(hipDeviceSynchronize());
     if (fg) {
       copy_to_host(F, dF, fg * sizeof(double));
       copy_to_host(G, dG, fg * sizeof(double));
     }
     if (M) BIE_HIP_CHECK(hipMemcpy(C, dC, 16 * M, hipMemcpyDeviceToHost));
+    if (targets) {  // the M x E pair queue of explicit targets is not kept in the cache
+      std::lock_guard<std::mutex> lk(dm->mu);
+      dfree(dm->d_mid_queue);
+      dm->d_mid_queue = nullptr;
+      dm->mid_cap = 0;
+    }
   });
   dfree(dT);
   dfree(dF);
   dfree(dG);
   dfree(dC);
   return rc;
+}
+
+}  // extern "C"
+
+// ---- sparse near field ------------------------------------------------------------------
+
+struct bie_sparse {
+  bie_dmesh* dm = nullptr;       // not owned
+  int depth = 1;
+  int64_t nnz = 0, n_pairs = 0;
+  std::vector<int64_t> row_ptr;  // host pattern
+  std::vector<int> cols;
+  int2* d_pairs = nullptr;
+  int* d_pos = nullptr;
+  double* d_gsel = nullptr;
+  double* d_rules = nullptr;
+  int off[6] = {0, 0, 0, 0, 0, 0};
+  int len[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace bie {
+namespace {
+
+void sparse_destroy_impl(bie_sparse* sp) {
+  if (!sp) return;
+  if (sp->dm) (void)hipSetDevice(sp->dm->device);
+  (void)hipDeviceSynchronize();  // a launch may still read the tables
+  dfree(sp->d_pairs);
+  dfree(sp->d_pos);
+  dfree(sp->d_gsel);
+  dfree(sp->d_rules);
+  delete sp;
+}
+
+// Pattern, pairs, selection table and subdivision rules of a sparse assembly on dm.
+bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
+  BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+  BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
+  BIE_REQUIRE(!std::isinf(offset), BIE_EINVAL, "offset must be finite (NaN: the default 0)");
+  const double offr = std::isnan(offset) ? 0.0 : offset;
+  BIE_HIP_CHECK(hipSetDevice(dm->device));
+  std::unique_ptr<bie_sparse> sp(new bie_sparse);
+  sp->dm = dm;
+  sp->depth = depth;
+  const int NP = dm->pk, NG = dm->gk;
+  std::vector<int64_t> pair_ptr;
+  std::vector<int> pair_elem;
+  sparse_pattern(dm->h_ptopo.data(), NP, dm->E, dm->N, depth, sp->row_ptr, sp->cols, pair_ptr,
+                 pair_elem);
+  sp->nnz = (int64_t)sp->cols.size();
+  sp->n_pairs = (int64_t)pair_elem.size();
+  BIE_REQUIRE(sp->nnz < (int64_t)INT32_MAX, BIE_EINVAL, "sparse pattern too large");
+  std::vector<int2> pairs((size_t)sp->n_pairs);
+  std::vector<int> pos((size_t)sp->n_pairs * NP);
+  for (int64_t i = 0; i < dm->N; ++i)
+    for (int64_t t = pair_ptr[i]; t < pair_ptr[i + 1]; ++t) {
+      const int e = pair_elem[t];
+      pairs[t] = make_int2((int)i, e);
+      const int* c0 = sp->cols.data() + sp->row_ptr[i];
+      const int* c1 = sp->cols.data() + sp->row_ptr[i + 1];
+      for (int s = 0; s < NP; ++s) {
+        const int col = dm->h_ptopo[(size_t)e * NP + s];
+        const int* it = std::lower_bound(c0, c1, col);
+        BIE_REQUIRE(it != c1 && *it == col, BIE_EINVAL, "internal: sparse column missing");
+        pos[(size_t)t * NP + s] = (int)(sp->row_ptr[i] + (it - c0));
+      }
+    }
+  // geometry node coordinates per element, 4 doubles each (the selection's layout)
+  std::vector<double> gsel((size_t)dm->E * NG * 4, 0.0);
+  for (int64_t e = 0; e < dm->E; ++e)
+    for (int g = 0; g < NG; ++g)
+      for (int d = 0; d < 3; ++d)
+        gsel[((size_t)e * NG + g) * 4 + d] = dm->h_coords[3 * (size_t)dm->h_topo[e * NG + g] + d];
+  // the six rules (sources at the vertices, then the edge midpoints 1-2, 2-3, 3-1;
+  // assembly_collocation_losses.jl:227-232), geometry and physics bases at their points
+  const double srcs[6][3] = {{1.0, offr, offr}, {offr, 1.0, offr}, {offr, offr, 1.0},
+                             {0.5, 0.5, offr},  {offr, 0.5, 0.5},  {0.5, offr, 0.5}};
+  const int RS = sparse_record_doubles();
+  std::vector<double> rec;
+  for (int c = 0; c < NG; ++c) {
+    std::vector<double> u, v, w;
+    subdivision_rule(srcs[c], u, v, w);
+    sp->off[c] = (int)(rec.size() / RS);
+    sp->len[c] = (int)u.size();
+    for (size_t q = 0; q < u.size(); ++q) {
+      double N[6], du[6], dv[6], Np[6], dup[6], dvp[6];
+      geometry_basis(NG, u[q], v[q], N, du, dv);
+      geometry_basis(NP, u[q], v[q], Np, dup, dvp);
+      std::vector<double> r(RS, 0.0);
+      r[0] = w[q];
+      for (int g = 0; g < 6; ++g) {
+        r[1 + g] = N[g];
+        r[7 + g] = du[g];
+        r[13 + g] = dv[g];
+        r[19 + g] = Np[g];
+      }
+      rec.insert(rec.end(), r.begin(), r.end());
+    }
+  }
+  sp->d_pairs = upload_vec(pairs, nullptr);
+  sp->d_pos = upload_vec(pos, nullptr);
+  sp->d_gsel = upload_vec(gsel, nullptr);
+  sp->d_rules = upload_vec(rec, nullptr);
+  BIE_HIP_CHECK(hipDeviceSynchronize());
+  return sp.release();
+}
+
+void sparse_assemble_impl(bie_sparse* sp, double kr, double ki, const double* d_sources,
+                          uint32_t flags, double* dF, double* dG, hipStream_t s) {
+  BIE_REQUIRE(sp != nullptr, BIE_EINVAL, "sparse handle is NULL");
+  BIE_REQUIRE((flags & ~(BIE_F | BIE_G)) == 0, BIE_EINVAL, "flags: BIE_F and BIE_G only");
+  BIE_REQUIRE(std::isfinite(kr) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+  BIE_REQUIRE(((flags & BIE_F) == 0 || dF) && ((flags & BIE_G) == 0 || dG), BIE_EINVAL,
+              "output pointer is NULL");
+  bie_dmesh* dm = sp->dm;
+  BIE_HIP_CHECK(hipSetDevice(dm->device));
+  std::lock_guard<std::mutex> lk(dm->mu);
+  order_after_previous(dm, s);
+  if (dF) launch_fill_zero(dF, 2 * sp->nnz, s);
+  if (dG) launch_fill_zero(dG, 2 * sp->nnz, s);
+  SparseLaunch L{};
+  L.src = d_sources ? d_sources : dm->d_sources;
+  L.pairs = sp->d_pairs;
+  L.pos = sp->d_pos;
+  L.n_pairs = (flags & (BIE_F | BIE_G)) ? sp->n_pairs : 0;
+  L.coords = dm->d_coords;
+  L.topo = dm->d_topo;
+  L.gsel = sp->d_gsel;
+  L.rules = sp->d_rules;
+  for (int c = 0; c < 6; ++c) {
+    L.off[c] = sp->off[c];
+    L.len[c] = sp->len[c];
+  }
+  L.NG = dm->gk;
+  L.NP = dm->pk;
+  L.hyp = dm->hyp;
+  L.kr = kr;
+  L.ki = ki;
+  L.flags = flags;
+  L.F = dF;
+  L.G = dG;
+  launch_sparse_pairs(L, s);
+  mark_done(dm, s);
+}
+
+}  // namespace
+}  // namespace bie
+
+extern "C" {
+
+int bie_sparse_create(bie_dmesh* dm, int32_t depth, double offset, bie_sparse** out) {
+  return guarded([&] {
+    BIE_REQUIRE(out != nullptr, BIE_EINVAL, "out is NULL");
+    *out = nullptr;
+    *out = sparse_create_impl(dm, depth, offset);
+  });
+}
+
+int bie_sparse_destroy(bie_sparse* sp) {
+  sparse_destroy_impl(sp);
+  return BIE_OK;
+}
+
+int64_t bie_sparse_nnz(const bie_sparse* sp) { return sp ? sp->nnz : -1; }
+
+int bie_sparse_pattern(const bie_sparse* sp, int32_t index_base, int64_t* I, int64_t* J) {
+  return guarded([&] {
+    BIE_REQUIRE(sp && I && J, BIE_EINVAL, "NULL argument");
+    BIE_REQUIRE(index_base == 0 || index_base == 1, BIE_EINVAL, "index_base must be 0 or 1");
+    for (size_t i = 0; i + 1 < sp->row_ptr.size(); ++i)
+      for (int64_t t = sp->row_ptr[i]; t < sp->row_ptr[i + 1]; ++t) {
+        I[t] = (int64_t)i + index_base;
+        J[t] = (int64_t)sp->cols[t] + index_base;
+      }
+  });
+}
+
+int bie_sparse_assemble_device(bie_sparse* sp, double k_re, double k_im, const double* d_sources,
+                               uint32_t flags, double* dF, double* dG, void* stream) {
+  return guarded([&] {
+    sparse_assemble_impl(sp, k_re, k_im, d_sources, flags, dF, dG,
+                         static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz) {
+  return guarded([&] {
+    validate_mesh(mesh);
+    BIE_REQUIRE(nnz != nullptr, BIE_EINVAL, "nnz is NULL");
+    std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
+                                              mesh->n_elements, mesh->index_base,
+                                              mesh->n_sources, "physics_topology");
+    std::vector<int64_t> row_ptr, pair_ptr;
+    std::vector<int> cols, pair_elem;
+    sparse_pattern(ptopo.data(), mesh->physics_kind, mesh->n_elements, mesh->n_sources, depth,
+                   row_ptr, cols, pair_ptr, pair_elem);
+    *nnz = (int64_t)cols.size();
+  });
+}
+
+int bie_sparse_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* sources,
+                        int64_t n_sources, int32_t depth, double offset, uint32_t flags,
+                        int64_t nnz, int64_t* I, int64_t* J, double* F, double* G) {
+  std::shared_ptr<bie_dmesh> handle;
+  int rc = host_mesh_handle(mesh, &handle);
+  if (rc != BIE_OK) return rc;
+  bie_dmesh* dm = handle.get();
+  bie_sparse* sp = nullptr;
+  double* dT = nullptr;
+  double* dF = nullptr;
+  double* dG = nullptr;
+  rc = guarded([&] {
+    BIE_REQUIRE(I && J && F && G, BIE_EINVAL, "output pointer is NULL");
+    BIE_REQUIRE(sources == nullptr || n_sources == dm->N, BIE_EINVAL,
+                "the sparse assembly needs one source per mesh source (n_sources)");
+    sp = sparse_create_impl(dm, depth, offset);
+    BIE_REQUIRE(nnz == sp->nnz, BIE_EINVAL,
+                "nnz does not match the pattern (bie_sparse_count gives " +
+                    std::to_string(sp->nnz) + ")");
+    dF = dalloc<double>((size_t)2 * nnz);
+    dG = dalloc<double>((size_t)2 * nnz);
+    if (sources) {
+      dT = dalloc<double>((size_t)3 * n_sources);
+      if (n_sources)
+        BIE_HIP_CHECK(hipMemcpy(dT, sources, 24 * n_sources, hipMemcpyHostToDevice));
+    }
+    sparse_assemble_impl(sp, k_re, k_im, dT, flags, dF, dG, nullptr);
+    BIE_HIP_CHECK(hipDeviceSynchronize());
+    if (nnz) {
+      BIE_HIP_CHECK(hipMemcpy(F, dF, 16 * nnz, hipMemcpyDeviceToHost));
+      BIE_HIP_CHECK(hipMemcpy(G, dG, 16 * nnz, hipMemcpyDeviceToHost));
+    }
+    const int base = mesh->index_base;
+    for (size_t i = 0; i + 1 < sp->row_ptr.size(); ++i)
+      for (int64_t t = sp->row_ptr[i]; t < sp->row_ptr[i + 1]; ++t) {
+        I[t] = (int64_t)i + base;
+        J[t] = (int64_t)sp->cols[t] + base;
+      }
+  });
+  sparse_destroy_impl(sp);
+  dfree(dT);
+  dfree(dF);
+  dfree(dG);
+  return rc;
+}
+
+int bie_host_cache_info(int64_t* hits, int64_t* builds, int64_t* n_sources,
+                        int64_t* queue_entries) {
+  return guarded([&] {
+    HostMeshCache& c = host_mesh_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (hits) *hits = c.hits;
+    if (builds) *builds = c.builds;
+    if (n_sources) *n_sources = c.dm ? c.dm->N : -1;
+    if (queue_entries) {
+      *queue_entries = 0;
+      if (c.dm) {
+        std::lock_guard<std::mutex> dl(c.dm->mu);
+        *queue_entries = c.dm->d_mid_queue ? c.dm->mid_cap : 0;
+      }
+    }
+  });
+}
+
+int bie_host_cache_release(void) {
+  return guarded([&] {
+    {
+      HostMeshCache& c = host_mesh_cache();
+      std::lock_guard<std::mutex> lk(c.mu);
+      c.dm.reset();  // destroyed now, or when a call still using it returns
+      c.key.clear();
+    }
+    HostStaging& h = host_staging();
+    std::lock_guard<std::mutex> lk(h.mu);
+    if (h.s) BIE_HIP_CHECK(hipStreamSynchronize(h.s));
+    for (int i = 0; i < 2; ++i) {
+      if (h.buf[i]) (void)hipHostFree(h.buf[i]);
+      if (h.ev[i]) (void)hipEventDestroy(h.ev[i]);
+      h.buf[i] = nullptr;
+      h.ev[i] = nullptr;
+    }
+    if (h.s) (void)hipStreamDestroy(h.s);
+    h.s = nullptr;
+    h.tried = false;  // the next large copy allocates them again
+  });
 }
 
 int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stats) {
@@ -1084,6 +1553,7 @@ int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, in
     std::lock_guard<std::mutex> lk(dm->mu);
     BIE_HIP_CHECK(hipSetDevice(dm->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    order_after_previous(dm, s);
     ensure_rules(dm, n, m, s);
     AdjointArgs a{};
     a.targets = d_targets;
@@ -1105,10 +1575,12 @@ int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, in
     }
     a.R = dm->rs.R;
     a.E = (int)dm->E;
+    a.hyp = dm->hyp;
     a.k = k;
     a.gradX = d_grad_coords;
     a.gradk = d_grad_k;
     launch_adjoint(a, dm->gk, dm->pk, s);
+    mark_done(dm, s);
   });
 }
 
@@ -1116,6 +1588,34 @@ int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream) {
   return guarded([&] {
     zgemm_device(M, nrhs, K, dA, lda, dX, ldx, dY, ldy, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_lu_factor_device(int64_t n, double* dA, int64_t lda, const double* dC, int32_t* d_ipiv,
+                         void* stream) {
+  return guarded([&] { lu_factor_device(n, dA, lda, dC, d_ipiv, static_cast<hipStream_t>(stream)); });
+}
+
+int bie_lu_solve_device(int64_t n, const double* dA, int64_t lda, const int32_t* d_ipiv,
+                        int32_t trans, double* dB, int64_t ldb, int64_t nrhs, void* stream) {
+  return guarded([&] {
+    lu_solve_device(n, dA, lda, d_ipiv, trans, dB, ldb, nrhs, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_zgemv_device(int32_t trans, int64_t m, int64_t n, const double* dA, int64_t lda,
+                     const double* dx, double* dy, void* stream) {
+  return guarded([&] { zgemv_device(trans, m, n, dA, lda, dx, dy, static_cast<hipStream_t>(stream)); });
+}
+
+int bie_gmres_batched_device(int64_t n, int32_t n_systems, const double* dF, int64_t strideF,
+                             int64_t ldf, const double* dC, const double* dB, int64_t ldb,
+                             double* dX, int64_t ldx, double tol, int32_t restart,
+                             int32_t maxiter, int32_t* iterations, double* residuals,
+                             void* stream) {
+  return guarded([&] {
+    gmres_batched_device(n, n_systems, dF, strideF, ldf, dC, dB, ldb, dX, ldx, tol, restart,
+                         maxiter, iterations, residuals, static_cast<hipStream_t>(stream));
   });
 }
 
@@ -1314,6 +1814,14 @@ int bie_batch_assemble(bie_batch* b, const double* k_re, const double* k_im, uin
                        double* const* dF, double* const* dG, double* const* dC, void* stream) {
   return guarded([&] {
     batch_assemble_impl(b, k_re, k_im, flags, dF, dG, dC, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_batch_dfdk_device(bie_batch* b, const double* k_re, const double* k_im,
+                          const double* const* d_alpha, const double* const* d_p, double* d_w,
+                          void* stream) {
+  return guarded([&] {
+    batch_dfdk_impl(b, k_re, k_im, d_alpha, d_p, d_w, static_cast<hipStream_t>(stream));
   });
 }
 

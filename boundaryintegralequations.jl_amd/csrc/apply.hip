@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <string>
 
 #include "bie_common.hpp"
 #include "device_common.hpp"
@@ -210,58 +211,62 @@ k_zgemv_batched(const double2* __restrict__ A, int64_t strideA, int64_t lda,
   unsafeAtomicAdd(y + 1, yi);
 }
 
+// Per-device packed-X workspace shared by all calls.  Its mutex is held while a call
+// enqueues its kernels; the call's end is recorded in done[dev], and a call on another
+// stream waits for it before overwriting the workspace (a growth waits on the host before
+// the old buffer is freed).
 struct Workspace {
   std::mutex mu;
   double2* p[16] = {};
   size_t bytes[16] = {};
+  hipEvent_t done[16] = {};
+  hipStream_t done_stream[16] = {};
+  bool done_valid[16] = {};
 };
 Workspace g_ws;
 
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  BIE_HIP_CHECK(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
 }  // namespace
 
-void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
-                  const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s) {
+namespace {
+
+// Checks the shapes of Y = A X; false when there is nothing to compute.
+bool zgemm_validate(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                    const double* dX, int64_t ldx, double* dY, int64_t ldy) {
   BIE_REQUIRE(M >= 0 && nrhs >= 0 && K >= 0, BIE_EINVAL, "negative size");
   BIE_REQUIRE(lda >= (M > 0 ? M : 1) && ldx >= (K > 0 ? K : 1) && ldy >= (M > 0 ? M : 1),
               BIE_EINVAL, "leading dimension too small");
   BIE_REQUIRE(nrhs <= (1 << 20), BIE_EINVAL, "too many right-hand sides");
-  if (M == 0 || nrhs == 0) return;
+  if (M == 0 || nrhs == 0) return false;
   BIE_REQUIRE(dA && dX && dY, BIE_EINVAL, "NULL pointer");
-  const int ncp = (int)((nrhs + 31) / 32 * 32);
-  // 32-bit buffer offsets in the kernel: (klen + 64) 16 max(lda, ncp) < 2^31 with klen >= 16
+  const int64_t ncp = (nrhs + 31) / 32 * 32;
   const int64_t wide = std::max<int64_t>(lda, ncp) * 16;
   BIE_REQUIRE(wide * (64 + 4 * kRing) < (1LL << 31), BIE_EINVAL,
               "apply: leading dimension too large (lda <= 1.6e6)");
+  return true;
+}
+
+// The kernels of Y = A X with the packed copy of X in Xp (K x ncp complex entries), all on
+// stream s: Y zeroed by a fill kernel, X packed, the MFMA product with K splits.
+void zgemm_launch(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                  const double* dX, int64_t ldx, double* dY, int64_t ldy, double2* Xp, int dev,
+                  hipStream_t s) {
+  const int ncp = (int)((nrhs + 31) / 32 * 32);
+  const int64_t wide = std::max<int64_t>(lda, ncp) * 16;
   {
     int64_t blocks = (M * nrhs + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_zero_y, dim3((unsigned)blocks), dim3(256), 0, s,
                        reinterpret_cast<double2*>(dY), ldy, M, (int)nrhs);
   }
-  if (K == 0) return;
-  int dev = 0;
-  BIE_HIP_CHECK(hipGetDevice(&dev));
-  BIE_REQUIRE(dev >= 0 && dev < 16, BIE_EINVAL, "device ordinal out of range");
-  double2* Xp = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_ws.mu);
-    const size_t need = (size_t)K * ncp * sizeof(double2);
-    if (g_ws.bytes[dev] < need) {
-      if (g_ws.p[dev]) {
-        BIE_HIP_CHECK(hipDeviceSynchronize());
-        (void)hipFree(g_ws.p[dev]);
-      }
-      g_ws.p[dev] = nullptr;
-      g_ws.bytes[dev] = 0;
-      void* p = nullptr;
-      if (hipMalloc(&p, need) != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error{BIE_ENOMEM, "apply workspace allocation failed"};
-      }
-      g_ws.p[dev] = static_cast<double2*>(p);
-      g_ws.bytes[dev] = need;
-    }
-    Xp = g_ws.p[dev];
+  if (K == 0) {
+    BIE_HIP_CHECK(hipGetLastError());
+    return;
   }
   {
     int64_t blocks = (K * ncp + 255) / 256;
@@ -289,6 +294,56 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
                      dim3(256), 0, s, reinterpret_cast<const double2*>(dA), lda, Xp, ncp,
                      reinterpret_cast<double2*>(dY), ldy, M, K, (int)nrhs, klen);
   BIE_HIP_CHECK(hipGetLastError());
+}
+
+int current_device() {
+  int dev = 0;
+  BIE_HIP_CHECK(hipGetDevice(&dev));
+  BIE_REQUIRE(dev >= 0 && dev < 16, BIE_EINVAL, "device ordinal out of range");
+  return dev;
+}
+
+}  // namespace
+
+void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                  const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s) {
+  if (!zgemm_validate(M, nrhs, K, dA, lda, dX, ldx, dY, ldy)) return;
+  const int ncp = (int)((nrhs + 31) / 32 * 32);
+  const int dev = current_device();
+  std::lock_guard<std::mutex> lk(g_ws.mu);  // held until this call's kernels are enqueued
+  const bool cap = stream_capturing(s);
+  if (!g_ws.done[dev]) BIE_HIP_CHECK(hipEventCreateWithFlags(&g_ws.done[dev], hipEventDisableTiming));
+  double2* Xp = nullptr;
+  if (K > 0) {
+    const size_t need = (size_t)K * ncp * sizeof(double2);
+    if (g_ws.bytes[dev] < need) {
+      BIE_REQUIRE(!cap, BIE_EINVAL,
+                  "apply: the workspace must grow, which a stream under graph capture cannot "
+                  "do (call once outside the capture first)");
+      if (g_ws.p[dev]) {
+        if (g_ws.done_valid[dev]) BIE_HIP_CHECK(hipEventSynchronize(g_ws.done[dev]));
+        (void)hipFree(g_ws.p[dev]);
+      }
+      g_ws.p[dev] = nullptr;
+      g_ws.bytes[dev] = 0;
+      void* p = nullptr;
+      if (hipMalloc(&p, need) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{BIE_ENOMEM, "apply workspace allocation failed"};
+      }
+      g_ws.p[dev] = static_cast<double2*>(p);
+      g_ws.bytes[dev] = need;
+    } else if (g_ws.done_valid[dev] && g_ws.done_stream[dev] != s && !cap) {
+      BIE_HIP_CHECK(hipStreamWaitEvent(s, g_ws.done[dev], 0));
+    }
+    Xp = g_ws.p[dev];
+  }
+  zgemm_launch(M, nrhs, K, dA, lda, dX, ldx, dY, ldy, Xp, dev, s);
+  if (!cap && K > 0) {
+    BIE_HIP_CHECK(hipEventRecord(g_ws.done[dev], s));
+    g_ws.done_stream[dev] = s;
+    g_ws.done_valid[dev] = true;
+  }
 }
 
 void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA, int64_t lda,

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "bie_common.hpp"
 #include "device_common.hpp"
@@ -80,14 +81,14 @@ __global__ void k_gather_selnodes(const double* __restrict__ sources,
 template <int NP>
 __global__ void __launch_bounds__(256)
 k_select(const double* __restrict__ targets, int64_t M, const double* __restrict__ sel,
-         int E, int R, int8_t* __restrict__ out) {
+         int E, int R, int hyp, int8_t* __restrict__ out) {
   const int64_t total = M * (int64_t)E;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += stride) {
     const int64_t i = t / E;
     const int e = (int)(t - i * E);
     out[t] = (int8_t)select_rule<NP>(targets[3 * i], targets[3 * i + 1], targets[3 * i + 2],
-                                      sel + (size_t)e * NP * 4, R);
+                                      sel + (size_t)e * NP * 4, R, hyp);
   }
 }
 
@@ -96,8 +97,9 @@ k_select(const double* __restrict__ targets, int64_t M, const double* __restrict
 // C integrand.  Single-layer g = e^{ikr}/(4 pi r), double-layer
 // f = e^{ikr}(ikr - 1)(n.d)/(4 pi r^3) and k0 = -(n.d)/(4 pi r^3) (kernels.jl:13-55) share
 // one e^{ikr} per point; 1/(4 pi) and J w are folded into tab[6].  With TAB, e^{ikr}
-// comes from the LDS trig table trig (sincos_tab), else from sincos_f64.
-template <int NP, bool CPLX, bool TAB = false, bool GON = true>
+// comes from the LDS trig table trig (sincos_tab; sincos_tab_fast with FAST, for launches
+// whose arguments k r are known to stay below its bound), else from sincos_f64.
+template <int NP, bool CPLX, bool TAB = false, bool GON = true, bool FAST = false>
 __device__ __forceinline__ void point_kernel(double x0, double x1, double x2, const double4 A,
                                              const double4 B, const double* __restrict__ p,
                                              double kr, double ki, double* fr, double* fi,
@@ -112,7 +114,9 @@ __device__ __forceinline__ void point_kernel(double x0, double x1, double x2, co
   const double ak = nd * rinv * rinv * g;  // (n.d) J w / (4 pi r^3)
   csum -= ak;
   double sn, cs;
-  if (TAB)
+  if (FAST)
+    sincos_tab_fast(kr * r, trig, &sn, &cs);
+  else if (TAB)
     sincos_tab(kr * r, trig, &sn, &cs);
   else
     sincos_f64(kr * r, &sn, &cs);
@@ -198,18 +202,39 @@ __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double
 }
 
 // As pair_quadrature over table rows that this wave staged in LDS (rows[8 q .. 8 q + 8)).
-template <int NP, bool CPLX, bool TAB = false, bool GON = true>
+// QC > 0: the rule length is the compile-time QC (== Q), and the loop is unrolled whole, so
+// every LDS read has a constant offset and the compiler can interleave the points.
+template <int NP, bool CPLX, bool TAB = false, bool GON = true, int QC = 0, bool FAST = false,
+          bool ROLLED = false>
 __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double x2,
                                                     const double* rows, const double* ph, int Q,
                                                     double kr, double ki, double* fr, double* fi,
                                                     double* gr, double* gi, double& csum,
                                                     const double2* trig = nullptr) {
+  if constexpr (QC > 0 && ROLLED) {  // one point at a time: fewest registers live
+#pragma unroll 1
+    for (int q = 0; q < QC; ++q) {
+      const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
+      const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
+      point_kernel<NP, CPLX, TAB, GON, FAST>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr,
+                                             gi, csum, trig);
+    }
+  } else if constexpr (QC > 0) {
+#pragma unroll
+    for (int q = 0; q < QC; ++q) {
+      const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
+      const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
+      point_kernel<NP, CPLX, TAB, GON, FAST>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr,
+                                             gi, csum, trig);
+    }
+  } else {
 #pragma unroll 2
-  for (int q = 0; q < Q; ++q) {
-    const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
-    const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
-    point_kernel<NP, CPLX, TAB, GON>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr, gi,
-                                     csum, trig);
+    for (int q = 0; q < Q; ++q) {
+      const double4 A = *reinterpret_cast<const double4*>(rows + 8 * q);
+      const double4 B = *reinterpret_cast<const double4*>(rows + 8 * q + 4);
+      point_kernel<NP, CPLX, TAB, GON, FAST>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr,
+                                             gi, csum, trig);
+    }
   }
 }
 
@@ -229,7 +254,8 @@ k_assemble(const AssemblyArgs a, int ech) {
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   double csum = 0.0;
   for (int e = e0; e < e1; ++e) {
-    const int c = select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R) - 1;
+    const int c =
+        select_rule<NP>(x0, x1, x2, a.selnodes + (size_t)e * NP * 4, a.R, a.hyp) - 1;
     double fr[NP], fi[NP], gr[NP], gi[NP];
 #pragma unroll
     for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
@@ -323,7 +349,7 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
 // its element's vertex-rule rows in its LDS rows (s_rows + wave * 24 Qc) and loads the
 // next element's rows into registers a whole pair ahead: the global loads never wait
 // behind the flush's stores (vector-memory counters are in order).  Ends on a barrier.
-template <int NP, bool CPLX, int W, bool STAGED, bool GON = true>
+template <int NP, bool CPLX, int W, bool STAGED, bool GON = true, int QC = 0, bool FAST = false>
 __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
                                           int seg, double* acc, const double* phi,
                                           const double2* trig, double* s_rows, int2* s_q,
@@ -336,7 +362,7 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   const double kr = a.kr, ki = a.ki;
   double csum = 0.0;
-  const int Qc = a.len[0];
+  const int Qc = QC > 0 ? QC : a.len[0];
   const int nchunk = 12 * Qc;
   double* my_rows = STAGED ? s_rows + (size_t)wave * stage_wave_doubles(Qc) : nullptr;
   double* my_sel = STAGED ? my_rows + 24 * Qc : nullptr;  // the element's selection nodes
@@ -351,9 +377,9 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     if (STAGED) {
       stage_load(a.elem + (size_t)e * a.Pe * 8, nchunk, sel, nsel, lane, stg, stg_sel);
       stage_store(my_rows, nchunk, my_sel, nsel, lane, stg, stg_sel);
-      c = select_rule<NP>(x0, x1, x2, my_sel, a.R) - 1;
+      c = select_rule<NP>(x0, x1, x2, my_sel, a.R, a.hyp) - 1;
     } else {
-      c = select_rule<NP>(x0, x1, x2, sel, a.R) - 1;
+      c = select_rule<NP>(x0, x1, x2, sel, a.R, a.hyp) - 1;
       if (!(NP == 6 && c >= 3))
         load_rows01(a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, a.len[c], A0, B0, A1, B1);
     }
@@ -376,9 +402,9 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
 #pragma unroll
         for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
         if (STAGED)
-          pair_quadrature_lds<NP, CPLX, true, GON>(x0, x1, x2, my_rows + (size_t)c * Qc * 8,
-                                                   phi + c * a.Qmax * NP, Qc, kr, ki, fr, fi,
-                                                   gr, gi, csum, trig);
+          pair_quadrature_lds<NP, CPLX, true, GON, QC, FAST>(
+              x0, x1, x2, my_rows + (size_t)c * Qc * 8, phi + c * a.Qmax * NP, Qc, kr, ki, fr,
+              fi, gr, gi, csum, trig);
         else
           pair_quadrature_pre<NP, CPLX, true, GON>(
               x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, phi + c * a.Qmax * NP,
@@ -420,9 +446,9 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     int c_nxt = 0;
     if (STAGED) {  // only this wave reads my_rows and my_sel, and its reads above are done
       stage_store(my_rows, nchunk, my_sel, nsel, lane, stg, stg_sel);
-      if (e_nxt >= 0) c_nxt = select_rule<NP>(x0, x1, x2, my_sel, a.R) - 1;
+      if (e_nxt >= 0) c_nxt = select_rule<NP>(x0, x1, x2, my_sel, a.R, a.hyp) - 1;
     } else if (e_nxt >= 0) {
-      c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
+      c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R, a.hyp) - 1;
       if (!(NP == 6 && c_nxt >= 3))
         load_rows01(a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8, a.len[c_nxt], A0, B0,
                     A1, B1);
@@ -452,9 +478,159 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   lds_barrier();
 }
 
+// Split form of slot_tile (real k, vertex rules of QC points staged in LDS, the branch-free
+// table sincos): SPLIT waves share each element of a step, so a workgroup of W * SPLIT
+// waves keeps W elements per step - the same slot schedule and LDS slots - with SPLIT
+// times the waves resident to hide each other's latency.  Wave h of an element integrates
+// the points h QH .. h QH + QH - 1 (QH = QC / SPLIT) of the pair's rule from its own staged
+// rows and adds its partial node sums to the slots (LDS atomics, as the waves of different
+// elements already do); every wave selects the rule, the first one queues mid-edge pairs;
+// the flush is spread over all waves.
+template <int NP, int W, bool GON, int QC, int SPLIT>
+__device__ __forceinline__ void slot_tile_split(const AssemblyArgs& a, const SlotArgs& sa,
+                                                int rb, int seg, double* acc, const double* phi,
+                                                const double2* trig, double* s_rows, int2* s_q,
+                                                unsigned* s_qn) {
+  constexpr int QH = QC / SPLIT;
+  static_assert(QH * SPLIT == QC, "the split must divide the rule length");
+  constexpr int NV = (12 * QH + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int we = wave / SPLIT, half = wave - we * SPLIT;
+  const int64_t i = (int64_t)rb * 64 + lane;
+  const bool active = i < a.M;
+  const int64_t ic = active ? i : (a.M - 1);
+  const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
+  const double kr = a.kr;
+  double csum = 0.0;
+  double* my_rows = s_rows + (size_t)wave * split_stage_wave_doubles(QH);
+  double* my_sel = my_rows + 24 * QH;
+  constexpr int nsel = NP * 2;
+  const double* my_phi = phi + half * QH * NP;
+  double2 stg[NV], stg_sel;
+  const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
+  int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * W + we) : -1;
+  int c = 0;
+  if (e >= 0) {
+    stage_load_half<QC, QH, NV>(a.elem + (size_t)e * a.Pe * 8, half,
+                                sa.step_sel + ((size_t)st0 * W + we) * NP * 4, nsel, lane, stg,
+                                stg_sel);
+    stage_store_half<QH, NV>(my_rows, my_sel, nsel, lane, stg, stg_sel);
+    c = select_rule<NP>(x0, x1, x2, my_sel, a.R, a.hyp) - 1;
+  }
+  for (int st = st0; st < st1; ++st) {
+    const size_t nidx = (size_t)(st + 1) * W + we;
+    const int e_nxt = (st + 1 < st1) ? sload(sa.step_elems + nidx) : -1;
+    // unconditional, as in slot_tile
+    stage_load_half<QC, QH, NV>(a.elem + (size_t)max(e_nxt, 0) * a.Pe * 8, half,
+                                sa.step_sel + (e_nxt >= 0 ? nidx : 0) * NP * 4,
+                                e_nxt >= 0 ? nsel : 0, lane, stg, stg_sel);
+    if (e >= 0) {
+      const bool mid = NP == 6 && c >= 3;
+      if (active && !mid) {
+        double fr[NP], fi[NP], gr[NP], gi[NP];
+#pragma unroll
+        for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
+        pair_quadrature_lds<NP, false, true, GON, QH, true, (NP == 6 && (GON || QH > 3))>(
+            x0, x1, x2, my_rows + (size_t)c * QH * 8, my_phi + c * a.Qmax * NP, QH, kr, 0.0, fr,
+            fi, gr, gi, csum, trig);
+        const int* so = sa.step_slot + ((size_t)st * W + we) * NP;
+#pragma unroll
+        for (int s = 0; s < NP; ++s) {
+          double* p = acc + (size_t)sload(so + s) * 256 + lane;
+          atomicAdd(p, fr[s]);
+          atomicAdd(p + 64, fi[s]);
+          if constexpr (GON) {
+            atomicAdd(p + 128, gr[s]);
+            atomicAdd(p + 192, gi[s]);
+          }
+        }
+      }
+      if (NP == 6 && sa.push_mid && half == 0) {
+        const bool push = mid && active;
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+          const int leader = __ffsll((long long)mask) - 1;
+          unsigned base = 0;
+          if (lane == leader) base = atomicAdd(s_qn, (unsigned)__popcll(mask));
+          base = __shfl(base, leader);
+          if (push) {
+            const unsigned pos = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            const int2 entry = make_int2((int)i, e | ((c - 3) << 28));
+            if (pos < (unsigned)kMidLds) {
+              s_q[pos] = entry;
+            } else {
+              const unsigned g = atomicAdd(sa.mid_count, 1u);
+              if ((int64_t)g < sa.mid_cap) sa.mid_queue[g] = entry;
+            }
+          }
+        }
+      }
+    }
+    int c_nxt = 0;
+    stage_store_half<QH, NV>(my_rows, my_sel, nsel, lane, stg, stg_sel);
+    if (e_nxt >= 0) c_nxt = select_rule<NP>(x0, x1, x2, my_sel, a.R, a.hyp) - 1;
+    if (st > st0) flush_step<W * SPLIT>(a, sa, st - 1, wave, lane, i, active, acc);
+    lds_barrier();
+    e = e_nxt;
+    c = c_nxt;
+  }
+  if (st1 > st0) {
+    flush_step<W * SPLIT>(a, sa, st1 - 1, wave, lane, i, active, acc);
+    lds_barrier();
+  }
+  if (NP == 6 && wave == 0) {  // after a barrier: the LDS queue is complete
+    const unsigned n = min(*s_qn, (unsigned)kMidLds);
+    if (n) {
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(sa.mid_count, n);
+      base = __shfl(base, 0);
+      for (unsigned t = lane; t < n; t += 64)
+        if ((int64_t)(base + t) < sa.mid_cap) sa.mid_queue[base + t] = s_q[t];
+    }
+    if (lane == 0) *s_qn = 0u;
+  }
+  if (active && (a.flags & BIE_C)) unsafeAtomicAdd(a.C + 2 * i, csum);
+  lds_barrier();
+}
+
+// The split slot kernel: W * SPLIT waves per workgroup (slot_tile_split).
+template <int NP, int W, bool GON, int QC, int SPLIT>
+__global__ void __launch_bounds__(64 * W * SPLIT)
+k_assemble_slots_split(const AssemblyArgs a, const SlotArgs sa) {
+  extern __shared__ double lds[];
+  double* acc = lds;
+  double* s_phi = lds + (size_t)sa.S * 256;
+  const int nphi = a.R * a.Qmax * NP;
+  const double2* s_trig = reinterpret_cast<const double2*>(s_phi + phi_pad(nphi));
+  double* s_rows = s_phi + phi_pad(nphi) + kTrigDoubles;
+  int2* s_q = reinterpret_cast<int2*>(s_rows + split_stage_doubles(QC, SPLIT));
+  unsigned* s_qn = reinterpret_cast<unsigned*>(s_q + kMidLds);
+  const int b = blockIdx.x;
+  const int per_xcd = (int)((sa.n_tiles + 7) / 8);
+  const int64_t tile = (int64_t)(b & 7) * per_xcd + (b >> 3);
+  if ((b >> 3) >= per_xcd || tile >= sa.n_tiles) return;
+  int rb, seg;
+  if (sa.seg_major) {
+    seg = (int)(tile / sa.n_rowblocks);
+    rb = (int)(tile - (int64_t)seg * sa.n_rowblocks);
+  } else {
+    rb = (int)(tile / sa.n_segments);
+    seg = (int)(tile - (int64_t)rb * sa.n_segments);
+  }
+  for (int t = threadIdx.x; t < phi_pad(nphi) + kTrigDoubles; t += blockDim.x) s_phi[t] = a.phi[t];
+  for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
+  if (threadIdx.x == 0) *s_qn = 0u;
+  lds_barrier();
+  slot_tile_split<NP, W, GON, QC, SPLIT>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
+}
+
 // GON is whether G is formed (the BIE_G flag): F-only launches use their own instance.
-template <int NP, bool CPLX, int W, bool GON>
-__global__ void __launch_bounds__(64 * W)
+// QC > 0: instance for vertex rules of exactly QC points (staged in LDS, QC <= kStageMaxQ)
+// with the e^{ikr} of sincos_tab_fast (FAST: real k, |k| r below its bound); QC == 0: any
+// rule length, staged or not.
+template <int NP, bool CPLX, int W, bool GON, int QC = 0, bool FAST = false, int OCC = 1>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2 * OCC)))
 k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   extern __shared__ double lds[];
   double* acc = lds;                              // [slot][4][64]
@@ -484,10 +660,15 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   for (int t = threadIdx.x; t < sa.S * 256; t += blockDim.x) acc[t] = 0.0;
   if (threadIdx.x == 0) *s_qn = 0u;
   lds_barrier();
-  if (nstage)
+  if constexpr (QC > 0) {
+    static_assert(QC <= kStageMaxQ, "compile-time rule lengths are staged in LDS");
+    slot_tile<NP, CPLX, W, true, GON, QC, FAST>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q,
+                                                s_qn);
+  } else if (nstage) {
     slot_tile<NP, CPLX, W, true, GON>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
-  else
+  } else {
     slot_tile<NP, CPLX, W, false, GON>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
+  }
 }
 
 // Uneven batch: many meshes (same element kinds and rule set) in one persistent launch.
@@ -744,15 +925,15 @@ void launch_gather_selnodes(const double* sources, const int* ptopo, int NP, int
 }
 
 void launch_select(const double* targets, int64_t M, const double* sel, int NP, int E, int R,
-                   int8_t* out, hipStream_t s) {
+                   int hyp, int8_t* out, hipStream_t s) {
   if (M == 0 || E == 0) return;
   int64_t blocks = (M * (int64_t)E + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   dim3 grid((unsigned)blocks);
   if (NP == 3)
-    hipLaunchKernelGGL(k_select<3>, grid, dim3(256), 0, s, targets, M, sel, E, R, out);
+    hipLaunchKernelGGL(k_select<3>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
   else
-    hipLaunchKernelGGL(k_select<6>, grid, dim3(256), 0, s, targets, M, sel, E, R, out);
+    hipLaunchKernelGGL(k_select<6>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
   BIE_HIP_CHECK(hipGetLastError());
 }
 
@@ -764,15 +945,18 @@ void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipSt
   BIE_HIP_CHECK(hipGetLastError());
 }
 
-void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s) {
+void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;
   BIE_REQUIRE(sa.W == kSlotWaves, BIE_EINVAL, "slot schedule built for another wave count");
   const int64_t per_xcd = (sa.n_tiles + 7) / 8;
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
   BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
+  const int split = (sa.split == 2 && fast && a.ki == 0.0) ? 2 : 1;
   const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + phi_pad(a.R * a.Qmax * a.NP) +
-                                         kTrigDoubles + stage_doubles(a.len[0])) +
+                                         kTrigDoubles +
+                                         (split == 2 ? split_stage_doubles(a.len[0], 2)
+                                                     : stage_doubles(a.len[0]))) +
                        sizeof(int2) * kMidLds + 16;
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
   const bool cplx = a.ki != 0.0;
@@ -780,19 +964,71 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_
   auto go = [&](auto kern) {
     BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    hipLaunchKernelGGL(kern, grid, dim3(64 * W), shmem, s, a, sa);
+    hipLaunchKernelGGL(kern, grid, dim3(64 * W * split), shmem, s, a, sa);
   };
   const bool gon = (a.flags & BIE_G) != 0;
-  if (a.NP == 3) {
-    if (cplx)
-      gon ? go(k_assemble_slots<3, true, W, true>) : go(k_assemble_slots<3, true, W, false>);
-    else
-      gon ? go(k_assemble_slots<3, false, W, true>) : go(k_assemble_slots<3, false, W, false>);
-  } else {
-    if (cplx)
-      gon ? go(k_assemble_slots<6, true, W, true>) : go(k_assemble_slots<6, true, W, false>);
-    else
-      gon ? go(k_assemble_slots<6, false, W, true>) : go(k_assemble_slots<6, false, W, false>);
+  if (split == 2) {
+    auto split_go = [&](auto np, auto g) -> bool {
+      constexpr int NPc = decltype(np)::value;
+      constexpr bool Gc = decltype(g)::value;
+      switch (a.len[0]) {
+        case 4: go(k_assemble_slots_split<NPc, W, Gc, 4, 2>); return true;
+        case 6: go(k_assemble_slots_split<NPc, W, Gc, 6, 2>); return true;
+        case 16: go(k_assemble_slots_split<NPc, W, Gc, 16, 2>); return true;
+        default: return false;
+      }
+    };
+    using N3 = std::integral_constant<int, 3>;
+    using N6 = std::integral_constant<int, 6>;
+    const bool ok = a.NP == 3 ? (gon ? split_go(N3{}, std::true_type{}) : split_go(N3{}, std::false_type{}))
+                              : (gon ? split_go(N6{}, std::true_type{}) : split_go(N6{}, std::false_type{}));
+    BIE_REQUIRE(ok, BIE_EINVAL, "internal: split slot kernel for an unsupported rule length");
+    BIE_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  // real k with k r below the fast sincos bound and a vertex-rule length with its own
+  // instance: the unrolled, branch-free point loop
+  const int qc = fast && !cplx ? a.len[0] : 0;
+  const bool occ2 = sa.occupancy == 2;
+  auto fast_go = [&](auto np, auto g) -> bool {
+    constexpr int NPc = decltype(np)::value;
+    constexpr bool Gc = decltype(g)::value;
+    if (occ2) {
+      switch (qc) {
+        case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 2>); return true;
+        case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 2>); return true;
+        case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 2>); return true;
+        default: return false;
+      }
+    }
+    switch (qc) {
+      case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true>); return true;
+      case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true>); return true;
+      case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true>); return true;
+      default: return false;
+    }
+  };
+  using T3 = std::integral_constant<int, 3>;
+  using T6 = std::integral_constant<int, 6>;
+  using GT = std::true_type;
+  using GF = std::false_type;
+  bool done = false;
+  if (a.NP == 3)
+    done = gon ? fast_go(T3{}, GT{}) : fast_go(T3{}, GF{});
+  else
+    done = gon ? fast_go(T6{}, GT{}) : fast_go(T6{}, GF{});
+  if (!done) {
+    if (a.NP == 3) {
+      if (cplx)
+        gon ? go(k_assemble_slots<3, true, W, true>) : go(k_assemble_slots<3, true, W, false>);
+      else
+        gon ? go(k_assemble_slots<3, false, W, true>) : go(k_assemble_slots<3, false, W, false>);
+    } else {
+      if (cplx)
+        gon ? go(k_assemble_slots<6, true, W, true>) : go(k_assemble_slots<6, true, W, false>);
+      else
+        gon ? go(k_assemble_slots<6, false, W, true>) : go(k_assemble_slots<6, false, W, false>);
+    }
   }
   BIE_HIP_CHECK(hipGetLastError());
 }

@@ -33,6 +33,12 @@ struct Error {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Bound on |x| for the branch-free table sincos of the slot kernels (sincos_tab_fast): 2^46,
+// so that 64 |x| / pi stays below 2^51 and its rounding by an added 1.5 * 2^52 is exact.
+// The host chooses that form only when |k| times a bound on every target-to-element-point
+// distance is below it.
+constexpr double kTabFastMaxArg = 70368744177664.0;
+
 // Point record of the concatenated rule set: u, v, w, then N, dN/du, dN/dv of the
 // geometry basis (6 slots each, unused slots zero).
 constexpr int kPtStride = 21;
@@ -71,6 +77,7 @@ struct AssemblyArgs {
   int off[6];
   int len[6];
   int R;
+  int hyp;                 // rule-selection distance: 0 julia_hypot3, 1 scaled_hypot3
   double kr, ki;
   unsigned flags;
   double* F;
@@ -107,12 +114,19 @@ __host__ __device__ inline int sweep_fixed_doubles(int nphi) {
 __host__ __device__ inline int stage_doubles(int Qc) {
   return Qc <= kStageMaxQ ? kSlotWaves * stage_wave_doubles(Qc) : 0;
 }
+// Split form of the slot kernel: `split` waves share each element of a step, wave h taking
+// the rule points h Qc/split .. (h+1) Qc/split - 1; each stages the rows of its points of
+// the three vertex rules and the selection nodes.
+__host__ __device__ inline int split_stage_wave_doubles(int Qh) { return 3 * Qh * 8 + kStageSel; }
+__host__ __device__ inline int split_stage_doubles(int Qc, int split) {
+  return kSlotWaves * split * split_stage_wave_doubles(Qc / split);
+}
 
 // LDS slots that fit beside `fixed_doubles` (padded basis table + staging) and the
 // mid-edge queue when each slot holds `per_slot` accumulators of 64 rows x 4 doubles.
-inline int fitting_slots(int fixed_doubles, int per_slot) {
+inline int fitting_slots(int fixed_doubles, int per_slot, long lds_bytes = kLdsBytes) {
   const long fixed = (long)sizeof(double) * fixed_doubles + 8L * kMidLds + 16;
-  const long S = (kLdsBytes - fixed) / (256L * (long)sizeof(double) * per_slot);
+  const long S = (lds_bytes - fixed) / (256L * (long)sizeof(double) * per_slot);
   return (int)(S < 120 ? S : 120);
 }
 
@@ -134,6 +148,8 @@ struct SlotArgs {
   int2* mid_queue;
   int64_t mid_cap;
   int push_mid;            // 1: queue mid-edge pairs; 0: they come from a cached sorted list
+  int occupancy;           // slot-kernel workgroups per CU the schedule's LDS size allows
+  int split;               // waves per element of a step (1, or 2 for the split kernel)
 };
 
 // One instance of an uneven batch (assemble.hip, k_assemble_batch): its assembly and
@@ -176,10 +192,59 @@ struct AdjointArgs {
   int len[6];
   int R;
   int E;
+  int hyp;                 // rule-selection distance (as AssemblyArgs::hyp)
   double k;
   double* gradX;           // 3 x Nc, accumulated
   double* gradk;           // 1, accumulated (or NULL)
 };
+
+// One instance of the batched k-derivative w = alpha^T (dF/dk) p (adjoint.hip).
+struct DfdkInst {
+  const double* targets;   // 3 x M (the instance's collocation nodes)
+  int64_t M;
+  int E;
+  int Pe;
+  const double* elem;      // element tables of the assembly (E x Pe x 8)
+  const double* selnodes;  // E x NP x 4
+  const int* ptopo;        // E x NP
+  const double* phi;       // R x Qmax x NP
+  int Qmax;
+  int R;
+  int hyp;
+  int off[6];
+  int len[6];
+  double kr, ki;
+  const double2* alpha;    // M
+  const double2* p;        // N
+  double* w;               // complex result (2 doubles), accumulated
+};
+
+// Sparse near-field assembly (sparse.hip).
+struct SparseLaunch {
+  const double* src;      // 3 x rows source points (device)
+  const int2* pairs;      // (row, element) per pair (device)
+  const int* pos;         // pairs x NP entry indices (device)
+  int64_t n_pairs;
+  const double* coords;   // 3 x Nc (device)
+  const int* topo;        // E x NG (device)
+  const double* gsel;     // E x NG x 4 geometry node coordinates (device)
+  const double* rules;    // point records, sparse_record_doubles() each (device)
+  int off[6], len[6];     // per geometry node: first record and point count
+  int NG, NP, hyp;
+  double kr, ki;
+  unsigned flags;
+  double* F;              // nnz complex (device), zeroed
+  double* G;
+};
+void subdivision_rule(const double src[3], std::vector<double>& u, std::vector<double>& v,
+                      std::vector<double>& w);
+void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,
+                    std::vector<int64_t>& row_ptr, std::vector<int>& cols,
+                    std::vector<int64_t>& pair_ptr, std::vector<int>& pair_elem);
+void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s);
+int sparse_record_doubles();
+// Geometry basis (kind 3 or 6) and its derivatives at (u, v), 6 slots each (rules.cpp).
+void geometry_basis(int kind, double u, double v, double* N, double* du, double* dv);
 
 // Launchers (assemble.hip)
 void launch_fill_zero(double* p, int64_t n_doubles, hipStream_t s);
@@ -189,7 +254,9 @@ void launch_geometry(const double* coords, const int* topo, int gk, int E,
 void launch_gather_selnodes(const double* sources, const int* ptopo, int NP, int E,
                             double* selnodes, hipStream_t s);
 void launch_assemble(const AssemblyArgs& a, hipStream_t s);
-void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
+// fast: every |k| r of the launch is below kTabFastMaxArg (kernels_math.hpp), so real-k
+// launches may use the branch-free sincos
+void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast, hipStream_t s);
 void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s);
 void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipStream_t s);
 void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_cols, hipStream_t s);
@@ -222,8 +289,22 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
 void sweep_prepare_attributes(int NP, int nphi, bool taylor, bool gon);
 void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s);
 void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s);
+// tiles: (instance, 256-row block, chunk of ech elements)
+void launch_dfdk_batch(const DfdkInst* d_inst, const int4* d_tiles, int n_tiles, int ech, int NP,
+                       hipStream_t s);
+// Solves on the device (solve.hip)
+void lu_factor_device(int64_t n, double* dA, int64_t lda, const double* dC, int32_t* d_ipiv,
+                      hipStream_t s);
+void lu_solve_device(int64_t n, const double* dA, int64_t lda, const int32_t* d_ipiv, int trans,
+                     double* dB, int64_t ldb, int64_t nrhs, hipStream_t s);
+void zgemv_device(int trans, int64_t m, int64_t n, const double* dA, int64_t lda,
+                  const double* dx, double* dy, hipStream_t s);
+void gmres_batched_device(int64_t n, int nsys, const double* dF, int64_t strideF, int64_t ldf,
+                          const double* dC, const double* dB, int64_t ldb, double* dX,
+                          int64_t ldx, double tol, int restart, int maxiter, int32_t* iters,
+                          double* resid, hipStream_t s);
 void launch_select(const double* targets, int64_t M, const double* selnodes, int NP, int E,
-                   int R, int8_t* out, hipStream_t s);
+                   int R, int hyp, int8_t* out, hipStream_t s);
 
 }  // namespace bie
 
@@ -231,6 +312,7 @@ void launch_select(const double* targets, int64_t M, const double* selnodes, int
 struct bie_dmesh {
   int device = 0;
   int gk = 3, pk = 3;
+  int hyp = 0;                  // rule-selection distance (BIE_MESH_SCALED_HYPOT)
   int64_t Nc = 0, E = 0, N = 0;
   double* d_coords = nullptr;
   int* d_topo = nullptr;        // E x gk, element-major
@@ -244,6 +326,9 @@ struct bie_dmesh {
   double* d_phi = nullptr;
   double* d_elem = nullptr;
   bool geom_valid = false;      // d_elem holds the geometry of the cached rule set
+  // bounding sphere of the mesh: centre, radius over the coordinates and over the sources
+  double center[3] = {0.0, 0.0, 0.0};
+  double coord_radius = 0.0, source_radius = 0.0;
   // host copies for scheduling
   std::vector<double> h_coords, h_sources;
   std::vector<int> h_topo, h_ptopo;
@@ -277,6 +362,11 @@ struct bie_dmesh {
   int* d_mid_off = nullptr;
   int* d_mid_refs = nullptr;
   int64_t n_mid_entries = 0;
+  // calls on the handle share the scratch above: each call's work waits for the previous
+  // call's (recorded in `done` on `done_stream`) when it runs on another stream
+  hipEvent_t done = nullptr;
+  hipStream_t done_stream = nullptr;
+  bool done_valid = false;
   std::mutex mu;
 };
 

@@ -47,15 +47,39 @@ __device__ __noinline__ double julia_hypot3(double x, double y, double z) {
   return mx * sqrt(s);
 }
 
+// The n-ary hypot that later Julia releases use for IEEE floats: the arguments scaled by
+// 2^-E with E the exponent of maxabs (clamped to the normal range), squared and summed left
+// to right, square-rooted and scaled back by 2^E.  The scalings are exact, so away from overflow and underflow this is
+// sqrt((x^2 + y^2) + z^2) with every operation rounded once; no fused multiply-add, so the
+// result is bit-identical to the CPU.
+__device__ __noinline__ double scaled_hypot3(double x, double y, double z) {
+#pragma clang fp contract(off)
+  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
+  if (isinf(ax) || isinf(ay) || isinf(az)) return INFINITY;
+  if (isnan(ax) || isnan(ay) || isnan(az)) return NAN;
+  double mx = ax > ay ? ax : ay;
+  mx = mx > az ? mx : az;
+  if (mx == 0.0) return mx;
+  int e = 0;
+  (void)frexp(mx, &e);  // 2^E <= mx < 2^(E+1) with E = e - 1, clamped to the normal range
+  const int E = max(-1022, min(1022, e - 1));
+  const double inv = ldexp(1.0, -E), scale = ldexp(1.0, E);
+  const double a = ax * inv, b = ay * inv, c = az * inv;
+  double s = a * a;
+  s = s + b * b;
+  s = s + c * c;
+  return scale * sqrt(s);
+}
+
 // 1-based rule index of the node nearest to x (first strict minimum of the Julia hypot
-// distance).  A squared-distance filter decides every pair whose two nearest nodes are
-// separated by more than 1e-12 relative (far beyond the few-ulp error of either
-// distance, so the exact argmin is the same); near ties, zero, inf and NaN distances go
-// through the exact sequence.  Index 0 (no distance below Inf) maps to the last rule,
-// as the reference's if/elseif chain does.
+// distance; hyp 0: julia_hypot3, 1: scaled_hypot3).  A squared-distance filter decides
+// every pair whose two nearest nodes are separated by more than 1e-12 relative (far beyond
+// the few-ulp error of either distance, so the exact argmin is the same under either
+// formula); near ties, zero, inf and NaN distances go through the exact sequence.  Index 0
+// (no distance below Inf) maps to the last rule, as the reference's if/elseif chain does.
 template <int NP>
 __device__ __forceinline__ int select_rule(double x0, double x1, double x2,
-                                           const double* __restrict__ P, int R) {
+                                           const double* __restrict__ P, int R, int hyp) {
   double d2[NP];
 #pragma unroll
   for (int s = 0; s < NP; ++s) {
@@ -82,7 +106,8 @@ __device__ __forceinline__ int select_rule(double x0, double x1, double x2,
     double best = INFINITY;
     int idx = 0;
     for (int s = 0; s < NP; ++s) {
-      const double d = julia_hypot3(P[4 * s] - x0, P[4 * s + 1] - x1, P[4 * s + 2] - x2);
+      const double dx = P[4 * s] - x0, dy = P[4 * s + 1] - x1, dz = P[4 * s + 2] - x2;
+      const double d = hyp ? scaled_hypot3(dx, dy, dz) : julia_hypot3(dx, dy, dz);
       if (d < best) {
         best = d;
         idx = s + 1;
@@ -135,6 +160,43 @@ __device__ __forceinline__ void stage_store(double* rows, int nchunk, double* se
   for (int t = 0; t < 3; ++t) {
     const int j = lane + 64 * t;
     if (j < nchunk) reinterpret_cast<double2*>(rows)[j] = v[t];
+  }
+  if (lane < nsel) reinterpret_cast<double2*>(sel_rows)[lane] = vs;
+}
+
+// Split-kernel staging: a wave takes the rows of points half*QH .. half*QH + QH - 1 of
+// each of the element's three vertex rules (Qc points each, 64 bytes per row): 12 QH chunks
+// of 16 bytes, chunk t from rule t / (4 QH), row chunk t % (4 QH), at the element's
+// table chunk rule * 4 Qc + half * 4 QH + t % (4 QH).  Bounds-checked buffer loads as in
+// stage_load (lanes past 12 QH read zeros), plus the nsel selection-node chunks.
+template <int QC, int QH, int NV>
+__device__ __forceinline__ void stage_load_half(const double* __restrict__ elem_e, int half,
+                                                const double* __restrict__ sel, int nsel,
+                                                int lane, double2 (&v)[NV], double2& vs) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double*>(elem_e), (short)0, 12 * QC * 16, 0x00020000);
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int t = lane + 64 * u;
+    const int c = t / (4 * QH);
+    const int src = c * 4 * QC + half * 4 * QH + (t - c * 4 * QH);
+    const int off = t < 12 * QH ? src * 16 : 12 * QC * 16;
+    v[u] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+  }
+  const __amdgpu_buffer_rsrc_t rsel = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double*>(sel), (short)0, nsel * 16, 0x00020000);
+  vs = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rsel, lane * 16, 0, 0));
+}
+
+template <int QH, int NV>
+__device__ __forceinline__ void stage_store_half(double* rows, double* sel_rows, int nsel,
+                                                 int lane, const double2 (&v)[NV],
+                                                 const double2& vs) {
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int t = lane + 64 * u;
+    if (t < 12 * QH) reinterpret_cast<double2*>(rows)[t] = v[u];
   }
   if (lane < nsel) reinterpret_cast<double2*>(sel_rows)[lane] = vs;
 }

@@ -77,13 +77,42 @@ __device__ __forceinline__ void sincos_tab(double x, const double2* __restrict__
   *c = fma(t.x, cd, -(t.y * sd));
 }
 
-// 1/sqrt(x) from the hardware estimate refined by two Newton steps.
+// sincos_tab without the large-argument branch, for |x| < kTabFastMaxArg (bie_common.hpp):
+// n = rint(64 x / pi) is formed by adding 1.5 * 2^52 to x * 64 / pi inside one fused multiply-add (the sum's
+// unit in the last place is 1, so it is rounded to an integer there), the reduction uses
+// n = sum - 1.5 * 2^52 (exact), and the table index n mod 128 is the low bits of the sum's
+// mantissa (two's complement for negative n), so no float-to-integer conversion is needed.
+// Same reduction, table and polynomials as sincos_tab otherwise; NaN and infinite x give
+// NaN (the index is masked into the table whatever the bits are).
+__device__ __forceinline__ void sincos_tab_fast(double x, const double2* __restrict__ tab,
+                                                double* s, double* c) {
+  constexpr double kInvStep = 20.371832715762602978;   // 64 / pi
+  constexpr double kStepHi = 1.5707963267948965580 / 32;   // pi/2 rounded, / 32 (exact)
+  constexpr double kStepLo = 6.1232339957367658e-17 / 32;  // its remainder, / 32 (exact)
+  constexpr double kShift = 6755399441055744.0;            // 1.5 * 2^52
+  const double sh = fma(x, kInvStep, kShift);
+  const double n = sh - kShift;
+  double d = fma(-n, kStepHi, x);
+  d = fma(-n, kStepLo, d);
+  const double2 t = tab[__double2loint(sh) & (kTrigN - 1)];
+  const double d2 = d * d;
+  const double sp = fma(d2, fma(d2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0);
+  const double sd = fma(d * d2, sp, d);
+  const double cp = fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5);
+  const double cd = fma(d2, cp, 1.0);
+  *s = fma(t.y, cd, t.x * sd);
+  *c = fma(t.x, cd, -(t.y * sd));
+}
+
+// 1/sqrt(x) from the hardware estimate y0 (v_rsq_f64; relative error below 6e-8 on
+// gfx950, measured over 4M arguments spanning e^-20..e^20) refined by one step of the
+// third-order iteration y = y0 (1 + e/2 + 3 e^2/8), e = 1 - x y0^2, whose truncation
+// (5/16) e^3 < 1e-22 leaves only the roundings of e and of the last fused multiply-add
+// (~2e-16): five instructions where two Newton steps take seven.
 __device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x;
-  y = y * fma(-h, y * y, 1.5);
-  y = y * fma(-h, y * y, 1.5);
-  return y;
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 }  // namespace bie

@@ -170,6 +170,10 @@ Pts rule_points(int c, int n, int m) {
 
 }  // namespace
 
+void geometry_basis(int kind, double u, double v, double* N, double* du, double* dv) {
+  eval_basis(kind, u, v, N, du, dv);
+}
+
 RuleSet build_ruleset(int gk, int pk, int n, int m) {
   if ((gk != 3 && gk != 6) || (pk != 3 && pk != 6))
     throw Error{BIE_EUNSUPPORTED, "only linear (3-node) and quadratic (6-node) triangles"};

@@ -288,7 +288,8 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
   int c = 0;
   double4 A0 = make_double4(0, 0, 0, 0), B0 = A0;
   if (e >= 0) {
-    c = select_rule<NP>(x0, x1, x2, sa.step_sel + ((size_t)st0 * W + wave) * NP * 4, a.R) - 1;
+    c = select_rule<NP>(x0, x1, x2, sa.step_sel + ((size_t)st0 * W + wave) * NP * 4, a.R,
+                        a.hyp) - 1;
     if (!(NP == 6 && c >= 3)) {
       const double* t0 = a.elem + ((size_t)e * a.Pe + a.off[c]) * 8;
       A0 = *reinterpret_cast<const double4*>(t0);
@@ -376,7 +377,7 @@ k_sweep_slots(const AssemblyArgs a, const SlotArgs sa, const SweepArgs sw) {
     }
     int c_nxt = 0;
     if (e_nxt >= 0) {
-      c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R) - 1;
+      c_nxt = select_rule<NP>(x0, x1, x2, sa.step_sel + nidx * NP * 4, a.R, a.hyp) - 1;
       if (!(NP == 6 && c_nxt >= 3)) {
         const double* t0 = a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8;
         A0 = *reinterpret_cast<const double4*>(t0);

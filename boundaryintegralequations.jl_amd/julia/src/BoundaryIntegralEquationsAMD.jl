@@ -38,8 +38,16 @@ struct BieMesh
     geometry_kind::Int32
     physics_kind::Int32
     index_base::Int32
-    reserved::Int32
+    flags::Int32
 end
+
+# Rule selection compares hypot distances, and Julia releases round the 3-argument hypot
+# differently (Julia 1.9: maxabs * sqrt(sum((x/maxabs)^2)); later: power-of-two scaled
+# sqrt(x^2 + y^2 + z^2)).  The flag follows the running Julia, found from one argument
+# triple on which the two formulas round differently.
+const BIE_MESH_SCALED_HYPOT = Int32(1)
+hypot_flag() = hypot(0.6732655185893088, 0.3428080423874833, 0.13687617154257523) ==
+               0.7678143647933542 ? BIE_MESH_SCALED_HYPOT : Int32(0)
 
 element_kind(::BIE.TriangularLinear) = Int32(3)
 element_kind(::BIE.TriangularQuadratic) = Int32(6)
@@ -70,7 +78,9 @@ function assemble_hip(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix;
     topo = Matrix{Int64}(mesh.topology)
     srcs = Matrix{Float64}(mesh.sources)
     ptopo = Matrix{Int64}(mesh.physics_topology)
-    collocation = in_sources === mesh.sources
+    # collocation (the sources themselves, or an equal array: the library also recognises
+    # value-identical targets) passes NULL, which selects the recorded mid-edge pairs
+    collocation = in_sources === mesh.sources || in_sources == mesh.sources
     targets = collocation ? srcs : Matrix{Float64}(in_sources)
     M = size(targets, 2)
     N = size(srcs, 2)
@@ -83,12 +93,12 @@ function assemble_hip(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix;
         bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
                      size(coords, 2), size(topo, 2), N,
                      element_kind(mesh.shape_function), element_kind(mesh.physics_function),
-                     Int32(1), Int32(0))
+                     Int32(1), hypot_flag())
         rc = ccall((:bie_assemble, libbie[]), Cint,
                    (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Int32, UInt32,
                     Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
-                   bm, real(kc), imag(kc), pointer(targets), M, Int32(n), Int32(m), flags,
-                   F, G, C)
+                   bm, real(kc), imag(kc), collocation ? Ptr{Float64}(C_NULL) : pointer(targets),
+                   M, Int32(n), Int32(m), flags, F, G, C)
         check(rc)
     end
     return F, G, C
@@ -106,6 +116,50 @@ function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true,
         return BIE.assemble_parallel!(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn,
                                       sparse=sparse, m=m, n=n, progress=progress,
                                       depth=depth, offset=offset)
+    end
+    return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
+end
+
+# ---------------------------------------------------------------------------------------
+# Routing hook inside the reference's own generic function
+# ---------------------------------------------------------------------------------------
+#
+# The reference's dispatcher (assembly_collocation.jl:259-268) calls the 4-argument methods
+# assemble_parallel!(mesh, k, in_sources, mesh.shape_function; ...) for Triangular (:270) and
+# TriangularQuadratic (:355) geometry.  The methods below are more specific in k and
+# in_sources, so every caller inside the reference - the examples, the dense lossy branch
+# assemble_parallel!(mesh, kₐ, sources; m, n) at LossyGlobalMatrix.jl:119 (complex kₐ) -
+# reaches the GPU once this package is loaded.  Unsupported physics and `use_gpu!(false)`
+# go back to the reference's own method through `invoke`.
+
+const GPU_ROUTING = Ref(true)
+
+"""
+    use_gpu!(on::Bool)
+
+Turn the routing of the reference's `assemble_parallel!` to the GPU on or off (on by
+default once the package is loaded).
+"""
+use_gpu!(on::Bool) = (GPU_ROUTING[] = on; on)
+
+function BIE.assemble_parallel!(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix{<:Real},
+                                shape_function::BIE.TriangularLinear; fOn=true, gOn=true,
+                                cOn=true, m=3, n=3, progress=true)
+    if !(GPU_ROUTING[] && supported(mesh))
+        return invoke(BIE.assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,BIE.Triangular}, mesh, k,
+                      in_sources, shape_function; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n,
+                      progress=progress)
+    end
+    return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
+end
+
+function BIE.assemble_parallel!(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix{<:Real},
+                                shape_function::BIE.TriangularQuadratic; fOn=true, gOn=true,
+                                cOn=true, m=3, n=3, progress=true)
+    if !(GPU_ROUTING[] && supported(mesh))
+        return invoke(BIE.assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,BIE.TriangularQuadratic},
+                      mesh, k, in_sources, shape_function; fOn=fOn, gOn=gOn, cOn=cOn, m=m,
+                      n=n, progress=progress)
     end
     return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
 end
@@ -157,7 +211,7 @@ mutable struct DeviceMesh
             bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
                          size(coords, 2), size(topo, 2), size(srcs, 2),
                          element_kind(mesh.shape_function), element_kind(mesh.physics_function),
-                         Int32(1), Int32(0))
+                         Int32(1), hypot_flag())
             check(ccall((:bie_dmesh_create, libbie[]), Cint, (Ref{BieMesh}, Cint, Ptr{Ptr{Cvoid}}),
                         bm, Cint(device), h))
         end
@@ -268,28 +322,89 @@ end
 plane_wave(x, k, d) = exp.(im * k .* vec(d' * x))
 
 """
+    assemble_device!(dm, k, dF, dG, dC; targets=nothing, M=n_sources(dm), n=4, m=4,
+                     fOn=true, gOn=true, cOn=true)
+
+F, G, C into device buffers (bie_assemble_device), collocation or at the M device targets.
+"""
+function assemble_device!(dm::DeviceMesh, k::Number, dF::DeviceBuffer, dG::DeviceBuffer,
+                          dC::DeviceBuffer; targets=nothing, M::Integer=n_sources(dm), n=4, m=4,
+                          fOn=true, gOn=true, cOn=true)
+    kc = ComplexF64(k)
+    check(ccall((:bie_assemble_device, libbie[]), Cint,
+                (Ptr{Cvoid}, Cdouble, Cdouble, Ptr{Cvoid}, Int64, Int32, Int32, UInt32,
+                 Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dm.handle, real(kc), imag(kc), targets === nothing ? C_NULL : targets.ptr,
+                Int64(M), Int32(n), Int32(m), flagbits(fOn, gOn, cOn), dF.ptr, dG.ptr, dC.ptr,
+                C_NULL))
+    return nothing
+end
+
+lu_factor!(dA::DeviceBuffer, N, ipiv::DeviceBuffer, dC) =
+    check(ccall((:bie_lu_factor_device, libbie[]), Cint,
+                (Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                N, dA.ptr, N, dC === nothing ? C_NULL : dC.ptr, ipiv.ptr, C_NULL))
+
+lu_solve!(dA::DeviceBuffer, N, ipiv::DeviceBuffer, dB::DeviceBuffer; trans=0) =
+    check(ccall((:bie_lu_solve_device, libbie[]), Cint,
+                (Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}),
+                N, dA.ptr, N, ipiv.ptr, Int32(trans), dB.ptr, N, 1, C_NULL))
+
+gemv!(dy::DeviceBuffer, dA::DeviceBuffer, m, n, dx::DeviceBuffer; trans=0) =
+    check(ccall((:bie_zgemv_device, libbie[]), Cint,
+                (Int32, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                Int32(trans), m, n, dA.ptr, max(m, 1), dx.ptr, dy.ptr, C_NULL))
+
+# The forward problem with F and F_field on the device: A = F + diag(1 - C) factored in
+# place, p = A \ b, u = p_inc - F_field p; only vectors of N and P entries are downloaded.
+function forward_device(dmesh::DeviceMesh, k::Real, field_points::AbstractMatrix, n, m,
+                        direction)
+    srcs = Matrix{Float64}(dmesh.mesh.sources)
+    pts = Matrix{Float64}(field_points)
+    N, P = size(srcs, 2), size(pts, 2)
+    dF = DeviceBuffer(16 * N * N)
+    dFf = DeviceBuffer(16 * P * N)
+    dG = DeviceBuffer(16 * N * max(N, P))      # G is not needed (zero-filled)
+    dC = DeviceBuffer(16 * N)
+    dCf = DeviceBuffer(16 * P)
+    dT = DeviceBuffer(pts)
+    assemble_device!(dmesh, k, dF, dG, dC; n=n, m=m, gOn=false)
+    assemble_device!(dmesh, k, dFf, dG, dCf; targets=dT, M=P, n=n, m=m, gOn=false, cOn=false)
+    ipiv = DeviceBuffer(4 * N)
+    lu_factor!(dF, N, ipiv, dC)
+    b = plane_wave(srcs, k, direction)
+    dp = DeviceBuffer(b)
+    lu_solve!(dF, N, ipiv, dp)
+    dfp = DeviceBuffer(16 * P)
+    gemv!(dfp, dFf, P, N, dp)
+    p = download!(Vector{ComplexF64}(undef, N), dp)
+    pinc = plane_wave(pts, k, direction)
+    u = pinc - download!(Vector{ComplexF64}(undef, P), dfp)
+    return (; dF, dFf, ipiv, b, p, pinc, u, pts, srcs, N, P)
+end
+
+"""
     scattering_gradient(mesh, k, field_points; n=4, m=4, direction=[0, 0, 1]) -> (J, dJdX, dJdk)
 
 J = sum |u|^2 of the field pressure u = p_inc - F_field p at `field_points`, with p solving
 (F + diag(1 - C)) p = p_inc on the rigid body (examples/3d_rigid_sphere.jl:47-52,
 examples/3d_rosebem.jl:81), and its gradient with respect to the mesh coordinates and k by
 the adjoint method: A' lambda = F_field' u, then two vector-Jacobian products of the
-assembly (rule selection held fixed).
+assembly (rule selection held fixed).  F, F_field and the LU factors of A stay on the
+device (bie_lu_factor_device, bie_lu_solve_device, bie_zgemv_device); only vectors of N
+and P entries and the gradient come back.
 """
 function scattering_gradient(mesh::BIE.Mesh3d, k::Real, field_points::AbstractMatrix; n=4,
                              m=4, direction=[0.0, 0.0, 1.0], dm=nothing)
     dmesh = dm === nothing ? DeviceMesh(mesh) : dm
-    srcs = Matrix{Float64}(mesh.sources)
-    F, _, C = assemble_hip(mesh, k, mesh.sources; gOn=false, n=n, m=m)
-    Ff, _, _ = assemble_hip(mesh, k, Matrix{Float64}(field_points); gOn=false, cOn=false,
-                            n=n, m=m)
-    A = F + LinearAlgebra.Diagonal(1 .- C)
-    b = plane_wave(srcs, k, direction)
-    p = A \ b
-    pinc = plane_wave(field_points, k, direction)
-    u = pinc - Ff * p
+    fw = forward_device(dmesh, k, field_points, n, m, direction)
+    srcs, pts, b, p, u, pinc = fw.srcs, fw.pts, fw.b, fw.p, fw.u, fw.pinc
     J = real(sum(abs2, u))
-    lam = A' \ (Ff' * u)
+    du = DeviceBuffer(u)
+    dlam = DeviceBuffer(16 * fw.N)
+    gemv!(dlam, fw.dFf, fw.P, fw.N, du; trans=2)      # F_field' u
+    lu_solve!(fw.dF, fw.N, fw.ipiv, dlam; trans=2)     # A' lambda = F_field' u
+    lam = download!(Vector{ComplexF64}(undef, fw.N), dlam)
     # surface nodes are mesh coordinates: find each source's coordinate index
     lut = Dict(Tuple(srcs[:, j]) => j for j in 1:size(srcs, 2))
     coord_of = zeros(Int, size(srcs, 2))
@@ -300,8 +415,7 @@ function scattering_gradient(mesh::BIE.Mesh3d, k::Real, field_points::AbstractMa
     all(>(0), coord_of) || throw(ArgumentError("collocation nodes are not mesh coordinates"))
     gs, gks = assembly_vjp(dmesh, k, conj.(lam), p; beta=-conj.(lam) .* p,
                            target_node=coord_of, n=n, m=m)
-    gf, gkf = assembly_vjp(dmesh, k, -conj.(u), p; targets=Matrix{Float64}(field_points),
-                           n=n, m=m)
+    gf, gkf = assembly_vjp(dmesh, k, -conj.(u), p; targets=pts, n=n, m=m)
     grad = 2 .* (gs .+ gf)
     dk = 2 * (gks + gkf)
     d = Vector{Float64}(direction)
@@ -310,7 +424,7 @@ function scattering_gradient(mesh::BIE.Mesh3d, k::Real, field_points::AbstractMa
         grad[c, coord_of[j]] += 2 * d[c] * coef[j]
     end
     dk += 2 * real(sum(-conj.(lam) .* im .* vec(d' * srcs) .* b))
-    dk += 2 * real(sum(conj.(u) .* im .* vec(d' * field_points) .* pinc))
+    dk += 2 * real(sum(conj.(u) .* im .* vec(d' * pts) .* pinc))
     return J, grad, dk
 end
 
@@ -364,12 +478,8 @@ returns the adjoint gradient with respect to the mesh coordinates (as a tangent 
 """
 function scattering_objective(mesh::BIE.Mesh3d, k::Real, field_points::AbstractMatrix; n=4,
                               m=4, direction=[0.0, 0.0, 1.0])
-    F, _, C = assemble_hip(mesh, k, mesh.sources; gOn=false, n=n, m=m)
-    Ff, _, _ = assemble_hip(mesh, k, Matrix{Float64}(field_points); gOn=false, cOn=false,
-                            n=n, m=m)
-    p = (F + LinearAlgebra.Diagonal(1 .- C)) \ plane_wave(Matrix{Float64}(mesh.sources), k, direction)
-    u = plane_wave(field_points, k, direction) - Ff * p
-    return real(sum(abs2, u))
+    fw = forward_device(DeviceMesh(mesh), k, field_points, n, m, direction)
+    return real(sum(abs2, fw.u))
 end
 
 # ---------------------------------------------------------------------------------------

@@ -32,6 +32,13 @@ extern "C" {
 #define BIE_C 4u            /* cOn */
 #define BIE_FGC 7u
 
+/* bie_mesh.flags: rule selection (find_closest_corner, assembly_collocation.jl:102-115)
+ * compares hypot distances, and Julia releases differ in how hypot(x, y, z) rounds:
+ * 0 = maxabs * sqrt(sum((x_i/maxabs)^2)) (Julia 1.9, the reference's CI version),
+ * BIE_MESH_SCALED_HYPOT = power-of-two scaled sqrt(x^2 + y^2 + z^2) (later releases).
+ * The choice decides the rule of exactly tied pairs, which symmetric meshes have. */
+#define BIE_MESH_SCALED_HYPOT 1
+
 /* Mesh3d (src/utils/mesh.jl:23-34), the fields assembly reads. */
 typedef struct bie_mesh {
   const double* coordinates;        /* 3 x n_coordinates                              */
@@ -44,12 +51,13 @@ typedef struct bie_mesh {
   int32_t geometry_kind;            /* BIE_TRI3 | BIE_TRI6 */
   int32_t physics_kind;             /* BIE_TRI3 | BIE_TRI6 */
   int32_t index_base;               /* 1 for Julia arrays, 0 for C/Python              */
-  int32_t reserved;
+  int32_t flags;                    /* 0 or BIE_MESH_SCALED_HYPOT                      */
 } bie_mesh;
 
 typedef struct bie_dmesh bie_dmesh;  /* device-resident mesh + cached rule tables */
 typedef struct bie_graph bie_graph;  /* an instantiated HIP graph of one batched assembly */
 typedef struct bie_batch bie_batch;  /* an uneven batch of device meshes, one rule set */
+typedef struct bie_sparse bie_sparse;  /* pattern and rules of a sparse near-field assembly */
 
 const char* bie_last_error(void);
 int bie_version(void);
@@ -71,6 +79,46 @@ int bie_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* t
                  int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* F,
                  double* G, double* C);
 
+/* State of the host-array entry point's kept mesh (no counterpart in the reference):
+ * calls that reused it (hits), meshes uploaded (builds), its n_sources (-1: none kept) and
+ * the entries of its mid-edge pair queue (0 when none is held; any pointer may be NULL).
+ * Targets equal to the mesh sources value for value are collocation; the pair queue of
+ * other targets is released when the call returns. */
+int bie_host_cache_info(int64_t* hits, int64_t* builds, int64_t* n_sources,
+                        int64_t* queue_entries);
+/* Releases the kept mesh (its device memory is freed when no call is using it) and the
+ * pinned staging buffers; the next bie_assemble builds them again. */
+int bie_host_cache_release(void);
+
+/* ---- Sparse near field -----------------------------------------------------------------
+ * Replaces sparse_assemble_parallel!(mesh, k, sources, physics_function; fOn, gOn, depth,
+ * offset) (src/3d/assembly_collocation_losses.jl:200-405; assemble_parallel! with
+ * sparse=true, src/3d/assembly_collocation.jl:261-263): F and G on the pairs (source i,
+ * element e) with e connected to node i through at most `depth` (1 or 2) elements
+ * (connected_sources, assembly_collocation_losses.jl:98-132), each integrated with the
+ * subdivision rule (singular_triangle_integration, src/3d/adaptive_integration.jl:172-246)
+ * clustered at the element's geometry node closest to source i.  The reference returns
+ * sparse(I, J, F), sparse(I, J, G): I = the row of every stored entry, J = its column (the
+ * connected sources of each row, ascending), in that order; these entry points give the
+ * same I, J (index_base of the mesh) and values.  offset: the reference's `offset` keyword
+ * (NaN = nothing, i.e. 0 for continuous physics).  No C (free term) is formed there. */
+int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz);
+/* sources: 3 x n_sources (NULL = mesh sources; n_sources must equal the mesh's), flags
+ * BIE_F | BIE_G (a disabled output is zero-filled), I, J (nnz int64), F, G (nnz complex)
+ * host arrays. */
+int bie_sparse_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* sources,
+                        int64_t n_sources, int32_t depth, double offset, uint32_t flags,
+                        int64_t nnz, int64_t* I, int64_t* J, double* F, double* G);
+/* Device-resident form: the pattern, pairs and rules built once per (mesh, depth, offset),
+ * then any number of assemblies into device arrays of nnz complex entries (asynchronous on
+ * stream; d_sources NULL = the mesh sources).  Destroy the handle before its dmesh. */
+int bie_sparse_create(bie_dmesh* dm, int32_t depth, double offset, bie_sparse** out);
+int bie_sparse_destroy(bie_sparse* sp);
+int64_t bie_sparse_nnz(const bie_sparse* sp);
+int bie_sparse_pattern(const bie_sparse* sp, int32_t index_base, int64_t* I, int64_t* J);
+int bie_sparse_assemble_device(bie_sparse* sp, double k_re, double k_im, const double* d_sources,
+                               uint32_t flags, double* dF, double* dG, void* stream);
+
 /* ---- Device-resident path ------------------------------------------------------------
  * Uploads the mesh once (the element loop's inputs, Mesh3d src/utils/mesh.jl:23-34). */
 int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out);
@@ -82,7 +130,12 @@ const double* bie_dmesh_sources(const bie_dmesh* dm);
 
 /* Same semantics as bie_assemble, with device pointers and an optional hipStream_t
  * (NULL = the legacy default stream).  d_targets NULL = collocation at mesh sources.
- * Asynchronous with respect to the host. */
+ * Asynchronous with respect to the host.  Calls on one handle share its device scratch
+ * (element tables, schedules, mid-edge pair lists): each call's work waits for the
+ * previous call's on that handle when the two use different streams, so calls may come
+ * from several threads and streams.  Work captured into a graph (bie_sweep_graph_create,
+ * or a caller's own capture) is not ordered this way: its launches must not overlap other
+ * work on the same handle. */
 int bie_assemble_device(bie_dmesh* dm, double k_re, double k_im, const double* d_targets,
                         int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* dF,
                         double* dG, double* dC, void* stream);
@@ -183,6 +236,20 @@ int bie_batch_tune(bie_batch* b, const double* k_re, const double* k_im, uint32_
                    double* const* dF, double* const* dG, double* const* dC, void* stream,
                    double* best_ms);
 
+/* Batched derivative in k, one launch over the batch's instances: for every instance i
+ *   w_i = alpha_i^T (dF_i/dk) p_i     (complex; F_i its collocation F at k_re[i] + i k_im[i])
+ * written to d_w[2 i], d_w[2 i + 1] (device array of 2 n_inst doubles, overwritten).  alpha_i
+ * (N_i complex) and p_i (N_i complex) are device arrays listed in host arrays; k_im may be
+ * NULL.  dF/dk = -k e^{ikr} (n.d) J w / (4 pi r) per quadrature point (the order-1 term of
+ * the reference's taylor_freens3d!, src/3d/kernels.jl:104-118), with the assembly's rules.
+ * For a real k and Phi_i = Re[alpha_i^T F_i p_i], dPhi_i/dk = Re(w_i): the k-part of an
+ * adjoint gradient (alpha_i from the adjoint solve of instance i).  Waits for the launch.
+ * No counterpart in the reference (it differentiates by finite differences,
+ * examples/2d_optimization.jl:184-216). */
+int bie_batch_dfdk_device(bie_batch* b, const double* k_re, const double* k_im,
+                          const double* const* d_alpha, const double* const* d_p, double* d_w,
+                          void* stream);
+
 /* ---- Adjoint building block ------------------------------------------------------------
  * Vector-Jacobian product of the assembly at real k with the rule selection held fixed:
  *   d_grad_coords[3*v + d] += d/dX[v,d]  Re[ sum_i alpha_i sum_j F_ij p_j + sum_i beta_i C_i ]
@@ -206,8 +273,11 @@ int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, in
  * Replaces the BLAS ZGEMM behind `F*X` / `G*vs` in the reference's callers
  * (examples/3d_pulsating_sphere.jl:42,62), batched like the FMM operators' 32-RHS path
  * (src/3d/fast_multipole_method.jl:249-287).  Uses a cached per-device workspace of
- * 16 * K * ceil(nrhs/32)*32 bytes.  The kernel addresses each K split with 32-bit buffer
- * offsets: lda and ceil(nrhs/32)*32 up to 1.6e6 (BIE_EINVAL beyond). */
+ * 16 * K * ceil(nrhs/32)*32 bytes, shared by all calls on the device: a call on another
+ * stream than the previous one waits for it on the device.  Under graph capture the
+ * workspace cannot grow (BIE_EINVAL; make one call of that size outside the capture first),
+ * and replays of the graph must not overlap other calls.  The kernel addresses each K split
+ * with 32-bit buffer offsets: lda and ceil(nrhs/32)*32 up to 1.6e6 (BIE_EINVAL beyond). */
 int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream);
 
@@ -222,6 +292,41 @@ int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64
 int bie_zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA,
                              int64_t lda, const double* dX, int64_t ldx, double* dY,
                              int64_t ldy, const int32_t* d_idx, int32_t n_idx, void* stream);
+
+/* ---- Solves on the device ----------------------------------------------------------
+ * The reference solves the collocation system A p = b with A = F + diag(1 - C) by Julia's
+ * `\` (LAPACK getrf/getrs; examples/3d_rigid_sphere.jl:47-52, test/3d/assembly_test.jl:37)
+ * or IterativeSolvers.gmres, and forms the field pressure p_inc - F_field p
+ * (examples/3d_rosebem.jl:81).  These keep F where the assembly wrote it.  Matrices are
+ * column-major complex128; dimensions and leading dimensions below 2^31. */
+
+/* A := F + diag(1 - C) (dC NULL: A := F) in place at dA (n x n, leading dimension lda),
+ * then its LU factorisation with partial pivoting in place (rocSOLVER zgetrf); pivots to
+ * d_ipiv (n int32, device).  Waits for the factorisation: a singular matrix returns
+ * BIE_EINVAL naming the zero pivot's column. */
+int bie_lu_factor_device(int64_t n, double* dA, int64_t lda, const double* dC, int32_t* d_ipiv,
+                         void* stream);
+/* B := op(A)^-1 B with the factors of bie_lu_factor_device: op(A) = A (trans 0), A^T (1)
+ * or A^H (2); B n x nrhs (leading dimension ldb).  Asynchronous (rocSOLVER zgetrs). */
+int bie_lu_solve_device(int64_t n, const double* dA, int64_t lda, const int32_t* d_ipiv,
+                        int32_t trans, double* dB, int64_t ldb, int64_t nrhs, void* stream);
+/* y := op(A) x for A m x n (leading dimension lda): trans 0 (y: m entries), 1 (A^T) or 2
+ * (A^H; y: n entries).  Asynchronous (rocBLAS zgemv). */
+int bie_zgemv_device(int32_t trans, int64_t m, int64_t n, const double* dA, int64_t lda,
+                     const double* dx, double* dy, void* stream);
+/* Restarted GMRES (Arnoldi with classical Gram-Schmidt applied twice, Givens rotations,
+ * restart after `restart` steps, at most maxiter steps, stop at ||b_q - A_q x_q|| <=
+ * tol ||b_q||) for the n_systems systems (F_q + diag(1 - C)) x_q = b_q at once: F_q at
+ * dF + 2*q*strideF doubles (leading dimension ldf: a frequency sweep's output), C shared
+ * (n complex), b_q at dB + 2*q*ldb doubles, x_q written at dX + 2*q*ldx doubles (x0 = 0).
+ * The Arnoldi vectors stay on the device; the small Hessenberg problems are solved on the
+ * host, one small transfer per step for all systems.  Synchronous; iterations (int32) and
+ * relative residuals (double) per system go to the host arrays. */
+int bie_gmres_batched_device(int64_t n, int32_t n_systems, const double* dF, int64_t strideF,
+                             int64_t ldf, const double* dC, const double* dB, int64_t ldb,
+                             double* dX, int64_t ldx, double tol, int32_t restart,
+                             int32_t maxiter, int32_t* iterations, double* residuals,
+                             void* stream);
 
 /* ---- Kernel timing ------------------------------------------------------------------
  * While enabled, every device assembly records four HIP events on its launch stream:

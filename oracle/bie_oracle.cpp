@@ -38,6 +38,8 @@
 
 extern "C" double orc_julia_hypot3(double x, double y, double z);
 extern "C" int orc_find_closest(const double* x, const double* nodes, int n_nodes);
+extern "C" int orc_find_closest_variant(const double* x, const double* nodes, int n_nodes,
+                                        int variant);
 
 namespace {
 
@@ -266,13 +268,13 @@ Tables build_tables(const MeshView& mv, int n, int m) {
 
 // Rule index 1..R used for (target x, element e), with the reference's fallthrough:
 // index 0 (no distance below Inf) lands on the last branch.
-inline int rule_for(const MeshView& mv, const double* x, int64_t e, int R) {
+inline int rule_for(const MeshView& mv, const double* x, int64_t e, int R, int variant = 0) {
   double P[3 * 6];
   for (int s = 0; s < mv.pk; ++s) {
     int64_t node = mv.ptopo[s + e * mv.pk] - mv.base;
     for (int d = 0; d < 3; ++d) P[3 * s + d] = mv.sources[d + 3 * node];
   }
-  int c = orc_find_closest(x, P, mv.pk);
+  int c = orc_find_closest_variant(x, P, mv.pk, variant);
   if (c < 1 || c > R) c = R;
   return c;
 }
@@ -441,6 +443,25 @@ int orc_select(const double* sources, int64_t N, const int64_t* ptopo, int pk, i
   for (int64_t i = 0; i < M; ++i)
     for (int64_t e = 0; e < E; ++e) out[i * E + e] = (int8_t)rule_for(mv, targets + 3 * i, e, R);
   return 0;
+}
+
+// Number of (target, element) pairs whose rule differs between the two hypot formulas
+// (variant 0 and 1 of orc_find_closest_variant); diff (M x E int8, may be NULL) receives the
+// variant-1 rule where they differ and 0 elsewhere.
+int64_t orc_select_diff(const double* sources, int64_t N, const int64_t* ptopo, int pk, int gk,
+                        int64_t E, int base, const double* targets, int64_t M, int8_t* diff) {
+  MeshView mv = view(nullptr, nullptr, gk, sources, N, ptopo, pk, E, base);
+  int R = (gk == 6) ? 6 : 3;
+  int64_t count = 0;
+#pragma omp parallel for schedule(static) reduction(+ : count)
+  for (int64_t i = 0; i < M; ++i)
+    for (int64_t e = 0; e < E; ++e) {
+      const int a = rule_for(mv, targets + 3 * i, e, R, 0);
+      const int b = rule_for(mv, targets + 3 * i, e, R, 1);
+      if (diff) diff[i * E + e] = (int8_t)(a != b ? b : 0);
+      count += a != b;
+    }
+  return count;
 }
 
 // Dense assembly; F, G: interleaved complex, column-major M x N; C: complex length M.

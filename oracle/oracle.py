@@ -60,6 +60,11 @@ def lib():
                                           ctypes.c_double, ctypes.c_double, _dp,
                                           ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, _i8p, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_select_diff.restype = ctypes.c_int64
+        L.orc_select_diff.argtypes = [_dp, ctypes.c_int64, _i64p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64, _i8p]
+        L.orc_hypot3_scaled.restype = ctypes.c_double
+        L.orc_hypot3_scaled.argtypes = [ctypes.c_double] * 3
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -117,15 +122,35 @@ def element_geometry(mesh, c: int, n: int, m: int):
     return y, nr, jw
 
 
-def select(mesh, targets=None):
-    """Rule index (1..R) per (target, element) as int8 [M, E]."""
+def select(mesh, targets=None, hypot="julia1.9"):
+    """Rule index (1..R) per (target, element) as int8 [M, E]; hypot "scaled" takes the
+    distances of orc_hypot3_scaled instead of orc_julia_hypot3."""
     _, _, src, ptop = _mesh_args(mesh)
     tg = np.asfortranarray(mesh.sources if targets is None else targets, dtype=np.float64)
     M = tg.shape[1]
     out = np.zeros((M, mesh.n_elements), dtype=np.int8)
     lib().orc_select(_p(src), mesh.n_sources, _p(ptop, _i64p), mesh.physics_kind,
                      mesh.geometry_kind, mesh.n_elements, 0, _p(tg), M, _p(out, _i8p))
+    if hypot == "scaled":
+        _, alt = select_diff(mesh, targets, want_rules=True)
+        out = np.where(alt != 0, alt, out)
+    elif hypot != "julia1.9":
+        raise ValueError("hypot must be 'julia1.9' or 'scaled'")
     return out
+
+
+def select_diff(mesh, targets=None, want_rules=False):
+    """Pairs whose rule differs between the two hypot formulas (orc_select_diff): the count,
+    and with want_rules the int8 [M, E] array holding the other formula's rule where they
+    differ (0 elsewhere)."""
+    _, _, src, ptop = _mesh_args(mesh)
+    tg = np.asfortranarray(mesh.sources if targets is None else targets, dtype=np.float64)
+    M = tg.shape[1]
+    out = np.zeros((M, mesh.n_elements), dtype=np.int8) if want_rules else None
+    n = lib().orc_select_diff(_p(src), mesh.n_sources, _p(ptop, _i64p), mesh.physics_kind,
+                              mesh.geometry_kind, mesh.n_elements, 0, _p(tg), M,
+                              _p(out, _i8p) if want_rules else None)
+    return (int(n), out) if want_rules else int(n)
 
 
 def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True, nthreads=0):

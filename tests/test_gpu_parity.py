@@ -48,8 +48,8 @@ class DevBuf:
             _lib.lib().bie_device_free(ctypes.c_void_p(self.ptr))
 
 
-def gpu_rules(mesh, targets=None):
-    dm = DeviceMesh(mesh)
+def gpu_rules(mesh, targets=None, hypot="julia1.9"):
+    dm = DeviceMesh(mesh, hypot=hypot)
     if targets is None:
         M_ = mesh.n_sources
         tbuf = None
@@ -90,6 +90,31 @@ def test_rule_selection_ties_and_field_points():
          1.7 * c[:, :200], 0.3 * c[:, :200], rng.standard_normal((3, 300))]
     T = np.hstack(T)
     assert np.array_equal(gpu_rules(mesh, T), oracle.select(mesh, T))
+
+
+@pytest.mark.parametrize("name", ["lin8", "quad4", "sphere_1m"])
+def test_rule_selection_with_the_scaled_hypot_bit_exact(name):
+    """BIE_MESH_SCALED_HYPOT: the tied pairs take the later Julia releases' hypot rounding;
+    rule indices bit-exact against the oracle's selection with that formula, which differs
+    from the default one on these symmetric meshes."""
+    mesh = MESHES[name]()
+    print(f"[case] scaled-hypot selection {name}", flush=True)
+    got = gpu_rules(mesh, hypot="scaled")
+    assert np.array_equal(got, oracle.select(mesh, hypot="scaled"))
+    if name != "sphere_1m":
+        assert not np.array_equal(got, oracle.select(mesh))
+
+
+def test_assembly_with_the_scaled_hypot():
+    """F, G, C with BIE_MESH_SCALED_HYPOT against the oracle's frozen-rule assembly at the
+    rules that formula selects, on a mesh where about 0.2 % of the pairs change rule."""
+    mesh = M.icosphere(8, "linear")
+    rules = oracle.select(mesh, hypot="scaled")
+    assert (rules != oracle.select(mesh)).sum() > 0
+    for k in (1.0, 4.0 + 0.5j):
+        F, G, C = assemble_parallel(mesh, k, mesh.sources, n=2, m=2, hypot="scaled")
+        Fo, Go, Co = oracle.assemble_frozen(mesh, k, rules, n=2, m=2)
+        assert rel(F, Fo) < TOL and rel(G, Go) < TOL and rel(C, Co) < TOL, k
 
 
 @pytest.mark.parametrize("rule", [1, 2, 3, 4, 5, 6])

@@ -113,8 +113,9 @@ def test_sweep_then_gmres_rigid_sphere():
     oracle dense solve and the analytic series (theta measured from -z for +z incidence)."""
     import torch
     import analytic
-    from bie_amd.solvers import (collocation_operator, collocation_operator_batched, gmres,
-                                 gmres_batched)
+    from bie_amd import solvers
+    from torch_solvers import (collocation_operator, collocation_operator_batched, gmres,
+                               gmres_batched)
 
     mesh = M.icosphere(8, "linear")
     N = mesh.n_sources
@@ -150,29 +151,76 @@ def test_sweep_then_gmres_rigid_sphere():
         p, it = sols[q]
         assert rrs[q] <= 1e-10 and abs(int(its[q]) - it) <= 1
         assert rel(X[q].cpu().numpy(), p) < 1e-9
+    # the library's native batched GMRES on the same systems: same iteration counts (the
+    # same algorithm) and the same solutions
+    Xn = torch.empty_like(B)
+    itn, rrn = solvers.gmres_batched(N, len(ks), F.data_ptr(), N * N, N, C.data_ptr(),
+                                     B.data_ptr(), N, Xn.data_ptr(), N, tol=1e-10)
+    for q in range(len(ks)):
+        p, it = sols[q]
+        assert rrn[q] <= 1e-10 and abs(int(itn[q]) - it) <= 1, (q, itn[q], it)
+        assert rel(Xn[q].cpu().numpy(), p) < 1e-9
+        r = B[q] - (torch.mv(F[q].t(), Xn[q]) + (1.0 - C) * Xn[q])     # explicit residual
+        assert float(torch.linalg.norm(r) / torch.linalg.norm(B[q])) <= 1.05e-10
 
 
-@pytest.mark.parametrize("overwrite", [False, True])
-def test_device_lu_solve_matches_oracle_dense_solve(overwrite):
-    """(F + diag(1 - C)) X = B by LU on the device, F assembled into device memory: against
-    a host solve of the oracle's matrices (two right-hand sides, complex k)."""
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_device_lu_solve_matches_oracle_dense_solve(trans):
+    """(F + diag(1 - C)) X = B, and the transposed and adjoint systems, by LU on the device
+    (bie_lu_factor_device / bie_lu_solve_device) on F where the assembly wrote it: against a
+    host solve of the oracle's matrices (two right-hand sides, complex k)."""
     import torch
-    from bie_amd.solvers import lu_solve
+    from bie_amd import solvers
 
     mesh = M.icosphere(4, "quadratic")
     N = mesh.n_sources
     k = 3.0 + 0.1j
     dev = torch.device("cuda:0")
     dm = DeviceMesh(mesh)
-    F = torch.empty((N, N), dtype=torch.complex128, device=dev)
+    F = torch.empty((N, N), dtype=torch.complex128, device=dev)   # F^T view (column-major F)
     G = torch.empty_like(F)
     C = torch.empty(N, dtype=torch.complex128, device=dev)
-    print(f"[case] device LU solve N={N} overwrite={overwrite}", flush=True)
+    ipiv = torch.empty(N, dtype=torch.int32, device=dev)
+    print(f"[case] device LU solve N={N} trans={trans}", flush=True)
     dm.assemble(k, F.data_ptr(), G.data_ptr(), C.data_ptr(), n=3, m=3, gOn=False)
-    torch.cuda.synchronize()
+    solvers.lu_factor(N, F.data_ptr(), N, ipiv.data_ptr(), C_ptr=C.data_ptr())
     z = mesh.sources[2]
     Bh = np.stack([np.exp(1j * k.real * z), np.cos(2.0 * z) + 0j], axis=1)
-    X = lu_solve(torch, F, C, torch.as_tensor(Bh, device=dev), overwrite=overwrite).cpu().numpy()
+    Bd = torch.as_tensor(np.asfortranarray(Bh).T.copy(), device=dev)   # column-major N x 2
+    solvers.lu_solve(N, F.data_ptr(), N, ipiv.data_ptr(), Bd.data_ptr(), N, 2, trans=trans)
+    torch.cuda.synchronize()
+    X = Bd.cpu().numpy().T
     Fo, _, Co = oracle.assemble(mesh, k, n=3, m=3, gOn=False)
-    Xo = np.linalg.solve(Fo + np.diag(1.0 - Co), Bh)
+    A = Fo + np.diag(1.0 - Co)
+    A = {"N": A, "T": A.T, "C": A.conj().T}[trans]
+    Xo = np.linalg.solve(A, Bh)
     assert rel(X, Xo) < 1e-10
+
+
+def test_device_lu_rejects_a_singular_matrix():
+    import torch
+    from bie_amd import solvers
+    dev = torch.device("cuda:0")
+    A = torch.zeros((4, 4), dtype=torch.complex128, device=dev)
+    A[0, 0] = 1.0
+    ipiv = torch.empty(4, dtype=torch.int32, device=dev)
+    with pytest.raises(ValueError, match="singular"):
+        solvers.lu_factor(4, A.data_ptr(), 4, ipiv.data_ptr())
+
+
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_device_gemv_against_numpy(trans):
+    import torch
+    from bie_amd import solvers
+    rng = np.random.default_rng(4)
+    m, n = 300, 170
+    A = np.asfortranarray(rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n)))
+    x = rng.standard_normal(n if trans == "N" else m) + 1j * rng.standard_normal(n if trans == "N" else m)
+    dev = torch.device("cuda:0")
+    Ad = torch.as_tensor(A.T.copy(), device=dev)
+    xd = torch.as_tensor(x, device=dev)
+    yd = torch.empty(m if trans == "N" else n, dtype=torch.complex128, device=dev)
+    solvers.zgemv(m, n, Ad.data_ptr(), m, xd.data_ptr(), yd.data_ptr(), trans=trans)
+    torch.cuda.synchronize()
+    ref = {"N": A @ x, "T": A.T @ x, "C": A.conj().T @ x}[trans]
+    assert rel(yd.cpu().numpy(), ref) < 1e-14

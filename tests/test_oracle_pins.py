@@ -263,3 +263,51 @@ def test_pulsating_sphere_interior_reference_workflow():
     Fp, Gp, _ = oracle.assemble(mesh, k, targets=X, n=2, m=2)
     pf = Fp @ p + Gp @ vs
     assert np.max(np.abs(pf - analytic.pulsating_sphere_interior(k, 1.0, z, Z0, 1.0))) / abs(pa) < 1e-2
+
+
+def test_scaled_hypot_is_the_plainly_rounded_sum_of_squares():
+    """orc_hypot3_scaled (the other n-ary hypot of Julia releases) equals
+    sqrt((x*x + y*y) + z*z) with every operation rounded once, computed by numpy, bit for
+    bit away from overflow and underflow; inf beats NaN, zero stays zero, and huge or tiny
+    arguments neither overflow nor underflow (math.hypot within 2 ulp)."""
+    import math
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((20000, 3)) * np.exp(rng.uniform(-30, 30, (20000, 1)))
+    ref = np.sqrt((X[:, 0] * X[:, 0] + X[:, 1] * X[:, 1]) + X[:, 2] * X[:, 2])
+    got = np.array([L.orc_hypot3_scaled(*x) for x in X])
+    assert np.array_equal(got, ref)
+    assert L.orc_hypot3_scaled(np.inf, np.nan, 1.0) == np.inf
+    assert np.isnan(L.orc_hypot3_scaled(np.nan, 1.0, 2.0))
+    assert L.orc_hypot3_scaled(0.0, -0.0, 0.0) == 0.0
+    for s in (1e300, 1e-300, 5e-324 * 7):
+        x = (3.0 * s, 4.0 * s, 12.0 * s)
+        h = L.orc_hypot3_scaled(*x)
+        assert np.isfinite(h) and h > 0 and abs(h - math.hypot(*x)) <= 2 * math.ulp(math.hypot(*x))
+    # the two formulas round differently on some arguments (here: in the last place)
+    assert L.orc_hypot3_scaled(0.6732655185893088, 0.3428080423874833, 0.13687617154257523) == \
+        0.7678143647933542
+    assert L.orc_julia_hypot3(0.6732655185893088, 0.3428080423874833, 0.13687617154257523) == \
+        0.7678143647933543
+
+
+def test_scaled_hypot_selection_by_brute_force():
+    """oracle.select(hypot="scaled") against a numpy brute force of find_closest_corner with
+    that distance, on a tie-heavy mesh and targets."""
+    mesh = M.icosphere(4, "linear")
+    rng = np.random.default_rng(9)
+    c = mesh.coordinates
+    i, j = rng.integers(0, c.shape[1], (2, 150))
+    T = np.hstack([0.5 * (c[:, i] + c[:, j]), c[:, :80]])
+    got = oracle.select(mesh, T, hypot="scaled")
+    P = mesh.sources[:, mesh.physics_topology]           # 3 x 3 x E
+    D = T[:, :, None, None] - P[:, None, :, :]            # 3 x M x 3 x E
+    d = np.sqrt((D[0] * D[0] + D[1] * D[1]) + D[2] * D[2])  # M x 3 x E
+    best = np.full(d[:, 0].shape, np.inf)
+    idx = np.zeros(best.shape, dtype=np.int8)
+    for s_ in range(3):                                  # first strict minimum
+        better = d[:, s_] < best
+        best = np.where(better, d[:, s_], best)
+        idx = np.where(better, s_ + 1, idx)
+    assert np.array_equal(got, idx)
+    assert not np.array_equal(got, oracle.select(mesh, T))   # ties do fall differently

@@ -1,9 +1,11 @@
-"""Restarted GMRES (the caller of the assembled operators) on the CPU with torch tensors."""
+"""The torch restatement of the library's restarted GMRES (tests/torch_solvers.py, the
+reference the GPU tests hold the native bie_gmres_batched_device to) against direct solves,
+on CPU tensors."""
 import numpy as np
 import torch
 
-from bie_amd.solvers import (collocation_operator, collocation_operator_batched, gmres,
-                              gmres_batched)
+from torch_solvers import (collocation_operator, collocation_operator_batched, gmres,
+                           gmres_batched)
 
 
 def test_gmres_matches_direct_solve_with_restarts():
