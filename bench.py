@@ -144,6 +144,11 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
             dm.assemble(float(ks[q]), Fp.data_ptr() + 16 * q * stride,
                         Gp.data_ptr() + 16 * q * stride, C.data_ptr(), n=n, m=m, stream=stream)
 
+    # tile-shape autotuner (elements per tile x LDS slots): the choice is kept for the graph
+    t0 = time.perf_counter()
+    ept, slots, tuned_ms = dm.sweep_tune(kd.data_ptr(), nk, F.data_ptr(), G.data_ptr(), stride,
+                                         C.data_ptr(), n=n, m=m, stream=stream)
+    tune_s = time.perf_counter() - t0
     g = dm.sweep_graph(kd.data_ptr(), nk, F.data_ptr(), G.data_ptr(), stride, C.data_ptr(),
                        n=n, m=m)
 
@@ -223,6 +228,9 @@ def measure_sweep(torch, bie_amd, steps=3, f=20, nk=64, n=4, m=4):
                     "iterations_min_mean_max": [int(its.min()), float(its.mean()), int(its.max())],
                     "max_rel_residual": float(rrs.max())}
     res["speedup"] = res["per_k_launches_ms"] / res["batched_graph_ms"]
+    res["tile"] = {"rows": 64, "elems_per_tile": ept or "whole surface",
+                   "slots": slots or "as many as fit", "tuned_ms": tuned_ms,
+                   "tune_wall_s": tune_s}
     res["max_rel_err_batched_vs_per_k"] = err
     res["workload"] = (f"linear geodesic sphere N={N}, E={mesh.n_elements}, {nk} k uniform in "
                        f"[0.5, 30], (n,m)=({n},{m}), F_k and G_k for every k")
@@ -381,6 +389,31 @@ def measure_batch(torch, bie_amd, steps=3, n=4, m=4):
         sl = slice(2 * offs[i], 2 * offs[i + 1])
         err = max(err, float(torch.linalg.norm(FB[sl] - FL[sl]) / torch.linalg.norm(FL[sl])),
                   float(torch.linalg.norm(GB[sl] - GL[sl]) / torch.linalg.norm(GL[sl])))
+    # the same batch captured into a HIP graph, replayed
+    g = b.graph(ks, Fp, Gp, Cp)
+    g.launch(stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        g.launch(stream)
+    e1.record()
+    torch.cuda.synchronize()
+    res["batched_graph_ms"] = e0.elapsed_time(e1) / steps
+    g.close()
+    # the batched k-derivative of every instance, w_i = alpha_i^T (dF_i/dk) p_i, one launch
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    al = [torch.randn(N, dtype=torch.complex128, generator=gen).to(dev) for N in Ns]
+    pv = [torch.randn(N, dtype=torch.complex128, generator=gen).to(dev) for N in Ns]
+    w = torch.empty(len(Ns), dtype=torch.complex128, device=dev)
+    b.dfdk(ks, [a.data_ptr() for a in al], [x.data_ptr() for x in pv], w.data_ptr(),
+           stream=stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.dfdk(ks, [a.data_ptr() for a in al], [x.data_ptr() for x in pv], w.data_ptr(),
+               stream=stream)
+    res["dfdk_batched_ms"] = (time.perf_counter() - t0) * 1e3 / steps
     entries = sum(2 * N * N + N for N in Ns)
     res.update({
         "speedup": res["loop_plain_ms"] / res["batched_persistent_ms"],
@@ -393,6 +426,40 @@ def measure_batch(torch, bie_amd, steps=3, n=4, m=4):
     })
     b.close()
     del FB, GB, FL, GL, CB, CL
+    torch.cuda.empty_cache()
+    return res
+
+
+def measure_sparse(torch, bie_amd, steps=3, f=16):
+    """The near field (sparse_assemble_parallel!) on the 10242-node quadratic sphere at
+    k = 10: depth 1 (the function's default) and depth 2 (assemble_parallel!(...;
+    sparse=true)'s default), F and G, from a device handle (pattern and rules built once)."""
+    from bie_amd.assembly import DeviceMesh, SparsePattern
+    mesh = bie_amd.icosphere(f, "quadratic")
+    dm = DeviceMesh(mesh, 0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    res = {"workload": f"quadratic geodesic sphere N={mesh.n_sources}, k=10, F and G"}
+    for depth in (1, 2):
+        t0 = time.perf_counter()
+        sp = SparsePattern(dm, depth=depth)
+        setup_s = time.perf_counter() - t0
+        F = torch.empty(sp.nnz, dtype=torch.complex128, device=dev)
+        G = torch.empty_like(F)
+        sp.assemble_device(10.0, F.data_ptr(), G.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            sp.assemble_device(10.0, F.data_ptr(), G.data_ptr(), stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        res[f"depth{depth}"] = {"nnz": sp.nnz, "ms": e0.elapsed_time(e1) / steps,
+                                "setup_s": setup_s}
+        sp.close()
+        del F, G
+    dm.close()
     torch.cuda.empty_cache()
     return res
 
@@ -518,6 +585,7 @@ def main():
         extra["uneven_batch"] = measure_batch(torch, bie_amd)
         extra["adjoint"] = measure_adjoint(torch)
         extra["taylor"] = measure_taylor(torch, bie_amd)
+        extra["sparse_near_field"] = measure_sparse(torch, bie_amd)
         G = torch.empty(2 * N * N, dtype=torch.float64, device=dev)
         run(n, m, 1, 0)
 

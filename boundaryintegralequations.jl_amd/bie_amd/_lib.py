@@ -76,6 +76,10 @@ SIGNATURES = {
     "bie_assembly_vjp_device": (ctypes.c_int, [_dp, ctypes.c_double, _dp, ctypes.c_int64, _dp,
                                                 _dp, _dp, _dp, ctypes.c_int32, ctypes.c_int32,
                                                 _dp, _dp, _dp]),
+    "bie_assembly_vjp_complex_device": (ctypes.c_int, [_dp, ctypes.c_double, ctypes.c_double,
+                                                        _dp, ctypes.c_int64, _dp, _dp, _dp, _dp,
+                                                        ctypes.c_int32, ctypes.c_int32, _dp, _dp,
+                                                        _dp]),
     "bie_zgemv_batched_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _dp,
                                                  ctypes.c_int64, ctypes.c_int64, _dp,
                                                  ctypes.c_int64, _dp, ctypes.c_int64, _dp,
@@ -100,6 +104,12 @@ SIGNATURES = {
                                                   ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.c_uint32, _dp, _dp, ctypes.c_int64,
                                                   _dp, _dp]),
+    "bie_sweep_tune": (ctypes.c_int, [_dp, _dp, ctypes.c_int32, _dp, ctypes.c_int64,
+                                      ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _dp, _dp,
+                                      ctypes.c_int64, _dp, _dp,
+                                      ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_double)]),
     "bie_sweep_graph_create": (ctypes.c_int, [_dp, _dp, ctypes.c_int32, _dp, ctypes.c_int64,
                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
                                               _dp, _dp, ctypes.c_int64, _dp,
@@ -130,6 +140,10 @@ SIGNATURES = {
                                       ctypes.POINTER(_dp), ctypes.POINTER(_dp),
                                       ctypes.POINTER(_dp), _dp,
                                       ctypes.POINTER(ctypes.c_double)]),
+    "bie_batch_graph_create": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
     "bie_batch_dfdk_device": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp,
@@ -193,6 +207,34 @@ def check(rc):
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class DeviceBuffer:
+    """Device buffer from the library's allocator (bie_device_malloc), optionally filled
+    from a host array; freed with the object."""
+
+    def __init__(self, arr: np.ndarray | None = None, nbytes: int = 0):
+        if arr is not None:
+            arr = np.ascontiguousarray(arr)
+            nbytes = arr.nbytes
+        p = ctypes.c_void_p()
+        check(lib().bie_device_malloc(ctypes.byref(p), int(max(nbytes, 8))))
+        self.ptr = p.value
+        self.nbytes = nbytes
+        if arr is not None and arr.nbytes:
+            check(lib().bie_memcpy_h2d(ctypes.c_void_p(self.ptr), _ptr(arr), arr.nbytes))
+
+    @classmethod
+    def empty(cls, nbytes: int) -> "DeviceBuffer":
+        return cls(None, nbytes)
+
+    def get(self, out: np.ndarray) -> np.ndarray:
+        check(lib().bie_memcpy_d2h(_ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().bie_device_free(ctypes.c_void_p(self.ptr))
 
 
 class HostMesh:

@@ -28,33 +28,7 @@ from .assembly import DeviceMesh
 from .mesh import Mesh3d
 
 
-class _Dev:
-    """Device buffer from the library's allocator (bie_device_malloc), optionally filled
-    from a host array."""
-
-    def __init__(self, arr: np.ndarray | None = None, nbytes: int = 0):
-        if arr is not None:
-            arr = np.ascontiguousarray(arr)
-            nbytes = arr.nbytes
-        p = ctypes.c_void_p()
-        _lib.check(_lib.lib().bie_device_malloc(ctypes.byref(p), int(max(nbytes, 8))))
-        self.ptr = p.value
-        self.nbytes = nbytes
-        if arr is not None and arr.nbytes:
-            _lib.check(_lib.lib().bie_memcpy_h2d(ctypes.c_void_p(self.ptr), _lib._ptr(arr),
-                                                 arr.nbytes))
-
-    @classmethod
-    def empty(cls, nbytes: int) -> "_Dev":
-        return cls(None, nbytes)
-
-    def get(self, out: np.ndarray) -> np.ndarray:
-        _lib.check(_lib.lib().bie_memcpy_d2h(_lib._ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
-        return out
-
-    def __del__(self):
-        if getattr(self, "ptr", None):
-            _lib.lib().bie_device_free(ctypes.c_void_p(self.ptr))
+_Dev = _lib.DeviceBuffer
 
 
 def source_nodes(mesh: Mesh3d) -> np.ndarray:
@@ -66,10 +40,15 @@ def source_nodes(mesh: Mesh3d) -> np.ndarray:
     return out
 
 
-def assembly_vjp(dmesh: DeviceMesh, k: float, alpha, p, *, beta=None, targets=None,
+def assembly_vjp(dmesh: DeviceMesh, k, alpha, p, *, beta=None, targets=None,
                  target_node=None, n=4, m=4):
-    """Gradient of Re[alpha^T F p + beta^T C] w.r.t. mesh coordinates (3 x Nc) and k."""
+    """Gradient of Re[alpha^T F p + beta^T C] w.r.t. mesh coordinates (3 x Nc) and k.
+
+    Real k: returns (grad_X, d/dk).  Complex k (the lossy systems' wavenumbers): returns
+    (grad_X, (d/dRe k, d/dIm k))."""
     mesh = dmesh.mesh
+    kc = complex(k)
+    cplx = kc.imag != 0.0
     M = mesh.n_sources if targets is None else targets.shape[1]
     dA = _Dev(np.asarray(alpha, dtype=np.complex128))
     dP = _Dev(np.asarray(p, dtype=np.complex128))
@@ -77,14 +56,21 @@ def assembly_vjp(dmesh: DeviceMesh, k: float, alpha, p, *, beta=None, targets=No
     dT = _Dev(np.asfortranarray(targets, dtype=np.float64).ravel(order="F")) if targets is not None else None
     dN = _Dev(np.asarray(target_node, dtype=np.int32)) if target_node is not None else None
     gX = _Dev(np.zeros(3 * mesh.n_coordinates))
-    gk = _Dev(np.zeros(1))
-    _lib.check(_lib.lib().bie_assembly_vjp_device(
-        dmesh.handle, float(k), ctypes.c_void_p(dT.ptr if dT else None), int(M),
-        ctypes.c_void_p(dN.ptr if dN else None), ctypes.c_void_p(dA.ptr),
-        ctypes.c_void_p(dB.ptr if dB else None), ctypes.c_void_p(dP.ptr), int(n), int(m),
-        ctypes.c_void_p(gX.ptr), ctypes.c_void_p(gk.ptr), None))
+    gk = _Dev(np.zeros(2 if cplx else 1))
+    common = (ctypes.c_void_p(dT.ptr if dT else None), int(M),
+              ctypes.c_void_p(dN.ptr if dN else None), ctypes.c_void_p(dA.ptr),
+              ctypes.c_void_p(dB.ptr if dB else None), ctypes.c_void_p(dP.ptr), int(n), int(m),
+              ctypes.c_void_p(gX.ptr), ctypes.c_void_p(gk.ptr), None)
+    if cplx:
+        _lib.check(_lib.lib().bie_assembly_vjp_complex_device(dmesh.handle, kc.real, kc.imag,
+                                                              *common))
+    else:
+        _lib.check(_lib.lib().bie_assembly_vjp_device(dmesh.handle, kc.real, *common))
     _lib.check(_lib.lib().bie_device_synchronize())
     g = gX.get(np.zeros(3 * mesh.n_coordinates)).reshape(mesh.n_coordinates, 3).T.copy()
+    if cplx:
+        v = gk.get(np.zeros(2))
+        return g, (float(v[0]), float(v[1]))
     return g, float(gk.get(np.zeros(1))[0])
 
 

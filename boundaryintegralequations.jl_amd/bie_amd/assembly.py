@@ -33,10 +33,9 @@ def assemble_parallel(mesh: Mesh3d, k, in_sources=None, *, fOn=True, gOn=True, c
     ``hypot`` (an addition to the reference's keywords) names the n-ary hypot whose
     rounding decides the rule of exactly tied (target, element) pairs: "julia1.9" (the
     reference's CI Julia) or "scaled" (later Julia releases)."""
-    if sparse:
-        raise NotImplementedError(
-            "sparse=true (the near-field assembly behind the fast operators) is not part of "
-            "this library; call the reference's sparse_assemble_parallel!")
+    if sparse:  # (F, G) of the near field, as the reference returns them
+        return sparse_assemble_parallel(mesh, k, in_sources, fOn=fOn, gOn=gOn, depth=depth,
+                                        offset=offset, hypot=hypot)
     del progress, depth, offset  # accepted for signature parity; unused by dense assembly
     hm = _lib.HostMesh(mesh, _lib.hypot_flag(hypot))
     if in_sources is None or in_sources is mesh.sources:
@@ -143,6 +142,21 @@ class DeviceMesh:
             ctypes.c_void_p(G_ptr), int(stride), ctypes.c_void_p(C_ptr),
             ctypes.c_void_p(stream or None)))
 
+    def sweep_tune(self, k_ptr: int, nk: int, F_ptr: int, G_ptr: int, stride: int, C_ptr: int,
+                   *, targets_ptr: int = 0, n_targets: int | None = None, n=4, m=4, fOn=True,
+                   gOn=True, cOn=True, stream: int = 0):
+        """Tile-shape autotuner of the sweep (bie_sweep_tune): times every candidate
+        (elements per tile x LDS slots), keeps the fastest for later sweeps on this mesh;
+        returns (elems_per_tile, slots, ms) with 0 meaning whole surface / as many as fit."""
+        M = self.n_sources if n_targets is None else int(n_targets)
+        e, sl, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        _lib.check(_lib.lib().bie_sweep_tune(
+            self.handle, ctypes.c_void_p(k_ptr), int(nk), ctypes.c_void_p(targets_ptr or None),
+            M, int(n), int(m), _flags(fOn, gOn, cOn), ctypes.c_void_p(F_ptr),
+            ctypes.c_void_p(G_ptr), int(stride), ctypes.c_void_p(C_ptr),
+            ctypes.c_void_p(stream or None), ctypes.byref(e), ctypes.byref(sl), ctypes.byref(ms)))
+        return e.value, sl.value, ms.value
+
     def sweep_graph(self, k_ptr: int, nk: int, F_ptr: int, G_ptr: int, stride: int, C_ptr: int,
                     *, targets_ptr: int = 0, n_targets: int | None = None, n=4, m=4,
                     fOn=True, gOn=True, cOn=True) -> "Graph":
@@ -234,3 +248,128 @@ class Batch:
         _lib.check(_lib.lib().bie_batch_tune(self.handle, kr, ki, _flags(fOn, gOn, cOn), F, G, C,
                                              ctypes.c_void_p(stream or None), ctypes.byref(best)))
         return best.value
+
+    def graph(self, ks, F_ptrs, G_ptrs, C_ptrs, *, fOn=True, gOn=True, cOn=True) -> Graph:
+        """The batched assembly with these arguments captured into a HIP graph
+        (bie_batch_graph_create); Graph.launch(stream) replays it."""
+        kr, ki, F, G, C = self._args(ks, F_ptrs, G_ptrs, C_ptrs)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().bie_batch_graph_create(self.handle, kr, ki, _flags(fOn, gOn, cOn),
+                                                     F, G, C, ctypes.byref(h)))
+        return Graph(h)
+
+    def dfdk(self, ks, alpha_ptrs, p_ptrs, w_ptr: int, *, stream: int = 0):
+        """w_i = alpha_i^T (dF_i/dk) p_i for every instance in one launch (bie_batch_dfdk_device):
+        alpha_i, p_i device arrays of N_i complex128, w a device array of len(batch) complex128.
+        Returns when the launch has finished."""
+        n = len(self.dmeshes)
+        ks = [complex(k) for k in ks]
+        if not (len(ks) == len(alpha_ptrs) == len(p_ptrs) == n):
+            raise ValueError("one k and one alpha, p pointer per instance")
+        kr = (ctypes.c_double * n)(*[k.real for k in ks])
+        ki = (ctypes.c_double * n)(*[k.imag for k in ks])
+        P = ctypes.c_void_p * n
+        _lib.check(_lib.lib().bie_batch_dfdk_device(
+            self.handle, kr, ki, P(*alpha_ptrs), P(*p_ptrs), ctypes.c_void_p(w_ptr),
+            ctypes.c_void_p(stream or None)))
+
+
+# ---- sparse near field -------------------------------------------------------------------
+
+def _sparse_matrices(I, J, F, G, n, fOn, gOn):
+    import scipy.sparse as sps  # the reference returns SparseMatrixCSC; scipy is its analogue
+    shape = (n, n)
+    Fm = sps.csr_matrix((F, (I, J)), shape=shape)
+    Gm = sps.csr_matrix((G, (I, J)), shape=shape)
+    return Fm, Gm
+
+
+def sparse_assemble_parallel(mesh: Mesh3d, k, sources=None, *, fOn=True, gOn=True, depth=1,
+                             offset=None, progress=True, hypot="julia1.9"):
+    """Near-field F and G (drop-in for sparse_assemble_parallel!,
+    src/3d/assembly_collocation_losses.jl:200-405): the entries (i, j) of the sources
+    connected to node i through at most ``depth`` elements (connected_sources, :98-132),
+    each element integrated with the subdivision rule clustered at its node nearest to
+    source i.  Returns scipy CSR matrices n_sources x n_sources holding exactly the
+    reference's stored entries (a disabled output is all zeros, as in the reference).
+    ``offset``: the rules' source offset (None: 0 for continuous physics)."""
+    del progress
+    if depth not in (1, 2):
+        raise ValueError("depth must be 1 or 2")
+    N = mesh.n_sources
+    if sources is None or sources is mesh.sources:
+        pts = None
+    else:
+        pts = np.asfortranarray(sources, dtype=np.float64)
+        if pts.shape != (3, N):
+            raise ValueError("sources must be 3 x n_sources (one point per mesh source)")
+    hm = _lib.HostMesh(mesh, _lib.hypot_flag(hypot))
+    L = _lib.lib()
+    nnz = ctypes.c_int64()
+    _lib.check(L.bie_sparse_count(ctypes.byref(hm.struct), int(depth), ctypes.byref(nnz)))
+    n = nnz.value
+    I = np.empty(n, dtype=np.int64)
+    J = np.empty(n, dtype=np.int64)
+    F = np.empty(n, dtype=np.complex128)
+    G = np.empty(n, dtype=np.complex128)
+    k = complex(k)
+    off = float("nan") if offset is None else float(offset)
+    rc = L.bie_sparse_assemble(ctypes.byref(hm.struct), k.real, k.imag, _lib._ptr(pts),
+                               N if pts is not None else 0, int(depth), off,
+                               _flags(fOn, gOn, False), n, _lib._ptr(I), _lib._ptr(J),
+                               _lib._ptr(F), _lib._ptr(G))
+    if rc == _lib.BIE_EINVAL:
+        raise ValueError(L.bie_last_error().decode())
+    _lib.check(rc)
+    base = hm.struct.index_base
+    return _sparse_matrices(I - base, J - base, F, G, N, fOn, gOn)
+
+
+class SparsePattern:
+    """Device-resident sparse near-field assembly (bie_sparse) on a DeviceMesh: pattern,
+    pairs and subdivision rules built once, then assemblies for any k."""
+
+    def __init__(self, dmesh: DeviceMesh, depth=1, offset=None):
+        h = ctypes.c_void_p()
+        off = float("nan") if offset is None else float(offset)
+        _lib.check(_lib.lib().bie_sparse_create(dmesh.handle, int(depth), off,
+                                                ctypes.byref(h)))
+        self.handle = h
+        self.dmesh = dmesh  # the pattern must not outlive its mesh
+        self.nnz = int(_lib.lib().bie_sparse_nnz(h))
+
+    def pattern(self, index_base=0):
+        I = np.empty(self.nnz, dtype=np.int64)
+        J = np.empty(self.nnz, dtype=np.int64)
+        _lib.check(_lib.lib().bie_sparse_pattern(self.handle, int(index_base), _lib._ptr(I),
+                                                 _lib._ptr(J)))
+        return I, J
+
+    def assemble_device(self, k, F_ptr: int, G_ptr: int, *, sources_ptr: int = 0, fOn=True,
+                        gOn=True, stream: int = 0):
+        """Asynchronous assembly into device arrays of nnz complex128 entries."""
+        k = complex(k)
+        _lib.check(_lib.lib().bie_sparse_assemble_device(
+            self.handle, k.real, k.imag, ctypes.c_void_p(sources_ptr or None),
+            _flags(fOn, gOn, False), ctypes.c_void_p(F_ptr or None),
+            ctypes.c_void_p(G_ptr or None), ctypes.c_void_p(stream or None)))
+
+    def assemble(self, k, *, fOn=True, gOn=True):
+        """Host copies of the nnz values of F and G (pattern order)."""
+        dF = _lib.DeviceBuffer.empty(16 * self.nnz)
+        dG = _lib.DeviceBuffer.empty(16 * self.nnz)
+        self.assemble_device(k, dF.ptr, dG.ptr, fOn=fOn, gOn=gOn)
+        _lib.check(_lib.lib().bie_device_synchronize())
+        return (dF.get(np.empty(self.nnz, dtype=np.complex128)),
+                dG.get(np.empty(self.nnz, dtype=np.complex128)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().bie_sparse_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -4,6 +4,7 @@ The library is plain HIP/C++ with a C ABI (include/bie_amd.h); it links no PyTor
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -15,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB_PATH = os.path.join(HERE, "libbie_amd.so")
 
 SOURCES = ["rules.cpp", "schedule.cpp", "assemble.hip", "sweep.hip", "adjoint.hip", "apply.hip",
-           "solve.hip", "sparse.hip", "api.hip"]
+           "solve.hip", "sparse.hip", "near_field.cpp", "api.hip"]
 # rocSOLVER (LU) and rocBLAS (GEMV) from the ROCm install, found at run time by rpath
 ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 LINK_LIBS = ["-L" + ROCM_LIB, "-Wl,-rpath," + ROCM_LIB, "-lrocsolver", "-lrocblas"]
@@ -44,14 +45,20 @@ def build(verbose: bool = False) -> str:
     objs = []
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
+    cmds = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
-        cmd = [cc] + HIP_FLAGS + ["-x", "hip", "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), file=sys.stderr)
-        subprocess.run(cmd, check=True)
+        cmds.append([cc] + HIP_FLAGS + ["-x", "hip", "-c", path, "-o", obj])
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), file=sys.stderr)
+        for f in [pool.submit(subprocess.run, cmd, check=True) for cmd in cmds]:
+            f.result()
     tmp = LIB_PATH + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + LINK_LIBS
     subprocess.run(cmd, check=True)

@@ -19,6 +19,9 @@
 // A1 = alpha p_h w (p_h = physics-basis interpolant of p), A2 = beta w, and
 //   dPsi/dd = H c + (c.d) H_r d / r,  dPsi/dc = H d,  dPsi/dk = (c.d) A1 k e^{ikr}/(4 pi r),
 //   H_r = -A1 h' + A2 (-3/(4 pi r^4)),  h' = -k^2 e^{ikr}/(4 pi r^2) - 3 h / r.
+// The same formulas hold for complex k (the lossy systems' wavenumbers) in complex
+// arithmetic; then Phi's derivatives in Re k and Im k are Re(D) and -Im(D) of
+// D = sum (c.d) A1 k e^{ikr} / (4 pi r) (F is holomorphic in k).
 #include <hip/hip_runtime.h>
 
 #include "bie_common.hpp"
@@ -36,7 +39,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int NG, int NP>
+template <int NG, int NP, bool CPLX>
 __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const bool active = i < a.M;
@@ -44,8 +47,8 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
   const double x0 = a.targets[3 * ic], x1 = a.targets[3 * ic + 1], x2 = a.targets[3 * ic + 2];
   double2 al = active ? a.alpha[ic] : make_double2(0.0, 0.0);
   double2 be = (active && a.beta) ? a.beta[ic] : make_double2(0.0, 0.0);
-  const double k = a.k;
-  double gx0 = 0, gx1 = 0, gx2 = 0, gk = 0;
+  const double k = a.k, ki = CPLX ? a.ki : 0.0;
+  double gx0 = 0, gx1 = 0, gx2 = 0, gk = 0, gki = 0;
   const int e0 = blockIdx.y * ech;
   const int e1 = min(a.E, e0 + ech);
   for (int e = e0; e < e1; ++e) {
@@ -104,17 +107,43 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
       const double A2r = w * be.x;
       double sn, cs;
       sincos_f64(k * r, &sn, &cs);
+      if (CPLX) {  // e^{ikr} = e^{-Im k r} e^{i Re k r}
+        const double ex = exp(-ki * r);
+        cs *= ex;
+        sn *= ex;
+      }
       const double i4r = kInv4Pi * rinv;            // 1/(4 pi r)
       const double i4r3 = i4r * rinv * rinv;        // 1/(4 pi r^3)
       const double kr = k * r;
-      const double hr = (-cs - kr * sn) * i4r3, hi = (kr * cs - sn) * i4r3;
-      const double k2i4r2 = k * k * i4r * rinv;     // k^2/(4 pi r^2)
-      const double hpr = -k2i4r2 * cs - 3.0 * hr * rinv, hpi = -k2i4r2 * sn - 3.0 * hi * rinv;
+      double hr, hi, hpr, hpi;
+      if (CPLX) {
+        // h = e^{ikr} (ikr - 1)/(4 pi r^3), ikr - 1 = (-Im k r - 1) + i Re k r
+        const double ar = fma(-ki, r, -1.0), ai = kr;
+        hr = (cs * ar - sn * ai) * i4r3;
+        hi = (cs * ai + sn * ar) * i4r3;
+        // h' = -k^2 e^{ikr}/(4 pi r^2) - 3 h / r, k^2 = (Re k^2 - Im k^2) + 2 i Re k Im k
+        const double k2r = k * k - ki * ki, k2i = 2.0 * k * ki, s2 = i4r * rinv;
+        hpr = -(k2r * cs - k2i * sn) * s2 - 3.0 * hr * rinv;
+        hpi = -(k2r * sn + k2i * cs) * s2 - 3.0 * hi * rinv;
+      } else {
+        hr = (-cs - kr * sn) * i4r3;
+        hi = (kr * cs - sn) * i4r3;
+        const double k2i4r2 = k * k * i4r * rinv;   // k^2/(4 pi r^2)
+        hpr = -k2i4r2 * cs - 3.0 * hr * rinv;
+        hpi = -k2i4r2 * sn - 3.0 * hi * rinv;
+      }
       // Re(H) = -Re(A1 h) + A2r/(4 pi r^3);  Re(H_r) = -Re(A1 h') - 3 A2r/(4 pi r^4)
       const double ReH = -(A1r * hr - A1i * hi) + A2r * i4r3;
       const double ReHr = -(A1r * hpr - A1i * hpi) - 3.0 * A2r * i4r3 * rinv;
-      // dPsi/dk = cd * Re(A1 k e^{ikr}) / (4 pi r)
-      gk += cd * k * i4r * (A1r * cs - A1i * sn);
+      if (CPLX) {
+        // D += cd A1 k e^{ikr} / (4 pi r): d/dRe k = Re D, d/dIm k = -Im D
+        const double er = k * cs - ki * sn, ei = k * sn + ki * cs;   // k e^{ikr}
+        gk += cd * i4r * (A1r * er - A1i * ei);
+        gki -= cd * i4r * (A1r * ei + A1i * er);
+      } else {
+        // dPsi/dk = cd * Re(A1 k e^{ikr}) / (4 pi r)
+        gk += cd * k * i4r * (A1r * cs - A1i * sn);
+      }
       const double t = ReHr * cd * rinv;
       const double gd[3] = {fma(t, d[0], ReH * cv[0]), fma(t, d[1], ReH * cv[1]),
                             fma(t, d[2], ReH * cv[2])};
@@ -156,6 +185,10 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
   }
   const double gks = wave_sum(gk);
   if ((threadIdx.x & 63) == 0 && a.gradk) unsafeAtomicAdd(a.gradk, gks);
+  if (CPLX) {
+    const double gkis = wave_sum(gki);
+    if ((threadIdx.x & 63) == 0 && a.gradk) unsafeAtomicAdd(a.gradk + 1, gkis);
+  }
 }
 
 // Derivative in k of the assembly, contracted: w = alpha^T (dF/dk) p for one instance of an
@@ -239,7 +272,7 @@ void launch_dfdk_batch(const DfdkInst* d_inst, const int4* d_tiles, int n_tiles,
   BIE_HIP_CHECK(hipGetLastError());
 }
 
-void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s) {
+void launch_adjoint(const AdjointArgs& a, int gk, int pk, bool cplx, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;
   const int64_t rowblocks = (a.M + 255) / 256;
   BIE_REQUIRE(rowblocks < 2147483647LL, BIE_EINVAL, "too many targets");
@@ -250,12 +283,13 @@ void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s) {
   const int ech = (int)((a.E + chunks - 1) / chunks);
   chunks = (a.E + ech - 1) / ech;
   const dim3 grid((unsigned)rowblocks, (unsigned)chunks);
+  auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, a, ech); };
   if (gk == 3 && pk == 3)
-    hipLaunchKernelGGL((k_adjoint<3, 3>), grid, dim3(256), 0, s, a, ech);
+    cplx ? go(k_adjoint<3, 3, true>) : go(k_adjoint<3, 3, false>);
   else if (gk == 6 && pk == 6)
-    hipLaunchKernelGGL((k_adjoint<6, 6>), grid, dim3(256), 0, s, a, ech);
+    cplx ? go(k_adjoint<6, 6, true>) : go(k_adjoint<6, 6, false>);
   else
-    hipLaunchKernelGGL((k_adjoint<6, 3>), grid, dim3(256), 0, s, a, ech);
+    cplx ? go(k_adjoint<6, 3, true>) : go(k_adjoint<6, 3, false>);
   BIE_HIP_CHECK(hipGetLastError());
 }
 

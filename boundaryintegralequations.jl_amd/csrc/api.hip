@@ -172,6 +172,12 @@ int slot_occupancy() {
   return (v && std::atoi(v) == 2) ? 2 : 1;
 }
 
+// Elements each wave takes per step of the single-k slot kernel (1 or 2).
+int slot_elems_per_wave() {
+  const char* v = std::getenv("BIE_AMD_V");
+  return (v && std::atoi(v) == 2) ? 2 : 1;
+}
+
 int slot_split() {
   const char* v = std::getenv("BIE_AMD_SPLIT");
   return (v && std::atoi(v) == 2) ? 2 : 1;
@@ -245,9 +251,9 @@ int choose_segment_count(int device, int64_t rowblocks, int64_t E, int per_cu = 
 // Schedule `which` (0: single-k kernel, 1: sweep kernel, 2: batch kernel) for S live
 // slots and at least min_segments segments.
 bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segments,
-                                  hipStream_t s) {
+                                  hipStream_t s, int wv = kSlotWaves) {
   bie_dmesh::Sched& out = dm->sched[which];
-  if (out.key == min_segments && out.S == S && out.d_step_elems) return out;
+  if (out.key == min_segments && out.S == S && out.wv == wv && out.d_step_elems) return out;
   free_schedule(out);
   std::vector<double> cent((size_t)dm->E * 3);
   for (int64_t e = 0; e < dm->E; ++e)
@@ -257,13 +263,13 @@ bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segme
       cent[3 * e + d] = acc / 3.0;
     }
   SlotSchedule sc = build_slot_schedule(dm->h_ptopo.data(), dm->pk, (int)dm->E, (int)dm->N,
-                                        cent.data(), kSlotWaves, S, min_segments);
+                                        cent.data(), wv, S, min_segments);
   out.d_seg_step_begin = upload_vec(sc.seg_step_begin, s);
   out.h_seg_step_begin = sc.seg_step_begin;
   out.d_step_elems = upload_vec(sc.step_elems, s);
   // per (step, wave) copies of the selection nodes and slots, in the order the kernel
   // reads them
-  const size_t nsw = (size_t)sc.n_steps * kSlotWaves;
+  const size_t nsw = (size_t)sc.n_steps * wv;
   std::vector<int> step_slot(nsw * dm->pk, 0);
   std::vector<double> step_sel(nsw * dm->pk * 4, 0.0);
   for (size_t t = 0; t < nsw; ++t) {
@@ -288,12 +294,13 @@ bie_dmesh::Sched& ensure_schedule(bie_dmesh* dm, int which, int S, int min_segme
   out.max_live = sc.max_live;
   out.key = min_segments;
   out.S = S;
+  out.wv = wv;
   return out;
 }
 
 SlotArgs slot_args(bie_dmesh* dm, const bie_dmesh::Sched& sc, int64_t rowblocks) {
   SlotArgs sa{};
-  sa.W = kSlotWaves;
+  sa.W = sc.wv;
   sa.S = sc.S;
   sa.n_segments = sc.segments;
   sa.n_rowblocks = (int)rowblocks;
@@ -504,9 +511,11 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       const bool qc_inst = fast && (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16);
       const int occ = qc_inst ? slot_occupancy() : 1;
       const int split = qc_inst && occ == 1 ? slot_split() : 1;
+      const int vpw = qc_inst && occ == 1 && split == 1 ? slot_elems_per_wave() : 1;
       const int min_segments = choose_segment_count(dm->device, rowblocks, dm->E, occ);
-      const bie_dmesh::Sched& sc = ensure_schedule(
-          dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0], occ, split), min_segments, s);
+      const bie_dmesh::Sched& sc =
+          ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0], occ, split),
+                          min_segments, s, kSlotWaves * vpw);
       // collocation on quadratic physics: after the first call the mid-edge pairs come
       // from the recorded sorted list instead of the slot kernel's queue
       const bool colloc = d_targets == dm->d_sources && M == dm->N;
@@ -575,9 +584,13 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
   const int KB = sweep_kb(dm->pk);
   BIE_REQUIRE(sizeof(double) * (size_t)dm->rs.R * dm->rs.Qmax * dm->pk <= 24 * 1024,
               BIE_EUNSUPPORTED, "sweep supports quadrature orders up to n*m' = 36");
-  const bie_dmesh::Sched& sc =
-      ensure_schedule(dm, 1, fitting_slots(sweep_fixed_doubles(dm->rs.R * dm->rs.Qmax * dm->pk), KB),
-                      1, s);
+  const int fit = fitting_slots(sweep_fixed_doubles(dm->rs.R * dm->rs.Qmax * dm->pk), KB);
+  const int S = dm->sweep_slots ? std::min(dm->sweep_slots, fit) : fit;
+  const int min_seg = dm->sweep_elems_per_tile
+                          ? (int)std::max<int64_t>(1, (dm->E + dm->sweep_elems_per_tile - 1) /
+                                                          dm->sweep_elems_per_tile)
+                          : 1;
+  const bie_dmesh::Sched& sc = ensure_schedule(dm, 1, S, min_seg, s);
   if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
   if (dm->pk == 6) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
   const int KBk = sweep_block_size(dm->pk, (flags & BIE_G) != 0);
@@ -697,9 +710,8 @@ struct MeshLocks {
   }
 };
 
-void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
-                         double* const* dF, double* const* dG, double* const* dC,
-                         hipStream_t s) {
+void batch_validate(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                    double* const* dF, double* const* dG, double* const* dC) {
   BIE_REQUIRE(b != nullptr, BIE_EINVAL, "batch is NULL");
   BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
   BIE_REQUIRE(k_re && dF && dG && dC, BIE_EINVAL, "NULL argument array");
@@ -711,11 +723,21 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
     BIE_REQUIRE(!(flags & BIE_F) || dF[ii], BIE_EINVAL, "F output pointer is NULL");
     BIE_REQUIRE(!(flags & BIE_G) || dG[ii], BIE_EINVAL, "G output pointer is NULL");
   }
-  BIE_HIP_CHECK(hipSetDevice(b->device));
-  std::lock_guard<std::mutex> bl(b->mu);
-  MeshLocks locks(b->meshes);
+}
+
+// What the batch's kernels need besides the instance table.
+struct BatchPlan {
+  bool cplx = false;
+  int64_t max_work = 1;
+};
+
+// Host half of a batched assembly (under the batch and mesh locks): every per-mesh cache,
+// the tile list, and the instance table in b->h_inst (after the previous upload has left
+// it).
+BatchPlan batch_fill(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                     double* const* dF, double* const* dG, double* const* dC, hipStream_t s) {
+  const int n_inst = (int)b->meshes.size();
   for (bie_dmesh* dm : b->meshes) {
-    order_after_previous(dm, s);
     ensure_rules(dm, b->n, b->m, s);
     if (!dm->geom_valid) refresh_geometry(dm, s);
     if (dm->pk == 6)
@@ -725,12 +747,11 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
   }
   batch_configure(b, s);
   BIE_HIP_CHECK(hipEventSynchronize(b->staged));  // the previous upload has left h_inst
-  bool cplx = false;
-  int64_t max_work = 1;
+  BatchPlan plan;
   for (int ii = 0; ii < n_inst; ++ii) {
     bie_dmesh* dm = b->meshes[ii];
     const double ki = k_im ? k_im[ii] : 0.0;
-    cplx = cplx || ki != 0.0;
+    plan.cplx = plan.cplx || ki != 0.0;
     const bie_dmesh::Sched& sc = dm->sched[2];
     BatchInst& bi = b->h_inst[ii];
     bi.a = make_args(dm, dm->d_sources, dm->N, k_re[ii], ki, flags, dF[ii], dG[ii], dC[ii]);
@@ -738,26 +759,47 @@ void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, u
     bi.zero_cols = sc.d_zero_cols;
     bi.n_zero = sc.zero_cols;
     bi.pad = 0;
-    max_work = std::max<int64_t>(max_work, sc.zero_cols);  // columns to zero per output
+    plan.max_work = std::max<int64_t>(plan.max_work, sc.zero_cols);  // columns to zero per output
     if ((!(flags & BIE_F) && dF[ii]) || (!(flags & BIE_G) && dG[ii]))
-      max_work = std::max<int64_t>(max_work, dm->N);
+      plan.max_work = std::max<int64_t>(plan.max_work, dm->N);
   }
-  BIE_HIP_CHECK(hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(BatchInst) * n_inst,
-                               hipMemcpyHostToDevice, s));
-  BIE_HIP_CHECK(hipEventRecord(b->staged, s));
-  launch_batch_prep(b->d_inst, n_inst, max_work, s);
+  return plan;
+}
+
+// Device half: zero fills, the persistent assembly over the tile list, the mid-edge pairs.
+// Kernels only (no memset, no host synchronisation), so it can be captured.
+void batch_kernels(bie_batch* b, const BatchInst* d_inst, const int4* d_tiles,
+                   unsigned* d_counter, const BatchPlan& plan, hipStream_t s) {
+  const int n_inst = (int)b->meshes.size();
+  launch_batch_prep(d_inst, n_inst, plan.max_work, s);
   if (b->n_tiles > 0) {
     bie_dmesh* d0 = b->meshes[0];
     const int nphi = d0->rs.R * d0->rs.Qmax * d0->pk;
     int cus = 0;
     BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
     const int nstage = stage_doubles(d0->rs.len[0]);
-    const int per_cu = batch_blocks_per_cu(b->pk, cplx, b->slots_used, nphi, nstage);
+    const int per_cu = batch_blocks_per_cu(b->pk, plan.cplx, b->slots_used, nphi, nstage);
     const int grid = (int)std::min<int64_t>((int64_t)std::max(cus, 1) * per_cu, b->n_tiles);
-    launch_assemble_batch(b->d_inst, b->d_tiles, b->n_tiles, b->d_counter, d0->d_phi, nphi,
-                          nstage, b->slots_used, b->pk, cplx, grid, s);
+    launch_assemble_batch(d_inst, d_tiles, b->n_tiles, d_counter, d0->d_phi, nphi, nstage,
+                          b->slots_used, b->pk, plan.cplx, grid, s);
   }
-  if (b->pk == 6) launch_assemble_mid_batch(b->d_inst, n_inst, cplx, s);
+  if (b->pk == 6) launch_assemble_mid_batch(d_inst, n_inst, plan.cplx, s);
+}
+
+void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
+                         double* const* dF, double* const* dG, double* const* dC,
+                         hipStream_t s) {
+  batch_validate(b, k_re, k_im, flags, dF, dG, dC);
+  const int n_inst = (int)b->meshes.size();
+  BIE_HIP_CHECK(hipSetDevice(b->device));
+  std::lock_guard<std::mutex> bl(b->mu);
+  MeshLocks locks(b->meshes);
+  for (bie_dmesh* dm : b->meshes) order_after_previous(dm, s);
+  const BatchPlan plan = batch_fill(b, k_re, k_im, flags, dF, dG, dC, s);
+  BIE_HIP_CHECK(hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(BatchInst) * n_inst,
+                               hipMemcpyHostToDevice, s));
+  BIE_HIP_CHECK(hipEventRecord(b->staged, s));
+  batch_kernels(b, b->d_inst, b->d_tiles, b->d_counter, plan, s);
   for (bie_dmesh* dm : b->meshes) mark_done(dm, s);
 }
 
@@ -1289,12 +1331,14 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
       for (int d = 0; d < 3; ++d)
         gsel[((size_t)e * NG + g) * 4 + d] = dm->h_coords[3 * (size_t)dm->h_topo[e * NG + g] + d];
   // the six rules (sources at the vertices, then the edge midpoints 1-2, 2-3, 3-1;
-  // assembly_collocation_losses.jl:227-232), geometry and physics bases at their points
+  // assembly_collocation_losses.jl:227-232), geometry and physics bases at their points;
+  // all six for linear elements too, as the reference builds them (the selection's
+  // fallthrough uses the sixth)
   const double srcs[6][3] = {{1.0, offr, offr}, {offr, 1.0, offr}, {offr, offr, 1.0},
                              {0.5, 0.5, offr},  {offr, 0.5, 0.5},  {0.5, offr, 0.5}};
   const int RS = sparse_record_doubles();
   std::vector<double> rec;
-  for (int c = 0; c < NG; ++c) {
+  for (int c = 0; c < 6; ++c) {
     std::vector<double> u, v, w;
     subdivision_rule(srcs[c], u, v, w);
     sp->off[c] = (int)(rec.size() / RS);
@@ -1526,6 +1570,11 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
                                           (int)mesh->n_sources, cent.data(), kSlotWaves,
                                           kSlotCount, min_segments);
     validate_slot_schedule(sc, ptopo.data(), mesh->physics_kind, (int)E, (int)mesh->n_sources);
+    // the schedule of two elements per wave and step as well
+    SlotSchedule sc2 = build_slot_schedule(ptopo.data(), mesh->physics_kind, (int)E,
+                                           (int)mesh->n_sources, cent.data(), 2 * kSlotWaves,
+                                           kSlotCount, min_segments);
+    validate_slot_schedule(sc2, ptopo.data(), mesh->physics_kind, (int)E, (int)mesh->n_sources);
     if (stats) {
       stats[0] = sc.n_segments;
       stats[1] = sc.n_steps;
@@ -1537,50 +1586,75 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
   });
 }
 
+namespace {
+
+void assembly_vjp_impl(bie_dmesh* dm, double k, double ki, bool cplx, const double* d_targets,
+                       int64_t n_targets, const int32_t* d_target_node, const double* d_alpha,
+                       const double* d_beta, const double* d_p, int32_t n, int32_t m,
+                       double* d_grad_coords, double* d_grad_k, hipStream_t s) {
+  BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+  BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
+  BIE_REQUIRE(d_alpha && d_p && d_grad_coords, BIE_EINVAL, "NULL argument");
+  BIE_REQUIRE(std::isfinite(k) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+  if (!d_targets) {
+    d_targets = dm->d_sources;
+    BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
+  }
+  std::lock_guard<std::mutex> lk(dm->mu);
+  BIE_HIP_CHECK(hipSetDevice(dm->device));
+  order_after_previous(dm, s);
+  ensure_rules(dm, n, m, s);
+  AdjointArgs a{};
+  a.targets = d_targets;
+  a.M = n_targets;
+  a.tvert = reinterpret_cast<const int*>(d_target_node);
+  a.alpha = reinterpret_cast<const double2*>(d_alpha);
+  a.beta = reinterpret_cast<const double2*>(d_beta);
+  a.p = reinterpret_cast<const double2*>(d_p);
+  a.coords = dm->d_coords;
+  a.topo = dm->d_topo;
+  a.ptopo = dm->d_ptopo;
+  a.selnodes = dm->d_selnodes;
+  a.pts = dm->d_pts;
+  a.phi = dm->d_phi;
+  a.Qmax = dm->rs.Qmax;
+  for (int c = 0; c < 6; ++c) {
+    a.off[c] = dm->rs.off[c];
+    a.len[c] = dm->rs.len[c];
+  }
+  a.R = dm->rs.R;
+  a.E = (int)dm->E;
+  a.hyp = dm->hyp;
+  a.k = k;
+  a.ki = ki;
+  a.gradX = d_grad_coords;
+  a.gradk = d_grad_k;
+  launch_adjoint(a, dm->gk, dm->pk, cplx, s);
+  mark_done(dm, s);
+}
+
+}  // namespace
+
 int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, int64_t n_targets,
                             const int32_t* d_target_node, const double* d_alpha,
                             const double* d_beta, const double* d_p, int32_t n, int32_t m,
                             double* d_grad_coords, double* d_grad_k, void* stream) {
   return guarded([&] {
-    BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
-    BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
-    BIE_REQUIRE(d_alpha && d_p && d_grad_coords, BIE_EINVAL, "NULL argument");
-    BIE_REQUIRE(std::isfinite(k), BIE_EINVAL, "k must be finite");
-    if (!d_targets) {
-      d_targets = dm->d_sources;
-      BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
-    }
-    std::lock_guard<std::mutex> lk(dm->mu);
-    BIE_HIP_CHECK(hipSetDevice(dm->device));
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    order_after_previous(dm, s);
-    ensure_rules(dm, n, m, s);
-    AdjointArgs a{};
-    a.targets = d_targets;
-    a.M = n_targets;
-    a.tvert = reinterpret_cast<const int*>(d_target_node);
-    a.alpha = reinterpret_cast<const double2*>(d_alpha);
-    a.beta = reinterpret_cast<const double2*>(d_beta);
-    a.p = reinterpret_cast<const double2*>(d_p);
-    a.coords = dm->d_coords;
-    a.topo = dm->d_topo;
-    a.ptopo = dm->d_ptopo;
-    a.selnodes = dm->d_selnodes;
-    a.pts = dm->d_pts;
-    a.phi = dm->d_phi;
-    a.Qmax = dm->rs.Qmax;
-    for (int c = 0; c < 6; ++c) {
-      a.off[c] = dm->rs.off[c];
-      a.len[c] = dm->rs.len[c];
-    }
-    a.R = dm->rs.R;
-    a.E = (int)dm->E;
-    a.hyp = dm->hyp;
-    a.k = k;
-    a.gradX = d_grad_coords;
-    a.gradk = d_grad_k;
-    launch_adjoint(a, dm->gk, dm->pk, s);
-    mark_done(dm, s);
+    assembly_vjp_impl(dm, k, 0.0, false, d_targets, n_targets, d_target_node, d_alpha, d_beta,
+                      d_p, n, m, d_grad_coords, d_grad_k, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bie_assembly_vjp_complex_device(bie_dmesh* dm, double k_re, double k_im,
+                                    const double* d_targets, int64_t n_targets,
+                                    const int32_t* d_target_node, const double* d_alpha,
+                                    const double* d_beta, const double* d_p, int32_t n,
+                                    int32_t m, double* d_grad_coords, double* d_grad_k,
+                                    void* stream) {
+  return guarded([&] {
+    assembly_vjp_impl(dm, k_re, k_im, true, d_targets, n_targets, d_target_node, d_alpha,
+                      d_beta, d_p, n, m, d_grad_coords, d_grad_k,
+                      static_cast<hipStream_t>(stream));
   });
 }
 
@@ -1640,6 +1714,63 @@ int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
   });
 }
 
+int bie_sweep_tune(bie_dmesh* dm, const double* d_k, int32_t nk, const double* d_targets,
+                   int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* dF,
+                   double* dG, int64_t stride, double* dC, void* stream,
+                   int32_t* elems_per_tile, int32_t* slots, double* best_ms) {
+  return guarded([&] {
+    BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(dm->mu);
+    BIE_HIP_CHECK(hipSetDevice(dm->device));
+    hipEvent_t e0, e1;
+    BIE_HIP_CHECK(hipEventCreate(&e0));
+    BIE_HIP_CHECK(hipEventCreate(&e1));
+    // tile = 64 targets (one per lane) x elems_per_tile elements (0: the whole surface, one
+    // segment per row block), with as many LDS slots as fit or 32
+    const int cand_elems[] = {16, 32, 64, 128, 0};
+    const int cand_slots[] = {0, 32};
+    const int old_e = dm->sweep_elems_per_tile, old_s = dm->sweep_slots;
+    double best = 1e300;
+    int be = old_e, bs = old_s;
+    try {
+      for (int ce : cand_elems)
+        for (int cs : cand_slots) {
+          dm->sweep_elems_per_tile = ce;
+          dm->sweep_slots = cs;
+          // builds the schedule and warms up, then one timed sweep
+          sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC,
+                            s, false);
+          BIE_HIP_CHECK(hipEventRecord(e0, s));
+          sweep_device_impl(dm, d_k, nk, d_targets, n_targets, n, m, flags, dF, dG, stride, dC,
+                            s, false);
+          BIE_HIP_CHECK(hipEventRecord(e1, s));
+          BIE_HIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          BIE_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            be = ce;
+            bs = cs;
+          }
+        }
+    } catch (...) {
+      dm->sweep_elems_per_tile = old_e;
+      dm->sweep_slots = old_s;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    dm->sweep_elems_per_tile = be;
+    dm->sweep_slots = bs;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elems_per_tile) *elems_per_tile = be;
+    if (slots) *slots = bs;
+    if (best_ms) *best_ms = best;
+  });
+}
+
 int bie_assemble_taylor_device(bie_dmesh* dm, double k_re, double k_im, int32_t n_orders,
                                const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
                                uint32_t flags, double* dF, double* dG, int64_t stride,
@@ -1665,6 +1796,7 @@ struct bie_graph {
   int device = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<void*> owned;  // device buffers the captured kernels read (freed with the graph)
 };
 
 int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
@@ -1723,8 +1855,83 @@ int bie_graph_destroy(bie_graph* g) {
   (void)hipSetDevice(g->device);
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (!g->owned.empty()) (void)hipDeviceSynchronize();  // a replay may still read them
+  for (void* p : g->owned) (void)hipFree(p);
   delete g;
   return BIE_OK;
+}
+
+int bie_batch_graph_create(bie_batch* b, const double* k_re, const double* k_im,
+                           uint32_t flags, double* const* dF, double* const* dG,
+                           double* const* dC, bie_graph** out) {
+  return guarded([&] {
+    BIE_REQUIRE(out != nullptr, BIE_EINVAL, "out is NULL");
+    *out = nullptr;
+    batch_validate(b, k_re, k_im, flags, dF, dG, dC);
+    const int n_inst = (int)b->meshes.size();
+    BIE_HIP_CHECK(hipSetDevice(b->device));
+    std::lock_guard<std::mutex> bl(b->mu);
+    MeshLocks locks(b->meshes);
+    hipStream_t cs = nullptr;
+    BIE_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    std::unique_ptr<bie_graph> g(new bie_graph);
+    g->device = b->device;
+    try {
+      // every cache, the tile list and the instance table outside the capture; the graph
+      // gets its own copies of the table, the tile list and the tile counter
+      for (bie_dmesh* dm : b->meshes) order_after_previous(dm, cs);
+      const BatchPlan plan = batch_fill(b, k_re, k_im, flags, dF, dG, dC, cs);
+      // quadratic instances queue their mid-edge pairs: into queues of the graph's own (a
+      // mesh's queue is released once its collocation pairs are recorded, and regrown by
+      // calls with more targets)
+      for (int ii = 0; ii < n_inst; ++ii) {
+        bie_dmesh* dm = b->meshes[ii];
+        if (dm->pk != 6) continue;
+        const int64_t cap = std::max<int64_t>(1, dm->N * dm->E);
+        int2* q = dalloc<int2>((size_t)cap);
+        g->owned.push_back(q);
+        unsigned* cnt = dalloc<unsigned>(1);
+        g->owned.push_back(cnt);
+        b->h_inst[ii].sa.mid_queue = q;
+        b->h_inst[ii].sa.mid_count = cnt;
+        b->h_inst[ii].sa.mid_cap = cap;
+      }
+      BatchInst* g_inst = dalloc<BatchInst>(std::max(n_inst, 1));
+      g->owned.push_back(g_inst);
+      int4* g_tiles = dalloc<int4>(std::max(b->n_tiles, 1));
+      g->owned.push_back(g_tiles);
+      unsigned* g_counter = dalloc<unsigned>(kXcds);
+      g->owned.push_back(g_counter);
+      BIE_HIP_CHECK(hipMemcpyAsync(g_inst, b->h_inst, sizeof(BatchInst) * n_inst,
+                                   hipMemcpyHostToDevice, cs));
+      if (b->n_tiles > 0)
+        BIE_HIP_CHECK(hipMemcpyAsync(g_tiles, b->d_tiles, sizeof(int4) * b->n_tiles,
+                                     hipMemcpyDeviceToDevice, cs));
+      // one eager run sets the kernels' attributes (LDS sizes) outside the capture
+      batch_kernels(b, g_inst, g_tiles, g_counter, plan, cs);
+      BIE_HIP_CHECK(hipStreamSynchronize(cs));
+      BIE_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      try {
+        batch_kernels(b, g_inst, g_tiles, g_counter, plan, cs);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(cs, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      BIE_HIP_CHECK(hipStreamEndCapture(cs, &g->graph));
+      BIE_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      (void)hipStreamDestroy(cs);
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      for (void* p : g->owned) (void)hipFree(p);
+      g->owned.clear();
+      throw;
+    }
+    (void)hipStreamDestroy(cs);
+    *out = g.release();
+  });
 }
 
 int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_t m,

@@ -23,6 +23,11 @@ namespace bie {
 namespace {
 
 constexpr double kInv4Pi = 0.079577471545947667884;  // 1 / (4 pi)
+// Points per stage of the software-pipelined real-k point loop (pair_quadrature_pipelined).
+#ifndef BIE_POINT_GROUP
+#define BIE_POINT_GROUP 1
+#endif
+constexpr int kPointGroup = BIE_POINT_GROUP;
 
 __global__ void k_fill_zero(double2* __restrict__ p, int64_t n2) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -201,9 +206,137 @@ __device__ __forceinline__ void pair_quadrature_pre(double x0, double x1, double
                                      csum, trig);
 }
 
+// The real-k point of point_kernel<NP, false, true, GON, true> in two stages, so that a
+// software-pipelined loop can put other points' work between them.  front: distance,
+// 1/r, n.d, the kernel scale factors and the argument reduction of sincos_tab_fast, ending
+// with the issue of the table read; back: the sincos polynomials, the kernel values and
+// their accumulation into the NP node sums.  Same operations, in the same order per point,
+// as point_kernel.
+struct PointFront {
+  double krr, d, gg, ak;
+  double2 t;
+};
+
+__device__ __forceinline__ PointFront point_front(double x0, double x1, double x2,
+                                                  const double4 A, const double4 B, double kr,
+                                                  double& csum,
+                                                  const double2* __restrict__ trig) {
+  constexpr double kInvStep = 20.371832715762602978;       // 64 / pi
+  constexpr double kStepHi = 1.5707963267948965580 / 32;   // pi/2 rounded, / 32 (exact)
+  constexpr double kStepLo = 6.1232339957367658e-17 / 32;  // its remainder, / 32 (exact)
+  constexpr double kShift = 6755399441055744.0;            // 1.5 * 2^52
+  PointFront f;
+  const double dx = A.x - x0, dy = A.y - x1, dz = A.z - x2;
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rinv = rsqrt_nr(r2);
+  const double r = r2 * rinv;
+  const double nd = fma(A.w, dx, fma(B.x, dy, B.y * dz));
+  f.gg = B.z * rinv;                  // J w / (4 pi r)
+  f.ak = nd * rinv * rinv * f.gg;     // (n.d) J w / (4 pi r^3)
+  csum -= f.ak;
+  f.krr = kr * r;
+  // sincos_tab_fast(k r): reduction and table read
+  const double sh = fma(f.krr, kInvStep, kShift);
+  const double n = sh - kShift;
+  double d = fma(-n, kStepHi, f.krr);
+  f.d = fma(-n, kStepLo, d);
+  f.t = trig[__double2loint(sh) & (kTrigN - 1)];
+  return f;
+}
+
+template <int NP, bool GON>
+__device__ __forceinline__ void point_back(const PointFront& f, const double* p, double* fr,
+                                           double* fi, double* gr, double* gi) {
+  const double d = f.d, d2 = d * d;
+  const double sp = fma(d2, fma(d2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0);
+  const double sd = fma(d * d2, sp, d);
+  const double cp = fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5);
+  const double cd = fma(d2, cp, 1.0);
+  const double sn = fma(f.t.y, cd, f.t.x * sd);
+  const double cs = fma(f.t.x, cd, -(f.t.y * sd));
+  const double Fr = -f.ak * fma(f.krr, sn, cs);
+  const double Fi = f.ak * fma(f.krr, cs, -sn);
+  const double Gr = f.gg * cs, Gi = f.gg * sn;
+#pragma unroll
+  for (int s = 0; s < NP; ++s) {
+    const double ps = p[s];
+    fr[s] = fma(Fr, ps, fr[s]);
+    fi[s] = fma(Fi, ps, fi[s]);
+    if constexpr (GON) {
+      gr[s] = fma(Gr, ps, gr[s]);
+      gi[s] = fma(Gi, ps, gi[s]);
+    }
+  }
+}
+
+// Real-k quadrature of QC staged points (rows[8 q .. 8 q + 8)), software-pipelined over
+// groups of G points: while group q's back stage runs, group q + G's table reads (issued
+// by its front stage) and group q + 2G's row reads are in flight.  With two waves per
+// SIMD (the slot kernel's LDS use) a plain point-by-point loop leaves the FP64 pipe idle
+// through every LDS latency.
+template <int NP, bool GON, int QC, int G>
+__device__ __forceinline__ void pair_quadrature_pipelined(double x0, double x1, double x2,
+                                                          const double* rows,
+                                                          const double* __restrict__ ph,
+                                                          double kr, double* fr, double* fi,
+                                                          double* gr, double* gi,
+                                                          double& csum,
+                                                          const double2* trig) {
+  static_assert(QC % G == 0, "the rule length must be a multiple of the group size");
+  constexpr int NG = QC / G;
+  double4 A[G], B[G];
+  PointFront cur[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    A[g] = *reinterpret_cast<const double4*>(rows + 8 * g);
+    B[g] = *reinterpret_cast<const double4*>(rows + 8 * g + 4);
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) cur[g] = point_front(x0, x1, x2, A[g], B[g], kr, csum, trig);
+  if constexpr (NG > 1) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      A[g] = *reinterpret_cast<const double4*>(rows + 8 * (G + g));
+      B[g] = *reinterpret_cast<const double4*>(rows + 8 * (G + g) + 4);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    // group k's basis values first: LDS reads complete in issue order, so reads issued
+    // after the next group's table reads would wait for those as well
+    double pv[G * NP];
+#pragma unroll
+    for (int t = 0; t < G * NP; ++t) pv[t] = ph[k * G * NP + t];
+    __builtin_amdgcn_sched_barrier(0);
+    PointFront nxt[G];
+    if (k + 1 < NG) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) nxt[g] = point_front(x0, x1, x2, A[g], B[g], kr, csum, trig);
+      if (k + 2 < NG) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          A[g] = *reinterpret_cast<const double4*>(rows + 8 * ((k + 2) * G + g));
+          B[g] = *reinterpret_cast<const double4*>(rows + 8 * ((k + 2) * G + g) + 4);
+        }
+      }
+    }
+    // keep the stages in this order: the scheduler's LDS latency model would otherwise
+    // move the next group's reads down to their first use
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int g = 0; g < G; ++g) point_back<NP, GON>(cur[g], pv + g * NP, fr, fi, gr, gi);
+    __builtin_amdgcn_sched_barrier(0);
+    if (k + 1 < NG) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) cur[g] = nxt[g];
+    }
+  }
+}
+
 // As pair_quadrature over table rows that this wave staged in LDS (rows[8 q .. 8 q + 8)).
 // QC > 0: the rule length is the compile-time QC (== Q), and the loop is unrolled whole, so
-// every LDS read has a constant offset and the compiler can interleave the points.
+// every LDS read has a constant offset; with FAST (real k) the loop is the software-
+// pipelined pair_quadrature_pipelined over groups of kPointGroup points.
 template <int NP, bool CPLX, bool TAB = false, bool GON = true, int QC = 0, bool FAST = false,
           bool ROLLED = false>
 __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double x2,
@@ -219,6 +352,9 @@ __device__ __forceinline__ void pair_quadrature_lds(double x0, double x1, double
       point_kernel<NP, CPLX, TAB, GON, FAST>(x0, x1, x2, A, B, ph + q * NP, kr, ki, fr, fi, gr,
                                              gi, csum, trig);
     }
+  } else if constexpr (QC > 0 && FAST && !CPLX && QC % kPointGroup == 0) {
+    pair_quadrature_pipelined<NP, GON, QC, kPointGroup>(x0, x1, x2, rows, ph, kr, fr, fi, gr,
+                                                        gi, csum, trig);
   } else if constexpr (QC > 0) {
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
@@ -296,15 +432,58 @@ k_assemble(const AssemblyArgs a, int ech) {
 // Flush of one schedule step's finished nodes: each wave takes every W-th entry, reads the
 // node's slot (F re, F im, G re, G im for 64 rows), zeroes it and writes the column
 // segment of F and G: a plain (streaming) store for owned columns, atomics for shared
-// ones.  Runs concurrently with the next step's quadrature, which never touches these
-// slots.
+// ones.  Entries go two at a time, both slots read before either is written out, so one
+// LDS wait serves two nodes; an owned column's (re, im) pairs leave as one 16-byte store
+// per lane (a wave writes 1 KB of the column in one instruction).  Runs concurrently with
+// the next step's quadrature, which never touches these slots.
+__device__ __forceinline__ void flush_write(const AssemblyArgs& a, int node, int ss, int64_t i,
+                                            bool doF, bool doG, double vr, double vi,
+                                            double wr, double wi) {
+  const size_t idx = 2 * ((size_t)i + (size_t)node * (size_t)a.M);
+  if (ss & kSharedBit) {
+    if (doF) {
+      unsafeAtomicAdd(a.F + idx, vr);
+      unsafeAtomicAdd(a.F + idx + 1, vi);
+    }
+    if (doG) {
+      unsafeAtomicAdd(a.G + idx, wr);
+      unsafeAtomicAdd(a.G + idx + 1, wi);
+    }
+  } else {
+    // the output is written once and not read back here: keep it out of the caches
+    if (doF) __builtin_nontemporal_store(d2v{vr, vi}, reinterpret_cast<d2v*>(a.F + idx));
+    if (doG) __builtin_nontemporal_store(d2v{wr, wi}, reinterpret_cast<d2v*>(a.G + idx));
+  }
+}
+
 template <int W>
 __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs& sa, int st,
                                            int wave, int lane, int64_t i, bool active,
                                            double* acc) {
   const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
   const int f1 = sload(sa.flush_begin + st + 1);
-  for (int f = sload(sa.flush_begin + st) + wave; f < f1; f += W) {
+  int f = sload(sa.flush_begin + st) + wave;
+  for (; f + W < f1; f += 2 * W) {
+    const int n0 = sload(sa.flush_node + f), s0 = sload(sa.flush_slot + f);
+    const int n1 = sload(sa.flush_node + f + W), s1 = sload(sa.flush_slot + f + W);
+    double* p0 = acc + (size_t)(s0 & 0xffff) * 256 + lane;
+    double* p1 = acc + (size_t)(s1 & 0xffff) * 256 + lane;
+    const double a0 = p0[0], b0 = p0[64], c0 = p0[128], d0 = p0[192];
+    const double a1 = p1[0], b1 = p1[64], c1 = p1[128], d1 = p1[192];
+    p0[0] = 0.0;
+    p0[64] = 0.0;
+    p0[128] = 0.0;
+    p0[192] = 0.0;
+    p1[0] = 0.0;
+    p1[64] = 0.0;
+    p1[128] = 0.0;
+    p1[192] = 0.0;
+    if (active) {
+      flush_write(a, n0, s0, i, doF, doG, a0, b0, c0, d0);
+      flush_write(a, n1, s1, i, doF, doG, a1, b1, c1, d1);
+    }
+  }
+  if (f < f1) {
     const int node = sload(sa.flush_node + f);
     const int ss = sload(sa.flush_slot + f);
     double* p = acc + (size_t)(ss & 0xffff) * 256 + lane;
@@ -313,29 +492,7 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
     p[64] = 0.0;
     p[128] = 0.0;
     p[192] = 0.0;
-    if (active) {
-      const size_t idx = 2 * ((size_t)i + (size_t)node * (size_t)a.M);
-      if (ss & kSharedBit) {
-        if (doF) {
-          unsafeAtomicAdd(a.F + idx, vr);
-          unsafeAtomicAdd(a.F + idx + 1, vi);
-        }
-        if (doG) {
-          unsafeAtomicAdd(a.G + idx, wr);
-          unsafeAtomicAdd(a.G + idx + 1, wi);
-        }
-      } else {
-        // the output is written once and not read back here: keep it out of the caches
-        if (doF) {
-          __builtin_nontemporal_store(vr, a.F + idx);
-          __builtin_nontemporal_store(vi, a.F + idx + 1);
-        }
-        if (doG) {
-          __builtin_nontemporal_store(wr, a.G + idx);
-          __builtin_nontemporal_store(wi, a.G + idx + 1);
-        }
-      }
-    }
+    if (active) flush_write(a, node, ss, i, doF, doG, vr, vi, wr, wi);
   }
 }
 
@@ -349,7 +506,8 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
 // its element's vertex-rule rows in its LDS rows (s_rows + wave * 24 Qc) and loads the
 // next element's rows into registers a whole pair ahead: the global loads never wait
 // behind the flush's stores (vector-memory counters are in order).  Ends on a barrier.
-template <int NP, bool CPLX, int W, bool STAGED, bool GON = true, int QC = 0, bool FAST = false>
+template <int NP, bool CPLX, int W, bool STAGED, bool GON = true, int QC = 0, bool FAST = false,
+          int V = 1>
 __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
                                           int seg, double* acc, const double* phi,
                                           const double2* trig, double* s_rows, int2* s_q,
@@ -368,12 +526,13 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   double* my_sel = STAGED ? my_rows + 24 * Qc : nullptr;  // the element's selection nodes
   constexpr int nsel = NP * 2;                             // 16-byte chunks of them
   double2 stg[3], stg_sel;
+  constexpr int WV = W * V;  // elements per step; wave w takes elements v * W + w, v < V
   const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
-  int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * W + wave) : -1;
+  int e = (st0 < st1) ? sload(sa.step_elems + (size_t)st0 * WV + wave) : -1;
   int c = 0;
   double4 A0 = make_double4(0, 0, 0, 0), B0 = A0, A1 = A0, B1 = A0;
   if (e >= 0) {
-    const double* sel = sa.step_sel + ((size_t)st0 * W + wave) * NP * 4;
+    const double* sel = sa.step_sel + ((size_t)st0 * WV + wave) * NP * 4;
     if (STAGED) {
       stage_load(a.elem + (size_t)e * a.Pe * 8, nchunk, sel, nsel, lane, stg, stg_sel);
       stage_store(my_rows, nchunk, my_sel, nsel, lane, stg, stg_sel);
@@ -385,9 +544,13 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     }
   }
   for (int st = st0; st < st1; ++st) {
-    const size_t nidx = (size_t)(st + 1) * W + wave;
-    const int e_nxt =
-        (st + 1 < st1) ? sload(sa.step_elems + nidx) : -1;
+#pragma unroll 1
+    for (int v = 0; v < V; ++v) {
+    const size_t cidx = (size_t)st * WV + v * W + wave;
+    // the wave's next element: the step's next sub-element, else the next step's first
+    const bool last_v = v + 1 == V;
+    const size_t nidx = last_v ? (size_t)(st + 1) * WV + wave : cidx + W;
+    const int e_nxt = (!last_v || st + 1 < st1) ? sload(sa.step_elems + nidx) : -1;
     // unconditional (element 0 stands in at the segment's end): a path that skipped the
     // LDS store below would leave these loads outstanding at the loop head, where the
     // compiler would wait for them, and with them for the previous step's column stores
@@ -409,7 +572,7 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
           pair_quadrature_pre<NP, CPLX, true, GON>(
               x0, x1, x2, a.elem + ((size_t)e * a.Pe + a.off[c]) * 8, phi + c * a.Qmax * NP,
               a.len[c], kr, ki, A0, B0, A1, B1, fr, fi, gr, gi, csum, trig);
-        const int* so = sa.step_slot + ((size_t)st * W + wave) * NP;
+        const int* so = sa.step_slot + cidx * NP;
 #pragma unroll
         for (int s = 0; s < NP; ++s) {
           double* p = acc + (size_t)sload(so + s) * 256 + lane;
@@ -453,11 +616,12 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
         load_rows01(a.elem + ((size_t)e_nxt * a.Pe + a.off[c_nxt]) * 8, a.len[c_nxt], A0, B0,
                     A1, B1);
     }
+    e = e_nxt;
+    c = c_nxt;
+    }
     // the previous step's finished nodes: their slots were complete at the last barrier
     if (st > st0) flush_step<W>(a, sa, st - 1, wave, lane, i, active, acc);
     lds_barrier();
-    e = e_nxt;
-    c = c_nxt;
   }
   if (st1 > st0) {
     flush_step<W>(a, sa, st1 - 1, wave, lane, i, active, acc);
@@ -629,7 +793,8 @@ k_assemble_slots_split(const AssemblyArgs a, const SlotArgs sa) {
 // QC > 0: instance for vertex rules of exactly QC points (staged in LDS, QC <= kStageMaxQ)
 // with the e^{ikr} of sincos_tab_fast (FAST: real k, |k| r below its bound); QC == 0: any
 // rule length, staged or not.
-template <int NP, bool CPLX, int W, bool GON, int QC = 0, bool FAST = false, int OCC = 1>
+template <int NP, bool CPLX, int W, bool GON, int QC = 0, bool FAST = false, int OCC = 1,
+          int V = 1>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2 * OCC)))
 k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   extern __shared__ double lds[];
@@ -662,8 +827,8 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   lds_barrier();
   if constexpr (QC > 0) {
     static_assert(QC <= kStageMaxQ, "compile-time rule lengths are staged in LDS");
-    slot_tile<NP, CPLX, W, true, GON, QC, FAST>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q,
-                                                s_qn);
+    slot_tile<NP, CPLX, W, true, GON, QC, FAST, V>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows,
+                                                   s_q, s_qn);
   } else if (nstage) {
     slot_tile<NP, CPLX, W, true, GON>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
   } else {
@@ -947,7 +1112,10 @@ void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipSt
 
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;
-  BIE_REQUIRE(sa.W == kSlotWaves, BIE_EINVAL, "slot schedule built for another wave count");
+  BIE_REQUIRE(sa.W == kSlotWaves || (sa.W == 2 * kSlotWaves && fast && a.ki == 0.0 &&
+                                      sa.occupancy == 1 && sa.split == 1 &&
+                                      (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16)),
+              BIE_EINVAL, "slot schedule built for another wave count");
   const int64_t per_xcd = (sa.n_tiles + 7) / 8;
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
@@ -998,6 +1166,14 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
         case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 2>); return true;
         case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 2>); return true;
         case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 2>); return true;
+        default: return false;
+      }
+    }
+    if (sa.W == 2 * W) {
+      switch (qc) {
+        case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 1, 2>); return true;
+        case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 1, 2>); return true;
+        case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 1, 2>); return true;
         default: return false;
       }
     }

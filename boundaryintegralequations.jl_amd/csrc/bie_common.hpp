@@ -193,9 +193,10 @@ struct AdjointArgs {
   int R;
   int E;
   int hyp;                 // rule-selection distance (as AssemblyArgs::hyp)
-  double k;
+  double k;               // Re k
+  double ki;              // Im k (complex-k entry point; 0 for real k)
   double* gradX;           // 3 x Nc, accumulated
-  double* gradk;           // 1, accumulated (or NULL)
+  double* gradk;           // d/dRe k (and d/dIm k after it for complex k), accumulated, or NULL
 };
 
 // One instance of the batched k-derivative w = alpha^T (dF/dk) p (adjoint.hip).
@@ -288,7 +289,7 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
                   hipStream_t s);
 void sweep_prepare_attributes(int NP, int nphi, bool taylor, bool gon);
 void launch_block_uniform(const double* k, int nk, int KB, int* out, hipStream_t s);
-void launch_adjoint(const AdjointArgs& a, int gk, int pk, hipStream_t s);
+void launch_adjoint(const AdjointArgs& a, int gk, int pk, bool cplx, hipStream_t s);
 // tiles: (instance, 256-row block, chunk of ech elements)
 void launch_dfdk_batch(const DfdkInst* d_inst, const int4* d_tiles, int n_tiles, int ech, int NP,
                        hipStream_t s);
@@ -334,7 +335,7 @@ struct bie_dmesh {
   std::vector<int> h_topo, h_ptopo;
   // slot schedules (mesh-only): [0] single-k kernel, [1] sweep kernel, [2] batch kernel
   struct Sched {
-    int S = -1, key = -1;
+    int S = -1, key = -1, wv = 0;  // wv: elements per step
     int segments = 0, zero_cols = 0, max_live = 0;
     int* d_seg_step_begin = nullptr;
     int* d_step_elems = nullptr;
@@ -348,6 +349,10 @@ struct bie_dmesh {
   } sched[3];
   int* d_block_uniform = nullptr;
   int block_uniform_cap = 0;
+  // tile shape of the frequency sweep (bie_sweep_tune): elements per tile (0: one segment per
+  // row block) and LDS slots (0: as many as fit)
+  int sweep_elems_per_tile = 0;
+  int sweep_slots = 0;
   // mid-edge pair queue
   unsigned* d_mid_count = nullptr;
   int2* d_mid_queue = nullptr;

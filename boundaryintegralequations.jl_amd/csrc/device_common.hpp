@@ -30,6 +30,9 @@ __device__ __forceinline__ T sload(const T* p) {
       reinterpret_cast<uintptr_t>(p));
 }
 
+// Two doubles as one 16-byte vector: a complex entry's (re, im) written by one store.
+typedef double d2v __attribute__((ext_vector_type(2)));
+
 // Julia Base n-ary hypot for Float64 (maxabs * sqrt(sum((x_i/maxabs)^2)), left to
 // right), with no fused multiply-add so that the result is bit-identical to the CPU.
 __device__ __noinline__ double julia_hypot3(double x, double y, double z) {

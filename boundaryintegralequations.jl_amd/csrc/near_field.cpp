@@ -1,0 +1,158 @@
+// Host half of the sparse near-field assembly (the kernel is in sparse.hip): the
+// reference's subdivision rule and connected_sources, restated for the GPU path.
+//
+// * subdivision_rule: singular_triangle_integration (src/3d/adaptive_integration.jl:215-246)
+//   with subdivide_triangle (:172-213), called with the identity as element coordinates
+//   and the source at a vertex or an edge midpoint (assembly_collocation_losses.jl:226-232):
+//   the reference triangle is split into four while a sub-triangle's largest side exceeds
+//   a quarter of its distance to the source (sides below 1e-6 of the element size count as
+//   zero); each sub-triangle then gets gauss_points_triangle(3) (adaptive_integration.jl:
+//   13-19), shifted to its centre and scaled by its shortest side, as the reference places
+//   it.
+// * sparse_pattern: connected_sources (assembly_collocation_losses.jl:98-132).
+// Plain C++ (no device code), so that the CPU sanitizer build covers it.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "bie_common.hpp"
+
+namespace bie {
+namespace {
+
+// ---- subdivision rules (host) ---------------------------------------------------------
+// Restated operation by operation (no fused multiply-add: the subdivision decisions are
+// floating-point comparisons, and the recursion must reach the same sub-triangles).
+#pragma clang fp contract(off)
+
+struct P2 {
+  double u, v;
+};
+struct V3 {
+  double x, y, z;
+};
+
+P2 mid(P2 a, P2 b) { return {(a.u + b.u) / 2, (a.v + b.v) / 2}; }
+
+// The linear basis [(1 - u) - v, u, v] of the element the reference subdivides (a
+// discontinuous linear triangle with beta = 0), times the identity coordinates.
+V3 lin(P2 p) { return {(1.0 - p.u) - p.v, p.u, p.v}; }
+
+double dist3(V3 a, V3 b) {
+  const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return std::sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+// maximum_side_length / minimum_side_length (adaptive_integration.jl:73-86): sides of the
+// closed polygon c1 -> c2 -> c3 -> c1.
+double max_side3(V3 a, V3 b, V3 c) {
+  return std::max(std::max(dist3(b, a), dist3(c, b)), dist3(a, c));
+}
+double side2(P2 a, P2 b) {
+  const double du = a.u - b.u, dv = a.v - b.v;
+  return std::sqrt(du * du + dv * dv);
+}
+
+struct Tri {
+  P2 a, b, c;
+};
+
+void subdivide(const std::vector<Tri>& in, V3 src, double esize, double tol,
+               std::vector<Tri>& out, int depth) {
+  BIE_REQUIRE(depth < 64, BIE_EINVAL, "subdivision does not terminate");
+  for (const Tri& t : in) {
+    const P2 p12 = mid(t.a, t.b), p23 = mid(t.b, t.c), p31 = mid(t.c, t.a);
+    const P2 cen = {((t.a.u + t.b.u) + t.c.u) / 3.0, ((t.a.v + t.b.v) + t.c.v) / 3.0};
+    const V3 s0 = lin(cen), s1 = lin(t.a), s2 = lin(t.b), s3 = lin(t.c);
+    const double dsub = std::min(std::min(std::min(dist3(s0, src), dist3(s1, src)),
+                                          dist3(s2, src)),
+                                 dist3(s3, src));
+    double size = max_side3(s1, s2, s3);
+    if (size < esize * tol) size = 0.0;
+    if (size > dsub / 4) {
+      const std::vector<Tri> nd = {{t.a, p12, p31}, {p12, t.b, p23}, {p31, p23, t.c},
+                                   {p12, p23, p31}};
+      subdivide(nd, src, esize, tol, out, depth + 1);
+    } else {
+      out.push_back(t);
+    }
+  }
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace
+
+// Points (u, v) and weights of the subdivision rule for a source at barycentric-like
+// position src = (1 - u - v, u, v) of the reference triangle.
+void subdivision_rule(const double src[3], std::vector<double>& u, std::vector<double>& v,
+                      std::vector<double>& w) {
+#pragma clang fp contract(off)
+  const V3 s = {src[0], src[1], src[2]};
+  const double esize = max_side3({1, 0, 0}, {0, 1, 0}, {0, 0, 1});
+  std::vector<Tri> out;
+  subdivide({{{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}}}, s, esize, 1e-6, out, 0);
+  const double gx[3] = {1.0 / 6, 2.0 / 3, 1.0 / 6}, gy[3] = {1.0 / 6, 1.0 / 6, 2.0 / 3};
+  const double gw = 1.0 / 6, a3 = 1.0 / 3.0;
+  u.clear();
+  v.clear();
+  w.clear();
+  for (const Tri& t : out) {
+    const double dim = std::min(std::min(side2(t.b, t.a), side2(t.c, t.b)), side2(t.a, t.c));
+    const double cu = ((t.a.u + t.b.u) + t.c.u) / 3, cv = ((t.a.v + t.b.v) + t.c.v) / 3;
+    for (int g = 0; g < 3; ++g) {
+      u.push_back((gx[g] - a3) * dim + cu);
+      v.push_back((gy[g] - a3) * dim + cv);
+      w.push_back(gw * (dim * dim));
+    }
+  }
+}
+
+
+// Pattern and pairs of the sparse assembly (connected_sources, depth 1 or 2).
+void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,
+                    std::vector<int64_t>& row_ptr, std::vector<int>& cols,
+                    std::vector<int64_t>& pair_ptr, std::vector<int>& pair_elem) {
+  BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
+  std::vector<std::vector<int>> el(N), sc(N);
+  for (int64_t e = 0; e < E; ++e)
+    for (int i = 0; i < NP; ++i) {
+      const int node = ptopo[e * NP + i];
+      el[node].push_back((int)e);
+      for (int j = 0; j < NP; ++j) sc[node].push_back(ptopo[e * NP + j]);
+    }
+  auto uniq = [](std::vector<int>& v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  for (int64_t i = 0; i < N; ++i) {
+    uniq(el[i]);
+    uniq(sc[i]);
+  }
+  if (depth == 2) {
+    std::vector<std::vector<int>> el2(N), sc2(N);
+    for (int64_t i = 0; i < N; ++i) {
+      for (int j : sc[i]) {
+        el2[i].insert(el2[i].end(), el[j].begin(), el[j].end());
+        sc2[i].insert(sc2[i].end(), sc[j].begin(), sc[j].end());
+      }
+      uniq(el2[i]);
+      uniq(sc2[i]);
+    }
+    el.swap(el2);
+    sc.swap(sc2);
+  }
+  row_ptr.assign(1, 0);
+  pair_ptr.assign(1, 0);
+  cols.clear();
+  pair_elem.clear();
+  for (int64_t i = 0; i < N; ++i) {
+    cols.insert(cols.end(), sc[i].begin(), sc[i].end());
+    row_ptr.push_back((int64_t)cols.size());
+    pair_elem.insert(pair_elem.end(), el[i].begin(), el[i].end());
+    pair_ptr.push_back((int64_t)pair_elem.size());
+  }
+}
+
+}  // namespace bie

@@ -195,8 +195,7 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
               unsafeAtomicAdd(a.F + idx, xr);
               unsafeAtomicAdd(a.F + idx + 1, xi);
             } else {
-              __builtin_nontemporal_store(xr, a.F + idx);
-              __builtin_nontemporal_store(xi, a.F + idx + 1);
+              __builtin_nontemporal_store(d2v{xr, xi}, reinterpret_cast<d2v*>(a.F + idx));
             }
           }
         }
@@ -215,14 +214,8 @@ __device__ __forceinline__ void sweep_flush(const AssemblyArgs& a, const SlotArg
             unsafeAtomicAdd(a.G + idx + 1, wi);
           }
         } else {
-          if (doF) {
-            __builtin_nontemporal_store(vr, a.F + idx);
-            __builtin_nontemporal_store(vi, a.F + idx + 1);
-          }
-          if (doG) {
-            __builtin_nontemporal_store(wr, a.G + idx);
-            __builtin_nontemporal_store(wi, a.G + idx + 1);
-          }
+          if (doF) __builtin_nontemporal_store(d2v{vr, vi}, reinterpret_cast<d2v*>(a.F + idx));
+          if (doG) __builtin_nontemporal_store(d2v{wr, wi}, reinterpret_cast<d2v*>(a.G + idx));
         }
       }
     }

@@ -10,8 +10,9 @@ Drop-in MI355X backend for `BoundaryIntegralEquations.assemble_parallel!`
     F, G, C = assemble_parallel!(mesh, k, mesh.sources; n=2, m=2)
 
 Supported meshes (TriangularLinear / TriangularQuadratic geometry with geometry or linear
-physics) run on the GPU; everything else (quads, discontinuous physics, `sparse=true`)
-is forwarded to the reference method unchanged, so the call stays a drop-in.  Errors from
+physics) run on the GPU, dense and `sparse=true` (the near field,
+`sparse_assemble_parallel!`); everything else (quads, discontinuous physics) is forwarded
+to the reference method unchanged, so the call stays a drop-in.  Errors from
 the library are thrown as `ArgumentError` (bad input) or `ErrorException` (HIP failure).
 
 This file is not executed in the build container (no Julia there); the same ABI is
@@ -21,6 +22,7 @@ module BoundaryIntegralEquationsAMD
 
 import BoundaryIntegralEquations
 import LinearAlgebra
+import SparseArrays
 const BIE = BoundaryIntegralEquations
 
 const libbie = Ref{String}(get(ENV, "BIE_AMD_LIB",
@@ -112,12 +114,53 @@ GPU and everything else to `BoundaryIntegralEquations.assemble_parallel!`.
 """
 function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true, cOn=true,
                             sparse=false, m=4, n=4, progress=true, depth=2, offset=nothing)
-    if sparse || !supported(mesh)
+    if !supported(mesh)
         return BIE.assemble_parallel!(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn,
                                       sparse=sparse, m=m, n=n, progress=progress,
                                       depth=depth, offset=offset)
     end
+    sparse && return sparse_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, depth=depth,
+                                offset=offset)
     return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
+end
+
+"""
+    sparse_hip(mesh, k, sources; fOn=true, gOn=true, depth=1, offset=nothing)
+
+Near-field F, G on the GPU as `SparseMatrixCSC`s (n_sources x n_sources) with the
+reference's stored entries (`sparse_assemble_parallel!`,
+src/3d/assembly_collocation_losses.jl:200-405): `sparse(I, J, F), sparse(I, J, G)`.
+"""
+function sparse_hip(mesh::BIE.Mesh3d, k::Number, sources::AbstractMatrix; fOn=true, gOn=true,
+                    depth=1, offset=nothing)
+    coords = Matrix{Float64}(mesh.coordinates)
+    topo = Matrix{Int64}(mesh.topology)
+    srcs = Matrix{Float64}(mesh.sources)
+    ptopo = Matrix{Int64}(mesh.physics_topology)
+    N = size(srcs, 2)
+    size(sources) == (3, N) || throw(ArgumentError("one source point per mesh source"))
+    own = sources === mesh.sources
+    pts = own ? srcs : Matrix{Float64}(sources)
+    flags = UInt32((fOn ? 1 : 0) | (gOn ? 2 : 0))
+    kc = ComplexF64(k)
+    off = offset === nothing ? NaN : Float64(offset)
+    GC.@preserve coords topo srcs ptopo pts begin
+        bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
+                     size(coords, 2), size(topo, 2), N,
+                     element_kind(mesh.shape_function), element_kind(mesh.physics_function),
+                     Int32(1), hypot_flag())
+        nnz = Ref{Int64}(0)
+        check(ccall((:bie_sparse_count, libbie[]), Cint, (Ref{BieMesh}, Int32, Ref{Int64}),
+                    bm, Int32(depth), nnz))
+        I = Vector{Int64}(undef, nnz[]); J = similar(I)
+        F = Vector{ComplexF64}(undef, nnz[]); G = similar(F)
+        check(ccall((:bie_sparse_assemble, libbie[]), Cint,
+                    (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Cdouble, UInt32,
+                     Int64, Ptr{Int64}, Ptr{Int64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                    bm, real(kc), imag(kc), own ? Ptr{Float64}(C_NULL) : pointer(pts),
+                    own ? 0 : N, Int32(depth), off, flags, nnz[], I, J, F, G))
+        return SparseArrays.sparse(I, J, F, N, N), SparseArrays.sparse(I, J, G, N, N)
+    end
 end
 
 # ---------------------------------------------------------------------------------------
@@ -162,6 +205,26 @@ function BIE.assemble_parallel!(mesh::BIE.Mesh3d, k::Number, in_sources::Abstrac
                       n=n, progress=progress)
     end
     return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
+end
+
+# The near field: sparse_assemble_parallel!(mesh, k, sources, physics_function; fOn, gOn,
+# depth, offset, progress, Ngauss) (assembly_collocation_losses.jl:200-405), called by the
+# reference's fast operators (fast_multipole_method.jl:181,187) and lossy systems through
+# assemble_parallel!(...; sparse=true).  Continuous linear and quadratic physics run on the
+# GPU; other physics (the polar rules of discontinuous elements) and a source count that
+# differs from the mesh's go to the reference's method.
+function BIE.sparse_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
+                                       sources::AbstractMatrix{<:Real},
+                                       physics_function::Union{BIE.TriangularLinear,
+                                                               BIE.TriangularQuadratic};
+                                       fOn=true, gOn=true, depth=1, offset=nothing,
+                                       progress=true, Ngauss=40)
+    if !(GPU_ROUTING[] && supported(mesh) && size(sources, 2) == size(mesh.sources, 2))
+        return invoke(BIE.sparse_assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,Any}, mesh, k,
+                      sources, physics_function; fOn=fOn, gOn=gOn, depth=depth,
+                      offset=offset, progress=progress, Ngauss=Ngauss)
+    end
+    return sparse_hip(mesh, k, sources; fOn=fOn, gOn=gOn, depth=depth, offset=offset)
 end
 
 # ---------------------------------------------------------------------------------------

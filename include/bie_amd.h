@@ -153,8 +153,9 @@ int bie_element_geometry(bie_dmesh* dm, int32_t rule, int32_t n, int32_t m, doub
 
 /* Builds the LDS-slot schedule of the assembly kernel for a host mesh and replays it,
  * checking every invariant the kernel relies on (no counterpart in the reference; the
- * reference's threads own whole rows, assembly_collocation.jl:314).  stats (6 values):
- * segments, steps, max live slots, shared nodes, zeroed columns, flush entries. */
+ * reference's threads own whole rows, assembly_collocation.jl:314), for steps of one and of
+ * two elements per wave.  stats (6 values, one element per wave): segments, steps, max
+ * live slots, shared nodes, zeroed columns, flush entries. */
 int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stats);
 
 /* ---- Batched frequency sweep ------------------------------------------------------------
@@ -170,6 +171,17 @@ int bie_assemble_sweep_device(bie_dmesh* dm, const double* d_k, int32_t nk,
                               const double* d_targets, int64_t n_targets, int32_t n, int32_t m,
                               uint32_t flags, double* dF, double* dG, int64_t stride,
                               double* dC, void* stream);
+/* Tile-shape autotuner of the sweep: times the sweep with these arguments for every
+ * candidate tile, 64 targets (one per lane) x elements per tile in {16, 32, 64, 128, the
+ * whole surface}, each with as many LDS slots as fit or with 32, and keeps the fastest for
+ * later sweeps and sweep graphs on this handle (the outputs hold the last candidate's
+ * result, equal to any other's up to the order of shared-column additions).  Reports the
+ * chosen elements per tile (0: whole surface), slots (0: as many as fit) and its time.  No
+ * counterpart in the reference.  Synchronous. */
+int bie_sweep_tune(bie_dmesh* dm, const double* d_k, int32_t nk, const double* d_targets,
+                   int64_t n_targets, int32_t n, int32_t m, uint32_t flags, double* dF,
+                   double* dG, int64_t stride, double* dC, void* stream,
+                   int32_t* elems_per_tile, int32_t* slots, double* best_ms);
 /* The same sweep captured once into a HIP graph (kernels only, no memset nodes); launching
  * the graph replays it on any stream.  The graph keeps the pointers it was built with. */
 int bie_sweep_graph_create(bie_dmesh* dm, const double* d_k, int32_t nk,
@@ -246,6 +258,19 @@ int bie_batch_tune(bie_batch* b, const double* k_re, const double* k_im, uint32_
  * adjoint gradient (alpha_i from the adjoint solve of instance i).  Waits for the launch.
  * No counterpart in the reference (it differentiates by finite differences,
  * examples/2d_optimization.jl:184-216). */
+/* The batched assembly (bie_batch_assemble with these arguments) captured into a HIP graph:
+ * kernels only (fill kernels for the zeroing, no memset nodes), with the graph's own copies of
+ * the instance table, the tile list and the tile counter, and its own mid-edge pair queues
+ * (quadratic physics); launching it replays the batch on any stream (bie_graph_launch),
+ * destroying it frees those (bie_graph_destroy).  The graph keeps the output pointers and
+ * the meshes' rule tables and schedules: replays must not overlap other work on the batch
+ * or its meshes, and reconfiguring or tuning the batch, assembling its meshes with another
+ * rule set (n, m), or destroying either invalidates it.
+ * No counterpart in the reference (a persistent launch captured in a graph is the GPU form
+ * of its per-mesh loop). */
+int bie_batch_graph_create(bie_batch* b, const double* k_re, const double* k_im,
+                           uint32_t flags, double* const* dF, double* const* dG,
+                           double* const* dC, bie_graph** out);
 int bie_batch_dfdk_device(bie_batch* b, const double* k_re, const double* k_im,
                           const double* const* d_alpha, const double* const* d_p, double* d_w,
                           void* stream);
@@ -266,6 +291,18 @@ int bie_assembly_vjp_device(bie_dmesh* dm, double k, const double* d_targets, in
                             const int32_t* d_target_node, const double* d_alpha,
                             const double* d_beta, const double* d_p, int32_t n, int32_t m,
                             double* d_grad_coords, double* d_grad_k, void* stream);
+
+/* As bie_assembly_vjp_device at a complex wavenumber k_re + i k_im (the lossy systems'
+ * kernels, e.g. assemble_parallel!(mesh, k_a, sources; ...) at
+ * src/3d/LossyGlobalMatrix.jl:119 with complex k_a): d_grad_k (2 doubles, accumulated, or
+ * NULL) receives d/dRe k and d/dIm k of the same functional.  No counterpart in the
+ * reference. */
+int bie_assembly_vjp_complex_device(bie_dmesh* dm, double k_re, double k_im,
+                                    const double* d_targets, int64_t n_targets,
+                                    const int32_t* d_target_node, const double* d_alpha,
+                                    const double* d_beta, const double* d_p, int32_t n,
+                                    int32_t m, double* d_grad_coords, double* d_grad_k,
+                                    void* stream);
 
 /* ---- Dense operator apply --------------------------------------------------------------
  * Y = A X for complex128 column-major A (M x K, leading dimension lda), X (K x nrhs, ldx),

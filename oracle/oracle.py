@@ -66,6 +66,14 @@ def lib():
         L.orc_hypot3_scaled.restype = ctypes.c_double
         L.orc_hypot3_scaled.argtypes = [ctypes.c_double] * 3
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_subdivision_rule.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_connected_sources.argtypes = [_i64p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int, ctypes.c_int, _i64p, _i64p, _i64p,
+                                            _i64p]
+        L.orc_sparse_assemble.argtypes = [_dp, _i64p, ctypes.c_int, _dp, _i64p, ctypes.c_int,
+                                          ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -213,3 +221,54 @@ def assemble_frozen(mesh, k, rules, targets=None, n=4, m=4, fOn=True, gOn=True, 
 
 def max_threads() -> int:
     return lib().orc_max_threads()
+
+
+# ---- sparse near field (sparse_oracle.cpp) ---------------------------------------------
+
+def subdivision_rule(src, tol=1e-6):
+    """(u, v, w) of singular_triangle_integration(., 3, src, I, tol)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    Q = lib().orc_subdivision_rule(_p(src), tol, None, None, None, 0)
+    u, v, w = np.zeros(Q), np.zeros(Q), np.zeros(Q)
+    lib().orc_subdivision_rule(_p(src), tol, _p(u), _p(v), _p(w), Q)
+    return u, v, w
+
+
+def connected_sources(mesh, depth=1):
+    """connected_sources(mesh, depth): (element lists, source lists), 0-based, sorted."""
+    ptop = np.asfortranarray(mesh.physics_topology)
+    N, E = mesh.n_sources, mesh.n_elements
+    rp, ep = np.zeros(N + 1, dtype=np.int64), np.zeros(N + 1, dtype=np.int64)
+    if lib().orc_connected_sources(_p(ptop, _i64p), mesh.physics_kind, E, N, 0, depth, _p(rp, _i64p),
+                                   None, _p(ep, _i64p), None) != 0:
+        raise ValueError("depth must be 1 or 2")
+    cols, els = np.zeros(rp[-1], dtype=np.int64), np.zeros(ep[-1], dtype=np.int64)
+    lib().orc_connected_sources(_p(ptop, _i64p), mesh.physics_kind, E, N, 0, depth, _p(rp, _i64p),
+                                _p(cols, _i64p), _p(ep, _i64p), _p(els, _i64p))
+    return ([els[ep[i]:ep[i + 1]] for i in range(N)], [cols[rp[i]:rp[i + 1]] for i in range(N)])
+
+
+def sparse_assemble(mesh, k, sources=None, depth=1, offset=0.0, fOn=True, gOn=True,
+                    hypot="julia1.9"):
+    """sparse_assemble_parallel! as (I, J, F, G): 0-based rows and columns of the stored
+    entries in the reference's order, complex128 values."""
+    coords, topo, src, ptop = _mesh_args(mesh)
+    pts = np.asfortranarray(src if sources is None else sources, dtype=np.float64)
+    N, E = mesh.n_sources, mesh.n_elements
+    if pts.shape != (3, N):
+        raise ValueError("one source point per mesh source")
+    _, cols = connected_sources(mesh, depth)
+    nnz = sum(len(c) for c in cols)
+    F = np.zeros(nnz, dtype=np.complex128)
+    G = np.zeros(nnz, dtype=np.complex128)
+    k = complex(k)
+    fg = (1 if fOn else 0) | (2 if gOn else 0)
+    variant = {"julia1.9": 0, "scaled": 1}[hypot]
+    rc = lib().orc_sparse_assemble(_p(coords), _p(topo, _i64p), mesh.geometry_kind, _p(pts),
+                                   _p(ptop, _i64p), mesh.physics_kind, E, N, 0, k.real, k.imag,
+                                   depth, float(offset), fg, variant, _p(F), _p(G))
+    if rc != 0:
+        raise ValueError(f"oracle sparse assemble failed ({rc})")
+    I = np.concatenate([np.full(len(c), i, dtype=np.int64) for i, c in enumerate(cols)])
+    J = np.concatenate(cols) if nnz else np.zeros(0, dtype=np.int64)
+    return I, J, F, G

@@ -61,6 +61,66 @@ def test_vjp_matches_frozen_rule_finite_differences(kind, field):
     assert abs(gk - ref) <= 1e-9 * abs(ref)
 
 
+@pytest.mark.parametrize("kind,field", [("linear", False), ("quadratic", False),
+                                         ("quadratic", True)])
+def test_vjp_matches_dual_number_derivatives(kind, field):
+    """The VJP dotted with random directions against forward-mode dual numbers through
+    the assembly (tests/dual_oracle.py): exact derivatives, so a tight tolerance."""
+    from dual_oracle import assemble_F_C
+    mesh = M.perturb_radial(M.icosphere(3, kind), 0.01, 4)
+    rng = np.random.default_rng(8)
+    n, m, k = 3, 2, 6.0
+    targets = M.fibonacci_sphere(20, 2.0) if field else None
+    Mt = 20 if field else mesh.n_sources
+    beta = None if field else crandn(rng, Mt)
+    alpha, p = crandn(rng, Mt), crandn(rng, mesh.n_sources)
+    print(f"[case] vjp dual {kind} field={field}", flush=True)
+    g, gk = assembly_vjp(DeviceMesh(mesh), k, alpha, p, beta=beta, targets=targets,
+                         target_node=None if field else source_nodes(mesh), n=n, m=m)
+    rules = oracle.select(mesh, targets)
+    X = mesh.coordinates
+    for trial in range(2):
+        V = rng.standard_normal(X.shape)
+        F, C = assemble_F_C(mesh, X, V, k, 0.0, rules, targets=targets, n=n, m=m)
+        dd = np.real(alpha @ (F.d @ p) + (beta @ C.d if beta is not None else 0.0))
+        assert abs(float(np.sum(g * V)) - dd) <= 1e-10 * abs(dd), (trial, dd)
+    F, C = assemble_F_C(mesh, X, np.zeros_like(X), k, 1.0, rules, targets=targets, n=n, m=m)
+    ddk = np.real(alpha @ (F.d @ p))
+    assert abs(gk - ddk) <= 1e-10 * abs(ddk)
+
+
+@pytest.mark.parametrize("kind,field", [("linear", False), ("quadratic", True)])
+def test_vjp_complex_k_matches_dual_number_derivatives(kind, field):
+    """Complex wavenumber (the lossy systems' case): the VJP in X and the two k-derivatives
+    (d/dRe k, d/dIm k) against forward-mode dual numbers and against central differences
+    of the oracle's frozen-rule assembly."""
+    from dual_oracle import assemble_F_C
+    mesh = M.perturb_radial(M.icosphere(3, kind), 0.01, 6)
+    rng = np.random.default_rng(9)
+    n, m, k = 3, 3, 4.0 + 0.7j
+    targets = M.fibonacci_sphere(16, 2.2) if field else None
+    Mt = 16 if field else mesh.n_sources
+    beta = None if field else crandn(rng, Mt)
+    alpha, p = crandn(rng, Mt), crandn(rng, mesh.n_sources)
+    print(f"[case] vjp complex k {kind} field={field}", flush=True)
+    g, (gkr, gki) = assembly_vjp(DeviceMesh(mesh), k, alpha, p, beta=beta, targets=targets,
+                                 target_node=None if field else source_nodes(mesh), n=n, m=m)
+    rules = oracle.select(mesh, targets)
+    X = mesh.coordinates
+    V = rng.standard_normal(X.shape)
+    F, C = assemble_F_C(mesh, X, V, k, 0.0, rules, targets=targets, n=n, m=m)
+    dd = np.real(alpha @ (F.d @ p) + (beta @ C.d if beta is not None else 0.0))
+    assert abs(float(np.sum(g * V)) - dd) <= 1e-10 * abs(dd)
+    h = 1e-6
+    fp = functional(mesh, X + h * V, k, rules, alpha, beta, p, targets, n, m)
+    fm = functional(mesh, X - h * V, k, rules, alpha, beta, p, targets, n, m)
+    assert abs(float(np.sum(g * V)) - (fp - fm) / (2 * h)) <= 1e-6 * abs(dd)
+    for dk, got in ((1.0, gkr), (1j, gki)):
+        F, _ = assemble_F_C(mesh, X, np.zeros_like(X), k, dk, rules, targets=targets, n=n, m=m)
+        ref = np.real(alpha @ (F.d @ p))
+        assert abs(got - ref) <= 1e-10 * abs(ref), dk
+
+
 def test_scattering_gradient_matches_finite_differences():
     # config-4 shape at reduced size: perturbed linear sphere, k = 5, field shell r = 3
     mesh = M.perturb_radial(M.icosphere(4, "linear"), 0.01, 1)

@@ -214,3 +214,45 @@ def test_uneven_batch_48_ellipsoids():
         # angles below a half, by up to 0.074 (the 492-node instance 5, oracle) and by 0.047
         # on average there
         assert np.abs(Ci.real - 0.5).max() < 0.15 and abs(Ci.real.mean() - 0.5) < 0.06, i
+
+
+def test_uneven_batch_48_ellipsoids_dfdk():
+    """The batched k-derivative w_i = alpha_i^T (dF_i/dk) p_i of all 48 instances in one
+    launch, against central differences of each instance's own F (GPU assembly at
+    k_i +- h) and, on the two smallest instances, the oracle's derivative operator."""
+    torch, dev = _torch()
+    meshes, ks = M.uneven_ellipsoid_batch()
+    Ns = [x.n_sources for x in meshes]
+    print(f"[case] 48 ellipsoids dF/dk, N {min(Ns)}..{max(Ns)}", flush=True)
+    dms = [DeviceMesh(x) for x in meshes]
+    b = Batch(dms, n=4, m=4)
+    rng = np.random.default_rng(21)
+    alphas = [torch.as_tensor(rng.standard_normal(N) + 1j * rng.standard_normal(N),
+                              device=dev) for N in Ns]
+    ps = [torch.as_tensor(rng.standard_normal(N) + 1j * rng.standard_normal(N), device=dev)
+          for N in Ns]
+    w = torch.zeros(len(Ns), dtype=torch.complex128, device=dev)
+    b.dfdk(ks, [a.data_ptr() for a in alphas], [p.data_ptr() for p in ps], w.data_ptr())
+    torch.cuda.synchronize()
+    w = w.cpu().numpy()
+    F1 = torch.empty(max(Ns) ** 2, dtype=torch.complex128, device=dev)
+    G1 = torch.empty_like(F1)
+    C1 = torch.empty(max(Ns), dtype=torch.complex128, device=dev)
+    h = 1e-6
+    worst = 0.0
+    for i, N in enumerate(Ns):
+        vals = []
+        for kk in (ks[i] + h, ks[i] - h):
+            dms[i].assemble(kk, F1.data_ptr(), G1.data_ptr(), C1.data_ptr(), n=4, m=4,
+                            gOn=False)
+            Fi = _colmajor(F1[:N * N], N)
+            vals.append(complex((alphas[i] @ (Fi @ ps[i])).item()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        err = abs(w[i] - fd) / abs(fd)
+        worst = max(worst, err)
+        assert err < 1e-7, (i, w[i], fd)
+    print(f"  worst dF/dk vs central differences {worst:.2e}", flush=True)
+    for i in np.argsort(Ns)[:2]:
+        D1, _ = oracle.assemble_taylor(meshes[i], ks[i], 1, n=4, m=4)
+        ref = alphas[i].cpu().numpy() @ (D1 @ ps[i].cpu().numpy())
+        assert abs(w[i] - ref) <= 1e-10 * abs(ref), i

@@ -98,6 +98,37 @@ def test_sweep_of_one_wavenumber(gOn):
     assert (rel(Gs[0], G1) < TOL) if gOn else not Gs[0].any()
 
 
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_sweep_tune_then_sweep_and_graph(quadratic):
+    """The tile autotuner tries every (elements per tile, slots) candidate and keeps the
+    fastest; later sweeps and graphs use it and agree with the untuned sweep."""
+    mesh = M.icosphere(3, "quadratic") if quadratic else M.icosphere(4, "linear")
+    ks = np.linspace(1.0, 9.0, 7)
+    n, m = (3, 2) if quadratic else (3, 3)
+    print(f"[case] sweep tune quadratic={quadratic}", flush=True)
+    (F0, G0, C0), = run_sweep(mesh, ks, n, m)
+    dm = DeviceMesh(mesh)
+    N, nk = mesh.n_sources, len(ks)
+    stride = N * N
+    kb = DevBuf(8 * nk)
+    kb.upload(np.asarray(ks, dtype=np.float64))
+    F, G, C = DevBuf(16 * stride * nk), DevBuf(16 * stride * nk), DevBuf(16 * N)
+    ept, slots, ms = dm.sweep_tune(kb.ptr, nk, F.ptr, G.ptr, stride, C.ptr, n=n, m=m)
+    assert ept in (16, 32, 64, 128, 0) and slots in (0, 32) and ms > 0
+    dm.assemble_sweep(kb.ptr, nk, F.ptr, G.ptr, stride, C.ptr, n=n, m=m)
+    sync()
+    Fs, Gs, c = read(F, G, C, N, nk, stride)
+    for q in range(nk):
+        assert rel(Fs[q], F0[q]) < 1e-14 and rel(Gs[q], G0[q]) < 1e-14
+    assert rel(c, C0) < 1e-14
+    g = dm.sweep_graph(kb.ptr, nk, F.ptr, G.ptr, stride, C.ptr, n=n, m=m)
+    g.launch()
+    sync()
+    Fs, Gs, c = read(F, G, C, N, nk, stride)
+    assert all(rel(Fs[q], F0[q]) < 1e-14 and rel(Gs[q], G0[q]) < 1e-14 for q in range(nk))
+    g.close()
+
+
 def test_sweep_graph_replays():
     mesh = M.icosphere(3, "linear")
     ks = np.linspace(2.0, 9.0, 8)
@@ -222,5 +253,6 @@ def test_device_gemv_against_numpy(trans):
     yd = torch.empty(m if trans == "N" else n, dtype=torch.complex128, device=dev)
     solvers.zgemv(m, n, Ad.data_ptr(), m, xd.data_ptr(), yd.data_ptr(), trans=trans)
     torch.cuda.synchronize()
-    ref = {"N": A @ x, "T": A.T @ x, "C": A.conj().T @ x}[trans]
+    op = {"N": A, "T": A.T, "C": A.conj().T}[trans]
+    ref = op @ x
     assert rel(yd.cpu().numpy(), ref) < 1e-14

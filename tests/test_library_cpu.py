@@ -85,10 +85,28 @@ def test_one_based_indices_are_validated_too():
 
 def test_python_mirror_argument_errors():
     mesh = M.icosphere(1, "linear")
-    with pytest.raises(NotImplementedError):
-        assemble_parallel(mesh, 1.0, sparse=True)
+    # checked before any HIP call
+    with pytest.raises(ValueError):
+        assemble_parallel(mesh, 1.0, sparse=True, depth=3)
+    with pytest.raises(ValueError):
+        assemble_parallel(mesh, 1.0, mesh.sources[:, :5], sparse=True)
     with pytest.raises(ValueError):
         assemble_parallel(mesh, 1.0, np.zeros((2, 5)))
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_sparse_pattern_count_matches_oracle(depth):
+    # bie_sparse_count is host code (connected_sources), no GPU needed
+    import oracle
+    for mesh in (M.icosphere(2, "linear"), M.icosphere(1, "quadratic")):
+        hm = _lib.HostMesh(mesh)
+        nnz = ctypes.c_int64()
+        _lib.check(_lib.lib().bie_sparse_count(ctypes.byref(hm.struct), depth,
+                                               ctypes.byref(nnz)))
+        _, cols = oracle.connected_sources(mesh, depth)
+        assert nnz.value == sum(len(c) for c in cols)
+    assert _lib.lib().bie_sparse_count(ctypes.byref(hm.struct), 3,
+                                       ctypes.byref(nnz)) == _lib.BIE_EINVAL
 
 
 def test_apply_rejects_leading_dimension_beyond_buffer_offsets():
