@@ -87,6 +87,9 @@ SIGNATURES = {
     "bie_zgemm_device": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp, ctypes.c_int64, _dp,
                                          ctypes.c_int64, _dp]),
+    "bie_zgemm_device_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                 _dp, ctypes.c_int64, _dp, ctypes.c_int64, _dp,
+                                                 ctypes.c_int64, _dp]),
     "bie_host_cache_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64)] * 4),
     "bie_lu_factor_device": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, _dp, _dp, _dp]),
     "bie_lu_solve_device": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, _dp,

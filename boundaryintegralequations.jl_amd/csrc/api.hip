@@ -1693,6 +1693,14 @@ int bie_gmres_batched_device(int64_t n, int32_t n_systems, const double* dF, int
   });
 }
 
+int bie_zgemm_device_noalloc(int64_t M, int64_t nrhs, int64_t K, const double* dA,
+                             int64_t lda, const double* dX, int64_t ldx, double* dY,
+                             int64_t ldy, void* stream) {
+  return guarded([&] {
+    zgemm_device_noalloc(M, nrhs, K, dA, lda, dX, ldx, dY, ldy, static_cast<hipStream_t>(stream));
+  });
+}
+
 int bie_zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA,
                              int64_t lda, const double* dX, int64_t ldx, double* dY, int64_t ldy,
                              const int32_t* d_idx, int32_t n_idx, void* stream) {

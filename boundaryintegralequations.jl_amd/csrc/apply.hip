@@ -305,6 +305,44 @@ int current_device() {
 
 }  // namespace
 
+// Packed-X workspace of the allocation-free apply (zgemm_device_noalloc): a file-scope
+// device array, one copy on each GPU the library runs on, held until the process ends.
+// Its host-side name is not a device address: host code takes the address each device
+// reports for it (scratch_address) and passes that to kernels.
+static __device__ __align__(256) double2 scratch[kApplyScratchComplex];
+
+namespace {
+
+// Device address of `scratch` on device dev (the current device), looked up once per
+// device.
+double2* scratch_address(int dev) {
+  static std::mutex mu;
+  static double2* addr[16] = {};
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    BIE_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BIE_REQUIRE(p != nullptr, BIE_EHIP, "no device address for the apply scratch array");
+    addr[dev] = static_cast<double2*>(p);
+  }
+  return addr[dev];
+}
+
+}  // namespace
+
+void zgemm_device_noalloc(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                          const double* dX, int64_t ldx, double* dY, int64_t ldy,
+                          hipStream_t s) {
+  if (!zgemm_validate(M, nrhs, K, dA, lda, dX, ldx, dY, ldy)) return;
+  const int64_t ncp = (nrhs + 31) / 32 * 32;
+  BIE_REQUIRE(K <= kApplyScratchComplex / ncp, BIE_EINVAL,
+              "apply without allocation: K * ceil(nrhs/32)*32 = " + std::to_string(K * ncp) +
+                  " complex entries exceed the scratch array's " +
+                  std::to_string(kApplyScratchComplex) + " (16 MiB)");
+  const int dev = current_device();
+  zgemm_launch(M, nrhs, K, dA, lda, dX, ldx, dY, ldy, scratch_address(dev), dev, s);
+}
+
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s) {
   if (!zgemm_validate(M, nrhs, K, dA, lda, dX, ldx, dY, ldy)) return;
@@ -319,7 +357,7 @@ void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t 
     if (g_ws.bytes[dev] < need) {
       BIE_REQUIRE(!cap, BIE_EINVAL,
                   "apply: the workspace must grow, which a stream under graph capture cannot "
-                  "do (call once outside the capture first)");
+                  "do (call once outside the capture first, or use bie_zgemm_device_noalloc)");
       if (g_ws.p[dev]) {
         if (g_ws.done_valid[dev]) BIE_HIP_CHECK(hipEventSynchronize(g_ws.done[dev]));
         (void)hipFree(g_ws.p[dev]);

@@ -280,6 +280,13 @@ void taylor_expand_device(int n_terms, int64_t n, const double* dA, int64_t stri
                           const double* coef, double* dY, hipStream_t s);
 void zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                   const double* dX, int64_t ldx, double* dY, int64_t ldy, hipStream_t s);
+// Capacity of the allocation-free apply's scratch array, in complex128 entries (16 MiB).
+constexpr int64_t kApplyScratchComplex = int64_t(1) << 20;
+// As zgemm_device with the packed X in the device scratch array instead of an allocated
+// workspace: K * ceil(nrhs/32)*32 <= kApplyScratchComplex.
+void zgemm_device_noalloc(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
+                          const double* dX, int64_t ldx, double* dY, int64_t ldy,
+                          hipStream_t s);
 void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, const int* cols,
                                int ncols, hipStream_t s);
 int sweep_kb(int NP);  // slot groups (wavenumbers) per k block of the sweep kernel

@@ -318,6 +318,18 @@ int bie_assembly_vjp_complex_device(bie_dmesh* dm, double k_re, double k_im,
 int bie_zgemm_device(int64_t M, int64_t nrhs, int64_t K, const double* dA, int64_t lda,
                      const double* dX, int64_t ldx, double* dY, int64_t ldy, void* stream);
 
+/* As bie_zgemm_device, allocating nothing: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).
+ * The packed X goes to a 16 MiB device array that the library holds on every GPU it runs
+ * on until the process ends, instead of the growable workspace: K * ceil(nrhs/32)*32 must
+ * not exceed 1048576 complex entries (e.g. K = 10242 with up to 96 right-hand sides);
+ * larger calls return BIE_EINVAL.  Y is cleared by a fill kernel, never by a memset.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other streams, other
+ * threads, replays of graphs that captured it) race and give wrong results; order them. */
+int bie_zgemm_device_noalloc(int64_t M, int64_t nrhs, int64_t K, const double* dA,
+                             int64_t lda, const double* dX, int64_t ldx, double* dY,
+                             int64_t ldy, void* stream);
+
 /* Batched matrix-vector product for Krylov solves over a frequency sweep's operators:
  *   y_q += A_q x_q   for q = d_idx[0 .. n_idx)  (device int32 array, n_idx <= 65535)
  * A_q column-major M x K complex128 at dA + 2 q strideA doubles (leading dimension lda),
