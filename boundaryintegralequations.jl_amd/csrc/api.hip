@@ -534,10 +534,28 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       // zeros (a disabled output is zeroed whole above and never written).
       if (flags & BIE_F) launch_zero_columns(dF, M, sc.d_zero_cols, sc.zero_cols, s);
       if (flags & BIE_G) launch_zero_columns(dG, M, sc.d_zero_cols, sc.zero_cols, s);
+      // recorded mid-edge pairs: their values do not depend on the slot kernel, so they are
+      // computed on the handle's side stream while it runs (in a captured graph: in order)
+      const bool side = recorded && dm->n_mid_keys > 0 && !capturing(s);
+      if (side) {
+        if (!dm->side) {
+          BIE_HIP_CHECK(hipStreamCreateWithFlags(&dm->side, hipStreamNonBlocking));
+          BIE_HIP_CHECK(hipEventCreateWithFlags(&dm->side_fork, hipEventDisableTiming));
+          BIE_HIP_CHECK(hipEventCreateWithFlags(&dm->side_join, hipEventDisableTiming));
+        }
+        BIE_HIP_CHECK(hipEventRecord(dm->side_fork, s));  // after the zero fills and geometry
+        BIE_HIP_CHECK(hipStreamWaitEvent(dm->side, dm->side_fork, 0));
+        launch_mid_compute(a, dm->d_mid_keys, dm->n_mid_keys, dm->d_mid_vals, dm->side);
+        BIE_HIP_CHECK(hipEventRecord(dm->side_join, dm->side));
+      }
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[1], s));
       launch_assemble_slots(a, sa, fast, s);
       if (timed) BIE_HIP_CHECK(hipEventRecord(ev[2], s));
-      if (recorded) {
+      if (side) {
+        BIE_HIP_CHECK(hipStreamWaitEvent(s, dm->side_join, 0));
+        launch_mid_scatter(a, dm->n_mid_keys, dm->d_mid_vals, dm->d_mid_entries, dm->d_mid_off,
+                           dm->d_mid_refs, dm->n_mid_entries, s);
+      } else if (recorded) {
         launch_mid_recorded(a, dm->d_mid_keys, dm->n_mid_keys, dm->d_mid_vals, dm->d_mid_entries,
                             dm->d_mid_off, dm->d_mid_refs, dm->n_mid_entries, s);
       } else {
@@ -973,6 +991,12 @@ int bie_dmesh_destroy(bie_dmesh* dm) {
   if (dm->done) {  // the handle's last call may still be running
     (void)hipEventSynchronize(dm->done);
     (void)hipEventDestroy(dm->done);
+  }
+  if (dm->side) {
+    (void)hipStreamSynchronize(dm->side);
+    (void)hipStreamDestroy(dm->side);
+    (void)hipEventDestroy(dm->side_fork);
+    (void)hipEventDestroy(dm->side_join);
   }
   dfree(dm->d_coords);
   dfree(dm->d_topo);

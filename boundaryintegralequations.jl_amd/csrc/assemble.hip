@@ -1288,17 +1288,33 @@ int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage) {
   return nb < 1 ? 1 : nb;
 }
 
-void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
-                         double* vals, const unsigned long long* entries, const int* off,
-                         const int* refs, int64_t n_entries, hipStream_t s) {
+void launch_mid_compute(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
+                        double* vals, hipStream_t s) {
   if (n_pairs <= 0) return;
   int64_t blocks = (n_pairs + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_mid_compute, dim3((unsigned)blocks), dim3(256), 0, s, a, keys, n_pairs,
                      vals, a.ki != 0.0 ? 1 : 0);
   BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mid_scatter(const AssemblyArgs& a, int64_t n_pairs, const double* vals,
+                        const unsigned long long* entries, const int* off, const int* refs,
+                        int64_t n_entries, hipStream_t s) {
+  if (n_pairs <= 0 || n_entries <= 0 || !(a.flags & (BIE_F | BIE_G))) return;
+  const int64_t blocks = std::min<int64_t>((n_entries + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_mid_scatter, dim3((unsigned)blocks), dim3(256), 0, s, a, entries, off, refs,
+                     n_entries, vals);
+  BIE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
+                         double* vals, const unsigned long long* entries, const int* off,
+                         const int* refs, int64_t n_entries, hipStream_t s) {
+  if (n_pairs <= 0) return;
+  launch_mid_compute(a, keys, n_pairs, vals, s);
   if (n_entries <= 0 || !(a.flags & (BIE_F | BIE_G))) return;
-  blocks = (n_entries + 255) / 256;
+  int64_t blocks = (n_entries + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_mid_scatter, dim3((unsigned)blocks), dim3(256), 0, s, a, entries, off, refs,
                      n_entries, vals);

@@ -269,6 +269,13 @@ void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, h
 void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
                          double* vals, const unsigned long long* entries, const int* off,
                          const int* refs, int64_t n_entries, hipStream_t s);
+// The two halves of launch_mid_recorded: the pairs' values (reads the element tables, adds
+// C), then their addition into F and G (after the slot kernel's column writes).
+void launch_mid_compute(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
+                        double* vals, hipStream_t s);
+void launch_mid_scatter(const AssemblyArgs& a, int64_t n_pairs, const double* vals,
+                        const unsigned long long* entries, const int* off, const int* refs,
+                        int64_t n_entries, hipStream_t s);
 int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage);
 void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA, int64_t lda,
                           const double* dX, int64_t ldx, double* dY, int64_t ldy,
@@ -379,6 +386,9 @@ struct bie_dmesh {
   hipEvent_t done = nullptr;
   hipStream_t done_stream = nullptr;
   bool done_valid = false;
+  // side stream of the recorded mid-edge pairs' values, computed while the slot kernel runs
+  hipStream_t side = nullptr;
+  hipEvent_t side_fork = nullptr, side_join = nullptr;
   std::mutex mu;
 };
 
