@@ -30,7 +30,8 @@ class BieMesh(ctypes.Structure):
                 ("physics_topology", _dp), ("n_coordinates", ctypes.c_int64),
                 ("n_elements", ctypes.c_int64), ("n_sources", ctypes.c_int64),
                 ("geometry_kind", ctypes.c_int32), ("physics_kind", ctypes.c_int32),
-                ("index_base", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("index_base", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("physics_beta", ctypes.c_double)]
 
 # bie_mesh.flags: which n-ary hypot rule selection compares distances with
 BIE_MESH_SCALED_HYPOT = 1
@@ -250,4 +251,5 @@ class HostMesh:
         self.ptopo = np.asfortranarray(m.physics_topology, dtype=np.int64)
         self.struct = BieMesh(_ptr(self.coords), _ptr(self.topo), _ptr(self.sources),
                               _ptr(self.ptopo), m.n_coordinates, m.n_elements, m.n_sources,
-                              m.geometry_kind, m.physics_kind, 0, flags)
+                              m.geometry_kind, m.physics_kind, 0, flags,
+                              float(getattr(m, "physics_beta", 0.0)))

@@ -28,6 +28,16 @@ import numpy as np
 
 TRI3 = 3
 TRI6 = 6
+# discontinuous physics kinds (physics_kind only): low byte = nodes per element
+DTRI1 = 0x101  # DiscontinuousTriangularConstant
+DTRI3 = 0x103  # DiscontinuousTriangularLinear
+DTRI6 = 0x106  # DiscontinuousTriangularQuadratic
+_DISC_ORDERS = {"disctriconstant": DTRI1, "disctrilinear": DTRI3, "disctriquadratic": DTRI6}
+
+
+def physics_nodes(kind: int) -> int:
+    """Nodes per element of a physics kind (its low byte)."""
+    return int(kind) & 0xFF
 
 
 @dataclasses.dataclass
@@ -39,7 +49,8 @@ class Mesh3d:
     sources: np.ndarray            # (3, N) float64, collocation nodes
     physics_topology: np.ndarray   # (n_p, E) int64, 0-based, indexes ``sources``
     geometry_kind: int             # TRI3 | TRI6
-    physics_kind: int              # TRI3 | TRI6
+    physics_kind: int              # TRI3 | TRI6 | DTRI1 | DTRI3 | DTRI6
+    physics_beta: float = 0.0      # node offset of DTRI3 / DTRI6
 
     def __post_init__(self):
         self.coordinates = np.ascontiguousarray(self.coordinates, dtype=np.float64)
@@ -50,7 +61,7 @@ class Mesh3d:
             raise ValueError("coordinates and sources must be 3 x n")
         if self.topology.shape[0] != self.geometry_kind:
             raise ValueError("topology rows must equal the geometry node count")
-        if self.physics_topology.shape[0] != self.physics_kind:
+        if self.physics_topology.shape[0] != physics_nodes(self.physics_kind):
             raise ValueError("physics_topology rows must equal the physics node count")
         if self.topology.shape[1] != self.physics_topology.shape[1]:
             raise ValueError("topology and physics_topology disagree on element count")
@@ -135,7 +146,7 @@ def _project(p, radius=1.0):
 
 
 def icosphere(f: int, order: str = "linear", radius: float = 1.0,
-              physics: str = "geometry") -> Mesh3d:
+              physics: str = "geometry", beta_type="legendre") -> Mesh3d:
     """Geodesic sphere mesh (nodes 10 f^2 + 2 linear, 40 f^2 + 2 quadratic)."""
     if f < 1:
         raise ValueError("f must be >= 1")
@@ -169,20 +180,98 @@ def icosphere(f: int, order: str = "linear", radius: float = 1.0,
         gk = TRI6
     else:
         raise ValueError("order must be 'linear' or 'quadratic'")
-    return _with_physics(coords.T.copy(), topo, gk, physics)
+    return _with_physics(coords.T.copy(), topo, gk, physics, beta_type)
 
 
 def ellipsoid(f: int, axes=(1.0, 1.0, 1.0), order: str = "linear",
-              physics: str = "geometry") -> Mesh3d:
+              physics: str = "geometry", beta_type="legendre") -> Mesh3d:
     """Icosphere scaled by semi-axes (a, b, c); orientation is preserved."""
     m = icosphere(f, order, 1.0, physics="geometry")
     s = np.asarray(axes, dtype=np.float64).reshape(3, 1)
-    return _with_physics(m.coordinates * s, m.topology, m.geometry_kind, physics)
+    return _with_physics(m.coordinates * s, m.topology, m.geometry_kind, physics, beta_type)
 
 
-def _with_physics(coords, topo, gk, physics):
+def discontinuous_beta(physics: str, beta_type="legendre") -> float:
+    """Node offset beta of a discontinuous triangle (get_beta_tri_linear /
+    get_beta_tri_quadratic, src/3d/mesh.jl:182-214): half the quadrilateral value, which is
+    1 + (the smallest Gauss-Legendre node of 2 or 3 points) for "legendre", 1/2 or 1/3 for
+    "equidistant", and the number itself when one is given (so a number is halved too)."""
+    if physics == "disctriconstant":
+        return 0.0
+    if physics not in ("disctrilinear", "disctriquadratic"):
+        raise ValueError("not a discontinuous triangular physics order")
+    lin = physics == "disctrilinear"
+    if isinstance(beta_type, (int, float)) and not isinstance(beta_type, bool):
+        quad = float(beta_type)
+    elif beta_type == "legendre":
+        # gausslegendre(2)[1][1] = -1/sqrt(3), gausslegendre(3)[1][1] = -sqrt(3/5)
+        quad = 1.0 + (-(1.0 / np.sqrt(3.0)) if lin else -np.sqrt(3.0 / 5.0))
+    elif beta_type == "equidistant":
+        quad = 0.5 if lin else 1.0 / 3.0
+    else:
+        raise ValueError("beta_type must be 'legendre', 'equidistant' or a number")
+    return quad / 2
+
+
+def geometry_basis(kind: int, u, v) -> np.ndarray:
+    """Geometry basis (kind x len(u)) of triangularLinear / triangularQuadratic
+    (src/3d/surface_functions.jl:191-213)."""
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    if kind == TRI3:
+        return np.stack([(1.0 - u) - v, u, v])
+    w = (1.0 - v) - u
+    return np.stack([w * ((1.0 - 2 * v) - 2 * u), u * (2 * u - 1.0), v * (2 * v - 1.0),
+                     (4 * u) * w, (4 * u) * v, (4 * v) * w])
+
+
+def discontinuous_nodes(kind: int, beta: float):
+    """Local (u, v) of the nodes of a discontinuous triangle (get_nodal_nodes_u/_v,
+    src/3d/SurfaceFunction.jl:429-438)."""
+    b = float(beta)
+    if kind == DTRI1:
+        return np.array([1 / 3]), np.array([1 / 3])
+    if kind == DTRI3:
+        return np.array([b, 1 - 2 * b, b]), np.array([b, b, 1 - 2 * b])
+    if kind == DTRI6:
+        h = (1 - b) / 2
+        return (np.array([b, 1 - 2 * b, b, h, h, b]), np.array([b, b, 1 - 2 * b, b, h, h]))
+    raise ValueError("not a discontinuous physics kind")
+
+
+def discontinuous_sources(coords, topo, gk, kind, beta):
+    """compute_sources (src/3d/mesh.jl:116-148): every element gets its own physics nodes,
+    numbered element by element, at the geometry interpolated at the local nodes."""
+    u, v = discontinuous_nodes(kind, beta)
+    N = geometry_basis(gk, u, v)                             # (gk, np)
+    X = np.asarray(coords)[:, np.asarray(topo)]              # (3, gk, E)
+    src = np.einsum("dke,kp->dpe", X, N)                     # (3, np, E)
+    npk, E = N.shape[1], X.shape[2]
+    sources = src.reshape(3, npk * E, order="F")
+    ptop = np.arange(npk * E, dtype=np.int64).reshape(npk, E, order="F")
+    return np.ascontiguousarray(sources), ptop
+
+
+def with_discontinuous_physics(mesh: "Mesh3d", physics: str, beta_type="legendre") -> "Mesh3d":
+    """The same geometry with discontinuous triangular physics ("disctriconstant",
+    "disctrilinear", "disctriquadratic"; set_physics_element + compute_sources,
+    src/3d/mesh.jl:116-180)."""
+    kind = _DISC_ORDERS.get(physics)
+    if kind is None:
+        raise ValueError("physics must be disctriconstant, disctrilinear or disctriquadratic")
+    beta = discontinuous_beta(physics, beta_type)
+    sources, ptop = discontinuous_sources(mesh.coordinates, mesh.topology, mesh.geometry_kind,
+                                          kind, beta)
+    return Mesh3d(mesh.coordinates, mesh.topology, sources, ptop, mesh.geometry_kind, kind,
+                  beta)
+
+
+def _with_physics(coords, topo, gk, physics, beta_type="legendre"):
     if physics == "geometry":
         return Mesh3d(coords, topo, coords.copy(), topo.copy(), gk, gk)
+    if physics in _DISC_ORDERS:
+        return with_discontinuous_physics(Mesh3d(coords, topo, coords.copy(), topo.copy(), gk,
+                                                 gk), physics, beta_type)
     if physics == "linear":
         # read_comsol.jl:87-93: corner connectivity; sources are the used corner nodes.
         ptop = topo[:3, :]
@@ -190,7 +279,7 @@ def _with_physics(coords, topo, gk, physics):
         remap = -np.ones(coords.shape[1], dtype=np.int64)
         remap[used] = np.arange(used.size)
         return Mesh3d(coords, topo, coords[:, used].copy(), remap[ptop], gk, TRI3)
-    raise ValueError("physics must be 'geometry' or 'linear'")
+    raise ValueError("physics must be 'geometry', 'linear' or a discontinuous triangular order")
 
 
 # ---------------------------------------------------------------------------------------
@@ -233,9 +322,11 @@ def _remove_unused(topo):
 
 
 def load_comsol(path: str, geometry_order: str = "quadratic",
-                physics_order: str = "geometry") -> Mesh3d:
+                physics_order: str = "geometry", beta_type="legendre") -> Mesh3d:
     """Restatement of ``load3dTriangularComsolMesh`` (read_comsol.jl:61-119) for
-    geometry in {linear, quadratic} and physics in {geometry, linear}."""
+    geometry in {linear, quadratic} and physics in {geometry, linear, disctriconstant,
+    disctrilinear, disctriquadratic} (the discontinuous ones: compute_sources with the
+    node offset of beta_type, src/3d/mesh.jl:116-214)."""
     coords0, topo0 = read_comsol(path)
     if geometry_order == "quadratic":
         if topo0.shape[0] != 6:
@@ -259,7 +350,11 @@ def load_comsol(path: str, geometry_order: str = "quadratic",
             raise ValueError("corner nodes are not numbered first; the reference's "
                              "linear-physics indexing would be inconsistent")
         return Mesh3d(coords, topo, coords[:, used_p].copy(), ptop.copy(), gk, TRI3)
-    raise ValueError("physics_order must be 'geometry' or 'linear'")
+    if physics_order in _DISC_ORDERS:
+        base = Mesh3d(coords, topo, coords.copy(), topo.copy(), gk, gk)
+        return with_discontinuous_physics(base, physics_order, beta_type)
+    raise ValueError("physics_order must be geometry, linear, disctriconstant, disctrilinear "
+                     "or disctriquadratic")
 
 
 def _read_obj(path):
@@ -456,24 +551,35 @@ def read_mesh_file(path: str):
 
 
 def load3d_triangular_mesh(path: str, geometry_order: str = "linear",
-                           physics_order: str = "geometry") -> Mesh3d:
+                           physics_order: str = "geometry", beta_type="legendre") -> Mesh3d:
     """Restatement of ``load3dTriangularMesh`` (src/3d/meshing.jl:48-106) for linear
-    geometry (the only order the reference accepts there) with geometry or linear physics:
-    unused nodes removed, collocation at the used geometry nodes."""
+    geometry (the only order the reference accepts there): unused nodes removed; geometry or
+    linear physics collocate at the used geometry nodes, the discontinuous triangular orders
+    at their element nodes (compute_sources, src/3d/mesh.jl:116-148)."""
     if geometry_order != "linear":
         raise ValueError("Currently only linear element supported")
-    if physics_order not in ("geometry", "linear"):
-        raise NotImplementedError("discontinuous physics is not part of this library")
+    if physics_order not in ("geometry", "linear") and physics_order not in _DISC_ORDERS:
+        raise ValueError("physics_order must be geometry, linear, disctriconstant, "
+                         "disctrilinear or disctriquadratic")
     coords0, topo0 = read_mesh_file(path)
     topo, used = _remove_unused(topo0)
     coords = coords0[:, used]
-    return Mesh3d(coords, topo, coords[:, np.unique(topo)].copy(), topo.copy(), TRI3, TRI3)
+    base = Mesh3d(coords, topo, coords[:, np.unique(topo)].copy(), topo.copy(), TRI3, TRI3)
+    if physics_order in _DISC_ORDERS:
+        return with_discontinuous_physics(base, physics_order, beta_type)
+    return base
 
 
 def with_coordinates(mesh: Mesh3d, coords) -> Mesh3d:
     """The same mesh with new coordinates; collocation nodes that are coordinates move
-    with them (physics = geometry, or the corner nodes for linear physics)."""
+    with them (physics = geometry, or the corner nodes for linear physics), discontinuous
+    physics nodes are interpolated again."""
     coords = np.asarray(coords, dtype=np.float64)
+    if mesh.physics_kind in (DTRI1, DTRI3, DTRI6):
+        src, ptop = discontinuous_sources(coords, mesh.topology, mesh.geometry_kind,
+                                          mesh.physics_kind, mesh.physics_beta)
+        return Mesh3d(coords, mesh.topology, src, ptop, mesh.geometry_kind, mesh.physics_kind,
+                      mesh.physics_beta)
     lut = {tuple(c): j for j, c in enumerate(mesh.coordinates.T)}
     idx = np.array([lut[tuple(s)] for s in mesh.sources.T], dtype=np.int64)
     return Mesh3d(coords, mesh.topology, coords[:, idx].copy(), mesh.physics_topology,

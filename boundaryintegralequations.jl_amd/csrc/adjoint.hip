@@ -263,7 +263,10 @@ k_dfdk_batch(const DfdkInst* __restrict__ inst, const int4* __restrict__ tiles, 
 void launch_dfdk_batch(const DfdkInst* d_inst, const int4* d_tiles, int n_tiles, int ech, int NP,
                        hipStream_t s) {
   if (n_tiles <= 0) return;
-  if (NP == 3)
+  if (NP == 1)
+    hipLaunchKernelGGL(k_dfdk_batch<1>, dim3((unsigned)n_tiles), dim3(256), 0, s, d_inst, d_tiles,
+                       ech);
+  else if (NP == 3)
     hipLaunchKernelGGL(k_dfdk_batch<3>, dim3((unsigned)n_tiles), dim3(256), 0, s, d_inst, d_tiles,
                        ech);
   else

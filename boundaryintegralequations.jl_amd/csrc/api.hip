@@ -66,10 +66,16 @@ void validate_mesh(const bie_mesh* m) {
   BIE_REQUIRE(m != nullptr, BIE_EINVAL, "mesh is NULL");
   BIE_REQUIRE(m->geometry_kind == BIE_TRI3 || m->geometry_kind == BIE_TRI6, BIE_EUNSUPPORTED,
               "geometry_kind must be BIE_TRI3 or BIE_TRI6");
-  BIE_REQUIRE(m->physics_kind == BIE_TRI3 || m->physics_kind == BIE_TRI6, BIE_EUNSUPPORTED,
-              "physics_kind must be BIE_TRI3 or BIE_TRI6");
-  BIE_REQUIRE(!(m->geometry_kind == BIE_TRI3 && m->physics_kind == BIE_TRI6), BIE_EUNSUPPORTED,
+  const int pkd = m->physics_kind;
+  BIE_REQUIRE(pkd == BIE_TRI3 || pkd == BIE_TRI6 || pkd == BIE_DTRI1 || pkd == BIE_DTRI3 ||
+                  pkd == BIE_DTRI6,
+              BIE_EUNSUPPORTED,
+              "physics_kind must be BIE_TRI3, BIE_TRI6, BIE_DTRI1, BIE_DTRI3 or BIE_DTRI6");
+  BIE_REQUIRE(!(m->geometry_kind == BIE_TRI3 && pkd == BIE_TRI6), BIE_EUNSUPPORTED,
               "quadratic physics on linear geometry is not a reference configuration");
+  if (pkd == BIE_DTRI3 || pkd == BIE_DTRI6)
+    BIE_REQUIRE(std::isfinite(m->physics_beta) && 1.0 - 3.0 * m->physics_beta != 0.0,
+                BIE_EINVAL, "physics_beta must be finite and differ from 1/3");
   BIE_REQUIRE(m->index_base == 0 || m->index_base == 1, BIE_EINVAL, "index_base must be 0 or 1");
   BIE_REQUIRE((m->flags & ~BIE_MESH_SCALED_HYPOT) == 0, BIE_EINVAL, "unknown mesh flag bits");
   BIE_REQUIRE(m->n_elements >= 0 && m->n_coordinates >= 0 && m->n_sources >= 0, BIE_EINVAL,
@@ -99,7 +105,7 @@ std::vector<int> convert_topology(const int64_t* t, int rows, int64_t E, int64_t
 
 void ensure_rules(bie_dmesh* dm, int n, int m, hipStream_t s) {
   if (dm->cn == n && dm->cm == m && dm->d_elem) return;
-  RuleSet rs = build_ruleset(dm->gk, dm->pk, n, m);
+  RuleSet rs = build_ruleset(dm->gk, dm->pkind, dm->beta, n, m);
   if (dm->d_elem) (void)hipDeviceSynchronize();  // in-flight kernels may read the old tables
   dfree(dm->d_pts);
   dfree(dm->d_phi);
@@ -509,9 +515,11 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       const int64_t rowblocks = (M + 63) / 64;
       const bool fast = fast_trig_ok(dm, kr, ki, target_radius);
       const bool qc_inst = fast && (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16);
-      const int occ = qc_inst ? slot_occupancy() : 1;
-      const int split = qc_inst && occ == 1 ? slot_split() : 1;
-      const int vpw = qc_inst && occ == 1 && split == 1 ? slot_elems_per_wave() : 1;
+      // the occupancy, split and two-element variants have no one-node physics instance
+      const bool tuned = qc_inst && dm->pk != 1;
+      const int occ = tuned ? slot_occupancy() : 1;
+      const int split = tuned && occ == 1 ? slot_split() : 1;
+      const int vpw = tuned && occ == 1 && split == 1 ? slot_elems_per_wave() : 1;
       const int min_segments = choose_segment_count(dm->device, rowblocks, dm->E, occ);
       const bie_dmesh::Sched& sc =
           ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0], occ, split),
@@ -519,8 +527,9 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       // collocation on quadratic physics: after the first call the mid-edge pairs come
       // from the recorded sorted list instead of the slot kernel's queue
       const bool colloc = d_targets == dm->d_sources && M == dm->N;
-      const bool recorded = dm->pk == 6 && colloc && dm->n_mid_keys >= 0;
-      if (dm->pk == 6 && !recorded) {
+      const bool mids = uses_mid_rules(dm->gk, dm->pk);
+      const bool recorded = mids && colloc && dm->n_mid_keys >= 0;
+      if (mids && !recorded) {
         ensure_mid_queue(dm, M * dm->E);
         launch_fill_zero_u32(dm->d_mid_count, s);
       } else if (!dm->d_mid_count) {
@@ -560,7 +569,7 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
                             dm->d_mid_off, dm->d_mid_refs, dm->n_mid_entries, s);
       } else {
         launch_assemble_mid(a, sa, s);
-        if (dm->pk == 6 && colloc) record_mid_pairs(dm, s);
+        if (mids && colloc) record_mid_pairs(dm, s);
       }
     }
   } else if (timed) {
@@ -610,7 +619,7 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
                           : 1;
   const bie_dmesh::Sched& sc = ensure_schedule(dm, 1, S, min_seg, s);
   if (!dm->d_mid_count) dm->d_mid_count = dalloc<unsigned>(1);
-  if (dm->pk == 6) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
+  if (uses_mid_rules(dm->gk, dm->pk)) ensure_mid_queue(dm, std::max<int64_t>(1, M * dm->E));
   const int KBk = sweep_block_size(dm->pk, (flags & BIE_G) != 0);
   const int nkb = (nk + KBk - 1) / KBk;
   if (dm->block_uniform_cap < std::max(nkb, 1)) {
@@ -746,7 +755,13 @@ void batch_validate(bie_batch* b, const double* k_re, const double* k_im, uint32
 // What the batch's kernels need besides the instance table.
 struct BatchPlan {
   bool cplx = false;
+  bool fast = true;  // every instance: real k with |k| r below the fast sincos bound
+  bool gon = true;
   int64_t max_work = 1;
+  // compile-time rule length of the fast kernel instance (0: the generic instance)
+  int qc(int len0) const {
+    return fast && !cplx && (len0 == 4 || len0 == 6 || len0 == 16) ? len0 : 0;
+  }
 };
 
 // Host half of a batched assembly (under the batch and mesh locks): every per-mesh cache,
@@ -758,7 +773,7 @@ BatchPlan batch_fill(bie_batch* b, const double* k_re, const double* k_im, uint3
   for (bie_dmesh* dm : b->meshes) {
     ensure_rules(dm, b->n, b->m, s);
     if (!dm->geom_valid) refresh_geometry(dm, s);
-    if (dm->pk == 6)
+    if (uses_mid_rules(dm->gk, dm->pk))
       ensure_mid_queue(dm, std::max<int64_t>(1, dm->N * dm->E));
     else if (!dm->d_mid_count)
       dm->d_mid_count = dalloc<unsigned>(1);
@@ -766,10 +781,12 @@ BatchPlan batch_fill(bie_batch* b, const double* k_re, const double* k_im, uint3
   batch_configure(b, s);
   BIE_HIP_CHECK(hipEventSynchronize(b->staged));  // the previous upload has left h_inst
   BatchPlan plan;
+  plan.gon = (flags & BIE_G) != 0;
   for (int ii = 0; ii < n_inst; ++ii) {
     bie_dmesh* dm = b->meshes[ii];
     const double ki = k_im ? k_im[ii] : 0.0;
     plan.cplx = plan.cplx || ki != 0.0;
+    plan.fast = plan.fast && fast_trig_ok(dm, k_re[ii], ki, dm->source_radius);
     const bie_dmesh::Sched& sc = dm->sched[2];
     BatchInst& bi = b->h_inst[ii];
     bi.a = make_args(dm, dm->d_sources, dm->N, k_re[ii], ki, flags, dF[ii], dG[ii], dC[ii]);
@@ -796,12 +813,14 @@ void batch_kernels(bie_batch* b, const BatchInst* d_inst, const int4* d_tiles,
     int cus = 0;
     BIE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
     const int nstage = stage_doubles(d0->rs.len[0]);
-    const int per_cu = batch_blocks_per_cu(b->pk, plan.cplx, b->slots_used, nphi, nstage);
+    const int qc = plan.qc(d0->rs.len[0]);
+    const int per_cu =
+        batch_blocks_per_cu(b->pk, plan.cplx, plan.gon, qc, b->slots_used, nphi, nstage);
     const int grid = (int)std::min<int64_t>((int64_t)std::max(cus, 1) * per_cu, b->n_tiles);
     launch_assemble_batch(d_inst, d_tiles, b->n_tiles, d_counter, d0->d_phi, nphi, nstage,
-                          b->slots_used, b->pk, plan.cplx, grid, s);
+                          b->slots_used, b->pk, plan.cplx, plan.gon, qc, grid, s);
   }
-  if (b->pk == 6) launch_assemble_mid_batch(d_inst, n_inst, plan.cplx, s);
+  if (uses_mid_rules(b->gk, b->pk)) launch_assemble_mid_batch(d_inst, n_inst, plan.cplx, s);
 }
 
 void batch_assemble_impl(bie_batch* b, const double* k_re, const double* k_im, uint32_t flags,
@@ -916,7 +935,7 @@ extern "C" {
 
 const char* bie_last_error(void) { return g_last_error.c_str(); }
 
-int bie_version(void) { return 1; }
+int bie_version(void) { return 2; }
 
 int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
   return guarded([&] {
@@ -928,7 +947,8 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     std::vector<int> topo = convert_topology(mesh->topology, mesh->geometry_kind,
                                              mesh->n_elements, base, mesh->n_coordinates,
                                              "topology");
-    std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
+    std::vector<int> ptopo = convert_topology(mesh->physics_topology,
+                                              physics_nodes(mesh->physics_kind),
                                               mesh->n_elements, base, mesh->n_sources,
                                               "physics_topology");
     BIE_REQUIRE(device >= 0, BIE_EINVAL, "device ordinal out of range");
@@ -944,7 +964,11 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     dm->device = device;
     dm->hyp = (mesh->flags & BIE_MESH_SCALED_HYPOT) ? 1 : 0;
     dm->gk = mesh->geometry_kind;
-    dm->pk = mesh->physics_kind;
+    dm->pk = physics_nodes(mesh->physics_kind);
+    dm->pkind = mesh->physics_kind;
+    dm->beta = (mesh->physics_kind == BIE_DTRI3 || mesh->physics_kind == BIE_DTRI6)
+                   ? mesh->physics_beta
+                   : 0.0;
     dm->Nc = mesh->n_coordinates;
     dm->E = mesh->n_elements;
     dm->N = mesh->n_sources;
@@ -1109,11 +1133,15 @@ std::vector<unsigned char> mesh_key(const bie_mesh* m) {
   const int64_t scal[7] = {m->n_coordinates, m->n_elements, m->n_sources, m->geometry_kind,
                            m->physics_kind, m->index_base, m->flags};
   append_bytes(key, scal, sizeof(scal));
+  const double beta = (m->physics_kind == BIE_DTRI3 || m->physics_kind == BIE_DTRI6)
+                          ? m->physics_beta
+                          : 0.0;
+  append_bytes(key, &beta, sizeof(beta));
   append_bytes(key, m->coordinates, sizeof(double) * 3 * (size_t)m->n_coordinates);
   append_bytes(key, m->topology, sizeof(int64_t) * (size_t)m->geometry_kind * m->n_elements);
   append_bytes(key, m->sources, sizeof(double) * 3 * (size_t)m->n_sources);
   append_bytes(key, m->physics_topology,
-               sizeof(int64_t) * (size_t)m->physics_kind * m->n_elements);
+               sizeof(int64_t) * (size_t)physics_nodes(m->physics_kind) * m->n_elements);
   return key;
 }
 
@@ -1319,6 +1347,8 @@ void sparse_destroy_impl(bie_sparse* sp) {
 bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
   BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
+  BIE_REQUIRE(!physics_discontinuous(dm->pkind), BIE_EUNSUPPORTED,
+              "sparse near field: continuous physics only");
   BIE_REQUIRE(!std::isinf(offset), BIE_EINVAL, "offset must be finite (NaN: the default 0)");
   const double offr = std::isnan(offset) ? 0.0 : offset;
   BIE_HIP_CHECK(hipSetDevice(dm->device));
@@ -1472,6 +1502,8 @@ int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz) {
   return guarded([&] {
     validate_mesh(mesh);
     BIE_REQUIRE(nnz != nullptr, BIE_EINVAL, "nnz is NULL");
+    BIE_REQUIRE(!physics_discontinuous(mesh->physics_kind), BIE_EUNSUPPORTED,
+                "sparse near field: continuous physics only");
     std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
                                               mesh->n_elements, mesh->index_base,
                                               mesh->n_sources, "physics_topology");
@@ -1578,7 +1610,8 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
     std::vector<int> topo = convert_topology(mesh->topology, mesh->geometry_kind,
                                              mesh->n_elements, base, mesh->n_coordinates,
                                              "topology");
-    std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
+    const int np = physics_nodes(mesh->physics_kind);
+    std::vector<int> ptopo = convert_topology(mesh->physics_topology, np,
                                               mesh->n_elements, base, mesh->n_sources,
                                               "physics_topology");
     const int64_t E = mesh->n_elements;
@@ -1590,15 +1623,15 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
           acc += mesh->coordinates[3 * (size_t)topo[e * mesh->geometry_kind + k] + d];
         cent[3 * e + d] = acc / 3.0;
       }
-    SlotSchedule sc = build_slot_schedule(ptopo.data(), mesh->physics_kind, (int)E,
+    SlotSchedule sc = build_slot_schedule(ptopo.data(), np, (int)E,
                                           (int)mesh->n_sources, cent.data(), kSlotWaves,
                                           kSlotCount, min_segments);
-    validate_slot_schedule(sc, ptopo.data(), mesh->physics_kind, (int)E, (int)mesh->n_sources);
+    validate_slot_schedule(sc, ptopo.data(), np, (int)E, (int)mesh->n_sources);
     // the schedule of two elements per wave and step as well
-    SlotSchedule sc2 = build_slot_schedule(ptopo.data(), mesh->physics_kind, (int)E,
+    SlotSchedule sc2 = build_slot_schedule(ptopo.data(), np, (int)E,
                                            (int)mesh->n_sources, cent.data(), 2 * kSlotWaves,
                                            kSlotCount, min_segments);
-    validate_slot_schedule(sc2, ptopo.data(), mesh->physics_kind, (int)E, (int)mesh->n_sources);
+    validate_slot_schedule(sc2, ptopo.data(), np, (int)E, (int)mesh->n_sources);
     if (stats) {
       stats[0] = sc.n_segments;
       stats[1] = sc.n_steps;
@@ -1620,6 +1653,8 @@ void assembly_vjp_impl(bie_dmesh* dm, double k, double ki, bool cplx, const doub
   BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
   BIE_REQUIRE(d_alpha && d_p && d_grad_coords, BIE_EINVAL, "NULL argument");
   BIE_REQUIRE(std::isfinite(k) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+  BIE_REQUIRE(!physics_discontinuous(dm->pkind), BIE_EUNSUPPORTED,
+              "assembly adjoint: continuous physics only");
   if (!d_targets) {
     d_targets = dm->d_sources;
     BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
@@ -1918,7 +1953,7 @@ int bie_batch_graph_create(bie_batch* b, const double* k_re, const double* k_im,
       // calls with more targets)
       for (int ii = 0; ii < n_inst; ++ii) {
         bie_dmesh* dm = b->meshes[ii];
-        if (dm->pk != 6) continue;
+        if (!uses_mid_rules(dm->gk, dm->pk)) continue;
         const int64_t cap = std::max<int64_t>(1, dm->N * dm->E);
         int2* q = dalloc<int2>((size_t)cap);
         g->owned.push_back(q);
@@ -1978,17 +2013,19 @@ int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_
       BIE_REQUIRE(meshes[ii] != nullptr, BIE_EINVAL, "batch mesh is NULL");
       BIE_REQUIRE(meshes[ii]->device == meshes[0]->device, BIE_EINVAL,
                   "batch meshes live on different devices");
-      BIE_REQUIRE(meshes[ii]->gk == meshes[0]->gk && meshes[ii]->pk == meshes[0]->pk,
+      BIE_REQUIRE(meshes[ii]->gk == meshes[0]->gk && meshes[ii]->pkind == meshes[0]->pkind &&
+                      meshes[ii]->beta == meshes[0]->beta,
                   BIE_EUNSUPPORTED, "batch meshes must share geometry and physics kinds");
     }
-    if (meshes[0]->pk == 6) {  // one mid-edge queue per mesh: each may appear once
+    // one mid-edge queue per mesh: each may appear once
+    if (uses_mid_rules(meshes[0]->gk, meshes[0]->pk)) {
       std::vector<bie_dmesh*> ms(meshes, meshes + n_inst);
       std::sort(ms.begin(), ms.end());
       BIE_REQUIRE(std::adjacent_find(ms.begin(), ms.end()) == ms.end(), BIE_EINVAL,
                   "a quadratic-physics mesh may appear only once in a batch");
     }
     {
-      RuleSet rs = build_ruleset(meshes[0]->gk, meshes[0]->pk, n, m);
+      RuleSet rs = build_ruleset(meshes[0]->gk, meshes[0]->pkind, meshes[0]->beta, n, m);
       BIE_REQUIRE(sizeof(double) * (size_t)rs.R * rs.Qmax * rs.pk <= 24 * 1024, BIE_EUNSUPPORTED,
                   "batch supports quadrature orders up to n*m' = 36");
     }
@@ -1997,6 +2034,8 @@ int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_
     b->device = meshes[0]->device;
     b->gk = meshes[0]->gk;
     b->pk = meshes[0]->pk;
+    b->pkind = meshes[0]->pkind;
+    b->beta = meshes[0]->beta;
     b->n = n;
     b->m = m;
     b->meshes.assign(meshes, meshes + n_inst);

@@ -843,8 +843,10 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
 // share a segment's element tables in its L2; a workgroup whose queue is empty takes tiles
 // from the following queues.  Every queue's counter only grows, so each workgroup leaves
 // after finding all queues exhausted once.
-template <int NP, bool CPLX, int W>
-__global__ void __launch_bounds__(64 * W)
+// QC > 0: the real-k instance for vertex rules of exactly QC points with the e^{ikr} of
+// sincos_tab_fast (every instance's |k| r below its bound), as k_assemble_slots.
+template <int NP, bool CPLX, int W, bool GON = true, int QC = 0>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2)))
 k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ tiles,
                  int n_tiles, unsigned* __restrict__ counters, const double* __restrict__ phi,
                  int nphi, int nstage, int S) {
@@ -877,7 +879,10 @@ k_assemble_batch(const BatchInst* __restrict__ inst, const int4* __restrict__ ti
       const int ii = __builtin_amdgcn_readfirstlane(td.x);
       const int rb = __builtin_amdgcn_readfirstlane(td.y);
       const int seg = __builtin_amdgcn_readfirstlane(td.z);
-      if (nstage)
+      if constexpr (QC > 0)
+        slot_tile<NP, false, W, true, GON, QC, true>(inst[ii].a, inst[ii].sa, rb, seg, acc,
+                                                     s_phi, s_trig, s_rows, s_q, s_qn);
+      else if (nstage)
         slot_tile<NP, CPLX, W, true>(inst[ii].a, inst[ii].sa, rb, seg, acc, s_phi, s_trig,
                                      s_rows, s_q, s_qn);
       else
@@ -1095,7 +1100,9 @@ void launch_select(const double* targets, int64_t M, const double* sel, int NP, 
   int64_t blocks = (M * (int64_t)E + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   dim3 grid((unsigned)blocks);
-  if (NP == 3)
+  if (NP == 1)
+    hipLaunchKernelGGL(k_select<1>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
+  else if (NP == 3)
     hipLaunchKernelGGL(k_select<3>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
   else
     hipLaunchKernelGGL(k_select<6>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
@@ -1112,7 +1119,7 @@ void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipSt
 
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;
-  BIE_REQUIRE(sa.W == kSlotWaves || (sa.W == 2 * kSlotWaves && fast && a.ki == 0.0 &&
+  BIE_REQUIRE(sa.W == kSlotWaves || (sa.W == 2 * kSlotWaves && fast && a.ki == 0.0 && a.NP != 1 &&
                                       sa.occupancy == 1 && sa.split == 1 &&
                                       (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16)),
               BIE_EINVAL, "slot schedule built for another wave count");
@@ -1120,7 +1127,8 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
   BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
-  const int split = (sa.split == 2 && fast && a.ki == 0.0) ? 2 : 1;
+  // (one-node physics has no split instance: its pairs are short)
+  const int split = (sa.split == 2 && fast && a.ki == 0.0 && a.NP != 1) ? 2 : 1;
   const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + phi_pad(a.R * a.Qmax * a.NP) +
                                          kTrigDoubles +
                                          (split == 2 ? split_stage_doubles(a.len[0], 2)
@@ -1161,40 +1169,57 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
   auto fast_go = [&](auto np, auto g) -> bool {
     constexpr int NPc = decltype(np)::value;
     constexpr bool Gc = decltype(g)::value;
-    if (occ2) {
+    if constexpr (NPc == 1) {  // one-node physics: the plain instance only
       switch (qc) {
-        case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 2>); return true;
-        case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 2>); return true;
-        case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 2>); return true;
+        case 4: go(k_assemble_slots<1, false, W, Gc, 4, true>); return true;
+        case 6: go(k_assemble_slots<1, false, W, Gc, 6, true>); return true;
+        case 16: go(k_assemble_slots<1, false, W, Gc, 16, true>); return true;
         default: return false;
       }
-    }
-    if (sa.W == 2 * W) {
+    } else {
+      if (occ2) {
+        switch (qc) {
+          case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 2>); return true;
+          case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 2>); return true;
+          case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 2>); return true;
+          default: return false;
+        }
+      }
+      if (sa.W == 2 * W) {
+        switch (qc) {
+          case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 1, 2>); return true;
+          case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 1, 2>); return true;
+          case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 1, 2>); return true;
+          default: return false;
+        }
+      }
       switch (qc) {
-        case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 1, 2>); return true;
-        case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 1, 2>); return true;
-        case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 1, 2>); return true;
+        case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true>); return true;
+        case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true>); return true;
+        case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true>); return true;
         default: return false;
       }
-    }
-    switch (qc) {
-      case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true>); return true;
-      case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true>); return true;
-      case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true>); return true;
-      default: return false;
     }
   };
+  using T1 = std::integral_constant<int, 1>;
   using T3 = std::integral_constant<int, 3>;
   using T6 = std::integral_constant<int, 6>;
   using GT = std::true_type;
   using GF = std::false_type;
   bool done = false;
-  if (a.NP == 3)
+  if (a.NP == 1)
+    done = gon ? fast_go(T1{}, GT{}) : fast_go(T1{}, GF{});
+  else if (a.NP == 3)
     done = gon ? fast_go(T3{}, GT{}) : fast_go(T3{}, GF{});
   else
     done = gon ? fast_go(T6{}, GT{}) : fast_go(T6{}, GF{});
   if (!done) {
-    if (a.NP == 3) {
+    if (a.NP == 1) {
+      if (cplx)
+        gon ? go(k_assemble_slots<1, true, W, true>) : go(k_assemble_slots<1, true, W, false>);
+      else
+        gon ? go(k_assemble_slots<1, false, W, true>) : go(k_assemble_slots<1, false, W, false>);
+    } else if (a.NP == 3) {
       if (cplx)
         gon ? go(k_assemble_slots<3, true, W, true>) : go(k_assemble_slots<3, true, W, false>);
       else
@@ -1210,7 +1235,7 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
 }
 
 void launch_assemble_mid(const AssemblyArgs& a, const SlotArgs& sa, hipStream_t s) {
-  if (a.NP != 6 || a.M == 0 || a.E == 0) return;
+  if (!uses_mid_rules(a.R, a.NP) || a.M == 0 || a.E == 0) return;
   hipLaunchKernelGGL(k_assemble_mid, dim3(2048), dim3(256), 0, s, a, sa.mid_count,
                      sa.mid_queue, sa.mid_cap, a.ki != 0.0 ? 1 : 0);
   BIE_HIP_CHECK(hipGetLastError());
@@ -1232,59 +1257,72 @@ size_t batch_shmem(int S, int nphi, int nstage) {
          sizeof(int2) * kMidLds + 16;
 }
 
+// Calls fn with the batch kernel instance for (NP, cplx, gon, qc): qc > 0 (real k only)
+// selects the unrolled fast-trig instance for rules of qc points, qc == 0 the generic one.
+template <class Fn>
+static void with_batch_kernel(int NP, bool cplx, bool gon, int qc, Fn&& fn) {
+  constexpr int W = kSlotWaves;
+  auto pick = [&](auto np) {
+    constexpr int N = decltype(np)::value;
+    if (qc > 0 && !cplx) {
+      auto fast = [&](auto g) -> bool {
+        constexpr bool G = decltype(g)::value;
+        switch (qc) {
+          case 4: fn(k_assemble_batch<N, false, W, G, 4>); return true;
+          case 6: fn(k_assemble_batch<N, false, W, G, 6>); return true;
+          case 16: fn(k_assemble_batch<N, false, W, G, 16>); return true;
+          default: return false;
+        }
+      };
+      if (gon ? fast(std::true_type{}) : fast(std::false_type{})) return;
+    }
+    if (cplx)
+      fn(k_assemble_batch<N, true, W>);
+    else
+      fn(k_assemble_batch<N, false, W>);
+  };
+  if (NP == 1)
+    pick(std::integral_constant<int, 1>{});
+  else if (NP == 3)
+    pick(std::integral_constant<int, 3>{});
+  else
+    pick(std::integral_constant<int, 6>{});
+}
+
 void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
                            unsigned* d_counter, const double* d_phi, int nphi, int nstage,
-                           int S, int NP, bool cplx, int grid, hipStream_t s) {
+                           int S, int NP, bool cplx, bool gon, int qc, int grid,
+                           hipStream_t s) {
   if (n_tiles <= 0) return;
   const size_t shmem = batch_shmem(S, nphi, nstage);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
   BIE_REQUIRE(grid >= 1, BIE_EINVAL, "empty persistent grid");
+  BIE_REQUIRE(qc == 0 || (!cplx && stage_doubles(qc) == nstage), BIE_EINVAL,
+              "internal: fast batch instance for another rule length");
   hipLaunchKernelGGL(k_zero_u32_n, dim3(1), dim3(64), 0, s, d_counter, kXcds);
   BIE_HIP_CHECK(hipGetLastError());
   constexpr int W = kSlotWaves;
-  auto go = [&](auto kern) {
+  with_batch_kernel(NP, cplx, gon, qc, [&](auto kern) {
     BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), shmem, s, d_inst, d_tiles,
                        n_tiles, d_counter, d_phi, nphi, nstage, S);
-  };
-  if (NP == 3) {
-    if (cplx)
-      go(k_assemble_batch<3, true, W>);
-    else
-      go(k_assemble_batch<3, false, W>);
-  } else {
-    if (cplx)
-      go(k_assemble_batch<6, true, W>);
-    else
-      go(k_assemble_batch<6, false, W>);
-  }
+  });
   BIE_HIP_CHECK(hipGetLastError());
 }
 
 // Resident workgroups of the batch kernel per CU for slot count S (LDS-bound).
-int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage) {
+int batch_blocks_per_cu(int NP, bool cplx, bool gon, int qc, int S, int nphi, int nstage) {
   const size_t shmem = batch_shmem(S, nphi, nstage);
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "batch kernel LDS request exceeds 160 KiB");
   constexpr int W = kSlotWaves;
   int nb = 0;
-  auto q = [&](auto kern) {
+  with_batch_kernel(NP, cplx, gon, qc, [&](auto kern) {
     BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     BIE_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &nb, reinterpret_cast<const void*>(kern), 64 * W, shmem));
-  };
-  if (NP == 3) {
-    if (cplx)
-      q(k_assemble_batch<3, true, W>);
-    else
-      q(k_assemble_batch<3, false, W>);
-  } else {
-    if (cplx)
-      q(k_assemble_batch<6, true, W>);
-    else
-      q(k_assemble_batch<6, false, W>);
-  }
+  });
   return nb < 1 ? 1 : nb;
 }
 
@@ -1340,7 +1378,12 @@ void launch_assemble(const AssemblyArgs& a, hipStream_t s) {
   chunks = (a.E + ech - 1) / ech;
   dim3 grid((unsigned)rowblocks, (unsigned)chunks);
   const bool cplx = a.ki != 0.0;
-  if (a.NP == 3) {
+  if (a.NP == 1) {
+    if (cplx)
+      hipLaunchKernelGGL((k_assemble<1, true>), grid, dim3(256), 0, s, a, ech);
+    else
+      hipLaunchKernelGGL((k_assemble<1, false>), grid, dim3(256), 0, s, a, ech);
+  } else if (a.NP == 3) {
     if (cplx)
       hipLaunchKernelGGL((k_assemble<3, true>), grid, dim3(256), 0, s, a, ech);
     else

@@ -47,7 +47,9 @@ constexpr int kPtStride = 21;
 // Rules 1..3 cluster at the corners (Q = n*m each); rules 4..6 at the edge mid points of
 // quadratic elements (Q = 2*n*m', m' = m rounded up to even).
 struct RuleSet {
-  int gk = 3, pk = 3, n = 0, m = 0;
+  int gk = 3, pk = 3, n = 0, m = 0;  // pk: physics nodes per element
+  int pkind = 3;                     // physics kind (BIE_TRI3 .. BIE_DTRI6)
+  double beta = 0.0;                 // node offset of BIE_DTRI3 / BIE_DTRI6
   int R = 3;          // number of rules
   int Qc = 0, Qm = 0; // corner / mid-edge rule sizes
   int Pe = 0;         // points per element over all rules
@@ -58,7 +60,14 @@ struct RuleSet {
   std::vector<double> phi;  // R x Qmax x pk physics basis at the rule points
 };
 
-RuleSet build_ruleset(int gk, int pk, int n, int m);
+RuleSet build_ruleset(int gk, int pkind, double beta, int n, int m);
+
+// Physics kinds: the low byte is the node count; BIE_DTRI* are discontinuous.
+inline int physics_nodes(int pkind) { return pkind & 0xff; }
+inline bool physics_discontinuous(int pkind) { return (pkind & 0x100) != 0; }
+// The mid-edge split rules 4-6 exist on quadratic geometry (R = 6) and are chosen only when
+// the nearest of six physics nodes is an edge node.
+inline bool uses_mid_rules(int gk, int pk) { return gk == 6 && pk == 6; }
 void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
 
 // Device-side launch parameters for one assembly.
@@ -264,7 +273,8 @@ void launch_batch_prep(const BatchInst* d_inst, int n_inst, int64_t max_cols, hi
 size_t batch_shmem(int S, int nphi, int nstage);
 void launch_assemble_batch(const BatchInst* d_inst, const int4* d_tiles, int n_tiles,
                            unsigned* d_counter, const double* d_phi, int nphi, int nstage,
-                           int S, int NP, bool cplx, int grid, hipStream_t s);
+                           int S, int NP, bool cplx, bool gon, int qc, int grid,
+                           hipStream_t s);
 void launch_assemble_mid_batch(const BatchInst* d_inst, int n_inst, bool cplx, hipStream_t s);
 void launch_mid_recorded(const AssemblyArgs& a, const unsigned long long* keys, int64_t n_pairs,
                          double* vals, const unsigned long long* entries, const int* off,
@@ -276,7 +286,7 @@ void launch_mid_compute(const AssemblyArgs& a, const unsigned long long* keys, i
 void launch_mid_scatter(const AssemblyArgs& a, int64_t n_pairs, const double* vals,
                         const unsigned long long* entries, const int* off, const int* refs,
                         int64_t n_entries, hipStream_t s);
-int batch_blocks_per_cu(int NP, bool cplx, int S, int nphi, int nstage);
+int batch_blocks_per_cu(int NP, bool cplx, bool gon, int qc, int S, int nphi, int nstage);
 void zgemv_batched_device(int64_t M, int64_t K, const double* dA, int64_t strideA, int64_t lda,
                           const double* dX, int64_t ldx, double* dY, int64_t ldy,
                           const int32_t* d_idx, int32_t n_idx, hipStream_t s);
@@ -326,7 +336,9 @@ void launch_select(const double* targets, int64_t M, const double* selnodes, int
 // Device-resident mesh and its cached rule-dependent tables.
 struct bie_dmesh {
   int device = 0;
-  int gk = 3, pk = 3;
+  int gk = 3, pk = 3;           // geometry nodes, physics nodes per element
+  int pkind = 3;                // physics kind (BIE_TRI3 .. BIE_DTRI6)
+  double beta = 0.0;            // node offset of BIE_DTRI3 / BIE_DTRI6
   int hyp = 0;                  // rule-selection distance (BIE_MESH_SCALED_HYPOT)
   int64_t Nc = 0, E = 0, N = 0;
   double* d_coords = nullptr;
@@ -397,6 +409,8 @@ struct bie_dmesh {
 struct bie_batch {
   int device = 0;
   int gk = 3, pk = 3, n = 0, m = 0;
+  int pkind = 3;
+  double beta = 0.0;
   std::vector<bie_dmesh*> meshes;
   // schedule configuration (chosen by bie_batch_tune)
   int elems_per_tile = 64;   // target elements per segment

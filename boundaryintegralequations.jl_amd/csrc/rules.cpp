@@ -174,14 +174,50 @@ void geometry_basis(int kind, double u, double v, double* N, double* du, double*
   eval_basis(kind, u, v, N, du, dv);
 }
 
-RuleSet build_ruleset(int gk, int pk, int n, int m) {
-  if ((gk != 3 && gk != 6) || (pk != 3 && pk != 6))
+// Physics basis at (u, v).  Continuous kinds are the geometry bases; the discontinuous
+// ones (surface_functions.jl:234-318) use psi(s) = (s - beta) / (1 - 3 beta) of the three
+// barycentric coordinates 1-u-v, u, v, each operation in the reference's order.
+void physics_basis(int pkind, double beta, double u, double v, double* N) {
+  if (pkind == BIE_TRI3 || pkind == BIE_TRI6) {
+    double du[6], dv[6];
+    eval_basis(pkind, u, v, N, du, dv);
+    return;
+  }
+  if (pkind == BIE_DTRI1) {
+    N[0] = 1.0;
+    return;
+  }
+  const double den = 1.0 - 3.0 * beta;
+  const double l1 = (((1.0 - u) - v) - beta) / den;
+  const double l2 = (u - beta) / den;
+  const double l3 = (v - beta) / den;
+  if (pkind == BIE_DTRI3) {
+    N[0] = l1;
+    N[1] = l2;
+    N[2] = l3;
+    return;
+  }
+  N[0] = l1 * (2.0 * l1 - 1.0);
+  N[1] = l2 * (2.0 * l2 - 1.0);
+  N[2] = l3 * (2.0 * l3 - 1.0);
+  N[3] = (4.0 * l2) * l1;
+  N[4] = (4.0 * l2) * l3;
+  N[5] = (4.0 * l3) * l1;
+}
+
+RuleSet build_ruleset(int gk, int pkind, double beta, int n, int m) {
+  const bool pk_ok = pkind == BIE_TRI3 || pkind == BIE_TRI6 || pkind == BIE_DTRI1 ||
+                     pkind == BIE_DTRI3 || pkind == BIE_DTRI6;
+  if ((gk != 3 && gk != 6) || !pk_ok)
     throw Error{BIE_EUNSUPPORTED, "only linear (3-node) and quadratic (6-node) triangles"};
+  const int pk = physics_nodes(pkind);
   if (n < 1 || m < 1 || n > 64 || m > 64)
     throw Error{BIE_EINVAL, "quadrature orders n, m must be in 1..64"};
   RuleSet rs;
   rs.gk = gk;
   rs.pk = pk;
+  rs.pkind = pkind;
+  rs.beta = physics_discontinuous(pkind) && pkind != BIE_DTRI1 ? beta : 0.0;
   rs.n = n;
   rs.m = m;
   rs.R = (gk == 6) ? 6 : 3;
@@ -207,8 +243,8 @@ RuleSet build_ruleset(int gk, int pk, int n, int m) {
       p[1] = v;
       p[2] = rules[c].w[q];
       eval_basis(gk, u, v, p + 3, p + 9, p + 15);
-      double N[6], du[6], dv[6];
-      eval_basis(pk, u, v, N, du, dv);
+      double N[6];
+      physics_basis(pkind, rs.beta, u, v, N);
       for (int s = 0; s < pk; ++s) rs.phi[((size_t)c * rs.Qmax + q) * pk + s] = N[s];
     }
   }

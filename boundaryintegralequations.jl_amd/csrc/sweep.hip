@@ -474,7 +474,7 @@ void launch_zero_columns_multi(double* X, int64_t stride, int nk, int64_t M, con
   BIE_HIP_CHECK(hipGetLastError());
 }
 
-int sweep_kb(int NP) { return NP == 3 ? 4 : 2; }
+int sweep_kb(int NP) { return NP == 6 ? 2 : 4; }
 
 int sweep_block_size(int NP, bool gon) { return gon ? sweep_kb(NP) : 2 * sweep_kb(NP); }
 
@@ -495,7 +495,7 @@ __global__ void k_block_uniform(const double* __restrict__ k, int nk, int KB, in
   out[kb] = ok;
 }
 
-size_t g_sweep_attr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+size_t g_sweep_attr[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 template <int NP, int KB, int W>
 const void* sweep_kernel_np(bool taylor, bool gon) {
@@ -508,6 +508,7 @@ const void* sweep_kernel_np(bool taylor, bool gon) {
 
 template <int W>
 const void* sweep_kernel(int NP, bool taylor, bool gon) {
+  if (NP == 1) return sweep_kernel_np<1, 4, W>(taylor, gon);
   return NP == 3 ? sweep_kernel_np<3, 4, W>(taylor, gon) : sweep_kernel_np<6, 2, W>(taylor, gon);
 }
 
@@ -531,7 +532,7 @@ void sweep_prepare_attributes(int NP, int nphi, bool taylor, bool gon) {
       sweep_shmem(NP, nphi, fitting_slots(sweep_fixed_doubles(nphi), sweep_kb(NP)));
   constexpr int W = kSlotWaves;
   const void* kern = sweep_kernel<W>(NP, taylor, gon);
-  size_t& done = g_sweep_attr[(NP == 3 ? 0 : 1) + (taylor ? 2 : 0) + (gon ? 4 : 0)];
+  size_t& done = g_sweep_attr[(NP == 1 ? 0 : NP == 3 ? 1 : 2) + (taylor ? 3 : 0) + (gon ? 6 : 0)];
   if (shmem > done) {
     BIE_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)std::max(shmem, (size_t)(144 * 1024))));
@@ -561,7 +562,7 @@ void launch_sweep(const AssemblyArgs& a, const SlotArgs& sa, const SweepArgs& sw
   void* args[] = {&a_, &sa_, &sw_};
   BIE_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)(per_xcd * 8)), dim3(64 * W), args, shmem,
                                 s));
-  if (a.NP == 6) {
+  if (uses_mid_rules(a.R, a.NP)) {
     if (tay)
       hipLaunchKernelGGL(k_sweep_mid<true>, dim3(4096), dim3(256), 0, s, a, sa, sw);
     else

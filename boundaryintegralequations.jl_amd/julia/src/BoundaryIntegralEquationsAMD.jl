@@ -9,10 +9,11 @@ Drop-in MI355X backend for `BoundaryIntegralEquations.assemble_parallel!`
     import BoundaryIntegralEquationsAMD: assemble_parallel!
     F, G, C = assemble_parallel!(mesh, k, mesh.sources; n=2, m=2)
 
-Supported meshes (TriangularLinear / TriangularQuadratic geometry with geometry or linear
-physics) run on the GPU, dense and `sparse=true` (the near field,
-`sparse_assemble_parallel!`); everything else (quads, discontinuous physics) is forwarded
-to the reference method unchanged, so the call stays a drop-in.  Errors from
+Supported meshes (TriangularLinear / TriangularQuadratic geometry with geometry, linear or
+discontinuous triangular physics) run on the GPU, dense and, for continuous physics,
+`sparse=true` (the near field, `sparse_assemble_parallel!`); everything else (quads, the
+near field of discontinuous physics) is forwarded to the reference method unchanged, so
+the call stays a drop-in.  Errors from
 the library are thrown as `ArgumentError` (bad input) or `ErrorException` (HIP failure).
 
 This file is not executed in the build container (no Julia there); the same ABI is
@@ -41,6 +42,7 @@ struct BieMesh
     physics_kind::Int32
     index_base::Int32
     flags::Int32
+    physics_beta::Float64
 end
 
 # Rule selection compares hypot distances, and Julia releases round the 3-argument hypot
@@ -55,6 +57,23 @@ element_kind(::BIE.TriangularLinear) = Int32(3)
 element_kind(::BIE.TriangularQuadratic) = Int32(6)
 element_kind(::Any) = Int32(0)
 
+# physics_kind of the C ABI: the continuous kinds, or 0x100 | node count for the
+# discontinuous triangles (surface_functions.jl:234-318), whose beta field sets the nodes
+physics_kind(pf) = element_kind(pf)
+physics_kind(::BIE.DiscontinuousTriangularConstant) = Int32(0x101)
+physics_kind(::BIE.DiscontinuousTriangularLinear) = Int32(0x103)
+physics_kind(::BIE.DiscontinuousTriangularQuadratic) = Int32(0x106)
+physics_beta(pf) = 0.0
+physics_beta(pf::BIE.DiscontinuousTriangularLinear) = Float64(pf.beta)
+physics_beta(pf::BIE.DiscontinuousTriangularQuadratic) = Float64(pf.beta)
+discontinuous(mesh) = physics_kind(mesh.physics_function) > 0xff
+
+bie_mesh(mesh, coords, topo, srcs, ptopo) =
+    BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo), size(coords, 2),
+            size(topo, 2), size(srcs, 2), element_kind(mesh.shape_function),
+            physics_kind(mesh.physics_function), Int32(1), hypot_flag(),
+            physics_beta(mesh.physics_function))
+
 last_error() = unsafe_string(ccall((:bie_last_error, libbie[]), Cstring, ()))
 
 function check(rc::Integer)
@@ -66,8 +85,8 @@ function check(rc::Integer)
 end
 
 supported(mesh) = element_kind(mesh.shape_function) != 0 &&
-                  element_kind(mesh.physics_function) != 0 &&
-                  !(element_kind(mesh.shape_function) == 3 && element_kind(mesh.physics_function) == 6)
+                  physics_kind(mesh.physics_function) != 0 &&
+                  !(element_kind(mesh.shape_function) == 3 && physics_kind(mesh.physics_function) == 6)
 
 """
     assemble_hip(mesh, k, in_sources; fOn=true, gOn=true, cOn=true, m=4, n=4)
@@ -92,10 +111,7 @@ function assemble_hip(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix;
     flags = UInt32((fOn ? 1 : 0) | (gOn ? 2 : 0) | (cOn ? 4 : 0))
     kc = ComplexF64(k)
     GC.@preserve coords topo srcs ptopo targets F G C begin
-        bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
-                     size(coords, 2), size(topo, 2), N,
-                     element_kind(mesh.shape_function), element_kind(mesh.physics_function),
-                     Int32(1), hypot_flag())
+        bm = bie_mesh(mesh, coords, topo, srcs, ptopo)
         rc = ccall((:bie_assemble, libbie[]), Cint,
                    (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Int32, UInt32,
                     Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
@@ -114,7 +130,9 @@ GPU and everything else to `BoundaryIntegralEquations.assemble_parallel!`.
 """
 function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true, cOn=true,
                             sparse=false, m=4, n=4, progress=true, depth=2, offset=nothing)
-    if !supported(mesh)
+    # the near field of discontinuous physics (its own connectivity and polar rules) stays
+    # with the reference
+    if !supported(mesh) || (sparse && discontinuous(mesh))
         return BIE.assemble_parallel!(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn,
                                       sparse=sparse, m=m, n=n, progress=progress,
                                       depth=depth, offset=offset)
@@ -145,10 +163,7 @@ function sparse_hip(mesh::BIE.Mesh3d, k::Number, sources::AbstractMatrix; fOn=tr
     kc = ComplexF64(k)
     off = offset === nothing ? NaN : Float64(offset)
     GC.@preserve coords topo srcs ptopo pts begin
-        bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
-                     size(coords, 2), size(topo, 2), N,
-                     element_kind(mesh.shape_function), element_kind(mesh.physics_function),
-                     Int32(1), hypot_flag())
+        bm = bie_mesh(mesh, coords, topo, srcs, ptopo)
         nnz = Ref{Int64}(0)
         check(ccall((:bie_sparse_count, libbie[]), Cint, (Ref{BieMesh}, Int32, Ref{Int64}),
                     bm, Int32(depth), nnz))
@@ -219,7 +234,8 @@ function BIE.sparse_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
                                                                BIE.TriangularQuadratic};
                                        fOn=true, gOn=true, depth=1, offset=nothing,
                                        progress=true, Ngauss=40)
-    if !(GPU_ROUTING[] && supported(mesh) && size(sources, 2) == size(mesh.sources, 2))
+    if !(GPU_ROUTING[] && supported(mesh) && !discontinuous(mesh) &&
+         size(sources, 2) == size(mesh.sources, 2))
         return invoke(BIE.sparse_assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,Any}, mesh, k,
                       sources, physics_function; fOn=fOn, gOn=gOn, depth=depth,
                       offset=offset, progress=progress, Ngauss=Ngauss)
@@ -271,10 +287,7 @@ mutable struct DeviceMesh
         ptopo = Matrix{Int64}(mesh.physics_topology)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         GC.@preserve coords topo srcs ptopo begin
-            bm = BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo),
-                         size(coords, 2), size(topo, 2), size(srcs, 2),
-                         element_kind(mesh.shape_function), element_kind(mesh.physics_function),
-                         Int32(1), hypot_flag())
+            bm = bie_mesh(mesh, coords, topo, srcs, ptopo)
             check(ccall((:bie_dmesh_create, libbie[]), Cint, (Ref{BieMesh}, Cint, Ptr{Ptr{Cvoid}}),
                         bm, Cint(device), h))
         end

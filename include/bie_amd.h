@@ -20,12 +20,24 @@ extern "C" {
 
 #define BIE_OK 0
 #define BIE_EINVAL 1        /* bad argument: null pointer, size, index, kind  (ArgumentError) */
-#define BIE_EUNSUPPORTED 2  /* element family outside linear/quadratic triangles            */
+#define BIE_EUNSUPPORTED 2  /* element family or entry point outside what this library covers */
 #define BIE_ENOMEM 3        /* device or host allocation failed                              */
 #define BIE_EHIP 4          /* HIP runtime error                                             */
 
 #define BIE_TRI3 3          /* TriangularLinear    (src/3d/SurfaceFunction.jl:31)  */
 #define BIE_TRI6 6          /* TriangularQuadratic (src/3d/SurfaceFunction.jl:58)  */
+/* Discontinuous physics (physics_kind only; geometry stays BIE_TRI3 or BIE_TRI6): one set
+ * of nodes per element inside it, the low byte is the node count.  Nodes and bases
+ * (src/3d/SurfaceFunction.jl:429-438, src/3d/surface_functions.jl:234-318), with
+ * psi(s) = (s - beta) / (1 - 3 beta):
+ *   BIE_DTRI1  DiscontinuousTriangularConstant   N = 1, node (1/3, 1/3)
+ *   BIE_DTRI3  DiscontinuousTriangularLinear     N = psi(1-u-v), psi(u), psi(v)
+ *   BIE_DTRI6  DiscontinuousTriangularQuadratic  the quadratic Lagrange basis in those three
+ * The mesh's sources are the geometry at the nodes (compute_sources, src/3d/mesh.jl:116-148)
+ * and physics_topology numbers them element by element; beta is bie_mesh.physics_beta. */
+#define BIE_DTRI1 0x101
+#define BIE_DTRI3 0x103
+#define BIE_DTRI6 0x106
 
 #define BIE_F 1u            /* fOn */
 #define BIE_G 2u            /* gOn */
@@ -44,14 +56,17 @@ typedef struct bie_mesh {
   const double* coordinates;        /* 3 x n_coordinates                              */
   const int64_t* topology;          /* geometry_kind x n_elements                     */
   const double* sources;            /* 3 x n_sources (collocation nodes)               */
-  const int64_t* physics_topology;  /* physics_kind x n_elements, indexes sources      */
+  const int64_t* physics_topology;  /* nodes x n_elements (nodes = physics_kind & 0xff),
+                                       indexes sources                                  */
   int64_t n_coordinates;
   int64_t n_elements;
   int64_t n_sources;
   int32_t geometry_kind;            /* BIE_TRI3 | BIE_TRI6 */
-  int32_t physics_kind;             /* BIE_TRI3 | BIE_TRI6 */
+  int32_t physics_kind;             /* BIE_TRI3 | BIE_TRI6 | BIE_DTRI1 | BIE_DTRI3 | BIE_DTRI6 */
   int32_t index_base;               /* 1 for Julia arrays, 0 for C/Python              */
   int32_t flags;                    /* 0 or BIE_MESH_SCALED_HYPOT                      */
+  double physics_beta;              /* node offset beta of BIE_DTRI3 / BIE_DTRI6 (the
+                                       physics function's beta field), else ignored     */
 } bie_mesh;
 
 typedef struct bie_dmesh bie_dmesh;  /* device-resident mesh + cached rule tables */

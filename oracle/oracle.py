@@ -111,6 +111,9 @@ def basis(kind: int, u, v):
 
 
 def _mesh_args(mesh):
+    if mesh.physics_kind not in (3, 6):
+        raise ValueError("the C restatement covers continuous physics; discontinuous meshes "
+                         "are checked with oracle_np")
     coords = np.asfortranarray(mesh.coordinates)
     topo = np.asfortranarray(mesh.topology)
     src = np.asfortranarray(mesh.sources)

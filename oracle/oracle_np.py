@@ -10,6 +10,8 @@ small meshes.  Reference functions restated (file:line):
   rotated_midpoint_triangular_quadpoints src/3d/quadrature.jl:117-144
   create_rotated_element       src/3d/assembly_collocation.jl:122-136
   triangularLinear/Quadratic   src/3d/surface_functions.jl:191-233
+  discontinuousTriangular{Constant,Linear,Quadratic} src/3d/surface_functions.jl:234-318
+    (physics kinds 0x101, 0x103, 0x106 with the mesh's physics_beta)
   interpolate_elements/jacobian! src/3d/assembly_collocation.jl:27-66, src/3d/jacobian.jl:7-87
   find_closest_corner          src/3d/assembly_collocation.jl:102-115
   greens3d!/freens3d!/freens3dk0! src/3d/kernels.jl:13-55
@@ -54,8 +56,37 @@ def rule(c, n, m):
     return Y, (1.0 - X) - Y, W
 
 
-def basis(kind, u, v):
-    """Returns N, dN/du, dN/dv with shape (Q, kind)."""
+def disc_basis(kind, u, v, beta):
+    """Discontinuous triangle bases, N, dN/du, dN/dv with shape (Q, nodes): the linear or
+    quadratic Lagrange basis in the three shrunk barycentrics
+    lam = ((1-u-v, u, v) - beta) / (1 - 3 beta), whose nodes sit where lam is 0 or 1."""
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    z = np.zeros_like(u)
+    if kind == 0x101:
+        return np.ones((u.size, 1)), z[:, None], z[:, None]
+    c = 1.0 / (1.0 - 3.0 * beta)
+    lam = (np.stack([(1.0 - u) - v, u, v], axis=1) - beta) / (1.0 - 3.0 * beta)
+    dlu = np.stack([-c + z, c + z, z], axis=1)    # d lam / du
+    dlv = np.stack([-c + z, z, c + z], axis=1)
+    if kind == 0x103:
+        return lam, dlu, dlv
+    pairs = [(1, 0), (1, 2), (2, 0)]               # edge nodes 4, 5, 6
+    N = np.concatenate([lam * (2.0 * lam - 1.0)] +
+                       [4.0 * lam[:, [a]] * lam[:, [b]] for a, b in pairs], axis=1)
+    du = np.concatenate([(4.0 * lam - 1.0) * dlu] +
+                        [4.0 * (dlu[:, [a]] * lam[:, [b]] + lam[:, [a]] * dlu[:, [b]])
+                         for a, b in pairs], axis=1)
+    dv = np.concatenate([(4.0 * lam - 1.0) * dlv] +
+                        [4.0 * (dlv[:, [a]] * lam[:, [b]] + lam[:, [a]] * dlv[:, [b]])
+                         for a, b in pairs], axis=1)
+    return N, du, dv
+
+
+def basis(kind, u, v, beta=0.0):
+    """Returns N, dN/du, dN/dv with shape (Q, nodes)."""
+    if kind > 0xFF:
+        return disc_basis(kind, u, v, beta)
     u = np.asarray(u, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
     o, z = np.ones_like(u), np.zeros_like(u)
@@ -114,11 +145,13 @@ def select(mesh, targets):
     return np.minimum(idx, R).astype(np.int8)
 
 
-def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True):
+def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True, rules=None):
+    """F, G, C; rules (int8 M x E, 1-based) freezes the rule of every pair instead of
+    selecting it."""
     targets = mesh.sources if targets is None else np.asarray(targets, dtype=np.float64)
     M, N, E = targets.shape[1], mesh.n_sources, mesh.n_elements
     R = 6 if mesh.geometry_kind == 6 else 3
-    sel = select(mesh, targets)
+    sel = select(mesh, targets) if rules is None else np.asarray(rules)
     F = np.zeros((M, N), dtype=np.complex128)
     G = np.zeros((M, N), dtype=np.complex128)
     C = np.zeros(M, dtype=np.complex128)
@@ -129,7 +162,8 @@ def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True):
             continue
         y, nrm, jw = element_geometry(mesh, c, n, m)
         u, v, _ = rule(c, n, m)
-        phi, _, _ = basis(mesh.physics_kind, u, v)            # (Q, np)
+        phi, _, _ = basis(mesh.physics_kind, u, v,
+                          getattr(mesh, "physics_beta", 0.0))  # (Q, np)
         d = y[ee] - targets.T[ii][:, None, :]                 # (P, Q, 3)
         r = julia_hypot3(d[..., 0], d[..., 1], d[..., 2])
         nd = np.einsum("pqd,pqd->pq", nrm[ee], d)

@@ -77,12 +77,19 @@ bool read_mesh(const char* path, Mesh& m) {
 
 void check_rules() {
   for (int gk : {3, 6})
-    for (int pk : {3, 6}) {
+    for (int pkind : {BIE_TRI3, BIE_TRI6, BIE_DTRI1, BIE_DTRI3, BIE_DTRI6}) {
+      const int pk = bie::physics_nodes(pkind);
       for (int n : {1, 2, 3, 4, 6})
         for (int m : {1, 2, 3, 5}) {
-          bie::RuleSet rs = bie::build_ruleset(gk, pk, n, m);
+          bie::RuleSet rs = bie::build_ruleset(gk, pkind, 0.2113, n, m);
           expect(rs.R == (gk == 6 ? 6 : 3), "rule count");
           expect((int)rs.phi.size() == rs.R * rs.Qmax * pk, "phi size");
+          for (int t = 0; t < rs.R * rs.Qmax; ++t) {  // every basis sums to one
+            double s = 0;
+            for (int j = 0; j < pk; ++j) s += rs.phi[(size_t)t * pk + j];
+            const int c = t / rs.Qmax, q = t % rs.Qmax;
+            if (q < rs.len[c]) expect(std::fabs(s - 1) < 1e-13, "physics partition of unity");
+          }
           for (int c = 0; c < rs.R; ++c) {
             expect(rs.off[c] + rs.len[c] <= rs.Pe, "rule offsets");
             double wsum = 0;

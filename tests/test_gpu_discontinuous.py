@@ -1,0 +1,184 @@
+"""Discontinuous triangular physics on the GPU (BIE_DTRI1 / BIE_DTRI3 / BIE_DTRI6 through the
+C ABI) against the NumPy oracle (oracle_np, whose discontinuous bases and sources are
+pinned in test_discontinuous_cpu.py).
+
+The reference assembles these meshes with the same collocation code as continuous physics
+(assemble_parallel!, src/3d/assembly_collocation.jl:270-479): the rule of a pair is chosen
+by the nearest of the element's own physics nodes (for quadratic physics on linear
+geometry nodes 4-6 fall back to rule 3), and each element adds into its own columns.
+Bar: F, G, C within 1e-12 relative Frobenius error of the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_np
+from bie_amd import _lib
+from bie_amd import mesh as M
+from bie_amd.assembly import Batch, DeviceMesh, assemble_parallel
+from gpu_helpers import DevBuf, sync
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+ORDERS = ["disctriconstant", "disctrilinear", "disctriquadratic"]
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def own_blocks(mesh):
+    """Mask of the (row, column) entries of an element's own nodes."""
+    own = np.zeros((mesh.n_sources, mesh.n_sources), dtype=bool)
+    for e in range(mesh.n_elements):
+        idx = mesh.physics_topology[:, e]
+        own[np.ix_(idx, idx)] = True
+    return own
+
+
+def check_F(F, Fo, mesh, tag=""):
+    """Collocation F of discontinuous physics.  On an element's own nodes (which sit inside
+    it, away from the corner its rule is clustered at) the integrand carries the rounding
+    noise of n.d amplified by 1/r^3 at the rule points nearest the node: on flat elements n.d
+    is 0 exactly there and F is pure noise (~1e-12 of max |F| for quadratic physics, whose
+    edge nodes take rule 3 on linear geometry); on curved ones the noise sits on top of the
+    true value.  Those entries are held to 1e-9 of max |F|, every other entry to the bar."""
+    own = own_blocks(mesh)
+    fmax = np.max(np.abs(Fo))
+    err_own = np.max(np.abs(F[own] - Fo[own])) / fmax
+    print(f"  {tag} F off own blocks rel {rel(F[~own], Fo[~own]):.2e}, own blocks max "
+          f"{err_own:.2e} of max|F|", flush=True)
+    assert rel(F[~own], Fo[~own]) <= TOL
+    assert err_own <= 1e-9
+
+
+def disc_mesh(gorder, physics, f=3, beta_type="legendre"):
+    base = M.perturb_radial(M.icosphere(f, gorder), 0.02, 7)
+    return M.with_discontinuous_physics(base, physics, beta_type)
+
+
+@pytest.mark.parametrize("gorder", ["linear", "quadratic"])
+@pytest.mark.parametrize("physics", ORDERS)
+@pytest.mark.parametrize("k", [0.0, 3.0, 2.0 + 0.5j])
+def test_collocation_matches_oracle(gorder, physics, k):
+    # (On linear geometry at (n, m) = (3, 3) the Legendre node offset of quadratic physics,
+    # beta = (1 - sqrt(3/5)) / 2, is itself a 3-point Gauss node on [0, 1]: the edge nodes,
+    # which fall back to rule 3, then coincide with one of its points and r = 0 there - the
+    # reference's 1/r gives NaN.  Linear geometry is checked at (3, 2).)
+    n, m = (3, 2) if gorder == "linear" else (3, 3)
+    mesh = disc_mesh(gorder, physics)
+    print(f"[case] disc collocation {gorder} {physics} k={k} N={mesh.n_sources}", flush=True)
+    F, G, C = assemble_parallel(mesh, k, n=n, m=m)
+    Fo, Go, Co = oracle_np.assemble(mesh, k, n=n, m=m)
+    assert rel(G, Go) <= TOL and rel(C, Co) <= TOL
+    check_F(F, Fo, mesh, "collocation")
+
+
+@pytest.mark.parametrize("gorder", ["linear", "quadratic"])
+@pytest.mark.parametrize("physics", ORDERS)
+def test_field_points_match_oracle(gorder, physics):
+    mesh = disc_mesh(gorder, physics, f=2)
+    T = np.concatenate([M.fibonacci_sphere(70, 1.7), M.fibonacci_sphere(30, 0.5)], axis=1)
+    print(f"[case] disc field {gorder} {physics}", flush=True)
+    F, G, C = assemble_parallel(mesh, 4.0, T, n=2, m=2)
+    Fo, Go, Co = oracle_np.assemble(mesh, 4.0, T, n=2, m=2)
+    assert rel(F, Fo) <= TOL and rel(G, Go) <= TOL
+    # off the surface C is a cancelling sum (near 0 or 1 in magnitude): compare against
+    # the larger of its size and 1
+    assert np.max(np.abs(C - Co)) <= TOL * max(np.max(np.abs(Co)), 1.0)
+
+
+@pytest.mark.parametrize("physics", ORDERS)
+@pytest.mark.parametrize("nm", [(2, 2), (3, 2), (4, 4)])
+def test_device_fast_instances(physics, nm):
+    """Real k below the fast-sincos bound takes the unrolled instances (rules of 4, 6 and 16
+    points); F-only launches take the G-less ones."""
+    n, m = nm
+    mesh = disc_mesh("quadratic", physics, f=3)
+    N = mesh.n_sources
+    print(f"[case] disc device {physics} nm={nm} N={N}", flush=True)
+    dm = DeviceMesh(mesh)
+    F, G, C = DevBuf(16 * N * N), DevBuf(16 * N * N), DevBuf(16 * N)
+    Fo, Go, Co = oracle_np.assemble(mesh, 6.0, n=n, m=m)
+    for gOn in (True, False):
+        dm.assemble(6.0, F.ptr, G.ptr, C.ptr, n=n, m=m, gOn=gOn)
+        sync()
+        f = F.download(np.zeros((N, N), dtype=np.complex128, order="F"))
+        c = C.download(np.zeros(N, dtype=np.complex128))
+        check_F(f, Fo, mesh, f"gOn={gOn}")
+        assert rel(c, Co) <= TOL
+        if gOn:
+            g = G.download(np.zeros((N, N), dtype=np.complex128, order="F"))
+            assert rel(g, Go) <= TOL
+    dm.close()
+
+
+def test_beta_is_part_of_the_cached_mesh():
+    """The host drop-in keeps the last mesh on the device: a mesh that differs only in beta
+    (same arrays otherwise) must not hit that cache."""
+    a = disc_mesh("linear", "disctrilinear", f=2)
+    b = M.Mesh3d(a.coordinates, a.topology, a.sources, a.physics_topology, a.geometry_kind,
+                 a.physics_kind, 0.25)
+    for mesh in (a, b, a):
+        F, G, _ = assemble_parallel(mesh, 2.0, n=2, m=2)
+        Fo, Go, _ = oracle_np.assemble(mesh, 2.0, n=2, m=2)
+        check_F(F, Fo, mesh, f"beta={mesh.physics_beta}")
+        assert rel(G, Go) <= TOL
+
+
+@pytest.mark.parametrize("physics", ["disctriconstant", "disctrilinear"])
+def test_batch_matches_single(physics):
+    meshes = [disc_mesh("linear", physics, f=f) for f in (2, 3, 4)]
+    ks = [1.5, 4.0, 2.5]
+    dms = [DeviceMesh(x) for x in meshes]
+    print(f"[case] disc batch {physics}", flush=True)
+    bufs = [(DevBuf(16 * x.n_sources ** 2), DevBuf(16 * x.n_sources ** 2),
+             DevBuf(16 * x.n_sources)) for x in meshes]
+    b = Batch(dms, n=3, m=3)
+    b.assemble(ks, [f.ptr for f, _, _ in bufs], [g.ptr for _, g, _ in bufs],
+               [c.ptr for _, _, c in bufs])
+    sync()
+    for x, k, (Fb, Gb, Cb) in zip(meshes, ks, bufs):
+        N = x.n_sources
+        Fo, Go, Co = oracle_np.assemble(x, k, n=3, m=3)
+        check_F(Fb.download(np.zeros((N, N), dtype=np.complex128, order="F")), Fo, x, "batch")
+        assert rel(Gb.download(np.zeros((N, N), dtype=np.complex128, order="F")), Go) <= TOL
+        assert rel(Cb.download(np.zeros(N, dtype=np.complex128)), Co) <= TOL
+    b.close()
+
+
+@pytest.mark.parametrize("gorder,physics", [("linear", "disctriconstant"),
+                                            ("quadratic", "disctriquadratic"),
+                                            ("linear", "disctriquadratic")])
+def test_sweep_matches_oracle(gorder, physics):
+    mesh = disc_mesh(gorder, physics, f=2)
+    N = mesh.n_sources
+    ks = np.linspace(1.0, 9.0, 5)
+    print(f"[case] disc sweep {gorder} {physics}", flush=True)
+    dm = DeviceMesh(mesh)
+    kb = DevBuf(8 * len(ks))
+    kb.upload(ks)
+    stride = N * N
+    F, G, C = DevBuf(16 * stride * len(ks)), DevBuf(16 * stride * len(ks)), DevBuf(16 * N)
+    dm.assemble_sweep(kb.ptr, len(ks), F.ptr, G.ptr, stride, C.ptr, n=3, m=2)
+    sync()
+    f = F.download(np.zeros(stride * len(ks), dtype=np.complex128))
+    g = G.download(np.zeros(stride * len(ks), dtype=np.complex128))
+    for q, k in enumerate(ks):
+        Fo, Go, _ = oracle_np.assemble(mesh, k, n=3, m=2)
+        check_F(f[q * stride:(q + 1) * stride].reshape(N, N, order="F"), Fo, mesh, f"k={k}")
+        assert rel(g[q * stride:(q + 1) * stride].reshape(N, N, order="F"), Go) <= TOL
+    dm.close()
+
+
+def test_adjoint_rejects_discontinuous_physics():
+    from bie_amd.adjoint import assembly_vjp
+    mesh = disc_mesh("linear", "disctrilinear", f=2)
+    N = mesh.n_sources
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(N) + 0j
+    dm = DeviceMesh(mesh)
+    with pytest.raises(Exception) as ei:
+        assembly_vjp(dm, 2.0, a, a)
+    assert "continuous physics" in str(ei.value)
+    dm.close()
