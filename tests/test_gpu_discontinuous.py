@@ -182,3 +182,26 @@ def test_adjoint_rejects_discontinuous_physics():
         assembly_vjp(dm, 2.0, a, a)
     assert "continuous physics" in str(ei.value)
     dm.close()
+
+
+def test_taylor_rosebem_settings():
+    """The reference's ROSEBEM example (examples/3d_rosebem.jl: quadratic geometry with
+    disctriquadratic physics, (n, m) = (2, 2), gOn = false, M = 25): order 0 against the
+    oracle's assembly, order 1 against its central difference in k (the rules do not depend
+    on k), and the 26-term expansion at a nearby k against the oracle's assembly there."""
+    from bie_amd.taylor import apply_taylor_expansion, taylor_assemble
+    mesh = disc_mesh("quadratic", "disctriquadratic", f=2)
+    k0, Mo = 2.0, 25
+    print(f"[case] disc taylor N={mesh.n_sources} orders 0..{Mo}", flush=True)
+    F, _, _ = taylor_assemble(mesh, k0, M=Mo, n=2, m=2, gOn=False)
+    F0, _, _ = oracle_np.assemble(mesh, k0, n=2, m=2, gOn=False)
+    check_F(F[:, :, 0], F0, mesh, "order 0")
+    h = 1e-4
+    Fp, _, _ = oracle_np.assemble(mesh, k0 + h, n=2, m=2, gOn=False)
+    Fm, _, _ = oracle_np.assemble(mesh, k0 - h, n=2, m=2, gOn=False)
+    fd = (Fp - Fm) / (2 * h)
+    assert rel(F[:, :, 1], fd) <= 1e-6
+    k1 = k0 + 0.25
+    Fx = apply_taylor_expansion(F, k1, k0)
+    F1, _, _ = oracle_np.assemble(mesh, k1, n=2, m=2, gOn=False)
+    assert rel(Fx, F1) <= 1e-10
