@@ -179,3 +179,28 @@ def test_loaders_build_discontinuous_meshes():
             assert mesh.topology.shape == (3 if order == "linear" else 6, 840)
             assert mesh.sources.shape == (3, len(np.unique(mesh.physics_topology)))
             assert mesh.topology.shape[1] == mesh.physics_topology.shape[1]
+
+
+@pytest.mark.parametrize("go,po,tol", [("linear", "disctrilinear", 0.0044),
+                                       ("quadratic", "disctriquadratic", 4.9e-5)])
+def test_rigid_sphere_reference_tolerance_table(go, po, tol):
+    """test/3d/assembly_test.jl:12-44 for discontinuous physics (the oracle, assembled in
+    row blocks): the rigid-sphere solution on the reference's sphere_1m at n = m = 4 within
+    the reference's own tolerance."""
+    import os
+
+    import analytic
+    k = 2 * np.pi * 100.0 / 340.0
+    mesh = M.load_comsol(os.path.join(M.golden_dir(), "sphere_1m"), go, po)
+    N = mesh.n_sources
+    F = np.zeros((N, N), dtype=np.complex128)
+    C = np.zeros(N, dtype=np.complex128)
+    for r0 in range(0, N, 420):
+        T = mesh.sources[:, r0:r0 + 420]
+        F[r0:r0 + T.shape[1]], _, C[r0:r0 + T.shape[1]] = oracle_np.assemble(mesh, k, T,
+                                                                             gOn=False)
+    pI = analytic.incoming_wave([np.pi / 2, 0.0], 1.0, mesh.sources, k)
+    p = np.linalg.solve(F + np.diag(1.0 - C), pI)
+    th = np.arccos(np.clip(mesh.sources[0], -1, 1))
+    pa = analytic.plane_wave_scattering_sphere(k, 1.0, 1.0, th, 1e-6)
+    assert np.linalg.norm(p - pa) / np.linalg.norm(pa) < tol

@@ -205,3 +205,28 @@ def test_taylor_rosebem_settings():
     Fx = apply_taylor_expansion(F, k1, k0)
     F1, _, _ = oracle_np.assemble(mesh, k1, n=2, m=2, gOn=False)
     assert rel(Fx, F1) <= 1e-10
+
+
+@pytest.mark.parametrize("go,po,tol", [
+    ("linear", "disctriconstant", 0.0073), ("linear", "disctrilinear", 0.0044),
+    ("linear", "disctriquadratic", 0.004), ("quadratic", "disctriconstant", 0.0008),
+    ("quadratic", "disctrilinear", 4.2e-4), ("quadratic", "disctriquadratic", 4.9e-5)])
+def test_rigid_sphere_reference_tolerance_table(go, po, tol):
+    """test/3d/assembly_test.jl:12-44: plane-wave scattering by the rigid unit sphere on the
+    reference's sphere_1m mesh with the default n = m = 4, the BEM solution against the
+    analytic one, to the reference's own tolerance for each geometry and physics order."""
+    import os
+
+    import analytic
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    k = 2 * np.pi * 100.0 / 340.0
+    mesh = M.load_comsol(os.path.join(gold, "sphere_1m"), go, po)
+    print(f"[case] rigid sphere {go} {po} N={mesh.n_sources}", flush=True)
+    F, _, C = assemble_parallel(mesh, k, gOn=False)
+    pI = analytic.incoming_wave([np.pi / 2, 0.0], 1.0, mesh.sources, k)
+    p = np.linalg.solve(F + np.diag(1.0 - C), pI)
+    th = np.arccos(np.clip(mesh.sources[0], -1, 1))
+    pa = analytic.plane_wave_scattering_sphere(k, 1.0, 1.0, th, 1e-6)
+    err = rel(p, pa)
+    print(f"  relative error {err:.3e} (reference tolerance {tol})", flush=True)
+    assert err < tol
