@@ -33,11 +33,30 @@ DTRI1 = 0x101  # DiscontinuousTriangularConstant
 DTRI3 = 0x103  # DiscontinuousTriangularLinear
 DTRI6 = 0x106  # DiscontinuousTriangularQuadratic
 _DISC_ORDERS = {"disctriconstant": DTRI1, "disctrilinear": DTRI3, "disctriquadratic": DTRI6}
+# quadrilaterals (geometry and physics kinds; low byte = nodes per element)
+QUAD4 = 0x204    # QuadrilateralLinear4 (COMSOL layout)
+QUAD4S = 0x404   # QuadrilateralLinearSerendipity
+QUAD8 = 0x208    # QuadrilateralQuadratic (8-node serendipity)
+QUAD9 = 0x209    # QuadrilateralQuadraticLagrange (COMSOL layout)
+DQUAD1 = 0x301   # DiscontinuousQuadrilateralConstant
+DQUAD4 = 0x304   # DiscontinuousQuadrilateralLinear4
+DQUAD9 = 0x309   # DiscontinuousQuadrilateralQuadraticLagrange
+_DISC_QUAD_ORDERS = {"discquadconstant": DQUAD1, "discquadlinear": DQUAD4,
+                     "discquadquadratic": DQUAD9}
 
 
 def physics_nodes(kind: int) -> int:
     """Nodes per element of a physics kind (its low byte)."""
     return int(kind) & 0xFF
+
+
+def geometry_nodes(kind: int) -> int:
+    """Nodes per element of a geometry kind (its low byte)."""
+    return int(kind) & 0xFF
+
+
+def is_quad(kind: int) -> bool:
+    return (int(kind) >> 9) != 0
 
 
 @dataclasses.dataclass
@@ -59,7 +78,7 @@ class Mesh3d:
         self.physics_topology = np.ascontiguousarray(self.physics_topology, dtype=np.int64)
         if self.coordinates.shape[0] != 3 or self.sources.shape[0] != 3:
             raise ValueError("coordinates and sources must be 3 x n")
-        if self.topology.shape[0] != self.geometry_kind:
+        if self.topology.shape[0] != geometry_nodes(self.geometry_kind):
             raise ValueError("topology rows must equal the geometry node count")
         if self.physics_topology.shape[0] != physics_nodes(self.physics_kind):
             raise ValueError("physics_topology rows must equal the physics node count")
@@ -196,11 +215,12 @@ def discontinuous_beta(physics: str, beta_type="legendre") -> float:
     get_beta_tri_quadratic, src/3d/mesh.jl:182-214): half the quadrilateral value, which is
     1 + (the smallest Gauss-Legendre node of 2 or 3 points) for "legendre", 1/2 or 1/3 for
     "equidistant", and the number itself when one is given (so a number is halved too)."""
-    if physics == "disctriconstant":
+    if physics in ("disctriconstant", "discquadconstant"):
         return 0.0
-    if physics not in ("disctrilinear", "disctriquadratic"):
-        raise ValueError("not a discontinuous triangular physics order")
-    lin = physics == "disctrilinear"
+    quad_order = physics in ("discquadlinear", "discquadquadratic")
+    if physics not in ("disctrilinear", "disctriquadratic") and not quad_order:
+        raise ValueError("not a discontinuous physics order")
+    lin = physics in ("disctrilinear", "discquadlinear")
     if isinstance(beta_type, (int, float)) and not isinstance(beta_type, bool):
         quad = float(beta_type)
     elif beta_type == "legendre":
@@ -210,14 +230,36 @@ def discontinuous_beta(physics: str, beta_type="legendre") -> float:
         quad = 0.5 if lin else 1.0 / 3.0
     else:
         raise ValueError("beta_type must be 'legendre', 'equidistant' or a number")
-    return quad / 2
+    # get_beta_quad_linear / _quadratic are used as they are on quadrilaterals
+    return quad if quad_order else quad / 2
+
+
+def _lagrange3(x):
+    """1-D quadratic Lagrange polynomials on -1, 0, 1."""
+    return [0.5 * x * (x - 1.0), (1.0 - x) * (1.0 + x), 0.5 * x * (x + 1.0)]
+
+
+_QUAD9_IJ = [(0, 0), (2, 0), (0, 2), (2, 2), (1, 0), (0, 1), (1, 1), (2, 1), (1, 2)]
 
 
 def geometry_basis(kind: int, u, v) -> np.ndarray:
-    """Geometry basis (kind x len(u)) of triangularLinear / triangularQuadratic
-    (src/3d/surface_functions.jl:191-213)."""
+    """Geometry basis (nodes x len(u)) of triangularLinear / triangularQuadratic and the
+    quadrilaterals (src/3d/surface_functions.jl:1-95, 191-213)."""
     u = np.asarray(u, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
+    if kind in (QUAD4, QUAD4S):
+        su = [-1, 1, -1, 1] if kind == QUAD4 else [-1, 1, 1, -1]
+        sv = [-1, -1, 1, 1]
+        return np.stack([(1 + a * u) * (1 + b * v) / 4 for a, b in zip(su, sv)])
+    if kind == QUAD8:
+        su, sv = [-1, 1, 1, -1], [-1, -1, 1, 1]
+        corners = [(1 + a * u) * (1 + b * v) * (a * u + b * v - 1) / 4 for a, b in zip(su, sv)]
+        mids = [(1 - v) * (1 - u * u) / 2, (1 + u) * (1 - v * v) / 2, (1 + v) * (1 - u * u) / 2,
+                (1 - u) * (1 - v * v) / 2]
+        return np.stack(corners + mids)
+    if kind == QUAD9:
+        lu, lv = _lagrange3(u), _lagrange3(v)
+        return np.stack([lu[i] * lv[j] for i, j in _QUAD9_IJ])
     if kind == TRI3:
         return np.stack([(1.0 - u) - v, u, v])
     w = (1.0 - v) - u
@@ -226,9 +268,17 @@ def geometry_basis(kind: int, u, v) -> np.ndarray:
 
 
 def discontinuous_nodes(kind: int, beta: float):
-    """Local (u, v) of the nodes of a discontinuous triangle (get_nodal_nodes_u/_v,
-    src/3d/SurfaceFunction.jl:429-438)."""
+    """Local (u, v) of the nodes of a discontinuous element (get_nodal_nodes_u/_v,
+    src/3d/SurfaceFunction.jl:429-470)."""
     b = float(beta)
+    if kind == DQUAD1:
+        return np.array([0.0]), np.array([0.0])
+    if kind == DQUAD4:
+        return (np.array([b - 1, 1 - b, b - 1, 1 - b]), np.array([b - 1, b - 1, 1 - b, 1 - b]))
+    if kind == DQUAD9:
+        lo, hi = b - 1, 1 - b
+        return (np.array([lo, hi, lo, hi, 0.0, lo, 0.0, hi, 0.0]),
+                np.array([lo, lo, hi, hi, lo, 0.0, 0.0, 0.0, hi]))
     if kind == DTRI1:
         return np.array([1 / 3]), np.array([1 / 3])
     if kind == DTRI3:
@@ -253,12 +303,14 @@ def discontinuous_sources(coords, topo, gk, kind, beta):
 
 
 def with_discontinuous_physics(mesh: "Mesh3d", physics: str, beta_type="legendre") -> "Mesh3d":
-    """The same geometry with discontinuous triangular physics ("disctriconstant",
-    "disctrilinear", "disctriquadratic"; set_physics_element + compute_sources,
+    """The same geometry with discontinuous physics ("disctriconstant", "disctrilinear",
+    "disctriquadratic" on triangles, "discquadconstant", "discquadlinear",
+    "discquadquadratic" on quadrilaterals; set_physics_element + compute_sources,
     src/3d/mesh.jl:116-180)."""
-    kind = _DISC_ORDERS.get(physics)
+    orders = _DISC_QUAD_ORDERS if is_quad(mesh.geometry_kind) else _DISC_ORDERS
+    kind = orders.get(physics)
     if kind is None:
-        raise ValueError("physics must be disctriconstant, disctrilinear or disctriquadratic")
+        raise ValueError("physics must be " + ", ".join(orders) + " on this geometry")
     beta = discontinuous_beta(physics, beta_type)
     sources, ptop = discontinuous_sources(mesh.coordinates, mesh.topology, mesh.geometry_kind,
                                           kind, beta)
@@ -530,6 +582,108 @@ def _read_stl(path):
     return verts, faces
 
 
+def cubed_sphere(f: int, order: str = "linear", radius: float = 1.0,
+                 physics: str = "geometry", beta_type="legendre") -> Mesh3d:
+    """Quadrilateral sphere: every cube face cut into f x f squares, nodes projected to the
+    sphere.  order "linear": QuadrilateralLinear4 (6 f^2 + 2 nodes); "quadratic":
+    QuadrilateralQuadraticLagrange (24 f^2 + 2 nodes).  COMSOL node order; (x2 - x1) x
+    (x3 - x1) points out of the body, as in the reference's quad_sphere.mphtxt."""
+    if f < 1:
+        raise ValueError("f must be >= 1")
+    q = 2 if order == "quadratic" else 1
+    if order not in ("linear", "quadratic"):
+        raise ValueError("order must be 'linear' or 'quadratic'")
+    g = q * f                                    # grid intervals per face side
+    ids: dict = {}
+    pts = []
+
+    def node(key):
+        i = ids.get(key)
+        if i is None:
+            i = len(pts)
+            ids[key] = i
+            pts.append(key)
+        return i
+
+    elems = []
+    for axis in range(3):
+        for sign in (-1, 1):
+            a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+            if sign < 0:
+                a1, a2 = a2, a1                   # e1 x e2 = sign * e_axis: outward
+            for j in range(f):
+                for i in range(f):
+                    def key(a, b):
+                        k = [0, 0, 0]
+                        k[axis] = sign * g
+                        k[a1] = -g + 2 * (q * i + a)
+                        k[a2] = -g + 2 * (q * j + b)
+                        return tuple(k)
+                    if q == 1:
+                        elems.append([node(key(0, 0)), node(key(1, 0)), node(key(0, 1)),
+                                      node(key(1, 1))])
+                    else:
+                        elems.append([node(key(a, b)) for a, b in _QUAD9_IJ])
+    P = np.array(pts, dtype=np.float64)
+    coords = (radius * P / np.linalg.norm(P, axis=1, keepdims=True)).T.copy()
+    topo = np.array(elems, dtype=np.int64).T.copy()
+    gk = QUAD9 if q == 2 else QUAD4
+    base = Mesh3d(coords, topo, coords.copy(), topo.copy(), gk, gk)
+    if physics == "geometry":
+        return base
+    if physics == "linear" and gk == QUAD9:
+        return _quad_linear_physics(coords, topo)
+    if physics in _DISC_QUAD_ORDERS:
+        return with_discontinuous_physics(base, physics, beta_type)
+    raise ValueError("physics must be geometry, linear (quadratic geometry) or a discquad order")
+
+
+def _quad_linear_physics(coords, topo):
+    """QuadrilateralLinear4 physics on 9-node geometry: the corner nodes (the first four
+    of every element), renumbered in order of appearance."""
+    ptop = topo[:4, :]
+    used = np.unique(ptop)
+    remap = -np.ones(coords.shape[1], dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    return Mesh3d(coords, topo, coords[:, used].copy(), remap[ptop], QUAD9, QUAD4)
+
+
+def load_comsol_quad(path: str, geometry_order: str = "quadratic",
+                     physics_order: str = "geometry", beta_type="legendre") -> Mesh3d:
+    """Restatement of ``load3dQuadComsolMesh`` (src/3d/read_comsol.jl:121-182): 9-node
+    COMSOL quadrilaterals as QuadrilateralQuadraticLagrange (quadratic) or their first four
+    nodes as QuadrilateralLinear4 (linear); physics geometry, linear (QuadrilateralLinear4
+    on the corner nodes) or discquadconstant / discquadlinear / discquadquadratic."""
+    coords0, topo0 = read_comsol(path)
+    if topo0.shape[0] != 9:
+        raise ValueError("a 9-node quadrilateral (quad2) mesh is expected")
+    if geometry_order == "quadratic":
+        gk = QUAD9
+    elif geometry_order == "linear":
+        topo0 = topo0[:4, :]
+        gk = QUAD4
+    else:
+        raise ValueError("Only QuadQuadratic and QuadLinear geometry types supported")
+    topo, used = _remove_unused(topo0)
+    coords = coords0[:, used]
+    base = Mesh3d(coords, topo, coords[:, np.unique(topo)].copy(), topo.copy(), gk, gk)
+    if physics_order == "geometry":
+        return base
+    if physics_order == "linear":
+        ptop = topo[:4, :]
+        used_p = np.unique(ptop)
+        # The reference indexes ``sources`` with ``topology[1:4,:]`` directly; that is only
+        # consistent when corner nodes come first.
+        if used_p.size and int(used_p.max()) != used_p.size - 1:
+            raise ValueError("corner nodes are not numbered first; the reference's "
+                             "linear-physics indexing would be inconsistent")
+        return Mesh3d(coords, topo, coords[:, used_p].copy(), ptop.copy(), gk, QUAD4)
+    if physics_order in _DISC_QUAD_ORDERS:
+        return with_discontinuous_physics(base, physics_order, beta_type)
+    raise ValueError("physics_order must be geometry, linear, discquadconstant, "
+                     "discquadlinear or discquadquadratic")
+
+
 def read_mesh_file(path: str):
     """(coordinates 3 x Nc, topology 3 x E, 0-based) of a triangle mesh in Wavefront OBJ,
     OFF, PLY (ASCII or binary) or STL (ASCII or binary): the formats ``load_mesh_file`` reads
@@ -575,7 +729,7 @@ def with_coordinates(mesh: Mesh3d, coords) -> Mesh3d:
     with them (physics = geometry, or the corner nodes for linear physics), discontinuous
     physics nodes are interpolated again."""
     coords = np.asarray(coords, dtype=np.float64)
-    if mesh.physics_kind in (DTRI1, DTRI3, DTRI6):
+    if mesh.physics_kind in (DTRI1, DTRI3, DTRI6, DQUAD1, DQUAD4, DQUAD9):
         src, ptop = discontinuous_sources(coords, mesh.topology, mesh.geometry_kind,
                                           mesh.physics_kind, mesh.physics_beta)
         return Mesh3d(coords, mesh.topology, src, ptop, mesh.geometry_kind, mesh.physics_kind,

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
       double y[3] = {0, 0, 0}, tu[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
-        const double N = P[3 + g], du = P[9 + g], dv = P[15 + g];
+        const double N = P[kPtN + g], du = P[kPtDu + g], dv = P[kPtDv + g];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           y[d] = fma(X[g][d], N, y[d]);
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjointArgs a, int ech) {
                              gc[0] * tu[1] - gc[1] * tu[0]};
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
-        const double N = P[3 + g], du = P[9 + g], dv = P[15 + g];
+        const double N = P[kPtN + g], du = P[kPtDu + g], dv = P[kPtDv + g];
 #pragma unroll
         for (int dd = 0; dd < 3; ++dd)
           gX[g][dd] = fma(N, gd[dd], fma(du, gtu[dd], fma(dv, gtv[dd], gX[g][dd])));

@@ -64,18 +64,33 @@ void dfree(void* p) {
 
 void validate_mesh(const bie_mesh* m) {
   BIE_REQUIRE(m != nullptr, BIE_EINVAL, "mesh is NULL");
-  BIE_REQUIRE(m->geometry_kind == BIE_TRI3 || m->geometry_kind == BIE_TRI6, BIE_EUNSUPPORTED,
-              "geometry_kind must be BIE_TRI3 or BIE_TRI6");
-  const int pkd = m->physics_kind;
-  BIE_REQUIRE(pkd == BIE_TRI3 || pkd == BIE_TRI6 || pkd == BIE_DTRI1 || pkd == BIE_DTRI3 ||
-                  pkd == BIE_DTRI6,
+  const int gkd = m->geometry_kind, pkd = m->physics_kind;
+  BIE_REQUIRE(gkd == BIE_TRI3 || gkd == BIE_TRI6 || gkd == BIE_QUAD4 || gkd == BIE_QUAD4S ||
+                  gkd == BIE_QUAD8 || gkd == BIE_QUAD9,
               BIE_EUNSUPPORTED,
-              "physics_kind must be BIE_TRI3, BIE_TRI6, BIE_DTRI1, BIE_DTRI3 or BIE_DTRI6");
-  BIE_REQUIRE(!(m->geometry_kind == BIE_TRI3 && pkd == BIE_TRI6), BIE_EUNSUPPORTED,
-              "quadratic physics on linear geometry is not a reference configuration");
-  if (pkd == BIE_DTRI3 || pkd == BIE_DTRI6)
-    BIE_REQUIRE(std::isfinite(m->physics_beta) && 1.0 - 3.0 * m->physics_beta != 0.0,
-                BIE_EINVAL, "physics_beta must be finite and differ from 1/3");
+              "geometry_kind must be BIE_TRI3, BIE_TRI6, BIE_QUAD4, BIE_QUAD4S, BIE_QUAD8 or "
+              "BIE_QUAD9");
+  if (kind_is_quad(gkd)) {
+    BIE_REQUIRE(pkd == gkd || (gkd == BIE_QUAD9 && pkd == BIE_QUAD4) || pkd == BIE_DQUAD1 ||
+                    pkd == BIE_DQUAD4 || pkd == BIE_DQUAD9,
+                BIE_EUNSUPPORTED,
+                "physics on quadrilateral geometry must be the geometry kind, BIE_QUAD4 on "
+                "BIE_QUAD9, or BIE_DQUAD1 / BIE_DQUAD4 / BIE_DQUAD9");
+    if (pkd == BIE_DQUAD4 || pkd == BIE_DQUAD9)
+      BIE_REQUIRE(std::isfinite(m->physics_beta) && 1.0 - m->physics_beta != 0.0, BIE_EINVAL,
+                  "physics_beta must be finite and differ from 1");
+  } else {
+    BIE_REQUIRE(pkd == BIE_TRI3 || pkd == BIE_TRI6 || pkd == BIE_DTRI1 || pkd == BIE_DTRI3 ||
+                    pkd == BIE_DTRI6,
+                BIE_EUNSUPPORTED,
+                "physics on triangles must be BIE_TRI3, BIE_TRI6, BIE_DTRI1, BIE_DTRI3 or "
+                "BIE_DTRI6");
+    BIE_REQUIRE(!(gkd == BIE_TRI3 && pkd == BIE_TRI6), BIE_EUNSUPPORTED,
+                "quadratic physics on linear geometry is not a reference configuration");
+    if (pkd == BIE_DTRI3 || pkd == BIE_DTRI6)
+      BIE_REQUIRE(std::isfinite(m->physics_beta) && 1.0 - 3.0 * m->physics_beta != 0.0,
+                  BIE_EINVAL, "physics_beta must be finite and differ from 1/3");
+  }
   BIE_REQUIRE(m->index_base == 0 || m->index_base == 1, BIE_EINVAL, "index_base must be 0 or 1");
   BIE_REQUIRE((m->flags & ~BIE_MESH_SCALED_HYPOT) == 0, BIE_EINVAL, "unknown mesh flag bits");
   BIE_REQUIRE(m->n_elements >= 0 && m->n_coordinates >= 0 && m->n_sources >= 0, BIE_EINVAL,
@@ -105,7 +120,7 @@ std::vector<int> convert_topology(const int64_t* t, int rows, int64_t E, int64_t
 
 void ensure_rules(bie_dmesh* dm, int n, int m, hipStream_t s) {
   if (dm->cn == n && dm->cm == m && dm->d_elem) return;
-  RuleSet rs = build_ruleset(dm->gk, dm->pkind, dm->beta, n, m);
+  RuleSet rs = build_ruleset(dm->gkind, dm->pkind, dm->beta, n, m);
   if (dm->d_elem) (void)hipDeviceSynchronize();  // in-flight kernels may read the old tables
   dfree(dm->d_pts);
   dfree(dm->d_phi);
@@ -515,8 +530,8 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       const int64_t rowblocks = (M + 63) / 64;
       const bool fast = fast_trig_ok(dm, kr, ki, target_radius);
       const bool qc_inst = fast && (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16);
-      // the occupancy, split and two-element variants have no one-node physics instance
-      const bool tuned = qc_inst && dm->pk != 1;
+      // the occupancy, split and two-element variants exist for three- and six-node physics
+      const bool tuned = qc_inst && (dm->pk == 3 || dm->pk == 6);
       const int occ = tuned ? slot_occupancy() : 1;
       const int split = tuned && occ == 1 ? slot_split() : 1;
       const int vpw = tuned && occ == 1 && split == 1 ? slot_elems_per_wave() : 1;
@@ -593,6 +608,8 @@ void sweep_device_impl(bie_dmesh* dm, const double* d_k, int nk, const double* d
                        int64_t stride, double* dC, hipStream_t s, bool prepare_only,
                        bool taylor = false, double k_re = 0.0, double k_im = 0.0) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
+  BIE_REQUIRE(!kind_is_quad(dm->gkind), BIE_EUNSUPPORTED,
+              "sweeps and k-derivatives: triangular meshes only");
   BIE_REQUIRE(nk >= 0 && M >= 0, BIE_EINVAL, "negative size");
   BIE_REQUIRE((flags & ~BIE_FGC) == 0, BIE_EINVAL, "unknown flag bits");
   BIE_REQUIRE(taylor || nk == 0 || d_k, BIE_EINVAL, "k list is NULL");
@@ -944,7 +961,8 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     validate_mesh(mesh);
     // Host-side validation first: a malformed mesh is rejected before any HIP call.
     const int64_t base = mesh->index_base;
-    std::vector<int> topo = convert_topology(mesh->topology, mesh->geometry_kind,
+    std::vector<int> topo = convert_topology(mesh->topology,
+                                             geometry_nodes(mesh->geometry_kind),
                                              mesh->n_elements, base, mesh->n_coordinates,
                                              "topology");
     std::vector<int> ptopo = convert_topology(mesh->physics_topology,
@@ -963,12 +981,11 @@ int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
     dm->h_sources.assign(mesh->sources, mesh->sources + 3 * mesh->n_sources);
     dm->device = device;
     dm->hyp = (mesh->flags & BIE_MESH_SCALED_HYPOT) ? 1 : 0;
-    dm->gk = mesh->geometry_kind;
+    dm->gk = geometry_nodes(mesh->geometry_kind);
+    dm->gkind = mesh->geometry_kind;
     dm->pk = physics_nodes(mesh->physics_kind);
     dm->pkind = mesh->physics_kind;
-    dm->beta = (mesh->physics_kind == BIE_DTRI3 || mesh->physics_kind == BIE_DTRI6)
-                   ? mesh->physics_beta
-                   : 0.0;
+    dm->beta = kind_has_beta(mesh->physics_kind) ? mesh->physics_beta : 0.0;
     dm->Nc = mesh->n_coordinates;
     dm->E = mesh->n_elements;
     dm->N = mesh->n_sources;
@@ -1066,8 +1083,9 @@ int bie_select_rules_device(bie_dmesh* dm, const double* d_targets, int64_t n_ta
       BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
     }
     BIE_HIP_CHECK(hipSetDevice(dm->device));
-    launch_select(d_targets, n_targets, dm->d_selnodes, dm->pk, (int)dm->E,
-                  dm->gk == 6 ? 6 : 3, dm->hyp, d_rules, static_cast<hipStream_t>(stream));
+    const int R = kind_is_quad(dm->gkind) ? 1 : (dm->gk == 6 ? 6 : 3);
+    launch_select(d_targets, n_targets, dm->d_selnodes, dm->pk, (int)dm->E, R, dm->hyp,
+                  d_rules, static_cast<hipStream_t>(stream));
   });
 }
 
@@ -1133,12 +1151,11 @@ std::vector<unsigned char> mesh_key(const bie_mesh* m) {
   const int64_t scal[7] = {m->n_coordinates, m->n_elements, m->n_sources, m->geometry_kind,
                            m->physics_kind, m->index_base, m->flags};
   append_bytes(key, scal, sizeof(scal));
-  const double beta = (m->physics_kind == BIE_DTRI3 || m->physics_kind == BIE_DTRI6)
-                          ? m->physics_beta
-                          : 0.0;
+  const double beta = kind_has_beta(m->physics_kind) ? m->physics_beta : 0.0;
   append_bytes(key, &beta, sizeof(beta));
   append_bytes(key, m->coordinates, sizeof(double) * 3 * (size_t)m->n_coordinates);
-  append_bytes(key, m->topology, sizeof(int64_t) * (size_t)m->geometry_kind * m->n_elements);
+  append_bytes(key, m->topology,
+               sizeof(int64_t) * (size_t)geometry_nodes(m->geometry_kind) * m->n_elements);
   append_bytes(key, m->sources, sizeof(double) * 3 * (size_t)m->n_sources);
   append_bytes(key, m->physics_topology,
                sizeof(int64_t) * (size_t)physics_nodes(m->physics_kind) * m->n_elements);
@@ -1347,8 +1364,8 @@ void sparse_destroy_impl(bie_sparse* sp) {
 bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
   BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
-  BIE_REQUIRE(!physics_discontinuous(dm->pkind), BIE_EUNSUPPORTED,
-              "sparse near field: continuous physics only");
+  BIE_REQUIRE(!physics_discontinuous(dm->pkind) && !kind_is_quad(dm->gkind), BIE_EUNSUPPORTED,
+              "sparse near field: continuous physics on triangles only");
   BIE_REQUIRE(!std::isinf(offset), BIE_EINVAL, "offset must be finite (NaN: the default 0)");
   const double offr = std::isnan(offset) ? 0.0 : offset;
   BIE_HIP_CHECK(hipSetDevice(dm->device));
@@ -1398,7 +1415,7 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
     sp->off[c] = (int)(rec.size() / RS);
     sp->len[c] = (int)u.size();
     for (size_t q = 0; q < u.size(); ++q) {
-      double N[6], du[6], dv[6], Np[6], dup[6], dvp[6];
+      double N[9], du[9], dv[9], Np[9], dup[9], dvp[9];
       geometry_basis(NG, u[q], v[q], N, du, dv);
       geometry_basis(NP, u[q], v[q], Np, dup, dvp);
       std::vector<double> r(RS, 0.0);
@@ -1502,8 +1519,9 @@ int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz) {
   return guarded([&] {
     validate_mesh(mesh);
     BIE_REQUIRE(nnz != nullptr, BIE_EINVAL, "nnz is NULL");
-    BIE_REQUIRE(!physics_discontinuous(mesh->physics_kind), BIE_EUNSUPPORTED,
-                "sparse near field: continuous physics only");
+    BIE_REQUIRE(!physics_discontinuous(mesh->physics_kind) &&
+                    !kind_is_quad(mesh->geometry_kind),
+                BIE_EUNSUPPORTED, "sparse near field: continuous physics on triangles only");
     std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
                                               mesh->n_elements, mesh->index_base,
                                               mesh->n_sources, "physics_topology");
@@ -1607,9 +1625,9 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
     validate_mesh(mesh);
     BIE_REQUIRE(min_segments >= 1, BIE_EINVAL, "min_segments must be >= 1");
     const int64_t base = mesh->index_base;
-    std::vector<int> topo = convert_topology(mesh->topology, mesh->geometry_kind,
-                                             mesh->n_elements, base, mesh->n_coordinates,
-                                             "topology");
+    const int ng = geometry_nodes(mesh->geometry_kind);
+    std::vector<int> topo = convert_topology(mesh->topology, ng, mesh->n_elements, base,
+                                             mesh->n_coordinates, "topology");
     const int np = physics_nodes(mesh->physics_kind);
     std::vector<int> ptopo = convert_topology(mesh->physics_topology, np,
                                               mesh->n_elements, base, mesh->n_sources,
@@ -1620,7 +1638,7 @@ int bie_schedule_check(const bie_mesh* mesh, int32_t min_segments, int64_t* stat
       for (int d = 0; d < 3; ++d) {
         double acc = 0;
         for (int k = 0; k < 3; ++k)
-          acc += mesh->coordinates[3 * (size_t)topo[e * mesh->geometry_kind + k] + d];
+          acc += mesh->coordinates[3 * (size_t)topo[e * ng + k] + d];
         cent[3 * e + d] = acc / 3.0;
       }
     SlotSchedule sc = build_slot_schedule(ptopo.data(), np, (int)E,
@@ -1653,8 +1671,8 @@ void assembly_vjp_impl(bie_dmesh* dm, double k, double ki, bool cplx, const doub
   BIE_REQUIRE(n_targets >= 0, BIE_EINVAL, "negative number of targets");
   BIE_REQUIRE(d_alpha && d_p && d_grad_coords, BIE_EINVAL, "NULL argument");
   BIE_REQUIRE(std::isfinite(k) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
-  BIE_REQUIRE(!physics_discontinuous(dm->pkind), BIE_EUNSUPPORTED,
-              "assembly adjoint: continuous physics only");
+  BIE_REQUIRE(!physics_discontinuous(dm->pkind) && !kind_is_quad(dm->gkind), BIE_EUNSUPPORTED,
+              "assembly adjoint: continuous physics on triangles only");
   if (!d_targets) {
     d_targets = dm->d_sources;
     BIE_REQUIRE(n_targets == dm->N, BIE_EINVAL, "collocation needs n_targets == n_sources");
@@ -2013,6 +2031,8 @@ int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_
       BIE_REQUIRE(meshes[ii] != nullptr, BIE_EINVAL, "batch mesh is NULL");
       BIE_REQUIRE(meshes[ii]->device == meshes[0]->device, BIE_EINVAL,
                   "batch meshes live on different devices");
+      BIE_REQUIRE(!kind_is_quad(meshes[ii]->gkind), BIE_EUNSUPPORTED,
+                  "batches: triangular meshes only");
       BIE_REQUIRE(meshes[ii]->gk == meshes[0]->gk && meshes[ii]->pkind == meshes[0]->pkind &&
                       meshes[ii]->beta == meshes[0]->beta,
                   BIE_EUNSUPPORTED, "batch meshes must share geometry and physics kinds");
@@ -2025,7 +2045,7 @@ int bie_batch_create(bie_dmesh* const* meshes, int32_t n_inst, int32_t n, int32_
                   "a quadratic-physics mesh may appear only once in a batch");
     }
     {
-      RuleSet rs = build_ruleset(meshes[0]->gk, meshes[0]->pkind, meshes[0]->beta, n, m);
+      RuleSet rs = build_ruleset(meshes[0]->gkind, meshes[0]->pkind, meshes[0]->beta, n, m);
       BIE_REQUIRE(sizeof(double) * (size_t)rs.R * rs.Qmax * rs.pk <= 24 * 1024, BIE_EUNSUPPORTED,
                   "batch supports quadrature orders up to n*m' = 36");
     }

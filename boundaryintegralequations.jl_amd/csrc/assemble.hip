@@ -50,7 +50,7 @@ k_geometry(const double* __restrict__ coords, const int* __restrict__ topo, int 
     const int node = topo[(size_t)e * gk + k];
     const double X0 = coords[3 * (size_t)node], X1 = coords[3 * (size_t)node + 1],
                  X2 = coords[3 * (size_t)node + 2];
-    const double N = P[3 + k], du = P[9 + k], dv = P[15 + k];
+    const double N = P[kPtN + k], du = P[kPtDu + k], dv = P[kPtDv + k];
     y0 += X0 * N;
     y1 += X1 * N;
     y2 += X2 * N;
@@ -521,10 +521,12 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   const double kr = a.kr, ki = a.ki;
   double csum = 0.0;
   const int Qc = QC > 0 ? QC : a.len[0];
-  const int nchunk = 12 * Qc;
+  // the rows of the element's vertex rules (three on triangles, the one rule of a
+  // quadrilateral) and, when there is a rule to select, its selection nodes
+  const int nchunk = 4 * Qc * (a.R < 3 ? a.R : 3);
   double* my_rows = STAGED ? s_rows + (size_t)wave * stage_wave_doubles(Qc) : nullptr;
   double* my_sel = STAGED ? my_rows + 24 * Qc : nullptr;  // the element's selection nodes
-  constexpr int nsel = NP * 2;                             // 16-byte chunks of them
+  const int nsel = a.R == 1 ? 0 : NP * 2;                  // 16-byte chunks of them
   double2 stg[3], stg_sel;
   constexpr int WV = W * V;  // elements per step; wave w takes elements v * W + w, v < V
   const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
@@ -1100,7 +1102,9 @@ void launch_select(const double* targets, int64_t M, const double* sel, int NP, 
   int64_t blocks = (M * (int64_t)E + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   dim3 grid((unsigned)blocks);
-  if (NP == 1)
+  if (R == 1)  // one rule (quadrilaterals): the selection is rule 1 whatever the nodes
+    hipLaunchKernelGGL(k_select<1>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
+  else if (NP == 1)
     hipLaunchKernelGGL(k_select<1>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
   else if (NP == 3)
     hipLaunchKernelGGL(k_select<3>, grid, dim3(256), 0, s, targets, M, sel, E, R, hyp, out);
@@ -1119,7 +1123,8 @@ void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipSt
 
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;
-  BIE_REQUIRE(sa.W == kSlotWaves || (sa.W == 2 * kSlotWaves && fast && a.ki == 0.0 && a.NP != 1 &&
+  BIE_REQUIRE(sa.W == kSlotWaves || (sa.W == 2 * kSlotWaves && fast && a.ki == 0.0 &&
+                                      (a.NP == 3 || a.NP == 6) &&
                                       sa.occupancy == 1 && sa.split == 1 &&
                                       (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16)),
               BIE_EINVAL, "slot schedule built for another wave count");
@@ -1127,8 +1132,9 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
   BIE_REQUIRE(sa.phi_in_lds, BIE_EINVAL, "slot kernel needs the basis table in LDS");
-  // (one-node physics has no split instance: its pairs are short)
-  const int split = (sa.split == 2 && fast && a.ki == 0.0 && a.NP != 1) ? 2 : 1;
+  // (only three- and six-node physics have split instances)
+  const int split =
+      (sa.split == 2 && fast && a.ki == 0.0 && (a.NP == 3 || a.NP == 6)) ? 2 : 1;
   const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + phi_pad(a.R * a.Qmax * a.NP) +
                                          kTrigDoubles +
                                          (split == 2 ? split_stage_doubles(a.len[0], 2)
@@ -1169,11 +1175,11 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
   auto fast_go = [&](auto np, auto g) -> bool {
     constexpr int NPc = decltype(np)::value;
     constexpr bool Gc = decltype(g)::value;
-    if constexpr (NPc == 1) {  // one-node physics: the plain instance only
+    if constexpr (NPc != 3 && NPc != 6) {  // other node counts: the plain instance only
       switch (qc) {
-        case 4: go(k_assemble_slots<1, false, W, Gc, 4, true>); return true;
-        case 6: go(k_assemble_slots<1, false, W, Gc, 6, true>); return true;
-        case 16: go(k_assemble_slots<1, false, W, Gc, 16, true>); return true;
+        case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true>); return true;
+        case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true>); return true;
+        case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true>); return true;
         default: return false;
       }
     } else {
@@ -1201,25 +1207,34 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
       }
     }
   };
-  using T1 = std::integral_constant<int, 1>;
-  using T3 = std::integral_constant<int, 3>;
-  using T6 = std::integral_constant<int, 6>;
   using GT = std::true_type;
   using GF = std::false_type;
-  bool done = false;
-  if (a.NP == 1)
-    done = gon ? fast_go(T1{}, GT{}) : fast_go(T1{}, GF{});
-  else if (a.NP == 3)
+  // fast instance, else the generic one, for the node count NPc
+  auto run = [&](auto np) {
+    constexpr int NPc = decltype(np)::value;
+    if (gon ? fast_go(np, GT{}) : fast_go(np, GF{})) return;
+    if (cplx)
+      gon ? go(k_assemble_slots<NPc, true, W, true>) : go(k_assemble_slots<NPc, true, W, false>);
+    else
+      gon ? go(k_assemble_slots<NPc, false, W, true>) : go(k_assemble_slots<NPc, false, W, false>);
+  };
+  bool done = true;
+  switch (a.NP) {
+    case 1: run(std::integral_constant<int, 1>{}); break;
+    case 4: run(std::integral_constant<int, 4>{}); break;
+    case 8: run(std::integral_constant<int, 8>{}); break;
+    case 9: run(std::integral_constant<int, 9>{}); break;
+    default: done = false;
+  }
+  using T3 = std::integral_constant<int, 3>;
+  using T6 = std::integral_constant<int, 6>;
+  if (!done && a.NP == 3)
     done = gon ? fast_go(T3{}, GT{}) : fast_go(T3{}, GF{});
-  else
+  else if (!done)
     done = gon ? fast_go(T6{}, GT{}) : fast_go(T6{}, GF{});
   if (!done) {
-    if (a.NP == 1) {
-      if (cplx)
-        gon ? go(k_assemble_slots<1, true, W, true>) : go(k_assemble_slots<1, true, W, false>);
-      else
-        gon ? go(k_assemble_slots<1, false, W, true>) : go(k_assemble_slots<1, false, W, false>);
-    } else if (a.NP == 3) {
+    BIE_REQUIRE(a.NP == 3 || a.NP == 6, BIE_EINVAL, "internal: no slot kernel for this node count");
+    if (a.NP == 3) {
       if (cplx)
         gon ? go(k_assemble_slots<3, true, W, true>) : go(k_assemble_slots<3, true, W, false>);
       else
@@ -1378,11 +1393,21 @@ void launch_assemble(const AssemblyArgs& a, hipStream_t s) {
   chunks = (a.E + ech - 1) / ech;
   dim3 grid((unsigned)rowblocks, (unsigned)chunks);
   const bool cplx = a.ki != 0.0;
-  if (a.NP == 1) {
+  auto go = [&](auto np) {
+    constexpr int NPc = decltype(np)::value;
     if (cplx)
-      hipLaunchKernelGGL((k_assemble<1, true>), grid, dim3(256), 0, s, a, ech);
+      hipLaunchKernelGGL((k_assemble<NPc, true>), grid, dim3(256), 0, s, a, ech);
     else
-      hipLaunchKernelGGL((k_assemble<1, false>), grid, dim3(256), 0, s, a, ech);
+      hipLaunchKernelGGL((k_assemble<NPc, false>), grid, dim3(256), 0, s, a, ech);
+  };
+  if (a.NP == 1) {
+    go(std::integral_constant<int, 1>{});
+  } else if (a.NP == 4) {
+    go(std::integral_constant<int, 4>{});
+  } else if (a.NP == 8) {
+    go(std::integral_constant<int, 8>{});
+  } else if (a.NP == 9) {
+    go(std::integral_constant<int, 9>{});
   } else if (a.NP == 3) {
     if (cplx)
       hipLaunchKernelGGL((k_assemble<3, true>), grid, dim3(256), 0, s, a, ech);

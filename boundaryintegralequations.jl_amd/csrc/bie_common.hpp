@@ -41,13 +41,17 @@ constexpr double kTabFastMaxArg = 70368744177664.0;
 
 // Point record of the concatenated rule set: u, v, w, then N, dN/du, dN/dv of the
 // geometry basis (6 slots each, unused slots zero).
-constexpr int kPtStride = 21;
+constexpr int kPtStride = 30;
+// offsets in a point record of N, dN/du and dN/dv (9 geometry nodes each)
+constexpr int kPtN = 3, kPtDu = 12, kPtDv = 21;
 
-// All quadrature rules one assembly uses, for geometry kind gk and physics kind pk.
-// Rules 1..3 cluster at the corners (Q = n*m each); rules 4..6 at the edge mid points of
-// quadratic elements (Q = 2*n*m', m' = m rounded up to even).
+// All quadrature rules one assembly uses, for a geometry and a physics kind.  Triangles:
+// rules 1..3 cluster at the corners (Q = n*m each); rules 4..6 at the edge mid points of
+// quadratic elements (Q = 2*n*m', m' = m rounded up to even).  Quadrilaterals: one tensor
+// Gauss-Legendre rule on [-1, 1]^2 (Q = n*m, R = 1).
 struct RuleSet {
-  int gk = 3, pk = 3, n = 0, m = 0;  // pk: physics nodes per element
+  int gk = 3, pk = 3, n = 0, m = 0;  // geometry and physics nodes per element
+  int gkind = 3;                     // geometry kind (BIE_TRI3 .. BIE_QUAD9)
   int pkind = 3;                     // physics kind (BIE_TRI3 .. BIE_DTRI6)
   double beta = 0.0;                 // node offset of BIE_DTRI3 / BIE_DTRI6
   int R = 3;          // number of rules
@@ -60,11 +64,18 @@ struct RuleSet {
   std::vector<double> phi;  // R x Qmax x pk physics basis at the rule points
 };
 
-RuleSet build_ruleset(int gk, int pkind, double beta, int n, int m);
+RuleSet build_ruleset(int gkind, int pkind, double beta, int n, int m);
 
-// Physics kinds: the low byte is the node count; BIE_DTRI* are discontinuous.
+// Element kinds: the low byte is the node count; BIE_DTRI* / BIE_DQUAD* are discontinuous,
+// BIE_QUAD* / BIE_DQUAD* quadrilaterals (one Gauss rule, no selection).
 inline int physics_nodes(int pkind) { return pkind & 0xff; }
+inline int geometry_nodes(int gkind) { return gkind & 0xff; }
 inline bool physics_discontinuous(int pkind) { return (pkind & 0x100) != 0; }
+inline bool kind_is_quad(int kind) { return (kind >> 9) != 0; }
+// physics kinds whose nodes sit at the offset bie_mesh.physics_beta
+inline bool kind_has_beta(int pkind) {
+  return pkind == BIE_DTRI3 || pkind == BIE_DTRI6 || pkind == BIE_DQUAD4 || pkind == BIE_DQUAD9;
+}
 // The mid-edge split rules 4-6 exist on quadratic geometry (R = 6) and are chosen only when
 // the nearest of six physics nodes is an edge node.
 inline bool uses_mid_rules(int gk, int pk) { return gk == 6 && pk == 6; }
@@ -253,7 +264,8 @@ void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,
                     std::vector<int64_t>& pair_ptr, std::vector<int>& pair_elem);
 void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s);
 int sparse_record_doubles();
-// Geometry basis (kind 3 or 6) and its derivatives at (u, v), 6 slots each (rules.cpp).
+// Geometry basis of a geometry kind and its derivatives at (u, v) (rules.cpp); N, du, dv
+// hold 9 entries each (slots beyond the kind's node count are zeroed).
 void geometry_basis(int kind, double u, double v, double* N, double* du, double* dv);
 
 // Launchers (assemble.hip)
@@ -337,6 +349,7 @@ void launch_select(const double* targets, int64_t M, const double* selnodes, int
 struct bie_dmesh {
   int device = 0;
   int gk = 3, pk = 3;           // geometry nodes, physics nodes per element
+  int gkind = 3;                // geometry kind (BIE_TRI3 .. BIE_QUAD9)
   int pkind = 3;                // physics kind (BIE_TRI3 .. BIE_DTRI6)
   double beta = 0.0;            // node offset of BIE_DTRI3 / BIE_DTRI6
   int hyp = 0;                  // rule-selection distance (BIE_MESH_SCALED_HYPOT)

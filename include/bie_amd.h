@@ -38,6 +38,29 @@ extern "C" {
 #define BIE_DTRI1 0x101
 #define BIE_DTRI3 0x103
 #define BIE_DTRI6 0x106
+/* Quadrilaterals (geometry and physics; the low byte is the node count).  They take the
+ * reference's generic path (assemble_parallel!(mesh, k, sources, ::SurfaceFunction),
+ * assembly_collocation.jl:481-538): one tensor Gauss-Legendre rule on [-1,1]^2 for every
+ * pair, m points in u and n in v (create_shape_function calls getQuadpoints(sf, m, n)), no
+ * rule selection.  Node orders (src/3d/surface_functions.jl:1-149, SurfaceFunction.jl:439-470):
+ *   BIE_QUAD4   QuadrilateralLinear4 (COMSOL): (-1,-1) (1,-1) (-1,1) (1,1)
+ *   BIE_QUAD4S  QuadrilateralLinearSerendipity: counter-clockwise from (-1,-1)
+ *   BIE_QUAD8   QuadrilateralQuadratic: corners counter-clockwise, then (0,-1) (1,0) (0,1) (-1,0)
+ *   BIE_QUAD9   QuadrilateralQuadraticLagrange (COMSOL): (-1,-1) (1,-1) (-1,1) (1,1) (0,-1)
+ *               (-1,0) (0,0) (1,0) (0,1); odd n and m together are refused as the reference
+ *               refuses them ("Gauss Node On Singularity", :497-504)
+ * Physics on quadrilateral geometry: the geometry kind itself, BIE_QUAD4 on BIE_QUAD9 (the
+ * loader's physics_order = :linear, corner nodes first), or the discontinuous
+ *   BIE_DQUAD1  DiscontinuousQuadrilateralConstant, node (0,0)
+ *   BIE_DQUAD4  DiscontinuousQuadrilateralLinear4: the BIE_QUAD4 basis of (u, v) / (1 - beta)
+ *   BIE_DQUAD9  DiscontinuousQuadrilateralQuadraticLagrange: BIE_QUAD9 of (u, v) / (1 - beta) */
+#define BIE_QUAD4 0x204
+#define BIE_QUAD4S 0x404
+#define BIE_QUAD8 0x208
+#define BIE_QUAD9 0x209
+#define BIE_DQUAD1 0x301
+#define BIE_DQUAD4 0x304
+#define BIE_DQUAD9 0x309
 
 #define BIE_F 1u            /* fOn */
 #define BIE_G 2u            /* gOn */
@@ -54,19 +77,22 @@ extern "C" {
 /* Mesh3d (src/utils/mesh.jl:23-34), the fields assembly reads. */
 typedef struct bie_mesh {
   const double* coordinates;        /* 3 x n_coordinates                              */
-  const int64_t* topology;          /* geometry_kind x n_elements                     */
+  const int64_t* topology;          /* nodes x n_elements (geometry_kind & 0xff)       */
   const double* sources;            /* 3 x n_sources (collocation nodes)               */
   const int64_t* physics_topology;  /* nodes x n_elements (nodes = physics_kind & 0xff),
                                        indexes sources                                  */
   int64_t n_coordinates;
   int64_t n_elements;
   int64_t n_sources;
-  int32_t geometry_kind;            /* BIE_TRI3 | BIE_TRI6 */
-  int32_t physics_kind;             /* BIE_TRI3 | BIE_TRI6 | BIE_DTRI1 | BIE_DTRI3 | BIE_DTRI6 */
+  int32_t geometry_kind;            /* BIE_TRI3 | BIE_TRI6 | BIE_QUAD4 | BIE_QUAD4S | BIE_QUAD8 |
+                                       BIE_QUAD9                                        */
+  int32_t physics_kind;             /* a geometry kind, BIE_DTRI1 | BIE_DTRI3 | BIE_DTRI6,
+                                       BIE_DQUAD1 | BIE_DQUAD4 | BIE_DQUAD9 (see above)   */
   int32_t index_base;               /* 1 for Julia arrays, 0 for C/Python              */
   int32_t flags;                    /* 0 or BIE_MESH_SCALED_HYPOT                      */
-  double physics_beta;              /* node offset beta of BIE_DTRI3 / BIE_DTRI6 (the
-                                       physics function's beta field), else ignored     */
+  double physics_beta;              /* node offset beta of BIE_DTRI3 / BIE_DTRI6 /
+                                       BIE_DQUAD4 / BIE_DQUAD9 (the physics function's
+                                       beta field), else ignored                        */
 } bie_mesh;
 
 typedef struct bie_dmesh bie_dmesh;  /* device-resident mesh + cached rule tables */

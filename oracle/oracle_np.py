@@ -12,6 +12,12 @@ small meshes.  Reference functions restated (file:line):
   triangularLinear/Quadratic   src/3d/surface_functions.jl:191-233
   discontinuousTriangular{Constant,Linear,Quadratic} src/3d/surface_functions.jl:234-318
     (physics kinds 0x101, 0x103, 0x106 with the mesh's physics_beta)
+  quadrilateralQuadpoints      src/3d/quadrature.jl:9-20, as create_shape_function calls it
+                               (getQuadpoints(sf, m, n), assembly_collocation.jl:12-20)
+  quadrilateral bases          src/3d/surface_functions.jl:1-149 (kinds 0x204 Linear4,
+                               0x404 LinearSerendipity, 0x208 Quadratic, 0x209
+                               QuadraticLagrange; discontinuous 0x301, 0x304, 0x309)
+  generic assemble_parallel!   src/3d/assembly_collocation.jl:481-538 (one rule, no selection)
   interpolate_elements/jacobian! src/3d/assembly_collocation.jl:27-66, src/3d/jacobian.jl:7-87
   find_closest_corner          src/3d/assembly_collocation.jl:102-115
   greens3d!/freens3d!/freens3dk0! src/3d/kernels.jl:13-55
@@ -36,7 +42,28 @@ def tensor(n, m):
     return u, v, w
 
 
-def rule(c, n, m):
+QUAD_KINDS = (0x204, 0x404, 0x208, 0x209)
+
+
+def is_quad(kind):
+    return kind in QUAD_KINDS or kind in (0x301, 0x304, 0x309)
+
+
+def quad_rule(n, m):
+    """The quadrilateral rule of assemble_parallel!'s generic method: create_shape_function
+    passes (m, n) to quadrilateralQuadpoints(n, m), so u runs over m Gauss-Legendre points on
+    [-1, 1] and v over n."""
+    xm, wm = np.polynomial.legendre.leggauss(m)
+    xn, wn = np.polynomial.legendre.leggauss(n)
+    u = np.kron(np.ones(n), xm)
+    v = np.kron(xn, np.ones(m))
+    w = np.kron(wn, wm)
+    return u, v, w
+
+
+def rule(c, n, m, quad=False):
+    if quad:
+        return quad_rule(n, m)
     if c == 1:
         u, v, w = tensor(n, m)
         return u * v, (1.0 - u) * v, w * v
@@ -83,8 +110,68 @@ def disc_basis(kind, u, v, beta):
     return N, du, dv
 
 
+def quad_basis(kind, u, v, beta=0.0):
+    """Quadrilateral bases, N, dN/du, dN/dv with shape (Q, nodes), in the reference's
+    formulas (surface_functions.jl:8-11, 22-49, 62-70, 81-90, 103, 123, 140)."""
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    o, z = np.ones_like(u), np.zeros_like(u)
+    if kind == 0x301:
+        return o[:, None], z[:, None], z[:, None]
+    if kind in (0x304, 0x309):
+        s = 1.0 / (1.0 - beta)
+        N, du, dv = quad_basis(0x204 if kind == 0x304 else 0x209, u * s, v * s)
+        return N, du * s, dv * s
+    if kind == 0x204:
+        N = [(1 - u) * (1 - v) / 4, (1 + u) * (1 - v) / 4, (1 - u) * (1 + v) / 4,
+             (1 + u) * (1 + v) / 4]
+        du = [-(1 - v) / 4, (1 - v) / 4, -(1 + v) / 4, (1 + v) / 4]
+        dv = [-(1 - u) / 4, -(1 + u) / 4, (1 - u) / 4, (1 + u) / 4]
+    elif kind == 0x404:
+        N = [(1 - u) * (1 - v) / 4, (1 + u) * (1 - v) / 4, (1 + u) * (1 + v) / 4,
+             (1 - u) * (1 + v) / 4]
+        du = [-(1 - v) / 4, (1 - v) / 4, (1 + v) / 4, -(1 + v) / 4]
+        dv = [-(1 - u) / 4, -(1 + u) / 4, (1 + u) / 4, (1 - u) / 4]
+    elif kind == 0x208:
+        N = [(1 - u) * (v - 1) * (u + v + 1) / 4, (1 + u) * (v - 1) * (v - u + 1) / 4,
+             (1 + u) * (v + 1) * (u + v - 1) / 4, (u - 1) * (v + 1) * (u - v + 1) / 4,
+             (1 - v) * (1 - u ** 2) / 2, (1 + u) * (1 - v ** 2) / 2,
+             (1 + v) * (1 - u ** 2) / 2, (1 - u) * (1 - v ** 2) / 2]
+        du = [-(v - 1) * (u + v + 1) / 4 + (1 - u) * (v - 1) / 4,
+              (v - 1) * (v - u + 1) / 4 - (1 + u) * (v - 1) / 4,
+              (v + 1) * (u + v - 1) / 4 + (1 + u) * (v + 1) / 4,
+              (v + 1) * (u - v + 1) / 4 + (u - 1) * (v + 1) / 4,
+              -(1 - v) * u, (1 - v ** 2) / 2, -(1 + v) * u, -(1 - v ** 2) / 2]
+        dv = [(1 - u) * (u + v + 1) / 4 + (1 - u) * (v - 1) / 4,
+              (1 + u) * (v - u + 1) / 4 + (1 + u) * (v - 1) / 4,
+              (1 + u) * (u + v - 1) / 4 + (1 + u) * (v + 1) / 4,
+              (u - 1) * (u - v + 1) / 4 - (u - 1) * (v + 1) / 4,
+              (u ** 2 - 1) / 2, -(1 + u) * v, (1 - u ** 2) / 2, -(1 - u) * v]
+    elif kind == 0x209:
+        N = [u * (1 - u) * v * (1 - v) / 4, -u * (1 + u) * v * (1 - v) / 4,
+             -u * (1 - u) * v * (1 + v) / 4, u * (1 + u) * v * (1 + v) / 4,
+             -(1 + u) * (1 - u) * v * (1 - v) / 2, -u * (1 - u) * (1 + v) * (1 - v) / 2,
+             (1 - u ** 2) * (1 - v ** 2), u * (1 + u) * (1 + v) * (1 - v) / 2,
+             (1 + u) * (1 - u) * (1 + v) * v / 2]
+        # derivatives of the products above (the reference takes them by ForwardDiff)
+        du = [(1 - 2 * u) * v * (1 - v) / 4, -(1 + 2 * u) * v * (1 - v) / 4,
+              -(1 - 2 * u) * v * (1 + v) / 4, (1 + 2 * u) * v * (1 + v) / 4,
+              2 * u * v * (1 - v) / 2, -(1 - 2 * u) * (1 - v * v) / 2,
+              -2 * u * (1 - v ** 2), (1 + 2 * u) * (1 - v * v) / 2, -2 * u * (1 + v) * v / 2]
+        dv = [u * (1 - u) * (1 - 2 * v) / 4, -u * (1 + u) * (1 - 2 * v) / 4,
+              -u * (1 - u) * (1 + 2 * v) / 4, u * (1 + u) * (1 + 2 * v) / 4,
+              -(1 - u * u) * (1 - 2 * v) / 2, u * (1 - u) * 2 * v / 2,
+              -(1 - u ** 2) * 2 * v, -u * (1 + u) * 2 * v / 2, (1 - u * u) * (1 + 2 * v) / 2]
+    else:
+        raise ValueError("not a quadrilateral kind")
+    return (np.stack([x + z for x in N], axis=1), np.stack([x + z for x in du], axis=1),
+            np.stack([x + z for x in dv], axis=1))
+
+
 def basis(kind, u, v, beta=0.0):
     """Returns N, dN/du, dN/dv with shape (Q, nodes)."""
+    if is_quad(kind):
+        return quad_basis(kind, u, v, beta)
     if kind > 0xFF:
         return disc_basis(kind, u, v, beta)
     u = np.asarray(u, dtype=np.float64)
@@ -120,8 +207,12 @@ def julia_hypot3(a, b, c):
     return out
 
 
+def n_rules(mesh):
+    return 1 if is_quad(mesh.geometry_kind) else (6 if mesh.geometry_kind == 6 else 3)
+
+
 def element_geometry(mesh, c, n, m):
-    u, v, w = rule(c, n, m)
+    u, v, w = rule(c, n, m, is_quad(mesh.geometry_kind))
     N, du, dv = basis(mesh.geometry_kind, u, v)
     X = mesh.coordinates[:, mesh.topology]            # (3, ng, E)
     y = np.einsum("dke,qk->eqd", X, N)
@@ -141,8 +232,7 @@ def select(mesh, targets):
                      P[2][None] - targets[2][:, None, None])   # (M, np, E)
     # first strict minimum: argmin returns the first occurrence of the minimum
     idx = np.argmin(np.where(np.isnan(d), np.inf, d), axis=1) + 1
-    R = 6 if mesh.geometry_kind == 6 else 3
-    return np.minimum(idx, R).astype(np.int8)
+    return np.minimum(idx, n_rules(mesh)).astype(np.int8)
 
 
 def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True, rules=None):
@@ -150,7 +240,10 @@ def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True, rule
     selecting it."""
     targets = mesh.sources if targets is None else np.asarray(targets, dtype=np.float64)
     M, N, E = targets.shape[1], mesh.n_sources, mesh.n_elements
-    R = 6 if mesh.geometry_kind == 6 else 3
+    if mesh.geometry_kind == 0x209 and n % 2 == 1 and m % 2 == 1:
+        # assembly_collocation.jl:497-504: a Gauss node on the centre node
+        raise ValueError("Gauss Node On Singularity.")
+    R = n_rules(mesh)
     sel = select(mesh, targets) if rules is None else np.asarray(rules)
     F = np.zeros((M, N), dtype=np.complex128)
     G = np.zeros((M, N), dtype=np.complex128)
@@ -161,7 +254,7 @@ def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True, rule
         if ii.size == 0:
             continue
         y, nrm, jw = element_geometry(mesh, c, n, m)
-        u, v, _ = rule(c, n, m)
+        u, v, _ = rule(c, n, m, is_quad(mesh.geometry_kind))
         phi, _, _ = basis(mesh.physics_kind, u, v,
                           getattr(mesh, "physics_beta", 0.0))  # (Q, np)
         d = y[ee] - targets.T[ii][:, None, :]                 # (P, Q, 3)

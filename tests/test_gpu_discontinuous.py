@@ -15,41 +15,11 @@ import oracle_np
 from bie_amd import _lib
 from bie_amd import mesh as M
 from bie_amd.assembly import Batch, DeviceMesh, assemble_parallel
-from gpu_helpers import DevBuf, sync
+from gpu_helpers import DevBuf, check_F, own_blocks, rel, sync
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 ORDERS = ["disctriconstant", "disctrilinear", "disctriquadratic"]
-
-
-def rel(a, b):
-    nb = np.linalg.norm(b)
-    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
-
-
-def own_blocks(mesh):
-    """Mask of the (row, column) entries of an element's own nodes."""
-    own = np.zeros((mesh.n_sources, mesh.n_sources), dtype=bool)
-    for e in range(mesh.n_elements):
-        idx = mesh.physics_topology[:, e]
-        own[np.ix_(idx, idx)] = True
-    return own
-
-
-def check_F(F, Fo, mesh, tag=""):
-    """Collocation F of discontinuous physics.  On an element's own nodes (which sit inside
-    it, away from the corner its rule is clustered at) the integrand carries the rounding
-    noise of n.d amplified by 1/r^3 at the rule points nearest the node: on flat elements n.d
-    is 0 exactly there and F is pure noise (~1e-12 of max |F| for quadratic physics, whose
-    edge nodes take rule 3 on linear geometry); on curved ones the noise sits on top of the
-    true value.  Those entries are held to 1e-9 of max |F|, every other entry to the bar."""
-    own = own_blocks(mesh)
-    fmax = np.max(np.abs(Fo))
-    err_own = np.max(np.abs(F[own] - Fo[own])) / fmax
-    print(f"  {tag} F off own blocks rel {rel(F[~own], Fo[~own]):.2e}, own blocks max "
-          f"{err_own:.2e} of max|F|", flush=True)
-    assert rel(F[~own], Fo[~own]) <= TOL
-    assert err_own <= 1e-9
 
 
 def disc_mesh(gorder, physics, f=3, beta_type="legendre"):
