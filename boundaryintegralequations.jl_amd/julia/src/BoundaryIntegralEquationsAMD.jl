@@ -9,11 +9,14 @@ Drop-in MI355X backend for `BoundaryIntegralEquations.assemble_parallel!`
     import BoundaryIntegralEquationsAMD: assemble_parallel!
     F, G, C = assemble_parallel!(mesh, k, mesh.sources; n=2, m=2)
 
-Supported meshes (TriangularLinear / TriangularQuadratic geometry with geometry, linear or
-discontinuous triangular physics) run on the GPU, dense and, for continuous physics,
-`sparse=true` (the near field, `sparse_assemble_parallel!`); everything else (quads, the
-near field of discontinuous physics) is forwarded to the reference method unchanged, so
-the call stays a drop-in.  Errors from
+Supported meshes run on the GPU: TriangularLinear / TriangularQuadratic geometry with
+geometry, linear or discontinuous triangular physics, and the quadrilaterals
+(QuadrilateralLinear4, QuadrilateralLinearSerendipity, QuadrilateralQuadratic,
+QuadrilateralQuadraticLagrange) with their own, linear or discontinuous quadrilateral
+physics; dense, and for continuous physics on triangles also `sparse=true` (the near
+field, `sparse_assemble_parallel!`).  Everything else (the near field of discontinuous
+physics and of quadrilaterals, other element types) is forwarded to the reference method
+unchanged, so the call stays a drop-in.  Errors from
 the library are thrown as `ArgumentError` (bad input) or `ErrorException` (HIP failure).
 
 This file is not executed in the build container (no Julia there); the same ABI is
@@ -55,6 +58,10 @@ hypot_flag() = hypot(0.6732655185893088, 0.3428080423874833, 0.13687617154257523
 
 element_kind(::BIE.TriangularLinear) = Int32(3)
 element_kind(::BIE.TriangularQuadratic) = Int32(6)
+element_kind(::BIE.QuadrilateralLinear4) = Int32(0x204)
+element_kind(::BIE.QuadrilateralLinearSerendipity) = Int32(0x404)
+element_kind(::BIE.QuadrilateralQuadratic) = Int32(0x208)
+element_kind(::BIE.QuadrilateralQuadraticLagrange) = Int32(0x209)
 element_kind(::Any) = Int32(0)
 
 # physics_kind of the C ABI: the continuous kinds, or 0x100 | node count for the
@@ -63,10 +70,16 @@ physics_kind(pf) = element_kind(pf)
 physics_kind(::BIE.DiscontinuousTriangularConstant) = Int32(0x101)
 physics_kind(::BIE.DiscontinuousTriangularLinear) = Int32(0x103)
 physics_kind(::BIE.DiscontinuousTriangularQuadratic) = Int32(0x106)
+physics_kind(::BIE.DiscontinuousQuadrilateralConstant) = Int32(0x301)
+physics_kind(::BIE.DiscontinuousQuadrilateralLinear4) = Int32(0x304)
+physics_kind(::BIE.DiscontinuousQuadrilateralQuadraticLagrange) = Int32(0x309)
 physics_beta(pf) = 0.0
 physics_beta(pf::BIE.DiscontinuousTriangularLinear) = Float64(pf.beta)
 physics_beta(pf::BIE.DiscontinuousTriangularQuadratic) = Float64(pf.beta)
-discontinuous(mesh) = physics_kind(mesh.physics_function) > 0xff
+physics_beta(pf::BIE.DiscontinuousQuadrilateralLinear4) = Float64(pf.beta)
+physics_beta(pf::BIE.DiscontinuousQuadrilateralQuadraticLagrange) = Float64(pf.beta)
+discontinuous(mesh) = (physics_kind(mesh.physics_function) & 0x100) != 0
+quadrilateral(mesh) = mesh.shape_function isa BIE.Quadrilateral
 
 bie_mesh(mesh, coords, topo, srcs, ptopo) =
     BieMesh(pointer(coords), pointer(topo), pointer(srcs), pointer(ptopo), size(coords, 2),
@@ -84,9 +97,15 @@ function check(rc::Integer)
     throw(ErrorException("bie_amd error $rc: " * msg))
 end
 
-supported(mesh) = element_kind(mesh.shape_function) != 0 &&
-                  physics_kind(mesh.physics_function) != 0 &&
-                  !(element_kind(mesh.shape_function) == 3 && physics_kind(mesh.physics_function) == 6)
+function supported(mesh)
+    g = element_kind(mesh.shape_function)
+    p = physics_kind(mesh.physics_function)
+    (g == 0 || p == 0) && return false
+    if mesh.shape_function isa BIE.Quadrilateral   # the combinations include/bie_amd.h lists
+        return p == g || (g == 0x209 && p == 0x204) || p in (0x301, 0x304, 0x309)
+    end
+    return p in (3, 6, 0x101, 0x103, 0x106) && !(g == 3 && p == 6)
+end
 
 """
     assemble_hip(mesh, k, in_sources; fOn=true, gOn=true, cOn=true, m=4, n=4)
@@ -130,9 +149,9 @@ GPU and everything else to `BoundaryIntegralEquations.assemble_parallel!`.
 """
 function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true, cOn=true,
                             sparse=false, m=4, n=4, progress=true, depth=2, offset=nothing)
-    # the near field of discontinuous physics (its own connectivity and polar rules) stays
-    # with the reference
-    if !supported(mesh) || (sparse && discontinuous(mesh))
+    # the near field of discontinuous physics (its own connectivity and polar rules) and of
+    # quadrilaterals stays with the reference
+    if !supported(mesh) || (sparse && (discontinuous(mesh) || quadrilateral(mesh)))
         return BIE.assemble_parallel!(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn,
                                       sparse=sparse, m=m, n=n, progress=progress,
                                       depth=depth, offset=offset)
@@ -222,6 +241,19 @@ function BIE.assemble_parallel!(mesh::BIE.Mesh3d, k::Number, in_sources::Abstrac
     return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
 end
 
+# Quadrilateral geometry takes the reference's generic method (assembly_collocation.jl:481-538,
+# defaults n = m = 4); this method is more specific than it for every Quadrilateral.
+function BIE.assemble_parallel!(mesh::BIE.Mesh3d, k::Number, in_sources::AbstractMatrix{<:Real},
+                                shape_function::BIE.Quadrilateral; fOn=true, gOn=true,
+                                cOn=true, m=4, n=4, progress=true)
+    if !(GPU_ROUTING[] && supported(mesh))
+        return invoke(BIE.assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,BIE.SurfaceFunction},
+                      mesh, k, in_sources, shape_function; fOn=fOn, gOn=gOn, cOn=cOn, m=m,
+                      n=n, progress=progress)
+    end
+    return assemble_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn, m=m, n=n)
+end
+
 # The near field: sparse_assemble_parallel!(mesh, k, sources, physics_function; fOn, gOn,
 # depth, offset, progress, Ngauss) (assembly_collocation_losses.jl:200-405), called by the
 # reference's fast operators (fast_multipole_method.jl:181,187) and lossy systems through
@@ -234,7 +266,7 @@ function BIE.sparse_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
                                                                BIE.TriangularQuadratic};
                                        fOn=true, gOn=true, depth=1, offset=nothing,
                                        progress=true, Ngauss=40)
-    if !(GPU_ROUTING[] && supported(mesh) && !discontinuous(mesh) &&
+    if !(GPU_ROUTING[] && supported(mesh) && !discontinuous(mesh) && !quadrilateral(mesh) &&
          size(sources, 2) == size(mesh.sources, 2))
         return invoke(BIE.sparse_assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,Any}, mesh, k,
                       sources, physics_function; fOn=fOn, gOn=gOn, depth=depth,
@@ -569,11 +601,11 @@ end
 Same signature and defaults as the reference's `taylor_assemble!` (src/3d/rosebem.jl:56-157):
 `From[:, :, q+1]` is the q-th k-derivative of F for q = 0..M, all orders from one device call
 (bie_assemble_taylor_device), reduced to `U' * F_q * U` on the host when a basis U is given.
-Unsupported meshes and M > 63 go to the reference method.
+Unsupported meshes, quadrilaterals and M > 63 go to the reference method.
 """
 function taylor_assemble!(mesh::BIE.Mesh3d, k, in_sources, shape_function; M=0, fOn=true,
                           gOn=true, cOn=true, m=3, n=3, progress=true, U=LinearAlgebra.I)
-    if !supported(mesh) || M > 63
+    if !supported(mesh) || quadrilateral(mesh) || M > 63
         return BIE.taylor_assemble!(mesh, k, in_sources, shape_function; M=M, fOn=fOn,
                                     gOn=gOn, cOn=cOn, m=m, n=n, progress=progress, U=U)
     end
