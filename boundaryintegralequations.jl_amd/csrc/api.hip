@@ -204,10 +204,22 @@ int slot_split() {
   return (v && std::atoi(v) == 2) ? 2 : 1;
 }
 
-int plain_slot_count(int nphi, int Qc, int occ = 1, int split = 1) {
+// Waves per workgroup of the single-k slot kernel on the real-k unrolled instances:
+// kWideWaves (three per SIMD) for three-node physics, kSlotWaves for six-node physics,
+// whose wide instance needs more registers than three waves per SIMD leave (measured
+// slower); BIE_AMD_WAVES=8 or 12 forces either.
+int slot_waves(int NP) {
+  if (const char* v = std::getenv("BIE_AMD_WAVES")) {
+    const int x = std::atoi(v);
+    if (x == kWideWaves || x == kSlotWaves) return x;
+  }
+  return NP == 3 ? kWideWaves : kSlotWaves;
+}
+
+int plain_slot_count(int nphi, int Qc, int occ = 1, int split = 1, int waves = kSlotWaves) {
   // LDS of one workgroup when occ of them share a CU (allocation granularity left spare)
   const long budget = occ == 2 ? kLdsBytes / 2 - 1024 : kLdsBytes;
-  const int stage = split == 2 ? split_stage_doubles(Qc, 2) : stage_doubles(Qc);
+  const int stage = split == 2 ? split_stage_doubles(Qc, 2) : stage_doubles(Qc, waves);
   int S = fitting_slots(phi_pad(nphi) + kTrigDoubles + stage, 1, budget);
   if (const char* v = std::getenv("BIE_AMD_SLOTS")) {
     const int x = std::atoi(v);
@@ -534,11 +546,13 @@ void assemble_device_impl(bie_dmesh* dm, double kr, double ki, const double* d_t
       const bool tuned = qc_inst && (dm->pk == 3 || dm->pk == 6);
       const int occ = tuned ? slot_occupancy() : 1;
       const int split = tuned && occ == 1 ? slot_split() : 1;
-      const int vpw = tuned && occ == 1 && split == 1 ? slot_elems_per_wave() : 1;
+      const int waves = tuned && occ == 1 && split == 1 ? slot_waves(dm->pk) : kSlotWaves;
+      const int vpw =
+          tuned && occ == 1 && split == 1 && waves == kSlotWaves ? slot_elems_per_wave() : 1;
       const int min_segments = choose_segment_count(dm->device, rowblocks, dm->E, occ);
-      const bie_dmesh::Sched& sc =
-          ensure_schedule(dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0], occ, split),
-                          min_segments, s, kSlotWaves * vpw);
+      const bie_dmesh::Sched& sc = ensure_schedule(
+          dm, 0, plain_slot_count(a.R * a.Qmax * a.NP, a.len[0], occ, split, waves),
+          min_segments, s, waves * vpw);
       // collocation on quadratic physics: after the first call the mid-edge pairs come
       // from the recorded sorted list instead of the slot kernel's queue
       const bool colloc = d_targets == dm->d_sources && M == dm->N;

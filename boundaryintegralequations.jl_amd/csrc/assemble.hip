@@ -506,8 +506,11 @@ __device__ __forceinline__ void flush_step(const AssemblyArgs& a, const SlotArgs
 // its element's vertex-rule rows in its LDS rows (s_rows + wave * 24 Qc) and loads the
 // next element's rows into registers a whole pair ahead: the global loads never wait
 // behind the flush's stores (vector-memory counters are in order).  Ends on a barrier.
+// HOLD (with STAGED): the next element's rows are loaded before the pair's quadrature and
+// held in registers through it; without HOLD they are loaded after it, which frees those
+// registers where more waves per SIMD cover the load latency instead (the wide instance).
 template <int NP, bool CPLX, int W, bool STAGED, bool GON = true, int QC = 0, bool FAST = false,
-          int V = 1>
+          int V = 1, bool HOLD = true>
 __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs& sa, int rb,
                                           int seg, double* acc, const double* phi,
                                           const double2* trig, double* s_rows, int2* s_q,
@@ -523,10 +526,12 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
   const int Qc = QC > 0 ? QC : a.len[0];
   // the rows of the element's vertex rules (three on triangles, the one rule of a
   // quadrilateral) and, when there is a rule to select, its selection nodes
-  const int nchunk = 4 * Qc * (a.R < 3 ? a.R : 3);
+  // (three- and six-node physics are triangles only: three vertex rules, compile-time sizes)
+  constexpr bool kTri = NP == 3 || NP == 6;
+  const int nchunk = kTri ? 12 * Qc : 4 * Qc * (a.R < 3 ? a.R : 3);
   double* my_rows = STAGED ? s_rows + (size_t)wave * stage_wave_doubles(Qc) : nullptr;
   double* my_sel = STAGED ? my_rows + 24 * Qc : nullptr;  // the element's selection nodes
-  const int nsel = a.R == 1 ? 0 : NP * 2;                  // 16-byte chunks of them
+  const int nsel = (kTri || a.R != 1) ? NP * 2 : 0;      // 16-byte chunks of them
   double2 stg[3], stg_sel;
   constexpr int WV = W * V;  // elements per step; wave w takes elements v * W + w, v < V
   const int st0 = sload(sa.seg_step_begin + seg), st1 = sload(sa.seg_step_begin + seg + 1);
@@ -556,7 +561,7 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     // unconditional (element 0 stands in at the segment's end): a path that skipped the
     // LDS store below would leave these loads outstanding at the loop head, where the
     // compiler would wait for them, and with them for the previous step's column stores
-    if (STAGED)
+    if (STAGED && HOLD)
       stage_load(a.elem + (size_t)max(e_nxt, 0) * a.Pe * 8, nchunk,
                  sa.step_sel + (e_nxt >= 0 ? nidx : 0) * NP * 4, e_nxt >= 0 ? nsel : 0, lane,
                  stg, stg_sel);
@@ -610,6 +615,10 @@ __device__ __forceinline__ void slot_tile(const AssemblyArgs& a, const SlotArgs&
     // next element: rule selection, and its rows (LDS: after this wave's reads above)
     int c_nxt = 0;
     if (STAGED) {  // only this wave reads my_rows and my_sel, and its reads above are done
+      if (!HOLD)
+        stage_load(a.elem + (size_t)max(e_nxt, 0) * a.Pe * 8, nchunk,
+                   sa.step_sel + (e_nxt >= 0 ? nidx : 0) * NP * 4, e_nxt >= 0 ? nsel : 0,
+                   lane, stg, stg_sel);
       stage_store(my_rows, nchunk, my_sel, nsel, lane, stg, stg_sel);
       if (e_nxt >= 0) c_nxt = select_rule<NP>(x0, x1, x2, my_sel, a.R, a.hyp) - 1;
     } else if (e_nxt >= 0) {
@@ -803,7 +812,7 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   double* acc = lds;                              // [slot][4][64]
   double* s_phi = lds + (size_t)sa.S * 256;
   const int nphi = a.R * a.Qmax * NP;
-  const int nstage = stage_doubles(a.len[0]);
+  const int nstage = stage_doubles(a.len[0], W);
   const double2* s_trig = reinterpret_cast<const double2*>(s_phi + phi_pad(nphi));
   double* s_rows = s_phi + phi_pad(nphi) + kTrigDoubles;
   int2* s_q = reinterpret_cast<int2*>(s_rows + nstage);
@@ -829,8 +838,8 @@ k_assemble_slots(const AssemblyArgs a, const SlotArgs sa) {
   lds_barrier();
   if constexpr (QC > 0) {
     static_assert(QC <= kStageMaxQ, "compile-time rule lengths are staged in LDS");
-    slot_tile<NP, CPLX, W, true, GON, QC, FAST, V>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows,
-                                                   s_q, s_qn);
+    slot_tile<NP, CPLX, W, true, GON, QC, FAST, V, W != kWideWaves>(a, sa, rb, seg, acc, s_phi,
+                                                                    s_trig, s_rows, s_q, s_qn);
   } else if (nstage) {
     slot_tile<NP, CPLX, W, true, GON>(a, sa, rb, seg, acc, s_phi, s_trig, s_rows, s_q, s_qn);
   } else {
@@ -1123,11 +1132,13 @@ void launch_zero_columns(double* X, int64_t M, const int* cols, int ncols, hipSt
 
 void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast, hipStream_t s) {
   if (a.M == 0 || a.E == 0) return;
-  BIE_REQUIRE(sa.W == kSlotWaves || (sa.W == 2 * kSlotWaves && fast && a.ki == 0.0 &&
-                                      (a.NP == 3 || a.NP == 6) &&
-                                      sa.occupancy == 1 && sa.split == 1 &&
-                                      (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16)),
+  const bool tuned_ok = fast && a.ki == 0.0 && (a.NP == 3 || a.NP == 6) && sa.occupancy == 1 &&
+                        sa.split == 1 && (a.len[0] == 4 || a.len[0] == 6 || a.len[0] == 16);
+  BIE_REQUIRE(sa.W == kSlotWaves || ((sa.W == 2 * kSlotWaves || sa.W == kWideWaves) && tuned_ok),
               BIE_EINVAL, "slot schedule built for another wave count");
+  // waves of the workgroup: kWideWaves for the wide instance, else kSlotWaves (the two-element
+  // instance keeps 8 waves with 16 elements per step)
+  const int kw = sa.W == kWideWaves ? kWideWaves : kSlotWaves;
   const int64_t per_xcd = (sa.n_tiles + 7) / 8;
   BIE_REQUIRE(per_xcd * 8 <= 2147483647LL, BIE_EINVAL, "too many tiles");
   const dim3 grid((unsigned)(per_xcd * 8));
@@ -1138,7 +1149,7 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
   const size_t shmem = sizeof(double) * ((size_t)sa.S * 256 + phi_pad(a.R * a.Qmax * a.NP) +
                                          kTrigDoubles +
                                          (split == 2 ? split_stage_doubles(a.len[0], 2)
-                                                     : stage_doubles(a.len[0]))) +
+                                                     : stage_doubles(a.len[0], kw))) +
                        sizeof(int2) * kMidLds + 16;
   BIE_REQUIRE(shmem <= 160 * 1024, BIE_EINVAL, "slot kernel LDS request exceeds 160 KiB");
   const bool cplx = a.ki != 0.0;
@@ -1146,7 +1157,7 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
   auto go = [&](auto kern) {
     BIE_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    hipLaunchKernelGGL(kern, grid, dim3(64 * W * split), shmem, s, a, sa);
+    hipLaunchKernelGGL(kern, grid, dim3(64 * kw * split), shmem, s, a, sa);
   };
   const bool gon = (a.flags & BIE_G) != 0;
   if (split == 2) {
@@ -1188,6 +1199,14 @@ void launch_assemble_slots(const AssemblyArgs& a, const SlotArgs& sa, bool fast,
           case 4: go(k_assemble_slots<NPc, false, W, Gc, 4, true, 2>); return true;
           case 6: go(k_assemble_slots<NPc, false, W, Gc, 6, true, 2>); return true;
           case 16: go(k_assemble_slots<NPc, false, W, Gc, 16, true, 2>); return true;
+          default: return false;
+        }
+      }
+      if (sa.W == kWideWaves) {
+        switch (qc) {
+          case 4: go(k_assemble_slots<NPc, false, kWideWaves, Gc, 4, true>); return true;
+          case 6: go(k_assemble_slots<NPc, false, kWideWaves, Gc, 6, true>); return true;
+          case 16: go(k_assemble_slots<NPc, false, kWideWaves, Gc, 16, true>); return true;
           default: return false;
         }
       }

@@ -112,7 +112,10 @@ constexpr int kMidLds = 512;    // per-workgroup LDS queue of mid-edge pairs
 constexpr int kXcds = 8;        // XCDs of an MI355X; workgroup b is dispatched to XCD b mod 8
 constexpr int kLdsBytes = 160 * 1024;  // LDS per CU (one slot-kernel workgroup per CU)
 
-constexpr int kStageMaxQ = 16;  // vertex rules up to this length are staged in LDS per wave
+constexpr int kStageMaxQ = 16;
+// Waves of the wide single-k slot kernel (three per SIMD, one workgroup per CU), an option
+// beside kSlotWaves for real k on rules of 4, 6 or 16 points.
+constexpr int kWideWaves = 12;  // vertex rules up to this length are staged in LDS per wave
 
 // Table of (cos, sin)(j pi / 64), j < kTrigN, for the table-driven sincos of the slot
 // kernels.  It follows the basis table in the same device buffer (at phi_pad(nphi)
@@ -131,8 +134,8 @@ __host__ __device__ inline int stage_wave_doubles(int Qc) { return 3 * Qc * 8 + 
 __host__ __device__ inline int sweep_fixed_doubles(int nphi) {
   return phi_pad(nphi) + kTrigDoubles;
 }
-__host__ __device__ inline int stage_doubles(int Qc) {
-  return Qc <= kStageMaxQ ? kSlotWaves * stage_wave_doubles(Qc) : 0;
+__host__ __device__ inline int stage_doubles(int Qc, int waves = kSlotWaves) {
+  return Qc <= kStageMaxQ ? waves * stage_wave_doubles(Qc) : 0;
 }
 // Split form of the slot kernel: `split` waves share each element of a step, wave h taking
 // the rule points h Qc/split .. (h+1) Qc/split - 1; each stages the rows of its points of

@@ -83,7 +83,8 @@ __device__ __noinline__ double scaled_hypot3(double x, double y, double z) {
 template <int NP>
 __device__ __forceinline__ int select_rule(double x0, double x1, double x2,
                                            const double* __restrict__ P, int R, int hyp) {
-  if (R == 1) return 1;  // quadrilaterals: one rule, nothing to select
+  // quadrilaterals (node counts other than 3 and 6): one rule, nothing to select
+  if (NP != 3 && NP != 6 && R == 1) return 1;
   double d2[NP];
 #pragma unroll
   for (int s = 0; s < NP; ++s) {
