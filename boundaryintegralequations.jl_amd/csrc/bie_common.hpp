@@ -112,10 +112,10 @@ constexpr int kMidLds = 512;    // per-workgroup LDS queue of mid-edge pairs
 constexpr int kXcds = 8;        // XCDs of an MI355X; workgroup b is dispatched to XCD b mod 8
 constexpr int kLdsBytes = 160 * 1024;  // LDS per CU (one slot-kernel workgroup per CU)
 
-constexpr int kStageMaxQ = 16;
+constexpr int kStageMaxQ = 16;  // vertex rules up to this length are staged in LDS per wave
 // Waves of the wide single-k slot kernel (three per SIMD, one workgroup per CU), an option
 // beside kSlotWaves for real k on rules of 4, 6 or 16 points.
-constexpr int kWideWaves = 12;  // vertex rules up to this length are staged in LDS per wave
+constexpr int kWideWaves = 12;
 
 // Table of (cos, sin)(j pi / 64), j < kTrigN, for the table-driven sincos of the slot
 // kernels.  It follows the basis table in the same device buffer (at phi_pad(nphi)
