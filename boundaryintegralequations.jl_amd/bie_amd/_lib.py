@@ -158,6 +158,10 @@ SIGNATURES = {
                                            ctypes.c_double, _dp, ctypes.c_int64,
                                            ctypes.c_int32, ctypes.c_double, ctypes.c_uint32,
                                            ctypes.c_int64, _dp, _dp, _dp, _dp]),
+    "bie_partial_assemble": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_double,
+                                            ctypes.c_double, _dp, ctypes.c_int64,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                            ctypes.c_int64, _dp, _dp, _dp, _dp]),
     "bie_sparse_create": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_double,
                                          ctypes.POINTER(_dp)]),
     "bie_sparse_destroy": (ctypes.c_int, [_dp]),
@@ -165,6 +169,9 @@ SIGNATURES = {
     "bie_sparse_pattern": (ctypes.c_int, [_dp, ctypes.c_int32, _dp, _dp]),
     "bie_sparse_assemble_device": (ctypes.c_int, [_dp, ctypes.c_double, ctypes.c_double, _dp,
                                                    ctypes.c_uint32, _dp, _dp, _dp]),
+    "bie_sparse_partial_device": (ctypes.c_int, [_dp, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_int32, _dp, ctypes.c_uint32, _dp,
+                                                  _dp, _dp]),
     "bie_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "bie_timing_collect": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),

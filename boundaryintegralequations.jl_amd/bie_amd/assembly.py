@@ -325,6 +325,43 @@ def sparse_assemble_parallel(mesh: Mesh3d, k, sources=None, *, fOn=True, gOn=Tru
     return _sparse_matrices(I - base, J - base, F, G, N, fOn, gOn)
 
 
+def partial_assemble_parallel(mesh: Mesh3d, k, in_sources=None, *, fOn=True, gOn=True, n=3,
+                              progress=False, depth=1, hypot="julia1.9"):
+    """The partial near field of a fast operator (drop-in for partial_assemble_parallel!,
+    src/3d/fast_multipole_method.jl:11-81): the pattern of sparse_assemble_parallel, every
+    connected (source, element) pair integrated with the one symmetric rule
+    gauss_points_triangle(n) (n = 1, 3, 4, 7, 12, 13).  Returns scipy CSR matrices."""
+    del progress
+    if depth not in (1, 2):
+        raise ValueError("depth must be 1 or 2")
+    N = mesh.n_sources
+    if in_sources is None or in_sources is mesh.sources:
+        pts = None
+    else:
+        pts = np.asfortranarray(in_sources, dtype=np.float64)
+        if pts.shape != (3, N):
+            raise ValueError("in_sources must be 3 x n_sources (one point per mesh source)")
+    hm = _lib.HostMesh(mesh, _lib.hypot_flag(hypot))
+    L = _lib.lib()
+    nnz = ctypes.c_int64()
+    _lib.check(L.bie_sparse_count(ctypes.byref(hm.struct), int(depth), ctypes.byref(nnz)))
+    m = nnz.value
+    I = np.empty(m, dtype=np.int64)
+    J = np.empty(m, dtype=np.int64)
+    F = np.empty(m, dtype=np.complex128)
+    G = np.empty(m, dtype=np.complex128)
+    k = complex(k)
+    rc = L.bie_partial_assemble(ctypes.byref(hm.struct), k.real, k.imag, _lib._ptr(pts),
+                                N if pts is not None else 0, int(depth), int(n),
+                                _flags(fOn, gOn, False), m, _lib._ptr(I), _lib._ptr(J),
+                                _lib._ptr(F), _lib._ptr(G))
+    if rc == _lib.BIE_EINVAL:
+        raise ValueError(L.bie_last_error().decode())
+    _lib.check(rc)
+    base = hm.struct.index_base
+    return _sparse_matrices(I - base, J - base, F, G, N, fOn, gOn)
+
+
 class SparsePattern:
     """Device-resident sparse near-field assembly (bie_sparse) on a DeviceMesh: pattern,
     pairs and subdivision rules built once, then assemblies for any k."""
@@ -351,6 +388,16 @@ class SparsePattern:
         k = complex(k)
         _lib.check(_lib.lib().bie_sparse_assemble_device(
             self.handle, k.real, k.imag, ctypes.c_void_p(sources_ptr or None),
+            _flags(fOn, gOn, False), ctypes.c_void_p(F_ptr or None),
+            ctypes.c_void_p(G_ptr or None), ctypes.c_void_p(stream or None)))
+
+    def partial_device(self, k, n_gauss, F_ptr: int, G_ptr: int, *, sources_ptr: int = 0,
+                       fOn=True, gOn=True, stream: int = 0):
+        """The partial near field (gauss_points_triangle(n_gauss) on every pair) into
+        device arrays of nnz complex entries.  Asynchronous."""
+        k = complex(k)
+        _lib.check(_lib.lib().bie_sparse_partial_device(
+            self.handle, k.real, k.imag, int(n_gauss), ctypes.c_void_p(sources_ptr or None),
             _flags(fOn, gOn, False), ctypes.c_void_p(F_ptr or None),
             ctypes.c_void_p(G_ptr or None), ctypes.c_void_p(stream or None)))
 

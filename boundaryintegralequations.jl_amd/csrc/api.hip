@@ -1358,6 +1358,9 @@ struct bie_sparse {
   double* d_rules = nullptr;
   int off[6] = {0, 0, 0, 0, 0, 0};
   int len[6] = {0, 0, 0, 0, 0, 0};
+  // the one rule of the partial assembly (gauss_points_triangle(partial_n)), built on demand
+  double* d_partial = nullptr;
+  int partial_n = -1, partial_len = 0;
 };
 
 namespace bie {
@@ -1371,10 +1374,33 @@ void sparse_destroy_impl(bie_sparse* sp) {
   dfree(sp->d_pos);
   dfree(sp->d_gsel);
   dfree(sp->d_rules);
+  dfree(sp->d_partial);
   delete sp;
 }
 
 // Pattern, pairs, selection table and subdivision rules of a sparse assembly on dm.
+// Point records of a rule for the sparse kernel: w, the geometry basis and its derivatives,
+// the physics basis (sparse_record_doubles() each).
+void append_rule_records(std::vector<double>& rec, const std::vector<double>& u,
+                         const std::vector<double>& v, const std::vector<double>& w, int NG,
+                         int NP) {
+  const int RS = sparse_record_doubles();
+  for (size_t q = 0; q < u.size(); ++q) {
+    double N[9], du[9], dv[9], Np[9], dup[9], dvp[9];
+    geometry_basis(NG, u[q], v[q], N, du, dv);
+    geometry_basis(NP, u[q], v[q], Np, dup, dvp);
+    std::vector<double> r(RS, 0.0);
+    r[0] = w[q];
+    for (int g = 0; g < 6; ++g) {
+      r[1 + g] = N[g];
+      r[7 + g] = du[g];
+      r[13 + g] = dv[g];
+      r[19 + g] = Np[g];
+    }
+    rec.insert(rec.end(), r.begin(), r.end());
+  }
+}
+
 bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
   BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
@@ -1428,20 +1454,7 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
     subdivision_rule(srcs[c], u, v, w);
     sp->off[c] = (int)(rec.size() / RS);
     sp->len[c] = (int)u.size();
-    for (size_t q = 0; q < u.size(); ++q) {
-      double N[9], du[9], dv[9], Np[9], dup[9], dvp[9];
-      geometry_basis(NG, u[q], v[q], N, du, dv);
-      geometry_basis(NP, u[q], v[q], Np, dup, dvp);
-      std::vector<double> r(RS, 0.0);
-      r[0] = w[q];
-      for (int g = 0; g < 6; ++g) {
-        r[1 + g] = N[g];
-        r[7 + g] = du[g];
-        r[13 + g] = dv[g];
-        r[19 + g] = Np[g];
-      }
-      rec.insert(rec.end(), r.begin(), r.end());
-    }
+    append_rule_records(rec, u, v, w, NG, NP);
   }
   sp->d_pairs = upload_vec(pairs, nullptr);
   sp->d_pos = upload_vec(pos, nullptr);
@@ -1476,6 +1489,67 @@ void sparse_assemble_impl(bie_sparse* sp, double kr, double ki, const double* d_
   for (int c = 0; c < 6; ++c) {
     L.off[c] = sp->off[c];
     L.len[c] = sp->len[c];
+  }
+  L.NG = dm->gk;
+  L.NP = dm->pk;
+  L.hyp = dm->hyp;
+  L.kr = kr;
+  L.ki = ki;
+  L.flags = flags;
+  L.F = dF;
+  L.G = dG;
+  launch_sparse_pairs(L, s);
+  mark_done(dm, s);
+}
+
+// The partial assembly (partial_assemble_parallel!, fast_multipole_method.jl:11-81): the
+// same pattern and pairs, every pair integrated with the one symmetric rule
+// gauss_points_triangle(n) - the quadrature a fast operator applies to the near field,
+// which its correction subtracts.
+void sparse_partial_impl(bie_sparse* sp, double kr, double ki, int n_gauss,
+                         const double* d_sources, uint32_t flags, double* dF, double* dG,
+                         hipStream_t s) {
+  BIE_REQUIRE(sp != nullptr, BIE_EINVAL, "sparse handle is NULL");
+  BIE_REQUIRE((flags & ~(BIE_F | BIE_G)) == 0, BIE_EINVAL, "flags: BIE_F and BIE_G only");
+  BIE_REQUIRE(std::isfinite(kr) && std::isfinite(ki), BIE_EINVAL, "k must be finite");
+  BIE_REQUIRE(((flags & BIE_F) == 0 || dF) && ((flags & BIE_G) == 0 || dG), BIE_EINVAL,
+              "output pointer is NULL");
+  bie_dmesh* dm = sp->dm;
+  BIE_HIP_CHECK(hipSetDevice(dm->device));
+  std::lock_guard<std::mutex> lk(dm->mu);
+  if (sp->partial_n != n_gauss) {
+    std::vector<double> u, v, w;
+    BIE_REQUIRE(gauss_points_triangle(n_gauss, u, v, w), BIE_EINVAL,
+                "Triangular quadrature not implemented for n=" + std::to_string(n_gauss) +
+                    " (gauss_points_triangle has n = 1, 3, 4, 7, 12, 13)");
+    std::vector<double> rec;
+    append_rule_records(rec, u, v, w, dm->gk, dm->pk);
+    if (sp->d_partial) {
+      BIE_HIP_CHECK(hipDeviceSynchronize());  // a launch may still read the old rule
+      dfree(sp->d_partial);
+      sp->d_partial = nullptr;
+      sp->partial_n = -1;
+    }
+    sp->d_partial = upload_vec(rec, nullptr);
+    BIE_HIP_CHECK(hipDeviceSynchronize());
+    sp->partial_n = n_gauss;
+    sp->partial_len = (int)u.size();
+  }
+  order_after_previous(dm, s);
+  if (dF) launch_fill_zero(dF, 2 * sp->nnz, s);
+  if (dG) launch_fill_zero(dG, 2 * sp->nnz, s);
+  SparseLaunch L{};
+  L.src = d_sources ? d_sources : dm->d_sources;
+  L.pairs = sp->d_pairs;
+  L.pos = sp->d_pos;
+  L.n_pairs = (flags & (BIE_F | BIE_G)) ? sp->n_pairs : 0;
+  L.coords = dm->d_coords;
+  L.topo = dm->d_topo;
+  L.gsel = sp->d_gsel;
+  L.rules = sp->d_partial;
+  for (int c = 0; c < 6; ++c) {  // whatever the selection, the one rule
+    L.off[c] = 0;
+    L.len[c] = sp->partial_len;
   }
   L.NG = dm->gk;
   L.NP = dm->pk;
@@ -1529,6 +1603,15 @@ int bie_sparse_assemble_device(bie_sparse* sp, double k_re, double k_im, const d
   });
 }
 
+int bie_sparse_partial_device(bie_sparse* sp, double k_re, double k_im, int32_t n_gauss,
+                              const double* d_sources, uint32_t flags, double* dF, double* dG,
+                              void* stream) {
+  return guarded([&] {
+    sparse_partial_impl(sp, k_re, k_im, n_gauss, d_sources, flags, dF, dG,
+                        static_cast<hipStream_t>(stream));
+  });
+}
+
 int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz) {
   return guarded([&] {
     validate_mesh(mesh);
@@ -1574,6 +1657,52 @@ int bie_sparse_assemble(const bie_mesh* mesh, double k_re, double k_im, const do
         BIE_HIP_CHECK(hipMemcpy(dT, sources, 24 * n_sources, hipMemcpyHostToDevice));
     }
     sparse_assemble_impl(sp, k_re, k_im, dT, flags, dF, dG, nullptr);
+    BIE_HIP_CHECK(hipDeviceSynchronize());
+    if (nnz) {
+      BIE_HIP_CHECK(hipMemcpy(F, dF, 16 * nnz, hipMemcpyDeviceToHost));
+      BIE_HIP_CHECK(hipMemcpy(G, dG, 16 * nnz, hipMemcpyDeviceToHost));
+    }
+    const int base = mesh->index_base;
+    for (size_t i = 0; i + 1 < sp->row_ptr.size(); ++i)
+      for (int64_t t = sp->row_ptr[i]; t < sp->row_ptr[i + 1]; ++t) {
+        I[t] = (int64_t)i + base;
+        J[t] = (int64_t)sp->cols[t] + base;
+      }
+  });
+  sparse_destroy_impl(sp);
+  dfree(dT);
+  dfree(dF);
+  dfree(dG);
+  return rc;
+}
+
+int bie_partial_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* sources,
+                         int64_t n_sources, int32_t depth, int32_t n_gauss, uint32_t flags,
+                         int64_t nnz, int64_t* I, int64_t* J, double* F, double* G) {
+  std::shared_ptr<bie_dmesh> handle;
+  int rc = host_mesh_handle(mesh, &handle);
+  if (rc != BIE_OK) return rc;
+  bie_dmesh* dm = handle.get();
+  bie_sparse* sp = nullptr;
+  double* dT = nullptr;
+  double* dF = nullptr;
+  double* dG = nullptr;
+  rc = guarded([&] {
+    BIE_REQUIRE(I && J && F && G, BIE_EINVAL, "output pointer is NULL");
+    BIE_REQUIRE(sources == nullptr || n_sources == dm->N, BIE_EINVAL,
+                "the partial assembly needs one source per mesh source (n_sources)");
+    sp = sparse_create_impl(dm, depth, NAN);
+    BIE_REQUIRE(nnz == sp->nnz, BIE_EINVAL,
+                "nnz does not match the pattern (bie_sparse_count gives " +
+                    std::to_string(sp->nnz) + ")");
+    dF = dalloc<double>((size_t)2 * nnz);
+    dG = dalloc<double>((size_t)2 * nnz);
+    if (sources) {
+      dT = dalloc<double>((size_t)3 * n_sources);
+      if (n_sources)
+        BIE_HIP_CHECK(hipMemcpy(dT, sources, 24 * n_sources, hipMemcpyHostToDevice));
+    }
+    sparse_partial_impl(sp, k_re, k_im, n_gauss, dT, flags, dF, dG, nullptr);
     BIE_HIP_CHECK(hipDeviceSynchronize());
     if (nnz) {
       BIE_HIP_CHECK(hipMemcpy(F, dF, 16 * nnz, hipMemcpyDeviceToHost));

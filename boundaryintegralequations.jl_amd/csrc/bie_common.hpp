@@ -262,6 +262,8 @@ struct SparseLaunch {
 };
 void subdivision_rule(const double src[3], std::vector<double>& u, std::vector<double>& v,
                       std::vector<double>& w);
+bool gauss_points_triangle(int n, std::vector<double>& u, std::vector<double>& v,
+                           std::vector<double>& w);
 void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,
                     std::vector<int64_t>& row_ptr, std::vector<int>& cols,
                     std::vector<int64_t>& pair_ptr, std::vector<int>& pair_elem);

@@ -10,6 +10,8 @@
 //   13-19), shifted to its centre and scaled by its shortest side, as the reference places
 //   it.
 // * sparse_pattern: connected_sources (assembly_collocation_losses.jl:98-132).
+// * gauss_points_triangle: the symmetric triangle rules of the partial assembly
+//   (partial_assemble_parallel!, fast_multipole_method.jl:11-81).
 // Plain C++ (no device code), so that the CPU sanitizer build covers it.
 #include <algorithm>
 #include <cmath>
@@ -109,6 +111,74 @@ void subdivision_rule(const double src[3], std::vector<double>& u, std::vector<d
   }
 }
 
+
+// The symmetric triangle rules of gauss_points_triangle (adaptive_integration.jl:13-64,
+// OpenBEM's table): n = 1, 3, 4, 7, 12 or 13 points on the reference triangle, weights
+// summing to its area 1/2.  Points are (u, v, w) rows; the table values are data that the
+// rules are defined by, kept as the reference lists them.
+bool gauss_points_triangle(int n, std::vector<double>& u, std::vector<double>& v,
+                           std::vector<double>& w) {
+  static const double t1[][3] = {{1.0 / 3, 1.0 / 3, 0.5}};
+  static const double t3[][3] = {{1.0 / 6, 1.0 / 6, 1.0 / 6},
+                                 {2.0 / 3, 1.0 / 6, 1.0 / 6},
+                                 {1.0 / 6, 2.0 / 3, 1.0 / 6}};
+  static const double t4[][3] = {{1.0 / 3, 1.0 / 3, -9.0 / 32},
+                                 {0.2, 0.6, 25.0 / 96},
+                                 {0.2, 0.2, 25.0 / 96},
+                                 {0.6, 0.2, 25.0 / 96}};
+  static const double t7[][3] = {{1.0 / 3, 1.0 / 3, 0.1125},
+                                 {0.797426985353087, 0.101286507323456, 0.0629695902724135},
+                                 {0.101286507323456, 0.797426985353087, 0.0629695902724135},
+                                 {0.101286507323456, 0.101286507323456, 0.0629695902724135},
+                                 {0.059715871789770, 0.470142064105115, 0.066197076394253},
+                                 {0.470142064105115, 0.059715871789770, 0.066197076394253},
+                                 {0.470142064105115, 0.470142064105115, 0.066197076394253}};
+  static const double t12[][3] = {{0.873821971016996, 0.063089014491502, 0.0254224531851035},
+                                  {0.063089014491502, 0.873821971016996, 0.0254224531851035},
+                                  {0.063089014491502, 0.063089014491502, 0.0254224531851035},
+                                  {0.501426509658279, 0.249286745170910, 0.0583931378631895},
+                                  {0.249286745170910, 0.501426509658279, 0.0583931378631895},
+                                  {0.249286745170910, 0.249286745170910, 0.0583931378631895},
+                                  {0.636502499121399, 0.310352451033785, 0.041425537809187},
+                                  {0.636502499121399, 0.053145049844816, 0.041425537809187},
+                                  {0.310352451033785, 0.636502499121399, 0.041425537809187},
+                                  {0.310352451033785, 0.053145049844816, 0.041425537809187},
+                                  {0.053145049844816, 0.636502499121399, 0.041425537809187},
+                                  {0.053145049844816, 0.310352451033785, 0.041425537809187}};
+  static const double t13[][3] = {{1.0 / 3, 1.0 / 3, -0.074785022233835},
+                                  {0.479308067841923, 0.260345966079038, 0.087807628716602},
+                                  {0.260345966079038, 0.479308067841923, 0.087807628716602},
+                                  {0.260345966079038, 0.260345966079038, 0.087807628716602},
+                                  {0.869739794195568, 0.065130102902216, 0.0266736178044195},
+                                  {0.065130102902216, 0.869739794195568, 0.0266736178044195},
+                                  {0.065130102902216, 0.065130102902216, 0.0266736178044195},
+                                  {0.638444188569809, 0.312865496004875, 0.0385568804451285},
+                                  {0.638444188569809, 0.048690315425316, 0.0385568804451285},
+                                  {0.312865496004875, 0.638444188569809, 0.0385568804451285},
+                                  {0.312865496004875, 0.048690315425316, 0.0385568804451285},
+                                  {0.048690315425316, 0.638444188569809, 0.0385568804451285},
+                                  {0.048690315425316, 0.312865496004875, 0.0385568804451285}};
+  const double(*t)[3] = nullptr;
+  int q = 0;
+  switch (n) {
+    case 1: t = t1; q = 1; break;
+    case 3: t = t3; q = 3; break;
+    case 4: t = t4; q = 4; break;
+    case 7: t = t7; q = 7; break;
+    case 12: t = t12; q = 12; break;
+    case 13: t = t13; q = 13; break;
+    default: return false;
+  }
+  u.clear();
+  v.clear();
+  w.clear();
+  for (int i = 0; i < q; ++i) {
+    u.push_back(t[i][0]);
+    v.push_back(t[i][1]);
+    w.push_back(t[i][2]);
+  }
+  return true;
+}
 
 // Pattern and pairs of the sparse assembly (connected_sources, depth 1 or 2).
 void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,

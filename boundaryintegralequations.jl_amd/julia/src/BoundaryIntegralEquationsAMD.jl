@@ -275,6 +275,57 @@ function BIE.sparse_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
     return sparse_hip(mesh, k, sources; fOn=fOn, gOn=gOn, depth=depth, offset=offset)
 end
 
+"""
+    partial_hip(mesh, k, sources; fOn=true, gOn=true, n=3, depth=1)
+
+The partial near field of a fast operator on the GPU (`partial_assemble_parallel!`,
+src/3d/fast_multipole_method.jl:11-81): the sparse pattern, every pair integrated with
+`gauss_points_triangle(n)`; `sparse(I, J, F), sparse(I, J, G)`.
+"""
+function partial_hip(mesh::BIE.Mesh3d, k::Number, sources::AbstractMatrix; fOn=true, gOn=true,
+                     n=3, depth=1)
+    coords = Matrix{Float64}(mesh.coordinates)
+    topo = Matrix{Int64}(mesh.topology)
+    srcs = Matrix{Float64}(mesh.sources)
+    ptopo = Matrix{Int64}(mesh.physics_topology)
+    N = size(srcs, 2)
+    size(sources) == (3, N) || throw(ArgumentError("one source point per mesh source"))
+    own = sources === mesh.sources
+    pts = own ? srcs : Matrix{Float64}(sources)
+    flags = UInt32((fOn ? 1 : 0) | (gOn ? 2 : 0))
+    kc = ComplexF64(k)
+    GC.@preserve coords topo srcs ptopo pts begin
+        bm = bie_mesh(mesh, coords, topo, srcs, ptopo)
+        nnz = Ref{Int64}(0)
+        check(ccall((:bie_sparse_count, libbie[]), Cint, (Ref{BieMesh}, Int32, Ref{Int64}),
+                    bm, Int32(depth), nnz))
+        I = Vector{Int64}(undef, nnz[]); J = similar(I)
+        F = Vector{ComplexF64}(undef, nnz[]); G = similar(F)
+        check(ccall((:bie_partial_assemble, libbie[]), Cint,
+                    (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Int32, UInt32,
+                     Int64, Ptr{Int64}, Ptr{Int64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                    bm, real(kc), imag(kc), own ? Ptr{Float64}(C_NULL) : pointer(pts),
+                    own ? 0 : N, Int32(depth), Int32(n), flags, nnz[], I, J, F, G))
+        return SparseArrays.sparse(I, J, F, N, N), SparseArrays.sparse(I, J, G, N, N)
+    end
+end
+
+# The reference's fast operators call partial_assemble_parallel!(mesh, zk, targets,
+# shape_function; fOn, gOn, depth) (fast_multipole_method.jl:179,185); continuous physics on
+# triangles runs on the GPU, the rest goes to the reference's method.
+function BIE.partial_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
+                                        in_sources::AbstractMatrix{<:Real},
+                                        shape_function::BIE.Triangular; fOn=true, gOn=true,
+                                        n=3, progress=false, depth=1)
+    if !(GPU_ROUTING[] && supported(mesh) && !discontinuous(mesh) && !quadrilateral(mesh) &&
+         n in (1, 3, 4, 7, 12, 13) && size(in_sources, 2) == size(mesh.sources, 2))
+        return invoke(BIE.partial_assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,BIE.Triangular},
+                      mesh, k, in_sources, shape_function; fOn=fOn, gOn=gOn, n=n,
+                      progress=progress, depth=depth)
+    end
+    return partial_hip(mesh, k, in_sources; fOn=fOn, gOn=gOn, n=n, depth=depth)
+end
+
 # ---------------------------------------------------------------------------------------
 # Device-resident meshes and the batched / adjoint / apply entry points
 # ---------------------------------------------------------------------------------------

@@ -160,6 +160,22 @@ int bie_sparse_pattern(const bie_sparse* sp, int32_t index_base, int64_t* I, int
 int bie_sparse_assemble_device(bie_sparse* sp, double k_re, double k_im, const double* d_sources,
                                uint32_t flags, double* dF, double* dG, void* stream);
 
+/* ---- Partial (fast-operator) near field -------------------------------------------------
+ * Replaces partial_assemble_parallel!(mesh, k, in_sources, shape_function::Triangular; fOn,
+ * gOn, n=3, depth=1) (src/3d/fast_multipole_method.jl:11-81): the same pattern as the
+ * sparse near field (connected_sources), every pair integrated with the one symmetric
+ * rule gauss_points_triangle(n_gauss) (src/3d/adaptive_integration.jl:13-64; n_gauss = 1,
+ * 3, 4, 7, 12 or 13), no selection and no subdivision - the quadrature a fast operator
+ * applies to the near field, which setup_fast_operator (:163-192) subtracts with it.  Same
+ * I, J and value layout as bie_sparse_assemble; nnz from bie_sparse_count. */
+int bie_partial_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* sources,
+                         int64_t n_sources, int32_t depth, int32_t n_gauss, uint32_t flags,
+                         int64_t nnz, int64_t* I, int64_t* J, double* F, double* G);
+/* Device-resident form on a bie_sparse handle (its pattern; its offset is not used). */
+int bie_sparse_partial_device(bie_sparse* sp, double k_re, double k_im, int32_t n_gauss,
+                              const double* d_sources, uint32_t flags, double* dF, double* dG,
+                              void* stream);
+
 /* ---- Device-resident path ------------------------------------------------------------
  * Uploads the mesh once (the element loop's inputs, Mesh3d src/utils/mesh.jl:23-34). */
 int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out);

@@ -192,6 +192,52 @@ def basis(kind, u, v, beta=0.0):
     return N, du, dv
 
 
+# gauss_points_triangle (adaptive_integration.jl:13-64): rows (u, v, w), as listed there
+_GPT = {
+    1: [[1 / 3, 1 / 3, 0.5]],
+    3: [[1 / 6, 1 / 6, 1 / 6], [2 / 3, 1 / 6, 1 / 6], [1 / 6, 2 / 3, 1 / 6]],
+    4: [[1 / 3, 1 / 3, -9 / 32], [0.2, 0.6, 25 / 96], [0.2, 0.2, 25 / 96],
+        [0.6, 0.2, 25 / 96]],
+    7: [[1 / 3, 1 / 3, 0.1125],
+        [0.797426985353087, 0.101286507323456, 0.0629695902724135],
+        [0.101286507323456, 0.797426985353087, 0.0629695902724135],
+        [0.101286507323456, 0.101286507323456, 0.0629695902724135],
+        [0.059715871789770, 0.470142064105115, 0.066197076394253],
+        [0.470142064105115, 0.059715871789770, 0.066197076394253],
+        [0.470142064105115, 0.470142064105115, 0.066197076394253]],
+    12: [[0.873821971016996, 0.063089014491502, 0.0254224531851035],
+         [0.063089014491502, 0.873821971016996, 0.0254224531851035],
+         [0.063089014491502, 0.063089014491502, 0.0254224531851035],
+         [0.501426509658279, 0.249286745170910, 0.0583931378631895],
+         [0.249286745170910, 0.501426509658279, 0.0583931378631895],
+         [0.249286745170910, 0.249286745170910, 0.0583931378631895],
+         [0.636502499121399, 0.310352451033785, 0.041425537809187],
+         [0.636502499121399, 0.053145049844816, 0.041425537809187],
+         [0.310352451033785, 0.636502499121399, 0.041425537809187],
+         [0.310352451033785, 0.053145049844816, 0.041425537809187],
+         [0.053145049844816, 0.636502499121399, 0.041425537809187],
+         [0.053145049844816, 0.310352451033785, 0.041425537809187]],
+    13: [[1 / 3, 1 / 3, -0.074785022233835],
+         [0.479308067841923, 0.260345966079038, 0.087807628716602],
+         [0.260345966079038, 0.479308067841923, 0.087807628716602],
+         [0.260345966079038, 0.260345966079038, 0.087807628716602],
+         [0.869739794195568, 0.065130102902216, 0.0266736178044195],
+         [0.065130102902216, 0.869739794195568, 0.0266736178044195],
+         [0.065130102902216, 0.065130102902216, 0.0266736178044195],
+         [0.638444188569809, 0.312865496004875, 0.0385568804451285],
+         [0.638444188569809, 0.048690315425316, 0.0385568804451285],
+         [0.312865496004875, 0.638444188569809, 0.0385568804451285],
+         [0.312865496004875, 0.048690315425316, 0.0385568804451285],
+         [0.048690315425316, 0.638444188569809, 0.0385568804451285],
+         [0.048690315425316, 0.312865496004875, 0.0385568804451285]],
+}
+
+
+def gauss_points_triangle(n):
+    t = np.asarray(_GPT[n], dtype=np.float64)
+    return t[:, 0].copy(), t[:, 1].copy(), t[:, 2].copy()
+
+
 def julia_hypot3(a, b, c):
     """Julia Base n-ary hypot, elementwise (NaN/Inf handling as in select_exact.cpp)."""
     a, b, c = np.abs(a), np.abs(b), np.abs(c)
@@ -272,3 +318,43 @@ def assemble(mesh, k, targets=None, n=4, m=4, fOn=True, gOn=True, cOn=True, rule
         if cOn:
             np.add.at(C, ii, k0.sum(axis=1))
     return F, G, C
+
+
+def partial_assemble(mesh, k, sources, element_lists, column_lists, n=3, fOn=True, gOn=True):
+    """partial_assemble_parallel! (fast_multipole_method.jl:11-81) as values on the pattern
+    (column_lists[i] the stored columns of row i, ascending; element_lists[i] its connected
+    elements): every (i, e) integrated with gauss_points_triangle(n), the kernels and the
+    physics basis of the mesh at those points, added into row i's entries of e's nodes."""
+    u, v, w = gauss_points_triangle(n)
+    N, du, dv = basis(mesh.geometry_kind, u, v)
+    phi, _, _ = basis(mesh.physics_kind, u, v, getattr(mesh, "physics_beta", 0.0))
+    X = mesh.coordinates[:, mesh.topology]                 # (3, ng, E)
+    y = np.einsum("dke,qk->eqd", X, N)
+    tu = np.einsum("dke,qk->eqd", X, du)
+    tv = np.einsum("dke,qk->eqd", X, dv)
+    cr = np.cross(tu, tv)
+    J = julia_hypot3(cr[..., 0], cr[..., 1], cr[..., 2])
+    nrm = -cr / J[..., None]
+    jw = J * w[None, :]
+    k = complex(k)
+    Fv, Gv = [], []
+    for i, (els, cols) in enumerate(zip(element_lists, column_lists)):
+        x = sources[:, i]
+        f_row = np.zeros(len(cols), dtype=np.complex128)
+        g_row = np.zeros(len(cols), dtype=np.complex128)
+        where = {int(c): t for t, c in enumerate(cols)}
+        for e in els:
+            d = y[e] - x[None, :]
+            r = julia_hypot3(d[:, 0], d[:, 1], d[:, 2])
+            nd = np.einsum("qd,qd->q", nrm[e], d)
+            ekr = np.exp(1j * k * r)
+            g = ekr / (4 * np.pi * r) * jw[e]
+            f = ekr * (1j * k * r - 1.0) * nd / (4 * np.pi * r ** 3) * jw[e]
+            idx = [where[int(c)] for c in mesh.physics_topology[:, e]]
+            if fOn:
+                np.add.at(f_row, idx, f @ phi)
+            if gOn:
+                np.add.at(g_row, idx, g @ phi)
+        Fv.append(f_row)
+        Gv.append(g_row)
+    return np.concatenate(Fv), np.concatenate(Gv)

@@ -105,3 +105,57 @@ def test_sparse_rejects_bad_arguments():
     assert rc == _lib.BIE_EINVAL and b"nnz" in L.bie_last_error()
     with pytest.raises(ValueError):
         sparse_assemble_parallel(mesh, 1.0, mesh.sources[:, :10], depth=1)
+
+
+# ---- the fast operators' partial near field (partial_assemble_parallel!,
+# src/3d/fast_multipole_method.jl:11-81) ----------------------------------------------------
+def _partial_ref(mesh, k, depth, n, sources=None, fOn=True, gOn=True):
+    import oracle_np
+    els, cols = oracle.connected_sources(mesh, depth)
+    src = mesh.sources if sources is None else sources
+    F, G = oracle_np.partial_assemble(mesh, k, src, els, cols, n=n, fOn=fOn, gOn=gOn)
+    I = np.concatenate([np.full(len(c), i) for i, c in enumerate(cols)])
+    return I, np.concatenate(cols), F, G
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("n", [1, 3, 7, 13])
+def test_partial_matches_oracle(name, depth, n):
+    from bie_amd.assembly import partial_assemble_parallel
+    mesh = CASES[name]()
+    k = 2.0 + (0.3j if n == 7 else 0.0)
+    print(f"partial {name} depth={depth} n={n} k={k}", flush=True)
+    Fs, Gs = partial_assemble_parallel(mesh, k, n=n, depth=depth)
+    ref = _partial_ref(mesh, k, depth, n)
+    Fc, Gc = Fs.tocoo(), Gs.tocoo()
+    _check(Fc.row, Fc.col, Fc.data, Gc.data, ref, name)
+
+
+def test_partial_device_handle_flags_and_sources():
+    from bie_amd.assembly import partial_assemble_parallel
+    mesh = CASES["quad_sphere"]()
+    dm = DeviceMesh(mesh)
+    sp = SparsePattern(dm, depth=2)
+    I, J = sp.pattern()
+    nnz = len(I)
+    from gpu_helpers import DevBuf, sync
+    F, G = DevBuf(16 * nnz), DevBuf(16 * nnz)
+    for n in (4, 12, 4):   # the rule table is rebuilt when n changes
+        sp.partial_device(3.0, n, F.ptr, G.ptr)
+        sync()
+        ref = _partial_ref(mesh, 3.0, 2, n)
+        f = F.download(np.zeros(nnz, dtype=np.complex128))
+        g = G.download(np.zeros(nnz, dtype=np.complex128))
+        _check(I, J, f, g, ref, f"device n={n}")
+    sp.close()
+    dm.close()
+    # explicit source points and the flags
+    pts = mesh.sources * 1.0
+    ref = _partial_ref(mesh, 1.5, 1, 3, sources=pts)
+    Fs, _ = partial_assemble_parallel(mesh, 1.5, pts, gOn=False)
+    _, Gs = partial_assemble_parallel(mesh, 1.5, pts, fOn=False)
+    Fc, Gc = Fs.tocoo(), Gs.tocoo()
+    _check(Fc.row, Fc.col, Fc.data, Gc.data, ref, "explicit sources")
+    with pytest.raises(ValueError, match="n=5"):
+        partial_assemble_parallel(mesh, 1.0, n=5)
