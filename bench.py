@@ -433,7 +433,8 @@ def measure_batch(torch, bie_amd, steps=3, n=4, m=4):
 def measure_sparse(torch, bie_amd, steps=3, f=16):
     """The near field (sparse_assemble_parallel!) on the 10242-node quadratic sphere at
     k = 10: depth 1 (the function's default) and depth 2 (assemble_parallel!(...;
-    sparse=true)'s default), F and G, from a device handle (pattern and rules built once)."""
+    sparse=true)'s default), F and G, from a device handle (pattern and rules built once);
+    and the partial near field (partial_assemble_parallel!, n = 3) on the same pattern."""
     from bie_amd.assembly import DeviceMesh, SparsePattern
     mesh = bie_amd.icosphere(f, "quadratic")
     dm = DeviceMesh(mesh, 0)
@@ -457,6 +458,15 @@ def measure_sparse(torch, bie_amd, steps=3, f=16):
         torch.cuda.synchronize()
         res[f"depth{depth}"] = {"nnz": sp.nnz, "ms": e0.elapsed_time(e1) / steps,
                                 "setup_s": setup_s}
+        # the fast operators' partial near field on the same pattern, 3-point rule
+        sp.partial_device(10.0, 3, F.data_ptr(), G.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            sp.partial_device(10.0, 3, F.data_ptr(), G.data_ptr(), stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        res[f"depth{depth}"]["partial_n3_ms"] = e0.elapsed_time(e1) / steps
         sp.close()
         del F, G
     dm.close()
@@ -479,7 +489,7 @@ def main():
                     help="timed CPU runs after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["sweep", "batch", "adjoint", "taylor"],
+    ap.add_argument("--only", choices=["sweep", "batch", "adjoint", "taylor", "sparse"],
                     help="run only this secondary workload and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3 times (PMC byte-counter calibration)")
@@ -507,7 +517,8 @@ def main():
         legs = {"sweep": lambda: measure_sweep(torch, bie_amd),
                 "batch": lambda: measure_batch(torch, bie_amd),
                 "adjoint": lambda: measure_adjoint(torch),
-                "taylor": lambda: measure_taylor(torch, bie_amd)}
+                "taylor": lambda: measure_taylor(torch, bie_amd),
+                "sparse": lambda: measure_sparse(torch, bie_amd)}
         print(json.dumps({"only": args.only, **legs[args.only]()}), flush=True)
         return
 
