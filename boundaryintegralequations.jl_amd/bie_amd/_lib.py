@@ -158,12 +158,19 @@ SIGNATURES = {
                                            ctypes.c_double, _dp, ctypes.c_int64,
                                            ctypes.c_int32, ctypes.c_double, ctypes.c_uint32,
                                            ctypes.c_int64, _dp, _dp, _dp, _dp]),
+    "bie_sparse_assemble_ex": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_double,
+                                              ctypes.c_double, _dp, ctypes.c_int64,
+                                              ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                              ctypes.c_uint32, ctypes.c_int64, _dp, _dp, _dp,
+                                              _dp]),
     "bie_partial_assemble": (ctypes.c_int, [ctypes.POINTER(BieMesh), ctypes.c_double,
                                             ctypes.c_double, _dp, ctypes.c_int64,
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
                                             ctypes.c_int64, _dp, _dp, _dp, _dp]),
     "bie_sparse_create": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_double,
                                          ctypes.POINTER(_dp)]),
+    "bie_sparse_create_ex": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_double,
+                                            ctypes.c_int32, ctypes.POINTER(_dp)]),
     "bie_sparse_destroy": (ctypes.c_int, [_dp]),
     "bie_sparse_nnz": (ctypes.c_int64, [_dp]),
     "bie_sparse_pattern": (ctypes.c_int, [_dp, ctypes.c_int32, _dp, _dp]),

@@ -285,14 +285,16 @@ def _sparse_matrices(I, J, F, G, n, fOn, gOn):
 
 
 def sparse_assemble_parallel(mesh: Mesh3d, k, sources=None, *, fOn=True, gOn=True, depth=1,
-                             offset=None, progress=True, hypot="julia1.9"):
+                             offset=None, progress=True, Ngauss=40, hypot="julia1.9"):
     """Near-field F and G (drop-in for sparse_assemble_parallel!,
     src/3d/assembly_collocation_losses.jl:200-405): the entries (i, j) of the sources
-    connected to node i through at most ``depth`` elements (connected_sources, :98-132),
-    each element integrated with the subdivision rule clustered at its node nearest to
-    source i.  Returns scipy CSR matrices n_sources x n_sources holding exactly the
-    reference's stored entries (a disabled output is all zeros, as in the reference).
-    ``offset``: the rules' source offset (None: 0 for continuous physics)."""
+    connected to node i through at most ``depth`` elements (connected_sources, :41-132),
+    each element integrated with the rule clustered at its node nearest to source i: the
+    subdivision rule for continuous physics, the polar rule (create_polar_elements,
+    :409-447, ``Ngauss``; 40 for quadratic physics) around the physics node for
+    discontinuous physics.  Returns scipy CSR matrices n_sources x n_sources holding
+    exactly the reference's stored entries (a disabled output is all zeros, as in the
+    reference).  ``offset``: the subdivision rules' source offset (None: 0)."""
     del progress
     if depth not in (1, 2):
         raise ValueError("depth must be 1 or 2")
@@ -314,10 +316,10 @@ def sparse_assemble_parallel(mesh: Mesh3d, k, sources=None, *, fOn=True, gOn=Tru
     G = np.empty(n, dtype=np.complex128)
     k = complex(k)
     off = float("nan") if offset is None else float(offset)
-    rc = L.bie_sparse_assemble(ctypes.byref(hm.struct), k.real, k.imag, _lib._ptr(pts),
-                               N if pts is not None else 0, int(depth), off,
-                               _flags(fOn, gOn, False), n, _lib._ptr(I), _lib._ptr(J),
-                               _lib._ptr(F), _lib._ptr(G))
+    rc = L.bie_sparse_assemble_ex(ctypes.byref(hm.struct), k.real, k.imag, _lib._ptr(pts),
+                                  N if pts is not None else 0, int(depth), off, int(Ngauss),
+                                  _flags(fOn, gOn, False), n, _lib._ptr(I), _lib._ptr(J),
+                                  _lib._ptr(F), _lib._ptr(G))
     if rc == _lib.BIE_EINVAL:
         raise ValueError(L.bie_last_error().decode())
     _lib.check(rc)
@@ -364,13 +366,14 @@ def partial_assemble_parallel(mesh: Mesh3d, k, in_sources=None, *, fOn=True, gOn
 
 class SparsePattern:
     """Device-resident sparse near-field assembly (bie_sparse) on a DeviceMesh: pattern,
-    pairs and subdivision rules built once, then assemblies for any k."""
+    pairs and rules (subdivision, or polar for discontinuous physics) built once, then
+    assemblies for any k."""
 
-    def __init__(self, dmesh: DeviceMesh, depth=1, offset=None):
+    def __init__(self, dmesh: DeviceMesh, depth=1, offset=None, Ngauss=40):
         h = ctypes.c_void_p()
         off = float("nan") if offset is None else float(offset)
-        _lib.check(_lib.lib().bie_sparse_create(dmesh.handle, int(depth), off,
-                                                ctypes.byref(h)))
+        _lib.check(_lib.lib().bie_sparse_create_ex(dmesh.handle, int(depth), off, int(Ngauss),
+                                                   ctypes.byref(h)))
         self.handle = h
         self.dmesh = dmesh  # the pattern must not outlive its mesh
         self.nnz = int(_lib.lib().bie_sparse_nnz(h))

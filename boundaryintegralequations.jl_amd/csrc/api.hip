@@ -966,7 +966,7 @@ extern "C" {
 
 const char* bie_last_error(void) { return g_last_error.c_str(); }
 
-int bie_version(void) { return 2; }
+int bie_version(void) { return 3; }
 
 int bie_dmesh_create(const bie_mesh* mesh, int device, bie_dmesh** out) {
   return guarded([&] {
@@ -1383,12 +1383,13 @@ void sparse_destroy_impl(bie_sparse* sp) {
 // the physics basis (sparse_record_doubles() each).
 void append_rule_records(std::vector<double>& rec, const std::vector<double>& u,
                          const std::vector<double>& v, const std::vector<double>& w, int NG,
-                         int NP) {
+                         int pkind, double beta) {
   const int RS = sparse_record_doubles();
   for (size_t q = 0; q < u.size(); ++q) {
-    double N[9], du[9], dv[9], Np[9], dup[9], dvp[9];
+    double N[9], du[9], dv[9], Np[9];
     geometry_basis(NG, u[q], v[q], N, du, dv);
-    geometry_basis(NP, u[q], v[q], Np, dup, dvp);
+    for (int g = 0; g < 9; ++g) Np[g] = 0.0;
+    physics_basis(pkind, beta, u[q], v[q], Np);
     std::vector<double> r(RS, 0.0);
     r[0] = w[q];
     for (int g = 0; g < 6; ++g) {
@@ -1401,13 +1402,34 @@ void append_rule_records(std::vector<double>& rec, const std::vector<double>& u,
   }
 }
 
-bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
+// connected_sources of either physics (continuous: shared nodes; discontinuous: the
+// element's own rows, shared corners at depth 2).
+void near_field_pattern(const int* topo, int gkind, const int* ptopo, int pkind, int64_t E,
+                        int64_t N, int depth, std::vector<int64_t>& row_ptr,
+                        std::vector<int>& cols, std::vector<int64_t>& pair_ptr,
+                        std::vector<int>& pair_elem) {
+  BIE_REQUIRE(!kind_is_quad(gkind), BIE_EUNSUPPORTED,
+              "sparse near field: triangular elements only");
+  if (physics_discontinuous(pkind))
+    sparse_pattern_disc(topo, geometry_nodes(gkind), ptopo, physics_nodes(pkind), E, N, depth,
+                        row_ptr, cols, pair_ptr, pair_elem);
+  else
+    sparse_pattern(ptopo, physics_nodes(pkind), E, N, depth, row_ptr, cols, pair_ptr,
+                   pair_elem);
+}
+
+bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset, int n_gauss) {
   BIE_REQUIRE(dm != nullptr, BIE_EINVAL, "dmesh is NULL");
   BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
-  BIE_REQUIRE(!physics_discontinuous(dm->pkind) && !kind_is_quad(dm->gkind), BIE_EUNSUPPORTED,
-              "sparse near field: continuous physics on triangles only");
+  BIE_REQUIRE(!kind_is_quad(dm->gkind), BIE_EUNSUPPORTED,
+              "sparse near field: triangular elements only");
   BIE_REQUIRE(!std::isinf(offset), BIE_EINVAL, "offset must be finite (NaN: the default 0)");
   const double offr = std::isnan(offset) ? 0.0 : offset;
+  const bool disc = physics_discontinuous(dm->pkind);
+  // the reference forces Ngauss = 40 for quadratic discontinuous physics
+  if (dm->pkind == BIE_DTRI6) n_gauss = 40;
+  BIE_REQUIRE(!disc || (n_gauss >= 1 && n_gauss <= 400), BIE_EINVAL,
+              "Ngauss must be between 1 and 400");
   BIE_HIP_CHECK(hipSetDevice(dm->device));
   std::unique_ptr<bie_sparse> sp(new bie_sparse);
   sp->dm = dm;
@@ -1415,8 +1437,8 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
   const int NP = dm->pk, NG = dm->gk;
   std::vector<int64_t> pair_ptr;
   std::vector<int> pair_elem;
-  sparse_pattern(dm->h_ptopo.data(), NP, dm->E, dm->N, depth, sp->row_ptr, sp->cols, pair_ptr,
-                 pair_elem);
+  near_field_pattern(dm->h_topo.data(), dm->gkind, dm->h_ptopo.data(), dm->pkind, dm->E, dm->N,
+                     depth, sp->row_ptr, sp->cols, pair_ptr, pair_elem);
   sp->nnz = (int64_t)sp->cols.size();
   sp->n_pairs = (int64_t)pair_elem.size();
   BIE_REQUIRE(sp->nnz < (int64_t)INT32_MAX, BIE_EINVAL, "sparse pattern too large");
@@ -1447,14 +1469,30 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset) {
   // fallthrough uses the sixth)
   const double srcs[6][3] = {{1.0, offr, offr}, {offr, 1.0, offr}, {offr, offr, 1.0},
                              {0.5, 0.5, offr},  {offr, 0.5, 0.5},  {0.5, offr, 0.5}};
+  // Discontinuous physics (losses.jl:233-257): the polar rules around the physics nodes;
+  // the rule of the closest geometry node c is the polar rule of physics node c, and the
+  // reference fills the slots beyond the node count with the rules of nodes 1, 2, 3
+  // (constant physics: node 1 for all six).
   const int RS = sparse_record_doubles();
   std::vector<double> rec;
   for (int c = 0; c < 6; ++c) {
     std::vector<double> u, v, w;
-    subdivision_rule(srcs[c], u, v, w);
+    if (disc) {
+      const int node = dm->pkind == BIE_DTRI1 ? 0 : (dm->pkind == BIE_DTRI3 ? c % 3 : c);
+      if (node != c) {  // a filler: the same records as the slot of that node
+        sp->off[c] = sp->off[node];
+        sp->len[c] = sp->len[node];
+        continue;
+      }
+      double pu, pv;
+      discontinuous_node(dm->pkind, dm->beta, node, &pu, &pv);
+      polar_rule(pu, pv, n_gauss, u, v, w);
+    } else {
+      subdivision_rule(srcs[c], u, v, w);
+    }
     sp->off[c] = (int)(rec.size() / RS);
     sp->len[c] = (int)u.size();
-    append_rule_records(rec, u, v, w, NG, NP);
+    append_rule_records(rec, u, v, w, NG, dm->pkind, dm->beta);
   }
   sp->d_pairs = upload_vec(pairs, nullptr);
   sp->d_pos = upload_vec(pos, nullptr);
@@ -1523,7 +1561,7 @@ void sparse_partial_impl(bie_sparse* sp, double kr, double ki, int n_gauss,
                 "Triangular quadrature not implemented for n=" + std::to_string(n_gauss) +
                     " (gauss_points_triangle has n = 1, 3, 4, 7, 12, 13)");
     std::vector<double> rec;
-    append_rule_records(rec, u, v, w, dm->gk, dm->pk);
+    append_rule_records(rec, u, v, w, dm->gk, dm->pkind, dm->beta);
     if (sp->d_partial) {
       BIE_HIP_CHECK(hipDeviceSynchronize());  // a launch may still read the old rule
       dfree(sp->d_partial);
@@ -1568,12 +1606,17 @@ void sparse_partial_impl(bie_sparse* sp, double kr, double ki, int n_gauss,
 
 extern "C" {
 
-int bie_sparse_create(bie_dmesh* dm, int32_t depth, double offset, bie_sparse** out) {
+int bie_sparse_create_ex(bie_dmesh* dm, int32_t depth, double offset, int32_t n_gauss,
+                         bie_sparse** out) {
   return guarded([&] {
     BIE_REQUIRE(out != nullptr, BIE_EINVAL, "out is NULL");
     *out = nullptr;
-    *out = sparse_create_impl(dm, depth, offset);
+    *out = sparse_create_impl(dm, depth, offset, n_gauss);
   });
+}
+
+int bie_sparse_create(bie_dmesh* dm, int32_t depth, double offset, bie_sparse** out) {
+  return bie_sparse_create_ex(dm, depth, offset, 40, out);
 }
 
 int bie_sparse_destroy(bie_sparse* sp) {
@@ -1616,16 +1659,20 @@ int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz) {
   return guarded([&] {
     validate_mesh(mesh);
     BIE_REQUIRE(nnz != nullptr, BIE_EINVAL, "nnz is NULL");
-    BIE_REQUIRE(!physics_discontinuous(mesh->physics_kind) &&
-                    !kind_is_quad(mesh->geometry_kind),
-                BIE_EUNSUPPORTED, "sparse near field: continuous physics on triangles only");
-    std::vector<int> ptopo = convert_topology(mesh->physics_topology, mesh->physics_kind,
+    BIE_REQUIRE(!kind_is_quad(mesh->geometry_kind), BIE_EUNSUPPORTED,
+                "sparse near field: triangular elements only");
+    std::vector<int> topo = convert_topology(mesh->topology, geometry_nodes(mesh->geometry_kind),
+                                             mesh->n_elements, mesh->index_base,
+                                             mesh->n_coordinates, "topology");
+    std::vector<int> ptopo = convert_topology(mesh->physics_topology,
+                                              physics_nodes(mesh->physics_kind),
                                               mesh->n_elements, mesh->index_base,
                                               mesh->n_sources, "physics_topology");
     std::vector<int64_t> row_ptr, pair_ptr;
     std::vector<int> cols, pair_elem;
-    sparse_pattern(ptopo.data(), mesh->physics_kind, mesh->n_elements, mesh->n_sources, depth,
-                   row_ptr, cols, pair_ptr, pair_elem);
+    near_field_pattern(topo.data(), mesh->geometry_kind, ptopo.data(), mesh->physics_kind,
+                       mesh->n_elements, mesh->n_sources, depth, row_ptr, cols, pair_ptr,
+                       pair_elem);
     *nnz = (int64_t)cols.size();
   });
 }
@@ -1633,6 +1680,14 @@ int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz) {
 int bie_sparse_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* sources,
                         int64_t n_sources, int32_t depth, double offset, uint32_t flags,
                         int64_t nnz, int64_t* I, int64_t* J, double* F, double* G) {
+  return bie_sparse_assemble_ex(mesh, k_re, k_im, sources, n_sources, depth, offset, 40, flags,
+                                nnz, I, J, F, G);
+}
+
+int bie_sparse_assemble_ex(const bie_mesh* mesh, double k_re, double k_im,
+                           const double* sources, int64_t n_sources, int32_t depth,
+                           double offset, int32_t n_gauss, uint32_t flags, int64_t nnz,
+                           int64_t* I, int64_t* J, double* F, double* G) {
   std::shared_ptr<bie_dmesh> handle;
   int rc = host_mesh_handle(mesh, &handle);
   if (rc != BIE_OK) return rc;
@@ -1645,7 +1700,7 @@ int bie_sparse_assemble(const bie_mesh* mesh, double k_re, double k_im, const do
     BIE_REQUIRE(I && J && F && G, BIE_EINVAL, "output pointer is NULL");
     BIE_REQUIRE(sources == nullptr || n_sources == dm->N, BIE_EINVAL,
                 "the sparse assembly needs one source per mesh source (n_sources)");
-    sp = sparse_create_impl(dm, depth, offset);
+    sp = sparse_create_impl(dm, depth, offset, n_gauss);
     BIE_REQUIRE(nnz == sp->nnz, BIE_EINVAL,
                 "nnz does not match the pattern (bie_sparse_count gives " +
                     std::to_string(sp->nnz) + ")");
@@ -1691,7 +1746,7 @@ int bie_partial_assemble(const bie_mesh* mesh, double k_re, double k_im, const d
     BIE_REQUIRE(I && J && F && G, BIE_EINVAL, "output pointer is NULL");
     BIE_REQUIRE(sources == nullptr || n_sources == dm->N, BIE_EINVAL,
                 "the partial assembly needs one source per mesh source (n_sources)");
-    sp = sparse_create_impl(dm, depth, NAN);
+    sp = sparse_create_impl(dm, depth, NAN, 40);
     BIE_REQUIRE(nnz == sp->nnz, BIE_EINVAL,
                 "nnz does not match the pattern (bie_sparse_count gives " +
                     std::to_string(sp->nnz) + ")");

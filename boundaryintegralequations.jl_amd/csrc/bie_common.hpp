@@ -80,6 +80,7 @@ inline bool kind_has_beta(int pkind) {
 // the nearest of six physics nodes is an edge node.
 inline bool uses_mid_rules(int gk, int pk) { return gk == 6 && pk == 6; }
 void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
+void gauss_legendre_pm1(int n, std::vector<double>& x, std::vector<double>& w);
 
 // Device-side launch parameters for one assembly.
 struct AssemblyArgs {
@@ -267,6 +268,20 @@ bool gauss_points_triangle(int n, std::vector<double>& u, std::vector<double>& v
 void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,
                     std::vector<int64_t>& row_ptr, std::vector<int>& cols,
                     std::vector<int64_t>& pair_ptr, std::vector<int>& pair_elem);
+// connected_sources of discontinuous physics (row r belongs to element r / NP; N == E * NP);
+// topo: the geometry topology (E x NG), whose first three nodes are the corners.
+void sparse_pattern_disc(const int* topo, int NG, const int* ptopo, int NP, int64_t E,
+                         int64_t N, int depth, std::vector<int64_t>& row_ptr,
+                         std::vector<int>& cols, std::vector<int64_t>& pair_ptr,
+                         std::vector<int>& pair_elem);
+// The polar rule of a discontinuous element around its physics node (pu, pv): three
+// sub-triangles from the node to the reference triangle's sides, polar_gaussian(n) on each.
+void polar_rule(double pu, double pv, int n, std::vector<double>& u, std::vector<double>& v,
+                std::vector<double>& w);
+// Local (u, v) of node c of a discontinuous triangular physics kind.
+void discontinuous_node(int pkind, double beta, int c, double* u, double* v);
+// Physics basis of any physics kind at (u, v) (rules.cpp); N holds 9 entries.
+void physics_basis(int pkind, double beta, double u, double v, double* N);
 void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s);
 int sparse_record_doubles();
 // Geometry basis of a geometry kind and its derivatives at (u, v) (rules.cpp); N, du, dv

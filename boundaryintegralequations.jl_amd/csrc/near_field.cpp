@@ -12,6 +12,10 @@
 // * sparse_pattern: connected_sources (assembly_collocation_losses.jl:98-132).
 // * gauss_points_triangle: the symmetric triangle rules of the partial assembly
 //   (partial_assemble_parallel!, fast_multipole_method.jl:11-81).
+// * polar_rule: create_polar_elements (assembly_collocation_losses.jl:409-447) with
+//   polar_gaussian (quadrature.jl:196-206) and polar_quadrature (:163-190), the rules of
+//   discontinuous physics; sparse_pattern_disc: connected_sources of discontinuous physics
+//   (assembly_collocation_losses.jl:41-96).
 // Plain C++ (no device code), so that the CPU sanitizer build covers it.
 #include <algorithm>
 #include <cmath>
@@ -221,6 +225,125 @@ void sparse_pattern(const int* ptopo, int NP, int64_t E, int64_t N, int depth,
     cols.insert(cols.end(), sc[i].begin(), sc[i].end());
     row_ptr.push_back((int64_t)cols.size());
     pair_elem.insert(pair_elem.end(), el[i].begin(), el[i].end());
+    pair_ptr.push_back((int64_t)pair_elem.size());
+  }
+}
+
+// Local (u, v) of node c (0-based) of a discontinuous triangular physics kind
+// (get_nodal_nodes_u/_v, SurfaceFunction.jl:429-438): the centroid of the constant element;
+// the corners moved inwards by beta, then (quadratic) the edge nodes 1-2, 2-3, 3-1.
+void discontinuous_node(int pkind, double beta, int c, double* u, double* v) {
+  const double b = beta, h = (1.0 - b) / 2.0;
+  const double lu[6] = {b, 1.0 - 2.0 * b, b, h, h, b};
+  const double lv[6] = {b, b, 1.0 - 2.0 * b, b, h, h};
+  if (pkind == BIE_DTRI1) {
+    *u = 1.0 / 3.0;
+    *v = 1.0 / 3.0;
+    return;
+  }
+  BIE_REQUIRE((pkind == BIE_DTRI3 && c >= 0 && c < 3) || (pkind == BIE_DTRI6 && c >= 0 && c < 6),
+              BIE_EINVAL, "internal: discontinuous node out of range");
+  *u = lu[c];
+  *v = lv[c];
+}
+
+// create_polar_elements(physics_function, n, c) for the node (pu, pv): polar_gaussian(n)
+// (n rounded up to even radii and as many angles; Gauss-Legendre angles on [0, pi/2], for
+// each angle Gauss-Legendre radii on [0, 1 / (cos + sin)], weight w_angle w_radius r) on
+// the reference triangle, whose vertex (0, 0) carries the singularity, mapped by the linear
+// basis [1 - X - Y, X, Y] onto the sub-triangles (P, (0,1), (0,0)), (P, (1,0), (0,1)) and
+// (P, (0,0), (1,0)), weights times |det| of each map; points of the three in that order.
+void polar_rule(double pu, double pv, int n, std::vector<double>& u, std::vector<double>& v,
+                std::vector<double>& w) {
+  BIE_REQUIRE(n >= 1 && n <= 400, BIE_EINVAL, "Ngauss must be between 1 and 400");
+  const int nr = ((n + 1) / 2) * 2, na = nr;
+  std::vector<double> ga, wa, gr, wr;
+  gauss_legendre_pm1(na, ga, wa);
+  gauss_legendre_pm1(nr, gr, wr);
+  std::vector<double> X, Y, W;
+  const double a0 = 0.0, a1 = kPi / 2.0;
+  for (int i = 0; i < na; ++i) {
+    const double th = (a0 + a1) / 2.0 + (a1 - a0) / 2.0 * ga[i];
+    const double thw = (a1 - a0) / 2.0 * wa[i];
+    const double rmax = 1.0 / (std::cos(th) + std::sin(th));
+    for (int j = 0; j < nr; ++j) {
+      const double rad = (0.0 + rmax) / 2.0 + (rmax - 0.0) / 2.0 * gr[j];
+      const double radw = (rmax - 0.0) / 2.0 * wr[j] * rad;
+      X.push_back(rad * std::cos(th));
+      Y.push_back(rad * std::sin(th));
+      W.push_back(thw * radw);
+    }
+  }
+  const double A[3][2] = {{0.0, 1.0}, {1.0, 0.0}, {0.0, 0.0}};
+  const double B[3][2] = {{0.0, 0.0}, {0.0, 1.0}, {1.0, 0.0}};
+  u.clear();
+  v.clear();
+  w.clear();
+  for (int t = 0; t < 3; ++t) {
+    const double du0 = A[t][0] - pu, du1 = A[t][1] - pv;  // T * d[1-X-Y, X, Y]/dX
+    const double dv0 = B[t][0] - pu, dv1 = B[t][1] - pv;  // T * d[...]/dY
+    const double jac = std::fabs(du0 * dv1 - du1 * dv0);
+    for (size_t q = 0; q < X.size(); ++q) {
+      const double l1 = (1.0 - X[q]) - Y[q];
+      u.push_back(pu * l1 + A[t][0] * X[q] + B[t][0] * Y[q]);
+      v.push_back(pv * l1 + A[t][1] * X[q] + B[t][1] * Y[q]);
+      w.push_back(jac * W[q]);
+    }
+  }
+}
+
+// connected_sources for discontinuous physics: row r is node r mod NP of element r / NP
+// (inverse_rle of the element lists, NP rows each).  Depth 1: the element itself and its
+// own physics nodes; depth 2: every element sharing a corner (the first three geometry
+// nodes) with it, and their physics nodes; lists sorted and unique.
+void sparse_pattern_disc(const int* topo, int NG, const int* ptopo, int NP, int64_t E,
+                         int64_t N, int depth, std::vector<int64_t>& row_ptr,
+                         std::vector<int>& cols, std::vector<int64_t>& pair_ptr,
+                         std::vector<int>& pair_elem) {
+  BIE_REQUIRE(depth == 1 || depth == 2, BIE_EINVAL, "depth must be 1 or 2");
+  BIE_REQUIRE(N == E * (int64_t)NP, BIE_EINVAL,
+              "discontinuous near field: one row per element node (n_sources = "
+              "n_elements * nodes per element)");
+  auto uniq = [](std::vector<int>& v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  std::vector<std::vector<int>> el(E), sc(E);
+  if (depth == 1) {
+    for (int64_t e = 0; e < E; ++e) {
+      el[e].push_back((int)e);
+      sc[e].assign(ptopo + e * NP, ptopo + (e + 1) * NP);
+      uniq(sc[e]);
+    }
+  } else {
+    int nc = 0;
+    for (int64_t e = 0; e < E; ++e)
+      for (int g = 0; g < 3; ++g) nc = std::max(nc, topo[e * NG + g] + 1);
+    std::vector<std::vector<int>> corner(nc);
+    for (int64_t e = 0; e < E; ++e)
+      for (int g = 0; g < 3; ++g) corner[topo[e * NG + g]].push_back((int)e);
+    for (int64_t e = 0; e < E; ++e) {
+      for (int g = 0; g < 3; ++g) {
+        const std::vector<int>& c = corner[topo[e * NG + g]];
+        el[e].insert(el[e].end(), c.begin(), c.end());
+      }
+      uniq(el[e]);
+      for (int f : el[e]) {
+        const int* p = ptopo + (int64_t)f * NP;
+        sc[e].insert(sc[e].end(), p, p + NP);
+      }
+      uniq(sc[e]);
+    }
+  }
+  row_ptr.assign(1, 0);
+  pair_ptr.assign(1, 0);
+  cols.clear();
+  pair_elem.clear();
+  for (int64_t r = 0; r < N; ++r) {
+    const int64_t e = r / NP;
+    cols.insert(cols.end(), sc[e].begin(), sc[e].end());
+    row_ptr.push_back((int64_t)cols.size());
+    pair_elem.insert(pair_elem.end(), el[e].begin(), el[e].end());
     pair_ptr.push_back((int64_t)pair_elem.size());
   }
 }

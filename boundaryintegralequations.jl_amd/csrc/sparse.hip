@@ -5,7 +5,8 @@
 //
 // * Pattern and rules: built on the host (near_field.cpp): connected_sources' rows and
 //   columns, and the six subdivision rules of singular_triangle_integration in the
-//   reference triangle.
+//   reference triangle (continuous physics) or the polar rules of create_polar_elements
+//   around the physics nodes (discontinuous physics, :233-257 there).
 // * Pairs: every (row i, connected element e) is integrated with the rule of the element's
 //   geometry node closest to source i (find_closest_corner on the element coordinates,
 //   :308-309 there), the kernels of sparse_computing_integrals! (:158-186: freens3d!,
@@ -20,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "bie_common.hpp"
@@ -202,12 +204,22 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
   const dim3 grid((unsigned)blocks);
   const bool cplx = L.ki != 0.0;
   auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, a); };
-  if (L.NG == 3 && L.NP == 3)
-    cplx ? go(k_sparse_pairs<3, 3, true>) : go(k_sparse_pairs<3, 3, false>);
-  else if (L.NG == 6 && L.NP == 6)
-    cplx ? go(k_sparse_pairs<6, 6, true>) : go(k_sparse_pairs<6, 6, false>);
+  // geometry nodes 3 or 6; physics nodes 1 (discontinuous constant), 3 or 6
+  auto by_np = [&](auto ng) {
+    constexpr int NG = decltype(ng)::value;
+    if (L.NP == 1)
+      cplx ? go(k_sparse_pairs<NG, 1, true>) : go(k_sparse_pairs<NG, 1, false>);
+    else if (L.NP == 3)
+      cplx ? go(k_sparse_pairs<NG, 3, true>) : go(k_sparse_pairs<NG, 3, false>);
+    else
+      cplx ? go(k_sparse_pairs<NG, 6, true>) : go(k_sparse_pairs<NG, 6, false>);
+  };
+  BIE_REQUIRE((L.NG == 3 || L.NG == 6) && (L.NP == 1 || L.NP == 3 || L.NP == 6), BIE_EINVAL,
+              "internal: sparse kernel node counts");
+  if (L.NG == 3)
+    by_np(std::integral_constant<int, 3>());
   else
-    cplx ? go(k_sparse_pairs<6, 3, true>) : go(k_sparse_pairs<6, 3, false>);
+    by_np(std::integral_constant<int, 6>());
   BIE_HIP_CHECK(hipGetLastError());
 }
 

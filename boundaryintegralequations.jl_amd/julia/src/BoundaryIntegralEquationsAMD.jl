@@ -13,10 +13,11 @@ Supported meshes run on the GPU: TriangularLinear / TriangularQuadratic geometry
 geometry, linear or discontinuous triangular physics, and the quadrilaterals
 (QuadrilateralLinear4, QuadrilateralLinearSerendipity, QuadrilateralQuadratic,
 QuadrilateralQuadraticLagrange) with their own, linear or discontinuous quadrilateral
-physics; dense, and for continuous physics on triangles also `sparse=true` (the near
-field, `sparse_assemble_parallel!`).  Everything else (the near field of discontinuous
-physics and of quadrilaterals, other element types) is forwarded to the reference method
-unchanged, so the call stays a drop-in.  Errors from
+physics; dense, and on triangles also `sparse=true` (the near field,
+`sparse_assemble_parallel!`, with the subdivision rules of continuous physics or the polar
+rules of discontinuous physics) and the fast operators' `partial_assemble_parallel!`.
+Everything else (the near field of quadrilaterals, other element types) is forwarded to the
+reference method unchanged, so the call stays a drop-in.  Errors from
 the library are thrown as `ArgumentError` (bad input) or `ErrorException` (HIP failure).
 
 This file is not executed in the build container (no Julia there); the same ABI is
@@ -78,7 +79,6 @@ physics_beta(pf::BIE.DiscontinuousTriangularLinear) = Float64(pf.beta)
 physics_beta(pf::BIE.DiscontinuousTriangularQuadratic) = Float64(pf.beta)
 physics_beta(pf::BIE.DiscontinuousQuadrilateralLinear4) = Float64(pf.beta)
 physics_beta(pf::BIE.DiscontinuousQuadrilateralQuadraticLagrange) = Float64(pf.beta)
-discontinuous(mesh) = (physics_kind(mesh.physics_function) & 0x100) != 0
 quadrilateral(mesh) = mesh.shape_function isa BIE.Quadrilateral
 
 bie_mesh(mesh, coords, topo, srcs, ptopo) =
@@ -149,9 +149,8 @@ GPU and everything else to `BoundaryIntegralEquations.assemble_parallel!`.
 """
 function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true, cOn=true,
                             sparse=false, m=4, n=4, progress=true, depth=2, offset=nothing)
-    # the near field of discontinuous physics (its own connectivity and polar rules) and of
-    # quadrilaterals stays with the reference
-    if !supported(mesh) || (sparse && (discontinuous(mesh) || quadrilateral(mesh)))
+    # the near field of quadrilaterals stays with the reference
+    if !supported(mesh) || (sparse && quadrilateral(mesh))
         return BIE.assemble_parallel!(mesh, k, in_sources; fOn=fOn, gOn=gOn, cOn=cOn,
                                       sparse=sparse, m=m, n=n, progress=progress,
                                       depth=depth, offset=offset)
@@ -162,14 +161,16 @@ function assemble_parallel!(mesh::BIE.Mesh3d, k, in_sources; fOn=true, gOn=true,
 end
 
 """
-    sparse_hip(mesh, k, sources; fOn=true, gOn=true, depth=1, offset=nothing)
+    sparse_hip(mesh, k, sources; fOn=true, gOn=true, depth=1, offset=nothing, Ngauss=40)
 
 Near-field F, G on the GPU as `SparseMatrixCSC`s (n_sources x n_sources) with the
 reference's stored entries (`sparse_assemble_parallel!`,
 src/3d/assembly_collocation_losses.jl:200-405): `sparse(I, J, F), sparse(I, J, G)`.
+`Ngauss`: the polar rules' order for discontinuous physics (40 for quadratic physics,
+as the reference forces).
 """
 function sparse_hip(mesh::BIE.Mesh3d, k::Number, sources::AbstractMatrix; fOn=true, gOn=true,
-                    depth=1, offset=nothing)
+                    depth=1, offset=nothing, Ngauss=40)
     coords = Matrix{Float64}(mesh.coordinates)
     topo = Matrix{Int64}(mesh.topology)
     srcs = Matrix{Float64}(mesh.sources)
@@ -188,11 +189,11 @@ function sparse_hip(mesh::BIE.Mesh3d, k::Number, sources::AbstractMatrix; fOn=tr
                     bm, Int32(depth), nnz))
         I = Vector{Int64}(undef, nnz[]); J = similar(I)
         F = Vector{ComplexF64}(undef, nnz[]); G = similar(F)
-        check(ccall((:bie_sparse_assemble, libbie[]), Cint,
-                    (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Cdouble, UInt32,
-                     Int64, Ptr{Int64}, Ptr{Int64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
+        check(ccall((:bie_sparse_assemble_ex, libbie[]), Cint,
+                    (Ref{BieMesh}, Cdouble, Cdouble, Ptr{Float64}, Int64, Int32, Cdouble, Int32,
+                     UInt32, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{ComplexF64}, Ptr{ComplexF64}),
                     bm, real(kc), imag(kc), own ? Ptr{Float64}(C_NULL) : pointer(pts),
-                    own ? 0 : N, Int32(depth), off, flags, nnz[], I, J, F, G))
+                    own ? 0 : N, Int32(depth), off, Int32(Ngauss), flags, nnz[], I, J, F, G))
         return SparseArrays.sparse(I, J, F, N, N), SparseArrays.sparse(I, J, G, N, N)
     end
 end
@@ -257,22 +258,27 @@ end
 # The near field: sparse_assemble_parallel!(mesh, k, sources, physics_function; fOn, gOn,
 # depth, offset, progress, Ngauss) (assembly_collocation_losses.jl:200-405), called by the
 # reference's fast operators (fast_multipole_method.jl:181,187) and lossy systems through
-# assemble_parallel!(...; sparse=true).  Continuous linear and quadratic physics run on the
-# GPU; other physics (the polar rules of discontinuous elements) and a source count that
-# differs from the mesh's go to the reference's method.
+# assemble_parallel!(...; sparse=true).  Continuous (subdivision rules) and discontinuous
+# (polar rules, Ngauss) triangular physics run on the GPU when the physics function is the
+# mesh's own (the kernel takes the mesh's physics); another physics function, quadrilaterals
+# and a source count that differs from the mesh's go to the reference's method.
 function BIE.sparse_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
                                        sources::AbstractMatrix{<:Real},
                                        physics_function::Union{BIE.TriangularLinear,
-                                                               BIE.TriangularQuadratic};
+                                                               BIE.TriangularQuadratic,
+                                                               BIE.DiscontinuousTriangular};
                                        fOn=true, gOn=true, depth=1, offset=nothing,
                                        progress=true, Ngauss=40)
-    if !(GPU_ROUTING[] && supported(mesh) && !discontinuous(mesh) && !quadrilateral(mesh) &&
+    same = typeof(physics_function) == typeof(mesh.physics_function) &&
+           physics_beta(physics_function) == physics_beta(mesh.physics_function)
+    if !(GPU_ROUTING[] && supported(mesh) && !quadrilateral(mesh) && same &&
          size(sources, 2) == size(mesh.sources, 2))
         return invoke(BIE.sparse_assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,Any}, mesh, k,
                       sources, physics_function; fOn=fOn, gOn=gOn, depth=depth,
                       offset=offset, progress=progress, Ngauss=Ngauss)
     end
-    return sparse_hip(mesh, k, sources; fOn=fOn, gOn=gOn, depth=depth, offset=offset)
+    return sparse_hip(mesh, k, sources; fOn=fOn, gOn=gOn, depth=depth, offset=offset,
+                      Ngauss=Ngauss)
 end
 
 """
@@ -311,13 +317,13 @@ function partial_hip(mesh::BIE.Mesh3d, k::Number, sources::AbstractMatrix; fOn=t
 end
 
 # The reference's fast operators call partial_assemble_parallel!(mesh, zk, targets,
-# shape_function; fOn, gOn, depth) (fast_multipole_method.jl:179,185); continuous physics on
-# triangles runs on the GPU, the rest goes to the reference's method.
+# shape_function; fOn, gOn, depth) (fast_multipole_method.jl:179,185); continuous and
+# discontinuous physics on triangles run on the GPU, the rest goes to the reference's method.
 function BIE.partial_assemble_parallel!(mesh::BIE.Mesh3d, k::Number,
                                         in_sources::AbstractMatrix{<:Real},
                                         shape_function::BIE.Triangular; fOn=true, gOn=true,
                                         n=3, progress=false, depth=1)
-    if !(GPU_ROUTING[] && supported(mesh) && !discontinuous(mesh) && !quadrilateral(mesh) &&
+    if !(GPU_ROUTING[] && supported(mesh) && !quadrilateral(mesh) &&
          n in (1, 3, 4, 7, 12, 13) && size(in_sources, 2) == size(mesh.sources, 2))
         return invoke(BIE.partial_assemble_parallel!, Tuple{BIE.Mesh3d,Any,Any,BIE.Triangular},
                       mesh, k, in_sources, shape_function; fOn=fOn, gOn=gOn, n=n,

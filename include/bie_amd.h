@@ -142,7 +142,16 @@ int bie_host_cache_release(void);
  * sparse(I, J, F), sparse(I, J, G): I = the row of every stored entry, J = its column (the
  * connected sources of each row, ascending), in that order; these entry points give the
  * same I, J (index_base of the mesh) and values.  offset: the reference's `offset` keyword
- * (NaN = nothing, i.e. 0 for continuous physics).  No C (free term) is formed there. */
+ * (NaN = nothing, i.e. 0 for continuous physics).  No C (free term) is formed there.
+ * Triangular geometry; continuous or discontinuous physics.  Discontinuous physics
+ * (BIE_DTRI1/3/6) follows the reference's branch for it (:233-257 and connected_sources,
+ * :41-96): row r is node r mod n of element r / n (n nodes per element, so n_sources must
+ * be n_elements * n), connected at depth 1 to its own element and at depth 2 to every
+ * element sharing a corner with it; each pair is integrated with the polar rule
+ * (create_polar_elements, :409-447: polar_gaussian(Ngauss), src/3d/quadrature.jl:196-206,
+ * on three sub-triangles) around the physics node whose index is that of the closest
+ * geometry node (the reference's filler rules for the missing ones: nodes 1, 2, 3; the
+ * constant element: node 1).  offset is not used there. */
 int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz);
 /* sources: 3 x n_sources (NULL = mesh sources; n_sources must equal the mesh's), flags
  * BIE_F | BIE_G (a disabled output is zero-filled), I, J (nnz int64), F, G (nnz complex)
@@ -150,10 +159,20 @@ int bie_sparse_count(const bie_mesh* mesh, int32_t depth, int64_t* nnz);
 int bie_sparse_assemble(const bie_mesh* mesh, double k_re, double k_im, const double* sources,
                         int64_t n_sources, int32_t depth, double offset, uint32_t flags,
                         int64_t nnz, int64_t* I, int64_t* J, double* F, double* G);
+/* The same with the reference's Ngauss keyword (the polar rules' order for discontinuous
+ * physics, 1..400; the reference's default 40, which bie_sparse_assemble uses, and forced to
+ * 40 for BIE_DTRI6 as the reference does; not used for continuous physics). */
+int bie_sparse_assemble_ex(const bie_mesh* mesh, double k_re, double k_im,
+                           const double* sources, int64_t n_sources, int32_t depth,
+                           double offset, int32_t n_gauss, uint32_t flags, int64_t nnz,
+                           int64_t* I, int64_t* J, double* F, double* G);
 /* Device-resident form: the pattern, pairs and rules built once per (mesh, depth, offset),
  * then any number of assemblies into device arrays of nnz complex entries (asynchronous on
  * stream; d_sources NULL = the mesh sources).  Destroy the handle before its dmesh. */
 int bie_sparse_create(bie_dmesh* dm, int32_t depth, double offset, bie_sparse** out);
+/* The same with Ngauss (as bie_sparse_assemble_ex). */
+int bie_sparse_create_ex(bie_dmesh* dm, int32_t depth, double offset, int32_t n_gauss,
+                         bie_sparse** out);
 int bie_sparse_destroy(bie_sparse* sp);
 int64_t bie_sparse_nnz(const bie_sparse* sp);
 int bie_sparse_pattern(const bie_sparse* sp, int32_t index_base, int64_t* I, int64_t* J);

@@ -22,6 +22,10 @@ small meshes.  Reference functions restated (file:line):
   find_closest_corner          src/3d/assembly_collocation.jl:102-115
   greens3d!/freens3d!/freens3dk0! src/3d/kernels.jl:13-55
   computing_integrals!         src/3d/assembly_collocation.jl:213-252
+  polar_quadrature / polar_gaussian src/3d/quadrature.jl:163-206
+  create_polar_elements        src/3d/assembly_collocation_losses.jl:409-447
+  connected_sources (discontinuous physics) src/3d/assembly_collocation_losses.jl:41-96
+  sparse_assemble_parallel! (discontinuous branch) src/3d/assembly_collocation_losses.jl:200-405
 """
 from __future__ import annotations
 
@@ -358,3 +362,134 @@ def partial_assemble(mesh, k, sources, element_lists, column_lists, n=3, fOn=Tru
         Fv.append(f_row)
         Gv.append(g_row)
     return np.concatenate(Fv), np.concatenate(Gv)
+
+
+# ---- near field of discontinuous physics ---------------------------------------------------
+
+DTRI1, DTRI3, DTRI6 = 0x101, 0x103, 0x106
+
+
+def polar_gaussian(n):
+    """polar_gaussian(n): n rounded up to even radii and as many angles; Gauss-Legendre
+    angles on [0, pi/2], radii on [0, 1 / (cos + sin)], weights w_angle w_radius r."""
+    nr = int(np.ceil(n / 2.0)) * 2
+    ga, wa = np.polynomial.legendre.leggauss(nr)
+    gr, wr = np.polynomial.legendre.leggauss(nr)
+    th = np.pi / 4 + np.pi / 4 * ga
+    thw = np.pi / 4 * wa
+    rmax = 1.0 / (np.cos(th) + np.sin(th))
+    rad = rmax[:, None] / 2 + rmax[:, None] / 2 * gr[None, :]     # (angle, radius)
+    radw = rmax[:, None] / 2 * wr[None, :] * rad
+    ang = np.repeat(th, nr)
+    rad = rad.ravel()
+    return rad * np.cos(ang), rad * np.sin(ang), np.repeat(thw, nr) * radw.ravel()
+
+
+def disc_nodes(kind, beta):
+    """get_nodal_nodes_u/_v of the discontinuous triangles (SurfaceFunction.jl:429-438)."""
+    b = float(beta)
+    if kind == DTRI1:
+        return np.array([1 / 3]), np.array([1 / 3])
+    if kind == DTRI3:
+        return np.array([b, 1 - 2 * b, b]), np.array([b, b, 1 - 2 * b])
+    h = (1 - b) / 2
+    return np.array([b, 1 - 2 * b, b, h, h, b]), np.array([b, b, 1 - 2 * b, b, h, h])
+
+
+def create_polar_elements(kind, beta, n, c):
+    """(u, v, w) of create_polar_elements(physics_function, n, c) (c 1-based): the polar
+    rule mapped onto the three sub-triangles (P, (0,1), (0,0)), (P, (1,0), (0,1)),
+    (P, (0,0), (1,0)) around the physics node P, weights times |det| of each map."""
+    X, Y, W = polar_gaussian(n)
+    un, vn = disc_nodes(kind, beta)
+    P = np.array([un[c - 1], vn[c - 1]])
+    us, vs, ws = [], [], []
+    for A, B in (((0.0, 1.0), (0.0, 0.0)), ((1.0, 0.0), (0.0, 1.0)), ((0.0, 0.0), (1.0, 0.0))):
+        A, B = np.array(A), np.array(B)
+        T = np.stack([P, A, B], axis=1)                     # 2 x 3
+        G = T @ np.stack([1 - X - Y, X, Y])                 # 2 x Q
+        Du, Dv = A - P, B - P
+        ws.append(abs(Du[0] * Dv[1] - Du[1] * Dv[0]) * W)
+        us.append(G[0])
+        vs.append(G[1])
+    return np.concatenate(us), np.concatenate(vs), np.concatenate(ws)
+
+
+def connected_sources_disc(mesh, depth=1):
+    """connected_sources for discontinuous physics, per ELEMENT (row r uses the lists of
+    element r // n): (element lists, source lists), 0-based, sorted."""
+    E = mesh.n_elements
+    ptop = mesh.physics_topology
+    if depth == 1:
+        return [np.array([e]) for e in range(E)], [np.unique(ptop[:, e]) for e in range(E)]
+    if depth != 2:
+        raise ValueError("depth must be 1 or 2")
+    corners = mesh.topology[:3]
+    touch = {}
+    for e in range(E):
+        for node in corners[:, e]:
+            touch.setdefault(int(node), []).append(e)
+    els, srcs = [], []
+    for e in range(E):
+        el = np.unique(np.concatenate([touch[int(nd)] for nd in corners[:, e]]))
+        els.append(el)
+        srcs.append(np.unique(ptop[:, el]))
+    return els, srcs
+
+
+def sparse_assemble_disc(mesh, k, sources=None, depth=1, Ngauss=40, fOn=True, gOn=True):
+    """sparse_assemble_parallel! for discontinuous triangular physics as (I, J, F, G):
+    rows in order, columns ascending (0-based), complex values.  Each (row, element) pair
+    takes the polar rule of the physics node whose index is that of the element's
+    geometry node closest to the source (the reference's filler rules: nodes 1, 2, 3 for
+    the missing ones, node 1 for constant physics; no finite distance: the sixth slot)."""
+    kind, beta = mesh.physics_kind, float(getattr(mesh, "physics_beta", 0.0))
+    if kind == DTRI6:
+        Ngauss = 40
+    npk, E, N = mesh.physics_topology.shape[0], mesh.n_elements, mesh.n_sources
+    if N != E * npk:
+        raise ValueError("one row per element node")
+    src = mesh.sources if sources is None else np.asarray(sources, dtype=np.float64)
+    rules = []
+    for slot in range(6):
+        node = 1 if kind == DTRI1 else (slot % 3 + 1 if kind == DTRI3 else slot + 1)
+        u, v, w = create_polar_elements(kind, beta, Ngauss, node)
+        Ng, du, dv = basis(mesh.geometry_kind, u, v)
+        phi, _, _ = basis(kind, u, v, beta)
+        rules.append((w, Ng, du, dv, phi))
+    els, cols = connected_sources_disc(mesh, depth)
+    k = complex(k)
+    I, Jc, Fv, Gv = [], [], [], []
+    for r in range(N):
+        e0 = r // npk
+        x = src[:, r]
+        c_row = cols[e0]
+        where = {int(c): t for t, c in enumerate(c_row)}
+        f_row = np.zeros(len(c_row), dtype=np.complex128)
+        g_row = np.zeros(len(c_row), dtype=np.complex128)
+        for e in els[e0]:
+            X = mesh.coordinates[:, mesh.topology[:, e]]            # 3 x ng
+            d = julia_hypot3(X[0] - x[0], X[1] - x[1], X[2] - x[2])
+            d = np.where(np.isnan(d), np.inf, d)
+            slot = int(np.argmin(d)) if np.isfinite(d.min()) else 5
+            w, Ng, du, dv, phi = rules[slot]
+            y, tu, tv = Ng @ X.T, du @ X.T, dv @ X.T                 # Q x 3
+            cr = np.cross(tu, tv)
+            J = julia_hypot3(cr[:, 0], cr[:, 1], cr[:, 2])
+            nrm = -cr / J[:, None]
+            jw = J * w
+            dd = y - x[None, :]
+            rr = julia_hypot3(dd[:, 0], dd[:, 1], dd[:, 2])
+            nd = np.einsum("qd,qd->q", nrm, dd)
+            ekr = np.exp(1j * k * rr)
+            idx = [where[int(c)] for c in mesh.physics_topology[:, e]]
+            if gOn:
+                np.add.at(g_row, idx, (ekr / (4 * np.pi * rr) * jw) @ phi)
+            if fOn:
+                f = ekr * (1j * k * rr - 1.0) * nd / (4 * np.pi * rr ** 3) * jw
+                np.add.at(f_row, idx, f @ phi)
+        I.append(np.full(len(c_row), r, dtype=np.int64))
+        Jc.append(np.asarray(c_row, dtype=np.int64))
+        Fv.append(f_row)
+        Gv.append(g_row)
+    return np.concatenate(I), np.concatenate(Jc), np.concatenate(Fv), np.concatenate(Gv)

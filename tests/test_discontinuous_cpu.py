@@ -150,9 +150,9 @@ def test_library_rejects_bad_discontinuous_meshes():
     mesh = M.with_discontinuous_physics(M.icosphere(1, "linear"), "disctrilinear")
     hm = _lib.HostMesh(mesh)
     nnz = ctypes.c_int64()
-    # the near field of discontinuous physics is not covered
-    assert L.bie_sparse_count(ctypes.byref(hm.struct), 1, ctypes.byref(nnz)) == \
-        _lib.BIE_EUNSUPPORTED
+    # the near field of discontinuous physics: one row per element node, its own element
+    assert L.bie_sparse_count(ctypes.byref(hm.struct), 1, ctypes.byref(nnz)) == _lib.BIE_OK
+    assert nnz.value == 3 * mesh.n_sources
     # beta = 1/3 collapses the nodes
     hm.struct.physics_beta = 1.0 / 3.0
     assert L.bie_sparse_count(ctypes.byref(hm.struct), 1, ctypes.byref(nnz)) == _lib.BIE_EINVAL
@@ -164,7 +164,7 @@ def test_library_rejects_bad_discontinuous_meshes():
     hm.struct.physics_kind = 0x102
     assert L.bie_sparse_count(ctypes.byref(hm.struct), 1, ctypes.byref(nnz)) == \
         _lib.BIE_EUNSUPPORTED
-    assert L.bie_version() >= 2
+    assert L.bie_version() >= 3
 
 
 def test_loaders_build_discontinuous_meshes():
