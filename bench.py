@@ -434,7 +434,8 @@ def measure_sparse(torch, bie_amd, steps=3, f=16):
     """The near field (sparse_assemble_parallel!) on the 10242-node quadratic sphere at
     k = 10: depth 1 (the function's default) and depth 2 (assemble_parallel!(...;
     sparse=true)'s default), F and G, from a device handle (pattern and rules built once);
-    and the partial near field (partial_assemble_parallel!, n = 3) on the same pattern."""
+    the partial near field (partial_assemble_parallel!, n = 3) on the same pattern; and the
+    near field of discontinuous linear physics on the same geometry (polar rules)."""
     from bie_amd.assembly import DeviceMesh, SparsePattern
     mesh = bie_amd.icosphere(f, "quadratic")
     dm = DeviceMesh(mesh, 0)
@@ -467,6 +468,27 @@ def measure_sparse(torch, bie_amd, steps=3, f=16):
         e1.record()
         torch.cuda.synchronize()
         res[f"depth{depth}"]["partial_n3_ms"] = e0.elapsed_time(e1) / steps
+        sp.close()
+        del F, G
+    dm.close()
+    # discontinuous linear physics on the same geometry (15360 element-wise rows): the polar
+    # rules at the reference's default Ngauss = 40 (4800 points per pair)
+    disc = bie_amd.with_discontinuous_physics(mesh, "disctrilinear")
+    dm = DeviceMesh(disc, 0)
+    res["disctrilinear"] = {"rows": disc.n_sources, "Ngauss": 40}
+    for depth in (1, 2):
+        sp = SparsePattern(dm, depth=depth, Ngauss=40)
+        F = torch.empty(sp.nnz, dtype=torch.complex128, device=dev)
+        G = torch.empty_like(F)
+        sp.assemble_device(10.0, F.data_ptr(), G.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            sp.assemble_device(10.0, F.data_ptr(), G.data_ptr(), stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        res["disctrilinear"][f"depth{depth}"] = {"nnz": sp.nnz,
+                                                 "ms": e0.elapsed_time(e1) / steps}
         sp.close()
         del F, G
     dm.close()

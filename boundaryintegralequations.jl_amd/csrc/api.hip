@@ -1379,26 +1379,16 @@ void sparse_destroy_impl(bie_sparse* sp) {
 }
 
 // Pattern, pairs, selection table and subdivision rules of a sparse assembly on dm.
-// Point records of a rule for the sparse kernel: w, the geometry basis and its derivatives,
-// the physics basis (sparse_record_doubles() each).
+// Point records of a rule for the sparse kernel: (u, v, w, 0), sparse_record_doubles() each;
+// the kernel evaluates the geometry and physics bases at (u, v).
 void append_rule_records(std::vector<double>& rec, const std::vector<double>& u,
-                         const std::vector<double>& v, const std::vector<double>& w, int NG,
-                         int pkind, double beta) {
-  const int RS = sparse_record_doubles();
+                         const std::vector<double>& v, const std::vector<double>& w) {
+  BIE_REQUIRE(sparse_record_doubles() == 4, BIE_EINVAL, "internal: sparse record layout");
   for (size_t q = 0; q < u.size(); ++q) {
-    double N[9], du[9], dv[9], Np[9];
-    geometry_basis(NG, u[q], v[q], N, du, dv);
-    for (int g = 0; g < 9; ++g) Np[g] = 0.0;
-    physics_basis(pkind, beta, u[q], v[q], Np);
-    std::vector<double> r(RS, 0.0);
-    r[0] = w[q];
-    for (int g = 0; g < 6; ++g) {
-      r[1 + g] = N[g];
-      r[7 + g] = du[g];
-      r[13 + g] = dv[g];
-      r[19 + g] = Np[g];
-    }
-    rec.insert(rec.end(), r.begin(), r.end());
+    rec.push_back(u[q]);
+    rec.push_back(v[q]);
+    rec.push_back(w[q]);
+    rec.push_back(0.0);
   }
 }
 
@@ -1492,7 +1482,7 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset, int n_ga
     }
     sp->off[c] = (int)(rec.size() / RS);
     sp->len[c] = (int)u.size();
-    append_rule_records(rec, u, v, w, NG, dm->pkind, dm->beta);
+    append_rule_records(rec, u, v, w);
   }
   sp->d_pairs = upload_vec(pairs, nullptr);
   sp->d_pos = upload_vec(pos, nullptr);
@@ -1531,6 +1521,8 @@ void sparse_assemble_impl(bie_sparse* sp, double kr, double ki, const double* d_
   L.NG = dm->gk;
   L.NP = dm->pk;
   L.hyp = dm->hyp;
+  L.disc = physics_discontinuous(dm->pkind);
+  L.beta = dm->beta;
   L.kr = kr;
   L.ki = ki;
   L.flags = flags;
@@ -1561,7 +1553,7 @@ void sparse_partial_impl(bie_sparse* sp, double kr, double ki, int n_gauss,
                 "Triangular quadrature not implemented for n=" + std::to_string(n_gauss) +
                     " (gauss_points_triangle has n = 1, 3, 4, 7, 12, 13)");
     std::vector<double> rec;
-    append_rule_records(rec, u, v, w, dm->gk, dm->pkind, dm->beta);
+    append_rule_records(rec, u, v, w);
     if (sp->d_partial) {
       BIE_HIP_CHECK(hipDeviceSynchronize());  // a launch may still read the old rule
       dfree(sp->d_partial);
@@ -1592,6 +1584,8 @@ void sparse_partial_impl(bie_sparse* sp, double kr, double ki, int n_gauss,
   L.NG = dm->gk;
   L.NP = dm->pk;
   L.hyp = dm->hyp;
+  L.disc = physics_discontinuous(dm->pkind);
+  L.beta = dm->beta;
   L.kr = kr;
   L.ki = ki;
   L.flags = flags;

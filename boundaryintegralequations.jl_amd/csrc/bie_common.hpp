@@ -253,9 +253,11 @@ struct SparseLaunch {
   const double* coords;   // 3 x Nc (device)
   const int* topo;        // E x NG (device)
   const double* gsel;     // E x NG x 4 geometry node coordinates (device)
-  const double* rules;    // point records, sparse_record_doubles() each (device)
+  const double* rules;    // point records (u, v, w, 0), sparse_record_doubles() each (device)
   int off[6], len[6];     // per geometry node: first record and point count
   int NG, NP, hyp;
+  bool disc;              // discontinuous physics (basis at beta)
+  double beta;
   double kr, ki;
   unsigned flags;
   double* F;              // nnz complex (device), zeroed

@@ -46,14 +46,91 @@ struct SparseArgs {
   const double* rules;    // point records (kSparseRec doubles)
   int off[6], len[6];
   int hyp;
+  bool disc;              // discontinuous physics, nodes at beta
+  double beta;
   double kr, ki;
   unsigned flags;
   double* F;              // nnz complex
   double* G;
 };
 
-// point record: w, N (6), dN/du (6), dN/dv (6), physics basis (6), pad
-constexpr int kSparseRec = 26;
+// point record: u, v, w, pad (the bases are evaluated per point from u, v: a 26-double
+// record with them precomputed made the kernel wait on L2 for 208 bytes per point)
+constexpr int kSparseRec = 4;
+
+// The geometry basis and its derivatives at (u, v), operation for operation as the host
+// tables (eval_basis, rules.cpp, compiled without fused multiply-adds) and the CPU
+// restatement compute them: the geometry below must round as theirs does.
+template <int NG>
+__device__ __forceinline__ void sparse_geom_basis(double u, double v, double (&N)[NG],
+                                                  double (&du)[NG], double (&dv)[NG]) {
+#pragma clang fp contract(off)
+  if constexpr (NG == 3) {
+    N[0] = (1.0 - u) - v;
+    N[1] = u;
+    N[2] = v;
+    du[0] = -1.0;
+    du[1] = 1.0;
+    du[2] = 0.0;
+    dv[0] = -1.0;
+    dv[1] = 0.0;
+    dv[2] = 1.0;
+  } else {
+    const double l = (1.0 - v) - u;
+    N[0] = l * ((1.0 - 2.0 * v) - 2.0 * u);
+    N[1] = u * (2.0 * u - 1.0);
+    N[2] = v * (2.0 * v - 1.0);
+    N[3] = 4.0 * u * l;
+    N[4] = 4.0 * u * v;
+    N[5] = 4.0 * v * l;
+    const double d0 = -(2.0 * l - 1.0) - 2.0 * l;
+    du[0] = d0;
+    du[1] = 4.0 * u - 1.0;
+    du[2] = 0.0;
+    du[3] = 4.0 * l - 4.0 * u;
+    du[4] = 4.0 * v;
+    du[5] = -4.0 * v;
+    dv[0] = d0;
+    dv[1] = 0.0;
+    dv[2] = 4.0 * v - 1.0;
+    dv[3] = -4.0 * u;
+    dv[4] = 4.0 * u;
+    dv[5] = 4.0 * l - 4.0 * v;
+  }
+}
+
+// The physics basis at (u, v): continuous three- or six-node triangles (the geometry
+// bases), or the discontinuous ones (surface_functions.jl:234-318) with psi(s) =
+// (s - beta) / (1 - 3 beta) of the barycentric coordinates 1 - u - v, u, v.
+template <int NP>
+__device__ __forceinline__ void sparse_phys_basis(bool disc, double beta, double u, double v,
+                                                  double (&N)[NP]) {
+  if constexpr (NP == 1) {
+    N[0] = 1.0;
+  } else {
+    if (!disc) {
+      double du[NP], dv[NP];
+      sparse_geom_basis<NP>(u, v, N, du, dv);
+      return;
+    }
+    const double den = 1.0 - 3.0 * beta;
+    const double l1 = (((1.0 - u) - v) - beta) / den;
+    const double l2 = (u - beta) / den;
+    const double l3 = (v - beta) / den;
+    if constexpr (NP == 3) {
+      N[0] = l1;
+      N[1] = l2;
+      N[2] = l3;
+    } else {
+      N[0] = l1 * (2.0 * l1 - 1.0);
+      N[1] = l2 * (2.0 * l2 - 1.0);
+      N[2] = l3 * (2.0 * l3 - 1.0);
+      N[3] = (4.0 * l2) * l1;
+      N[4] = (4.0 * l2) * l3;
+      N[5] = (4.0 * l3) * l1;
+    }
+  }
+}
 
 struct SparseGeom {
   double dx, dy, dz, nd, J;
@@ -66,7 +143,9 @@ struct SparseGeom {
 // part is evaluated exactly as the CPU restatement does.
 template <int NG>
 __device__ __forceinline__ SparseGeom sparse_geometry(const double (&X)[NG][3],
-                                                      const double* __restrict__ P, double x0,
+                                                      const double (&N)[NG],
+                                                      const double (&Du)[NG],
+                                                      const double (&Dv)[NG], double x0,
                                                       double x1, double x2) {
 #pragma clang fp contract(off)
   double y[3], tu[3], tv[3];
@@ -75,9 +154,9 @@ __device__ __forceinline__ SparseGeom sparse_geometry(const double (&X)[NG][3],
     double a = 0.0, b = 0.0, c = 0.0;
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
-      a = a + X[g][d] * P[1 + g];
-      b = b + X[g][d] * P[7 + g];
-      c = c + X[g][d] * P[13 + g];
+      a = a + X[g][d] * N[g];
+      b = b + X[g][d] * Du[g];
+      c = c + X[g][d] * Dv[g];
     }
     y[d] = a;
     tu[d] = b;
@@ -124,13 +203,17 @@ __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
   const double* R0 = a.rules + (size_t)a.off[c] * kSparseRec;
   const int Q = a.len[c];
   for (int q = lane; q < Q; q += 64) {
-    const double* P = R0 + (size_t)q * kSparseRec;
-    const SparseGeom sg = sparse_geometry<NG>(X, P, x0, x1, x2);
+    const double2 uv = *reinterpret_cast<const double2*>(R0 + (size_t)q * kSparseRec);
+    const double wq = R0[(size_t)q * kSparseRec + 2];
+    double Ng[NG], Du[NG], Dv[NG], Np[NP];
+    sparse_geom_basis<NG>(uv.x, uv.y, Ng, Du, Dv);
+    sparse_phys_basis<NP>(a.disc, a.beta, uv.x, uv.y, Np);
+    const SparseGeom sg = sparse_geometry<NG>(X, Ng, Du, Dv, x0, x1, x2);
     const double dx = sg.dx, dy = sg.dy, dz = sg.dz, J = sg.J, nd = sg.nd;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double rinv = rsqrt_nr(r2);
     const double r = r2 * rinv;
-    const double g = J * P[0] * kInv4PiS * rinv;       // J w / (4 pi r)
+    const double g = J * wq * kInv4PiS * rinv;         // J w / (4 pi r)
     const double ak = nd * rinv * rinv * g;             // (n.d) J w / (4 pi r^3)
     double sn, cs;
     sincos_f64(a.kr * r, &sn, &cs);
@@ -150,7 +233,7 @@ __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
     const double Gr = g * cs, Gi = g * sn;
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
-      const double ph = P[19 + s];
+      const double ph = Np[s];
       fr[s] = fma(Fr, ph, fr[s]);
       fi[s] = fma(Fi, ph, fi[s]);
       gr[s] = fma(Gr, ph, gr[s]);
@@ -194,6 +277,8 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
     a.len[c] = L.len[c];
   }
   a.hyp = L.hyp;
+  a.disc = L.disc;
+  a.beta = L.beta;
   a.kr = L.kr;
   a.ki = L.ki;
   a.flags = L.flags;
