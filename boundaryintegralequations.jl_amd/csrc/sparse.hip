@@ -173,6 +173,36 @@ __device__ __forceinline__ SparseGeom sparse_geometry(const double (&X)[NG][3],
   return sg;
 }
 
+// The kernels at one rule point: G = e^{ikr} J w / (4 pi r) and
+// F = e^{ikr} (ikr - 1) (n.d) J w / (4 pi r^3) (freens3d!, greens3d!, kernels.jl:13-37).
+template <bool CPLX>
+__device__ __forceinline__ void sparse_point_kernels(const SparseArgs& a, const SparseGeom& sg,
+                                                     double wq, double& Fr, double& Fi,
+                                                     double& Gr, double& Gi) {
+  const double dx = sg.dx, dy = sg.dy, dz = sg.dz, J = sg.J, nd = sg.nd;
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rinv = rsqrt_nr(r2);
+  const double r = r2 * rinv;
+  const double g = J * wq * kInv4PiS * rinv;         // J w / (4 pi r)
+  const double ak = nd * rinv * rinv * g;             // (n.d) J w / (4 pi r^3)
+  double sn, cs;
+  sincos_f64(a.kr * r, &sn, &cs);
+  if (CPLX) {
+    const double ex = exp(-a.ki * r);
+    cs *= ex;
+    sn *= ex;
+    const double ar = fma(-a.ki, r, -1.0), ai = a.kr * r;   // i k r - 1
+    Fr = ak * fma(cs, ar, -sn * ai);
+    Fi = ak * fma(cs, ai, sn * ar);
+  } else {
+    const double krr = a.kr * r;
+    Fr = -ak * fma(krr, sn, cs);
+    Fi = ak * fma(krr, cs, -sn);
+  }
+  Gr = g * cs;
+  Gi = g * sn;
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -209,28 +239,8 @@ __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
     sparse_geom_basis<NG>(uv.x, uv.y, Ng, Du, Dv);
     sparse_phys_basis<NP>(a.disc, a.beta, uv.x, uv.y, Np);
     const SparseGeom sg = sparse_geometry<NG>(X, Ng, Du, Dv, x0, x1, x2);
-    const double dx = sg.dx, dy = sg.dy, dz = sg.dz, J = sg.J, nd = sg.nd;
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double rinv = rsqrt_nr(r2);
-    const double r = r2 * rinv;
-    const double g = J * wq * kInv4PiS * rinv;         // J w / (4 pi r)
-    const double ak = nd * rinv * rinv * g;             // (n.d) J w / (4 pi r^3)
-    double sn, cs;
-    sincos_f64(a.kr * r, &sn, &cs);
-    double Fr, Fi;
-    if (CPLX) {
-      const double ex = exp(-a.ki * r);
-      cs *= ex;
-      sn *= ex;
-      const double ar = fma(-a.ki, r, -1.0), ai = a.kr * r;   // i k r - 1
-      Fr = ak * fma(cs, ar, -sn * ai);
-      Fi = ak * fma(cs, ai, sn * ar);
-    } else {
-      const double krr = a.kr * r;
-      Fr = -ak * fma(krr, sn, cs);
-      Fi = ak * fma(krr, cs, -sn);
-    }
-    const double Gr = g * cs, Gi = g * sn;
+    double Fr, Fi, Gr, Gi;
+    sparse_point_kernels<CPLX>(a, sg, wq, Fr, Fi, Gr, Gi);
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
       const double ph = Np[s];
@@ -255,6 +265,61 @@ __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
         unsafeAtomicAdd(a.G + e, vgr);
         unsafeAtomicAdd(a.G + e + 1, vgi);
       }
+    }
+  }
+}
+
+// Short rules (the partial near field's gauss_points_triangle(n), n <= 13): one thread per
+// pair instead of one wave, the same per-point evaluation; the thread adds its sums into
+// the entries with FP64 atomics.
+template <int NG, int NP, bool CPLX>
+__global__ void __launch_bounds__(256) k_sparse_pairs_short(const SparseArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n_pairs) return;
+  const int2 pr = a.pairs[p];
+  const double x0 = a.src[3 * (size_t)pr.x], x1 = a.src[3 * (size_t)pr.x + 1],
+               x2 = a.src[3 * (size_t)pr.x + 2];
+  const int c = select_rule<NG>(x0, x1, x2, a.gsel + (size_t)pr.y * NG * 4, 6, a.hyp) - 1;
+  double X[NG][3];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int node = a.topo[(size_t)pr.y * NG + g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[g][d] = a.coords[3 * (size_t)node + d];
+  }
+  double fr[NP], fi[NP], gr[NP], gi[NP];
+#pragma unroll
+  for (int s = 0; s < NP; ++s) fr[s] = fi[s] = gr[s] = gi[s] = 0.0;
+  const double* R0 = a.rules + (size_t)a.off[c] * kSparseRec;
+  const int Q = a.len[c];
+  for (int q = 0; q < Q; ++q) {
+    const double2 uv = *reinterpret_cast<const double2*>(R0 + (size_t)q * kSparseRec);
+    const double wq = R0[(size_t)q * kSparseRec + 2];
+    double Ng[NG], Du[NG], Dv[NG], Np[NP];
+    sparse_geom_basis<NG>(uv.x, uv.y, Ng, Du, Dv);
+    sparse_phys_basis<NP>(a.disc, a.beta, uv.x, uv.y, Np);
+    const SparseGeom sg = sparse_geometry<NG>(X, Ng, Du, Dv, x0, x1, x2);
+    double Fr, Fi, Gr, Gi;
+    sparse_point_kernels<CPLX>(a, sg, wq, Fr, Fi, Gr, Gi);
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      fr[s] = fma(Fr, Np[s], fr[s]);
+      fi[s] = fma(Fi, Np[s], fi[s]);
+      gr[s] = fma(Gr, Np[s], gr[s]);
+      gi[s] = fma(Gi, Np[s], gi[s]);
+    }
+  }
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+#pragma unroll
+  for (int s = 0; s < NP; ++s) {
+    const size_t e = 2 * (size_t)a.pos[p * NP + s];
+    if (doF) {
+      unsafeAtomicAdd(a.F + e, fr[s]);
+      unsafeAtomicAdd(a.F + e + 1, fi[s]);
+    }
+    if (doG) {
+      unsafeAtomicAdd(a.G + e, gr[s]);
+      unsafeAtomicAdd(a.G + e + 1, gi[s]);
     }
   }
 }
@@ -284,20 +349,31 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
   a.flags = L.flags;
   a.F = L.F;
   a.G = L.G;
-  const int64_t blocks = (L.n_pairs + 3) / 4;
+  // every slot's rule short (the partial near field): one thread per pair
+  int qmax = 0;
+  for (int c = 0; c < 6; ++c) qmax = std::max(qmax, L.len[c]);
+  const bool shortr = qmax <= 16;
+  const int64_t blocks = shortr ? (L.n_pairs + 255) / 256 : (L.n_pairs + 3) / 4;
   BIE_REQUIRE(blocks < 2147483647LL, BIE_EINVAL, "too many sparse pairs");
   const dim3 grid((unsigned)blocks);
   const bool cplx = L.ki != 0.0;
   auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, a); };
   // geometry nodes 3 or 6; physics nodes 1 (discontinuous constant), 3 or 6
+  auto pick = [&](auto ng, auto np, auto cx) {
+    constexpr int NG = decltype(ng)::value, NP = decltype(np)::value;
+    constexpr bool C = decltype(cx)::value;
+    shortr ? go(k_sparse_pairs_short<NG, NP, C>) : go(k_sparse_pairs<NG, NP, C>);
+  };
+  auto by_cx = [&](auto ng, auto np) {
+    cplx ? pick(ng, np, std::true_type()) : pick(ng, np, std::false_type());
+  };
   auto by_np = [&](auto ng) {
-    constexpr int NG = decltype(ng)::value;
     if (L.NP == 1)
-      cplx ? go(k_sparse_pairs<NG, 1, true>) : go(k_sparse_pairs<NG, 1, false>);
+      by_cx(ng, std::integral_constant<int, 1>());
     else if (L.NP == 3)
-      cplx ? go(k_sparse_pairs<NG, 3, true>) : go(k_sparse_pairs<NG, 3, false>);
+      by_cx(ng, std::integral_constant<int, 3>());
     else
-      cplx ? go(k_sparse_pairs<NG, 6, true>) : go(k_sparse_pairs<NG, 6, false>);
+      by_cx(ng, std::integral_constant<int, 6>());
   };
   BIE_REQUIRE((L.NG == 3 || L.NG == 6) && (L.NP == 1 || L.NP == 3 || L.NP == 6), BIE_EINVAL,
               "internal: sparse kernel node counts");
