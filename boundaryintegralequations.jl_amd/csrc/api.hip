@@ -1358,6 +1358,10 @@ struct bie_sparse {
   double* d_rules = nullptr;
   int off[6] = {0, 0, 0, 0, 0, 0};
   int len[6] = {0, 0, 0, 0, 0, 0};
+  // batches of pairs sharing an element (k_sparse_batch; 8 ints each), none when too few
+  // pairs would share
+  int* d_batches = nullptr;
+  int64_t n_batches = 0;
   // the one rule of the partial assembly (gauss_points_triangle(partial_n)), built on demand
   double* d_partial = nullptr;
   int partial_n = -1, partial_len = 0;
@@ -1374,6 +1378,7 @@ void sparse_destroy_impl(bie_sparse* sp) {
   dfree(sp->d_pos);
   dfree(sp->d_gsel);
   dfree(sp->d_rules);
+  dfree(sp->d_batches);
   dfree(sp->d_partial);
   delete sp;
 }
@@ -1484,10 +1489,62 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset, int n_ga
     sp->len[c] = (int)u.size();
     append_rule_records(rec, u, v, w);
   }
+  // Batches for k_sparse_batch: each element's pairs ordered by the rule the mesh sources
+  // select (nearest geometry node in plain squared distance: only a grouping, the kernel
+  // selects every row's rule exactly), cut into runs of at most B pairs of one rule.
+  std::vector<int> brec;
+  int64_t nb = 0;
+  {
+    const int B = sparse_batch_rows(NP);
+    std::vector<int64_t> first((size_t)dm->E + 1, 0);
+    for (int64_t t = 0; t < sp->n_pairs; ++t) ++first[(size_t)pairs[t].y + 1];
+    for (int64_t e = 0; e < dm->E; ++e) first[e + 1] += first[e];
+    std::vector<int64_t> order((size_t)sp->n_pairs), fill(first.begin(), first.end() - 1);
+    for (int64_t t = 0; t < sp->n_pairs; ++t) order[fill[pairs[t].y]++] = t;
+    std::vector<std::pair<int, int64_t>> key;
+    for (int64_t e = 0; e < dm->E; ++e) {
+      key.clear();
+      for (int64_t u = first[e]; u < first[e + 1]; ++u) {
+        const int64_t t = order[u];
+        const double* x = dm->h_sources.data() + 3 * (size_t)pairs[t].x;
+        int best = 0;
+        double bd = INFINITY;
+        for (int g = 0; g < NG; ++g) {
+          const double* y = gsel.data() + ((size_t)e * NG + g) * 4;
+          const double d = (y[0] - x[0]) * (y[0] - x[0]) + (y[1] - x[1]) * (y[1] - x[1]) +
+                           (y[2] - x[2]) * (y[2] - x[2]);
+          if (d < bd) {
+            bd = d;
+            best = g;
+          }
+        }
+        key.push_back({sp->off[best], t});  // slots sharing records share a rule
+      }
+      std::stable_sort(key.begin(), key.end(),
+                       [](const std::pair<int, int64_t>& l, const std::pair<int, int64_t>& r) {
+                         return l.first < r.first;
+                       });
+      size_t k = 0;
+      while (k < key.size()) {
+        int r8[8] = {(int)e, 0, -1, -1, -1, -1, 0, 0};
+        const int rule = key[k].first;
+        while (k < key.size() && r8[1] < B && key[k].first == rule)
+          r8[2 + r8[1]++] = (int)key[k++].second;
+        brec.insert(brec.end(), r8, r8 + 8);
+        ++nb;
+      }
+    }
+  }
   sp->d_pairs = upload_vec(pairs, nullptr);
   sp->d_pos = upload_vec(pos, nullptr);
   sp->d_gsel = upload_vec(gsel, nullptr);
   sp->d_rules = upload_vec(rec, nullptr);
+  // share only where it pays: on average at least 1.5 pairs per batch (the batched kernel
+  // holds B rows' sums and runs at lower occupancy)
+  if (nb > 0 && (double)sp->n_pairs >= 1.5 * (double)nb) {
+    sp->d_batches = upload_vec(brec, nullptr);
+    sp->n_batches = nb;
+  }
   BIE_HIP_CHECK(hipDeviceSynchronize());
   return sp.release();
 }
@@ -1514,6 +1571,8 @@ void sparse_assemble_impl(bie_sparse* sp, double kr, double ki, const double* d_
   L.topo = dm->d_topo;
   L.gsel = sp->d_gsel;
   L.rules = sp->d_rules;
+  L.batches = sp->d_batches;
+  L.n_batches = sp->n_batches;
   for (int c = 0; c < 6; ++c) {
     L.off[c] = sp->off[c];
     L.len[c] = sp->len[c];

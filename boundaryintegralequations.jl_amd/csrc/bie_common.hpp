@@ -256,6 +256,8 @@ struct SparseLaunch {
   const double* rules;    // point records (u, v, w, 0), sparse_record_doubles() each (device)
   int off[6], len[6];     // per geometry node: first record and point count
   int NG, NP, hyp;
+  const int* batches = nullptr;  // pair batches sharing an element (device; 8 ints each)
+  int64_t n_batches = 0;
   bool disc;              // discontinuous physics (basis at beta)
   double beta;
   double kr, ki;
@@ -286,6 +288,8 @@ void discontinuous_node(int pkind, double beta, int c, double* u, double* v);
 void physics_basis(int pkind, double beta, double u, double v, double* N);
 void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s);
 int sparse_record_doubles();
+// Rows per batch of the batched sparse kernel (k_sparse_batch) for NP physics nodes.
+int sparse_batch_rows(int NP);
 // Geometry basis of a geometry kind and its derivatives at (u, v) (rules.cpp); N, du, dv
 // hold 9 entries each (slots beyond the kind's node count are zeroed).
 void geometry_basis(int kind, double u, double v, double* N, double* du, double* dv);

@@ -45,6 +45,8 @@ struct SparseArgs {
   const double* gsel;     // E x NG x 4 geometry node coordinates (selection)
   const double* rules;    // point records (kSparseRec doubles)
   int off[6], len[6];
+  const int* batches;     // 8 ints per batch: element, count, up to 4 pair indices
+  int64_t n_batches;
   int hyp;
   bool disc;              // discontinuous physics, nodes at beta
   double beta;
@@ -170,6 +172,49 @@ __device__ __forceinline__ SparseGeom sparse_geometry(const double (&X)[NG][3],
   sg.dy = y[1] - x1;
   sg.dz = y[2] - x2;
   sg.nd = (-c0 / sg.J) * sg.dx + (-c1 / sg.J) * sg.dy + (-c2 / sg.J) * sg.dz;
+  return sg;
+}
+
+// The surface at one rule point without the source: y, the reference's normal
+// n = -(t_u x t_v) / J and J, operation for operation as sparse_geometry.
+template <int NG>
+__device__ __forceinline__ void sparse_surface(const double (&X)[NG][3], const double (&N)[NG],
+                                               const double (&Du)[NG], const double (&Dv)[NG],
+                                               double (&y)[3], double (&n)[3], double& J) {
+#pragma clang fp contract(off)
+  double tu[3], tv[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      a = a + X[g][d] * N[g];
+      b = b + X[g][d] * Du[g];
+      c = c + X[g][d] * Dv[g];
+    }
+    y[d] = a;
+    tu[d] = b;
+    tv[d] = c;
+  }
+  const double c0 = tu[1] * tv[2] - tu[2] * tv[1], c1 = tu[2] * tv[0] - tu[0] * tv[2],
+               c2 = tu[0] * tv[1] - tu[1] * tv[0];
+  J = julia_hypot3(c0, c1, c2);
+  n[0] = -c0 / J;
+  n[1] = -c1 / J;
+  n[2] = -c2 / J;
+}
+
+// d = y - x and n.d for one source, in sparse_geometry's order.
+__device__ __forceinline__ SparseGeom sparse_toward(const double (&y)[3], const double (&n)[3],
+                                                    double J, double x0, double x1,
+                                                    double x2) {
+#pragma clang fp contract(off)
+  SparseGeom sg;
+  sg.J = J;
+  sg.dx = y[0] - x0;
+  sg.dy = y[1] - x1;
+  sg.dz = y[2] - x2;
+  sg.nd = n[0] * sg.dx + n[1] * sg.dy + n[2] * sg.dz;
   return sg;
 }
 
@@ -324,6 +369,103 @@ __global__ void __launch_bounds__(256) k_sparse_pairs_short(const SparseArgs a) 
   }
 }
 
+// Batches of up to B pairs that share an element (built on the host, grouped by the rule
+// the mesh sources select): one wave per batch evaluates the surface at each point of a rule
+// once and the kernels for every row of the batch that selected that rule - the surface
+// (~2/3 of the per-point work) is shared.  Each row's rule is selected here exactly as in
+// k_sparse_pairs, so explicit sources stay correct (they only share less).
+template <int NG, int NP, int B, bool CPLX>
+__global__ void __launch_bounds__(256) k_sparse_batch(const SparseArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= a.n_batches) return;  // the whole wave leaves together
+  const int lane = threadIdx.x & 63;
+  const int* rec = a.batches + (size_t)b * 8;
+  const int e = rec[0], cnt = rec[1];
+  double X[NG][3];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int node = a.topo[(size_t)e * NG + g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[g][d] = a.coords[3 * (size_t)node + d];
+  }
+  double xs[B][3];
+  int cr[B], pj[B];
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    pj[j] = j < cnt ? rec[2 + j] : -1;
+    cr[j] = -1;
+    xs[j][0] = xs[j][1] = xs[j][2] = 0.0;
+    if (pj[j] >= 0) {
+      const int row = a.pairs[pj[j]].x;
+      xs[j][0] = a.src[3 * (size_t)row];
+      xs[j][1] = a.src[3 * (size_t)row + 1];
+      xs[j][2] = a.src[3 * (size_t)row + 2];
+      cr[j] = select_rule<NG>(xs[j][0], xs[j][1], xs[j][2], a.gsel + (size_t)e * NG * 4, 6,
+                              a.hyp) - 1;
+    }
+  }
+  double acc[B][NP][4];
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+#pragma unroll
+    for (int s = 0; s < NP; ++s) acc[j][s][0] = acc[j][s][1] = acc[j][s][2] = acc[j][s][3] = 0.0;
+  unsigned done = 0u;
+  for (int j0 = 0; j0 < B; ++j0) {
+    if (cr[j0] < 0 || ((done >> j0) & 1u)) continue;
+    const int c = cr[j0];
+    unsigned m = 0u;  // the rows whose rule has these records (filler slots share them)
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if (cr[j] >= 0 && a.off[cr[j]] == a.off[c] && a.len[cr[j]] == a.len[c]) m |= 1u << j;
+    done |= m;
+    const double* R0 = a.rules + (size_t)a.off[c] * kSparseRec;
+    const int Q = a.len[c];
+    for (int q = lane; q < Q; q += 64) {
+      const double2 uv = *reinterpret_cast<const double2*>(R0 + (size_t)q * kSparseRec);
+      const double wq = R0[(size_t)q * kSparseRec + 2];
+      double Ng[NG], Du[NG], Dv[NG], Np[NP], y[3], n[3], J;
+      sparse_geom_basis<NG>(uv.x, uv.y, Ng, Du, Dv);
+      sparse_phys_basis<NP>(a.disc, a.beta, uv.x, uv.y, Np);
+      sparse_surface<NG>(X, Ng, Du, Dv, y, n, J);
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        if (!((m >> j) & 1u)) continue;
+        const SparseGeom sg = sparse_toward(y, n, J, xs[j][0], xs[j][1], xs[j][2]);
+        double Fr, Fi, Gr, Gi;
+        sparse_point_kernels<CPLX>(a, sg, wq, Fr, Fi, Gr, Gi);
+#pragma unroll
+        for (int s = 0; s < NP; ++s) {
+          acc[j][s][0] = fma(Fr, Np[s], acc[j][s][0]);
+          acc[j][s][1] = fma(Fi, Np[s], acc[j][s][1]);
+          acc[j][s][2] = fma(Gr, Np[s], acc[j][s][2]);
+          acc[j][s][3] = fma(Gi, Np[s], acc[j][s][3]);
+        }
+      }
+    }
+  }
+  const bool doF = (a.flags & BIE_F) != 0, doG = (a.flags & BIE_G) != 0;
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    if (pj[j] < 0) continue;
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      const double vfr = wave_sum_d(acc[j][s][0]), vfi = wave_sum_d(acc[j][s][1]);
+      const double vgr = wave_sum_d(acc[j][s][2]), vgi = wave_sum_d(acc[j][s][3]);
+      if (lane == 0) {
+        const size_t ent = 2 * (size_t)a.pos[(size_t)pj[j] * NP + s];
+        if (doF) {
+          unsafeAtomicAdd(a.F + ent, vfr);
+          unsafeAtomicAdd(a.F + ent + 1, vfi);
+        }
+        if (doG) {
+          unsafeAtomicAdd(a.G + ent, vgr);
+          unsafeAtomicAdd(a.G + ent + 1, vgi);
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
@@ -342,6 +484,8 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
     a.len[c] = L.len[c];
   }
   a.hyp = L.hyp;
+  a.batches = L.batches;
+  a.n_batches = L.n_batches;
   a.disc = L.disc;
   a.beta = L.beta;
   a.kr = L.kr;
@@ -353,7 +497,10 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
   int qmax = 0;
   for (int c = 0; c < 6; ++c) qmax = std::max(qmax, L.len[c]);
   const bool shortr = qmax <= 16;
-  const int64_t blocks = shortr ? (L.n_pairs + 255) / 256 : (L.n_pairs + 3) / 4;
+  const bool batched = !shortr && L.batches != nullptr && L.n_batches > 0;
+  const int64_t blocks = shortr    ? (L.n_pairs + 255) / 256
+                         : batched ? (L.n_batches + 3) / 4
+                                   : (L.n_pairs + 3) / 4;
   BIE_REQUIRE(blocks < 2147483647LL, BIE_EINVAL, "too many sparse pairs");
   const dim3 grid((unsigned)blocks);
   const bool cplx = L.ki != 0.0;
@@ -362,7 +509,13 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
   auto pick = [&](auto ng, auto np, auto cx) {
     constexpr int NG = decltype(ng)::value, NP = decltype(np)::value;
     constexpr bool C = decltype(cx)::value;
-    shortr ? go(k_sparse_pairs_short<NG, NP, C>) : go(k_sparse_pairs<NG, NP, C>);
+    constexpr int B = NP == 1 ? 4 : 3;
+    if (shortr)
+      go(k_sparse_pairs_short<NG, NP, C>);
+    else if (batched)
+      go(k_sparse_batch<NG, NP, B, C>);
+    else
+      go(k_sparse_pairs<NG, NP, C>);
   };
   auto by_cx = [&](auto ng, auto np) {
     cplx ? pick(ng, np, std::true_type()) : pick(ng, np, std::false_type());
@@ -385,5 +538,7 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
 }
 
 int sparse_record_doubles() { return kSparseRec; }
+
+int sparse_batch_rows(int NP) { return NP == 1 ? 4 : 3; }
 
 }  // namespace bie
