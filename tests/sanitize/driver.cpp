@@ -201,6 +201,30 @@ void check_mesh(const Mesh& m) {
                                G.data()) == 0,
            "oracle sparse assembly");
   }
+  // the discontinuous pattern on the element-wise renumbering (three nodes per element):
+  // row r belongs to element r / 3; depth 1 its own nodes, depth 2 a superset of them
+  {
+    const int np = 3;
+    std::vector<int> dtopo((size_t)m.E * np);
+    for (int64_t t = 0; t < (int64_t)dtopo.size(); ++t) dtopo[t] = (int)t;
+    for (int depth : {1, 2}) {
+      std::vector<int64_t> row_ptr, pair_ptr;
+      std::vector<int> cols, pair_elem;
+      bie::sparse_pattern_disc(m.topo.data(), m.gk, dtopo.data(), np, m.E, m.E * np, depth,
+                               row_ptr, cols, pair_ptr, pair_elem);
+      bool ok = (int64_t)row_ptr.size() == m.E * np + 1;
+      for (int64_t r = 0; ok && r < m.E * np; ++r) {
+        const int64_t e = r / np;
+        int own = 0;
+        for (int64_t t = row_ptr[r]; t < row_ptr[r + 1]; ++t) {
+          own += cols[t] / np == e;
+          ok = ok && (t == row_ptr[r] || cols[t - 1] < cols[t]);
+        }
+        ok = ok && own == np && (depth == 2 || row_ptr[r + 1] - row_ptr[r] == np);
+      }
+      expect(ok, "discontinuous sparse pattern");
+    }
+  }
   // oracle selection and a dense assembly on a few targets
   const int64_t M = m.N < 16 ? m.N : 16;
   std::vector<int8_t> rules((size_t)M * m.E);
@@ -215,9 +239,26 @@ void check_mesh(const Mesh& m) {
 
 }  // namespace
 
+// The polar rules of discontinuous physics tile the reference triangle (weights sum to 1/2)
+// for every node of the three orders.
+void check_polar() {
+  for (int kind : {BIE_DTRI1, BIE_DTRI3, BIE_DTRI6})
+    for (int c = 0; c < (kind & 0xff); ++c) {
+      double pu, pv;
+      bie::discontinuous_node(kind, 0.2, c, &pu, &pv);
+      std::vector<double> u, v, w;
+      bie::polar_rule(pu, pv, 39, u, v, w);  // 40 radii and angles
+      double sum = 0.0;
+      for (double x : w) sum += x;
+      expect(u.size() == 3 * 1600 && std::fabs(sum - 0.5) < 1e-13,
+             "polar rule tiles the triangle");
+    }
+}
+
 int main(int argc, char** argv) {
   check_rules();
   check_subdivision();
+  check_polar();
   for (int a = 1; a < argc; ++a) {
     Mesh m;
     if (!read_mesh(argv[a], m)) {
