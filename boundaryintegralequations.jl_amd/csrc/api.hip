@@ -1436,7 +1436,8 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset, int n_ga
                      depth, sp->row_ptr, sp->cols, pair_ptr, pair_elem);
   sp->nnz = (int64_t)sp->cols.size();
   sp->n_pairs = (int64_t)pair_elem.size();
-  BIE_REQUIRE(sp->nnz < (int64_t)INT32_MAX, BIE_EINVAL, "sparse pattern too large");
+  BIE_REQUIRE(sp->nnz < (int64_t)INT32_MAX && sp->n_pairs * NP < (int64_t)INT32_MAX, BIE_EINVAL,
+              "sparse pattern too large");
   std::vector<int2> pairs((size_t)sp->n_pairs);
   std::vector<int> pos((size_t)sp->n_pairs * NP);
   for (int64_t i = 0; i < dm->N; ++i)
