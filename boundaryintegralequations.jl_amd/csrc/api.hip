@@ -1362,6 +1362,7 @@ struct bie_sparse {
   // pairs would share
   int* d_batches = nullptr;
   int64_t n_batches = 0;
+  double* d_trig = nullptr;  // the sincos table of the kernels (kTrigDoubles)
   // the one rule of the partial assembly (gauss_points_triangle(partial_n)), built on demand
   double* d_partial = nullptr;
   int partial_n = -1, partial_len = 0;
@@ -1379,6 +1380,7 @@ void sparse_destroy_impl(bie_sparse* sp) {
   dfree(sp->d_gsel);
   dfree(sp->d_rules);
   dfree(sp->d_batches);
+  dfree(sp->d_trig);
   dfree(sp->d_partial);
   delete sp;
 }
@@ -1540,6 +1542,15 @@ bie_sparse* sparse_create_impl(bie_dmesh* dm, int depth, double offset, int n_ga
   sp->d_pos = upload_vec(pos, nullptr);
   sp->d_gsel = upload_vec(gsel, nullptr);
   sp->d_rules = upload_vec(rec, nullptr);
+  {
+    std::vector<double> trig(kTrigDoubles);
+    for (int j = 0; j < kTrigN; ++j) {  // (cos, sin)(j pi / 64), rounded from long double
+      const long double th = (long double)j * (3.14159265358979323846264338327950288L / 64);
+      trig[2 * j] = (double)cosl(th);
+      trig[2 * j + 1] = (double)sinl(th);
+    }
+    sp->d_trig = upload_vec(trig, nullptr);
+  }
   // share only where it pays: on average at least 1.5 pairs per batch (the batched kernel
   // holds B rows' sums and runs at lower occupancy)
   if (nb > 0 && (double)sp->n_pairs >= 1.5 * (double)nb) {
@@ -1572,6 +1583,7 @@ void sparse_assemble_impl(bie_sparse* sp, double kr, double ki, const double* d_
   L.topo = dm->d_topo;
   L.gsel = sp->d_gsel;
   L.rules = sp->d_rules;
+  L.trig = sp->d_trig;
   L.batches = sp->d_batches;
   L.n_batches = sp->n_batches;
   for (int c = 0; c < 6; ++c) {
@@ -1637,6 +1649,7 @@ void sparse_partial_impl(bie_sparse* sp, double kr, double ki, int n_gauss,
   L.topo = dm->d_topo;
   L.gsel = sp->d_gsel;
   L.rules = sp->d_partial;
+  L.trig = sp->d_trig;
   for (int c = 0; c < 6; ++c) {  // whatever the selection, the one rule
     L.off[c] = 0;
     L.len[c] = sp->partial_len;

@@ -254,6 +254,7 @@ struct SparseLaunch {
   const int* topo;        // E x NG (device)
   const double* gsel;     // E x NG x 4 geometry node coordinates (device)
   const double* rules;    // point records (u, v, w, 0), sparse_record_doubles() each (device)
+  const double* trig;     // (cos, sin)(j pi / 64), j < kTrigN, interleaved (device)
   int off[6], len[6];     // per geometry node: first record and point count
   int NG, NP, hyp;
   const int* batches = nullptr;  // pair batches sharing an element (device; 8 ints each)

@@ -44,6 +44,7 @@ struct SparseArgs {
   const int* topo;        // E x NG
   const double* gsel;     // E x NG x 4 geometry node coordinates (selection)
   const double* rules;    // point records (kSparseRec doubles)
+  const double2* trig;    // (cos, sin)(j pi / 64), j < kTrigN
   int off[6], len[6];
   const int* batches;     // 8 ints per batch: element, count, up to 4 pair indices
   int64_t n_batches;
@@ -222,8 +223,8 @@ __device__ __forceinline__ SparseGeom sparse_toward(const double (&y)[3], const 
 // F = e^{ikr} (ikr - 1) (n.d) J w / (4 pi r^3) (freens3d!, greens3d!, kernels.jl:13-37).
 template <bool CPLX>
 __device__ __forceinline__ void sparse_point_kernels(const SparseArgs& a, const SparseGeom& sg,
-                                                     double wq, double& Fr, double& Fi,
-                                                     double& Gr, double& Gi) {
+                                                     double wq, const double2* tab, double& Fr,
+                                                     double& Fi, double& Gr, double& Gi) {
   const double dx = sg.dx, dy = sg.dy, dz = sg.dz, J = sg.J, nd = sg.nd;
   const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
   const double rinv = rsqrt_nr(r2);
@@ -231,7 +232,7 @@ __device__ __forceinline__ void sparse_point_kernels(const SparseArgs& a, const 
   const double g = J * wq * kInv4PiS * rinv;         // J w / (4 pi r)
   const double ak = nd * rinv * rinv * g;             // (n.d) J w / (4 pi r^3)
   double sn, cs;
-  sincos_f64(a.kr * r, &sn, &cs);
+  sincos_tab(a.kr * r, tab, &sn, &cs);
   if (CPLX) {
     const double ex = exp(-a.ki * r);
     cs *= ex;
@@ -248,6 +249,13 @@ __device__ __forceinline__ void sparse_point_kernels(const SparseArgs& a, const 
   Gi = g * sn;
 }
 
+// The block's copy of the trig table for sincos_tab (every thread of the block takes part,
+// before any leaves).
+__device__ __forceinline__ void load_trig(const SparseArgs& a, double2* s_trig) {
+  for (int j = threadIdx.x; j < kTrigN; j += blockDim.x) s_trig[j] = a.trig[j];
+  __syncthreads();
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -257,6 +265,8 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 template <int NG, int NP, bool CPLX>
 __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ double2 s_trig[kTrigN];
+  load_trig(a, s_trig);
   if (p >= a.n_pairs) return;  // the whole wave leaves together
   const int lane = threadIdx.x & 63;
   const int2 pr = a.pairs[p];
@@ -285,7 +295,7 @@ __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
     sparse_phys_basis<NP>(a.disc, a.beta, uv.x, uv.y, Np);
     const SparseGeom sg = sparse_geometry<NG>(X, Ng, Du, Dv, x0, x1, x2);
     double Fr, Fi, Gr, Gi;
-    sparse_point_kernels<CPLX>(a, sg, wq, Fr, Fi, Gr, Gi);
+    sparse_point_kernels<CPLX>(a, sg, wq, s_trig, Fr, Fi, Gr, Gi);
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
       const double ph = Np[s];
@@ -320,6 +330,8 @@ __global__ void __launch_bounds__(256) k_sparse_pairs(const SparseArgs a) {
 template <int NG, int NP, bool CPLX>
 __global__ void __launch_bounds__(256) k_sparse_pairs_short(const SparseArgs a) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double2 s_trig[kTrigN];
+  load_trig(a, s_trig);
   if (p >= a.n_pairs) return;
   const int2 pr = a.pairs[p];
   const double x0 = a.src[3 * (size_t)pr.x], x1 = a.src[3 * (size_t)pr.x + 1],
@@ -345,7 +357,7 @@ __global__ void __launch_bounds__(256) k_sparse_pairs_short(const SparseArgs a) 
     sparse_phys_basis<NP>(a.disc, a.beta, uv.x, uv.y, Np);
     const SparseGeom sg = sparse_geometry<NG>(X, Ng, Du, Dv, x0, x1, x2);
     double Fr, Fi, Gr, Gi;
-    sparse_point_kernels<CPLX>(a, sg, wq, Fr, Fi, Gr, Gi);
+    sparse_point_kernels<CPLX>(a, sg, wq, s_trig, Fr, Fi, Gr, Gi);
 #pragma unroll
     for (int s = 0; s < NP; ++s) {
       fr[s] = fma(Fr, Np[s], fr[s]);
@@ -377,6 +389,8 @@ __global__ void __launch_bounds__(256) k_sparse_pairs_short(const SparseArgs a) 
 template <int NG, int NP, int B, bool CPLX>
 __global__ void __launch_bounds__(256) k_sparse_batch(const SparseArgs a) {
   const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ double2 s_trig[kTrigN];
+  load_trig(a, s_trig);
   if (b >= a.n_batches) return;  // the whole wave leaves together
   const int lane = threadIdx.x & 63;
   const int* rec = a.batches + (size_t)b * 8;
@@ -432,7 +446,7 @@ __global__ void __launch_bounds__(256) k_sparse_batch(const SparseArgs a) {
         if (!((m >> j) & 1u)) continue;
         const SparseGeom sg = sparse_toward(y, n, J, xs[j][0], xs[j][1], xs[j][2]);
         double Fr, Fi, Gr, Gi;
-        sparse_point_kernels<CPLX>(a, sg, wq, Fr, Fi, Gr, Gi);
+        sparse_point_kernels<CPLX>(a, sg, wq, s_trig, Fr, Fi, Gr, Gi);
 #pragma unroll
         for (int s = 0; s < NP; ++s) {
           acc[j][s][0] = fma(Fr, Np[s], acc[j][s][0]);
@@ -479,6 +493,7 @@ void launch_sparse_pairs(const SparseLaunch& L, hipStream_t s) {
   a.topo = L.topo;
   a.gsel = L.gsel;
   a.rules = L.rules;
+  a.trig = reinterpret_cast<const double2*>(L.trig);
   for (int c = 0; c < 6; ++c) {
     a.off[c] = L.off[c];
     a.len[c] = L.len[c];
