@@ -110,3 +110,28 @@ def test_partial_near_field(physics, depth):
     ref = (I, np.concatenate(rows_col), Fo, Go)
     Fc, Gc = Fs.tocoo(), Gs.tocoo()
     _check(Fc.row, Fc.col, Fc.data, Gc.data, ref, mesh, f"partial {physics}")
+
+
+def test_batched_pairs_with_moved_sources():
+    """The batches are grouped by the rules the mesh sources select; explicit sources moved
+    off the nodes select other rules for some rows, which the kernel must follow (it selects
+    every row's rule itself).  Depth 2, continuous and discontinuous physics."""
+    import oracle
+    rng = np.random.default_rng(11)
+    for mesh in (M.icosphere(2, "quadratic"), disc_mesh("quadratic", "disctrilinear")):
+        N = mesh.n_sources
+        pts = mesh.sources + 0.02 * rng.standard_normal((3, N))
+        disc = mesh.physics_kind & 0x100
+        print(f"[case] batched pairs, moved sources, disc={bool(disc)} N={N}", flush=True)
+        Fs, Gs = sparse_assemble_parallel(mesh, 2.5, pts, depth=2, Ngauss=8)
+        if disc:
+            ref = oracle_np.sparse_assemble_disc(mesh, 2.5, pts, depth=2, Ngauss=8)
+        else:
+            ref = oracle.sparse_assemble(mesh, 2.5, pts, depth=2)
+        Fc, Gc = Fs.tocoo(), Gs.tocoo()
+        Io, Jo, Fo, Go = ref
+        assert np.array_equal(Fc.row, Io) and np.array_equal(Fc.col, Jo)
+        # off the surface nothing is rounding noise: every entry to 1e-12 of the largest
+        scale = max(np.max(np.abs(Fo)), np.max(np.abs(Go)))
+        assert np.max(np.abs(Gc.data - Go)) <= 1e-12 * np.max(np.abs(Go))
+        assert np.max(np.abs(Fc.data - Fo)) <= 1e-12 * scale
