@@ -135,3 +135,20 @@ def test_batched_pairs_with_moved_sources():
         scale = max(np.max(np.abs(Fo)), np.max(np.abs(Go)))
         assert np.max(np.abs(Gc.data - Go)) <= 1e-12 * np.max(np.abs(Go))
         assert np.max(np.abs(Fc.data - Fo)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("physics,depth,ng", [("disctrilinear", 1, 40), ("disctrilinear", 2, 6),
+                                              ("disctriquadratic", 1, 40)])
+def test_reference_mesh(physics, depth, ng):
+    """The reference's own 840-element sphere_1m (quadratic geometry) with discontinuous
+    physics from its loader, at the reference's default Ngauss and at a small one."""
+    import os
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    mesh = M.load_comsol(os.path.join(gold, "sphere_1m"), "quadratic", physics)
+    k = 2 * np.pi * 100.0 / 340.0
+    print(f"[case] disc sparse sphere_1m {physics} depth={depth} Ngauss={ng} "
+          f"N={mesh.n_sources}", flush=True)
+    Fs, Gs = sparse_assemble_parallel(mesh, k, depth=depth, Ngauss=ng)
+    ref = oracle_np.sparse_assemble_disc(mesh, k, depth=depth, Ngauss=ng)
+    Fc, Gc = Fs.tocoo(), Gs.tocoo()
+    _check(Fc.row, Fc.col, Fc.data, Gc.data, ref, mesh, f"sphere_1m {physics}")
