@@ -1,9 +1,10 @@
-"""The Julia boundary (julia/src/BoundaryIntegralEquationsAMD.jl; Julia is not installed
-here) against the C ABI it binds: every `ccall((:name, libbie[]), R, (T1, ..., Tn), ...)`
+"""The two host layers against the C ABI they bind.  The Julia boundary
+(julia/src/BoundaryIntegralEquationsAMD.jl; Julia is not installed here): every `ccall((:name, libbie[]), R, (T1, ..., Tn), ...)`
 names a function that include/bie_amd.h declares and the shared library exports, with as
 many argument types as the declaration has parameters, and a Cint result where the
-declaration returns int.  A renamed entry point or an added parameter that the Julia side
-missed fails here.
+declaration returns int.  The Python layer (bie_amd._lib.SIGNATURES): one ctypes signature
+per declared function, with the declared number of arguments.  A renamed entry point or an
+added parameter that either side missed fails here.
 """
 import ctypes
 import os
@@ -78,3 +79,17 @@ def test_julia_ccalls_match_the_header_and_the_library():
         assert hasattr(lib, name) and isinstance(getattr(lib, name), ctypes._CFuncPtr)
     # the near field with Ngauss is what the Julia hook calls
     assert any(n == "bie_sparse_assemble_ex" for n, _, _ in calls)
+
+
+def test_python_signatures_match_the_header():
+    """The ctypes table of the Python host layer (bie_amd._lib.SIGNATURES) against the
+    header: every entry is declared, with as many argument types as parameters, and every
+    function the header declares has an entry (so Python can reach all of them)."""
+    decls = _header_decls()
+    for name, (restype, argtypes) in _lib.SIGNATURES.items():
+        assert name in decls, f"{name} is not declared in include/bie_amd.h"
+        assert len(argtypes) == decls[name][1], f"{name}: {len(argtypes)} ctypes arguments"
+        if decls[name][0] == "int":
+            assert restype is ctypes.c_int, f"{name} returns int"
+    missing = sorted(set(decls) - set(_lib.SIGNATURES))
+    assert not missing, f"declared but without a ctypes signature: {missing}"
